@@ -1,0 +1,181 @@
+/*
+ * qla_mi355x.h -- drop-in C ABI for the QLA site-loop hot path on MI355X (gfx950).
+ *
+ * Every QLA_* prototype below has exactly the name, argument order and
+ * argument meaning of the symbol that the reference QLA 1.9.0 generator emits
+ * (lib/perl/prototype.pl:590-688 builds the argument list: dest, src1, src2,
+ * src2 extras `int mu, int sign`, gang `int *index`, `int n`; the leading
+ * `int nc` appears only for the N-colour library, prototype.pl:594-600).
+ * Type layouts are the reference's site-major AoS records
+ * (include/qla_types.h:29-33): ColorVector V[3], HalfFermion V[3][2],
+ * DiracFermion V[3][4], ColorMatrix V[3][3], complex = {re, im} with float
+ * complex aligned to 8 B and double complex to 16 B (include/qla_complex.h:27-35).
+ *
+ * Device semantics (the one deliberate difference from the CPU library):
+ *   - every field pointer, every pointer array, every pointee of a pointer
+ *     array and every index array is a DEVICE address;
+ *   - the result pointer of a global reduction (r_veq_norm2_*, c_veq_*_dot_*)
+ *     may be a host or a device address;
+ *   - by default each call returns with its results complete (QLA's
+ *     synchronous contract); QLA_MI355X_set_sync(0) turns a host thread's
+ *     calls into stream-ordered launches on the stream set by
+ *     QLA_MI355X_set_stream (reductions into host memory still synchronise).
+ * Behaviour kept from the reference: n <= 0 is a no-op (a reduction then
+ * writes 0); mu outside 0..4 is a silent no-op (expressions_gamma.pl:349-381
+ * emits a switch without default); any sign != 1 means -1; mu = 4 is the
+ * chiral projection.  Arguments are restrict: destinations must not overlap
+ * sources.  No 16 B alignment is required of any argument.
+ */
+#ifndef QLA_MI355X_H
+#define QLA_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#define QLA_MI355X_RESTRICT __restrict__
+#else
+#define QLA_MI355X_RESTRICT restrict
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types (layout-identical to include/qla_complex.h:27-35, qla_types.h:29-50) */
+/* ------------------------------------------------------------------------ */
+#ifndef _QLA_TYPES_H
+typedef float QLA_F_Real;
+typedef double QLA_D_Real;
+typedef struct { QLA_F_Real real; QLA_F_Real imag; } __attribute__((aligned(8))) QLA_F_Complex;
+typedef struct { QLA_D_Real real; QLA_D_Real imag; } __attribute__((aligned(16))) QLA_D_Complex;
+typedef QLA_F_Complex QLA_F3_ColorVector[3];
+typedef QLA_F_Complex QLA_F3_HalfFermion[3][2];
+typedef QLA_F_Complex QLA_F3_DiracFermion[3][4];
+typedef QLA_F_Complex QLA_F3_ColorMatrix[3][3];
+typedef QLA_D_Complex QLA_D3_ColorVector[3];
+typedef QLA_D_Complex QLA_D3_HalfFermion[3][2];
+typedef QLA_D_Complex QLA_D3_DiracFermion[3][4];
+typedef QLA_D_Complex QLA_D3_ColorMatrix[3][3];
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* a1/a2: matrix times colour vector.                                        */
+/* Reference: generated from lib/perl/expressions_tensor.pl:544-630 via     */
+/* build_qla.pl:1587-1622; accumulation x=0 (eq) or x=r (peq), then         */
+/* x += a[ic][kc]*b[kc] for kc = 0,1,2 with QLA_c_peq_c_times_c             */
+/* (include/qla_complex.h:330-333).                                          */
+/* ------------------------------------------------------------------------ */
+void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                            QLA_F3_ColorVector *QLA_MI355X_RESTRICT b, int n);
+void QLA_D3_V_veq_M_times_V(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                            QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                            QLA_D3_ColorVector *QLA_MI355X_RESTRICT b, int n);
+/* Reference (generic order): expressions_tensor.pl:581-583 (x = r start);
+ * SSE variant lib/optimized/sse/src/QLA_F3_V_vpeq_M_times_pV.c:59-103. */
+void QLA_F3_V_vpeq_M_times_pV(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b, int n);
+void QLA_D3_V_vpeq_M_times_pV(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b, int n);
+
+/* ------------------------------------------------------------------------ */
+/* a3/a4: global reductions.  Reference: make_code_unary.pl:318-366         */
+/* (norm2: per component the square is formed in the operand precision and */
+/* added to a sum promoted per datatypes.pl:355-359, F -> D, D -> D) and    */
+/* make_code_binary.pl:110-170 (dot: sum += conj(a)*b with operands         */
+/* promoted first, QLA_c_peq_ca_times_c, qla_complex.h:386-389).            */
+/* ------------------------------------------------------------------------ */
+void QLA_F3_r_veq_norm2_V(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a, int n);
+void QLA_D3_r_veq_norm2_V(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT a, int n);
+void QLA_F3_r_veq_norm2_D(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
+void QLA_D3_r_veq_norm2_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
+/* Mixed precision (lib qla_df3; build_qla.pl:2119-2156, 2273-2359). */
+void QLA_DF3_r_veq_norm2_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
+void QLA_F3_c_veq_V_dot_V(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT b, int n);
+void QLA_F3_c_veq_D_dot_D(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
+void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
+void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
+
+/* ------------------------------------------------------------------------ */
+/* a5/a6: spin projection of a gathered DiracFermion, then colour matrix    */
+/* times the projected HalfFermion.  Reference: build_qla.pl:1161-1190,     */
+/* make_code_binary.pl:189-222,259-271, tables expressions_gamma.pl:244-276.*/
+/* x-form: every operand at index[i] (gang indexing, indirection.pl:47-58). */
+/* ------------------------------------------------------------------------ */
+void QLA_F3_H_veq_spproj_M_times_pD(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int n);
+void QLA_F3_H_xeq_spproj_M_times_pD(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int *index, int n);
+void QLA_D3_H_veq_spproj_M_times_pD(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int n);
+void QLA_D3_H_xeq_spproj_M_times_pD(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int *index, int n);
+
+/* ------------------------------------------------------------------------ */
+/* a7/a8: colour matrix times a gathered HalfFermion, then spin             */
+/* reconstruction added into a DiracFermion.  Reference:                    */
+/* build_qla.pl:1191-1220, make_code_binary.pl:224-257, tables              */
+/* expressions_gamma.pl:311-337; SSE variant                                */
+/* lib/optimized/sse/src/QLA_F3_D_vpeq_sprecon_M_times_pH.c.                */
+/* ------------------------------------------------------------------------ */
+void QLA_F3_D_vpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int n);
+void QLA_F3_D_xpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int *index, int n);
+void QLA_D3_D_vpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int n);
+void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int *index, int n);
+
+/* ------------------------------------------------------------------------ */
+/* Library control (no counterpart in the reference, which has no device   */
+/* and no stream; these only select where and how the calls above run).    */
+/* ------------------------------------------------------------------------ */
+/* Per host thread: the HIP stream (hipStream_t) the calls launch on; NULL
+ * selects the default stream. */
+void QLA_MI355X_set_stream(void *stream);
+void *QLA_MI355X_get_stream(void);
+/* Per host thread: 1 (default) = each call returns with results complete,
+ * 0 = calls are stream-ordered and return after the launch. */
+void QLA_MI355X_set_sync(int on);
+/* Wait for the calling thread's stream. */
+void QLA_MI355X_synchronize(void);
+/* 0 if no HIP error has been seen on this thread since the last call, else
+ * the hipError_t value; QLA_MI355X_last_error_string describes it. The QLA
+ * entry points themselves return void, like the reference. */
+int QLA_MI355X_last_error(void);
+const char *QLA_MI355X_last_error_string(void);
+/* Library version string. */
+const char *QLA_MI355X_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QLA_MI355X_H */

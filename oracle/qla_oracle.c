@@ -1,0 +1,108 @@
+/*
+ * qla_oracle.c -- CPU restatement of the QLA 1.9.0 hot path (the oracle).
+ * TEST INFRASTRUCTURE ONLY: loaded by the tests, the smoke check and the
+ * CPU-baseline leg of the benchmark; never by the product library.
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
+ */
+#include <math.h>
+#include <float.h>
+#include <string.h>
+#include <stddef.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "qla_oracle.h"
+
+/* single precision instantiation */
+#define REAL float
+#define P(x) F_##x
+#define F_EPS FLT_EPSILON
+#include "qla_oracle_impl.inc"
+#undef REAL
+#undef P
+
+/* double precision instantiation */
+#define REAL double
+#define P(x) D_##x
+#define D_EPS DBL_EPSILON
+#include "qla_oracle_impl.inc"
+#undef REAL
+#undef P
+
+void orc_F3_V_eqop_M_times_V(int eqop, int adj, float *r, const float *a, const float *b,
+                             const float *const *pb, const int *index, int n)
+{ F_orc_3_V_eqop_M_times_V(eqop, adj, r, a, b, pb, index, n); }
+void orc_D3_V_eqop_M_times_V(int eqop, int adj, double *r, const double *a, const double *b,
+                             const double *const *pb, const int *index, int n)
+{ D_orc_3_V_eqop_M_times_V(eqop, adj, r, a, b, pb, index, n); }
+
+double orc_F_norm2(const float *a, int ncomp, int n) { return F_orc_norm2(a, ncomp, n); }
+double orc_D_norm2(const double *a, int ncomp, int n) { return D_orc_norm2(a, ncomp, n); }
+long double orc_F_norm2_ld(const float *a, int ncomp, int n) { return F_orc_norm2_ld(a, ncomp, n); }
+long double orc_D_norm2_ld(const double *a, int ncomp, int n) { return D_orc_norm2_ld(a, ncomp, n); }
+void orc_F_dot(double *re, double *im, const float *a, const float *b, int ncomp, int n)
+{ F_orc_dot(re, im, a, b, ncomp, n); }
+void orc_D_dot(double *re, double *im, const double *a, const double *b, int ncomp, int n)
+{ D_orc_dot(re, im, a, b, ncomp, n); }
+void orc_F_dot_ld(long double *re, long double *im, const float *a, const float *b, int ncomp, int n)
+{ F_orc_dot_ld(re, im, a, b, ncomp, n); }
+void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n)
+{ D_orc_dot_ld(re, im, a, b, ncomp, n); }
+
+void orc_F3_H_eqop_spproj_M_times_pD(int eqop, int adj, float *r, const float *a,
+                                     const float *const *b, int mu, int sign,
+                                     const int *index, int n)
+{ F_orc_3_H_eqop_spproj_M_times_pD(eqop, adj, r, a, b, mu, sign, index, n); }
+void orc_D3_H_eqop_spproj_M_times_pD(int eqop, int adj, double *r, const double *a,
+                                     const double *const *b, int mu, int sign,
+                                     const int *index, int n)
+{ D_orc_3_H_eqop_spproj_M_times_pD(eqop, adj, r, a, b, mu, sign, index, n); }
+void orc_F3_D_eqop_sprecon_M_times_pH(int eqop, int adj, float *r, const float *a,
+                                      const float *const *b, int mu, int sign,
+                                      const int *index, int n)
+{ F_orc_3_D_eqop_sprecon_M_times_pH(eqop, adj, r, a, b, mu, sign, index, n); }
+void orc_D3_D_eqop_sprecon_M_times_pH(int eqop, int adj, double *r, const double *a,
+                                      const double *const *b, int mu, int sign,
+                                      const int *index, int n)
+{ D_orc_3_D_eqop_sprecon_M_times_pH(eqop, adj, r, a, b, mu, sign, index, n); }
+
+void orc_D_spproj(double *h, const double *d, int mu, int sign)
+{
+    D_cpx *H = (D_cpx *)h;
+    const D_cpx *Dd = (const D_cpx *)d;
+    for (int ic = 0; ic < 3; ic++) D_proj_row(&H[2 * ic], &H[2 * ic + 1], Dd + 4 * ic, mu, sign);
+}
+void orc_D_sprecon(double *d, const double *h, int mu, int sign)
+{
+    D_recon_t tab;
+    if (!D_recon_table(&tab, mu, sign)) return;
+    D_cpx *Dd = (D_cpx *)d;
+    const D_cpx *H = (const D_cpx *)h;
+    for (int ic = 0; ic < 3; ic++)
+        for (int sp = 0; sp < 4; sp++)
+            D_recon_apply(ORC_EQ, tab.op[sp], &Dd[4 * ic + sp], H[2 * ic + tab.src[sp]]);
+}
+
+void orc_F_M_veq_exp_M(int nc, float *r, const float *a, int n) { F_orc_M_veq_exp_M(nc, r, a, n); }
+void orc_D_M_veq_exp_M(int nc, double *r, const double *a, int n) { D_orc_M_veq_exp_M(nc, r, a, n); }
+void orc_F_M_veq_sqrt_M(int nc, float *r, const float *a, int n) { F_orc_M_veq_sqrt_M(nc, r, a, n); }
+void orc_D_M_veq_sqrt_M(int nc, double *r, const double *a, int n) { D_orc_M_veq_sqrt_M(nc, r, a, n); }
+void orc_F_M_veq_invsqrt_M(int nc, float *r, const float *a, int n) { F_orc_M_veq_invsqrt_M(nc, r, a, n); }
+void orc_D_M_veq_invsqrt_M(int nc, double *r, const double *a, int n) { D_orc_M_veq_invsqrt_M(nc, r, a, n); }
+
+void orc_set_threads(int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+}
+int orc_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,222 @@
+// qla_capi.hip -- the C ABI: QLA symbols and argument lists (declared in
+// include/qla_mi355x.h) on device-resident arrays.
+//
+// Each entry point validates nothing the reference does not (QLA has no error
+// returns), launches its kernel on the calling thread's stream and, in the
+// default synchronous mode, waits for it, so a call returns with its results
+// complete as the CPU library's does.  Reductions accept a host or a device
+// result pointer (hipPointerGetAttributes tells which).
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <mutex>
+#include <unordered_map>
+#include "qla_launch.hpp"
+#include "../../include/qla_mi355x.h"
+
+#define QLA_MI355X_VERSION_STRING "qla_mi355x 0.1 (QLA 1.9.0 hot-path ABI, gfx950)"
+
+namespace {
+
+struct thread_state {
+    hipStream_t stream = nullptr;
+    int sync = 1;
+    hipError_t err = hipSuccess;
+    // per-device reduction scratch: partials + one result slot (2 doubles)
+    std::unordered_map<int, double *> scratch;
+};
+
+thread_local thread_state ts;
+
+void record(hipError_t e, const char *where)
+{
+    if (e == hipSuccess) return;
+    ts.err = e;
+    fprintf(stderr, "qla_mi355x: %s: %s\n", where, hipGetErrorString(e));
+}
+
+void finish(hipError_t e, const char *where)
+{
+    record(e, where);
+    if (e == hipSuccess && ts.sync) record(hipStreamSynchronize(ts.stream), where);
+}
+
+double *scratch_for_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = ts.scratch.find(dev);
+    if (it != ts.scratch.end()) return it->second;
+    double *p = nullptr;
+    size_t bytes = sizeof(double) * (qla_reduce_scratch_doubles() + 4);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess) { record(e, "reduction scratch"); return nullptr; }
+    ts.scratch[dev] = p;
+    return p;
+}
+
+bool is_device_pointer(const void *p)
+{
+    hipPointerAttribute_t at;
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) { (void)hipGetLastError(); return false; }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
+
+// run a reduction and deliver `bytes` of result to r (host or device)
+void reduce(int prec, int kind, const void *a, const void *b, long long m, void *r, int out_kind,
+            size_t bytes, const char *name)
+{
+    double *sc = scratch_for_device();
+    if (!sc) return;
+    if (is_device_pointer(r)) {
+        finish(qla_launch_reduce2(prec, kind, a, b, m, r, out_kind, sc, ts.stream), name);
+        return;
+    }
+    double *slot = sc + qla_reduce_scratch_doubles();
+    hipError_t e = qla_launch_reduce2(prec, kind, a, b, m, slot, out_kind, sc, ts.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    record(e, name);
+}
+
+} // namespace
+
+extern "C" {
+
+// ----------------------------------------------------------------- control
+void QLA_MI355X_set_stream(void *stream) { ts.stream = (hipStream_t)stream; }
+void *QLA_MI355X_get_stream(void) { return (void *)ts.stream; }
+void QLA_MI355X_set_sync(int on) { ts.sync = on ? 1 : 0; }
+void QLA_MI355X_synchronize(void) { record(hipStreamSynchronize(ts.stream), "synchronize"); }
+int QLA_MI355X_last_error(void)
+{
+    int e = (int)ts.err;
+    ts.err = hipSuccess;
+    return e;
+}
+const char *QLA_MI355X_last_error_string(void) { return hipGetErrorString(ts.err); }
+const char *QLA_MI355X_version(void) { return QLA_MI355X_VERSION_STRING; }
+
+// ----------------------------------------------------------------- a1 / a2
+void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *b, int n)
+{
+    if (n <= 0) return;
+    finish(qla_launch_matvec(0, 0, 0, r, a, b, nullptr, n, ts.stream), "QLA_F3_V_veq_M_times_V");
+}
+void QLA_D3_V_veq_M_times_V(QLA_D3_ColorVector *r, QLA_D3_ColorMatrix *a, QLA_D3_ColorVector *b, int n)
+{
+    if (n <= 0) return;
+    finish(qla_launch_matvec(1, 0, 0, r, a, b, nullptr, n, ts.stream), "QLA_D3_V_veq_M_times_V");
+}
+void QLA_F3_V_vpeq_M_times_pV(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b, int n)
+{
+    if (n <= 0) return;
+    finish(qla_launch_matvec(0, 1, 1, r, a, (const void *)b, nullptr, n, ts.stream), "QLA_F3_V_vpeq_M_times_pV");
+}
+void QLA_D3_V_vpeq_M_times_pV(QLA_D3_ColorVector *r, QLA_D3_ColorMatrix *a, QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b, int n)
+{
+    if (n <= 0) return;
+    finish(qla_launch_matvec(1, 1, 1, r, a, (const void *)b, nullptr, n, ts.stream), "QLA_D3_V_vpeq_M_times_pV");
+}
+
+// ----------------------------------------------------------------- a3 / a4
+void QLA_F3_r_veq_norm2_V(QLA_F_Real *r, QLA_F3_ColorVector *a, int n)
+{
+    reduce(0, 0, a, nullptr, n > 0 ? 3LL * n : 0, r, 1, sizeof(float), "QLA_F3_r_veq_norm2_V");
+}
+void QLA_D3_r_veq_norm2_V(QLA_D_Real *r, QLA_D3_ColorVector *a, int n)
+{
+    reduce(1, 0, a, nullptr, n > 0 ? 3LL * n : 0, r, 0, sizeof(double), "QLA_D3_r_veq_norm2_V");
+}
+void QLA_F3_r_veq_norm2_D(QLA_F_Real *r, QLA_F3_DiracFermion *a, int n)
+{
+    reduce(0, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 1, sizeof(float), "QLA_F3_r_veq_norm2_D");
+}
+void QLA_D3_r_veq_norm2_D(QLA_D_Real *r, QLA_D3_DiracFermion *a, int n)
+{
+    reduce(1, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 0, sizeof(double), "QLA_D3_r_veq_norm2_D");
+}
+void QLA_DF3_r_veq_norm2_D(QLA_D_Real *r, QLA_F3_DiracFermion *a, int n)
+{
+    reduce(0, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 0, sizeof(double), "QLA_DF3_r_veq_norm2_D");
+}
+void QLA_F3_c_veq_V_dot_V(QLA_F_Complex *r, QLA_F3_ColorVector *a, QLA_F3_ColorVector *b, int n)
+{
+    reduce(0, 1, a, b, n > 0 ? 3LL * n : 0, r, 3, 2 * sizeof(float), "QLA_F3_c_veq_V_dot_V");
+}
+void QLA_F3_c_veq_D_dot_D(QLA_F_Complex *r, QLA_F3_DiracFermion *a, QLA_F3_DiracFermion *b, int n)
+{
+    reduce(0, 1, a, b, n > 0 ? 12LL * n : 0, r, 3, 2 * sizeof(float), "QLA_F3_c_veq_D_dot_D");
+}
+void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_D3_DiracFermion *a, QLA_D3_DiracFermion *b, int n)
+{
+    reduce(1, 1, a, b, n > 0 ? 12LL * n : 0, r, 2, 2 * sizeof(double), "QLA_D3_c_veq_D_dot_D");
+}
+void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_F3_DiracFermion *a, QLA_F3_DiracFermion *b, int n)
+{
+    reduce(0, 1, a, b, n > 0 ? 12LL * n : 0, r, 2, 2 * sizeof(double), "QLA_DF3_c_veq_D_dot_D");
+}
+
+// ----------------------------------------------------------------- a5 / a6
+void QLA_F3_H_veq_spproj_M_times_pD(QLA_F3_HalfFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_spproj(0, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
+           "QLA_F3_H_veq_spproj_M_times_pD");
+}
+void QLA_F3_H_xeq_spproj_M_times_pD(QLA_F3_HalfFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int *index, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_spproj(0, r, a, (const void *)b, mu, sign, index, n, ts.stream),
+           "QLA_F3_H_xeq_spproj_M_times_pD");
+}
+void QLA_D3_H_veq_spproj_M_times_pD(QLA_D3_HalfFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_spproj(1, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
+           "QLA_D3_H_veq_spproj_M_times_pD");
+}
+void QLA_D3_H_xeq_spproj_M_times_pD(QLA_D3_HalfFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
+                                    int mu, int sign, int *index, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_spproj(1, r, a, (const void *)b, mu, sign, index, n, ts.stream),
+           "QLA_D3_H_xeq_spproj_M_times_pD");
+}
+
+// ----------------------------------------------------------------- a7 / a8
+void QLA_F3_D_vpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_sprecon(0, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
+           "QLA_F3_D_vpeq_sprecon_M_times_pH");
+}
+void QLA_F3_D_xpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int *index, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_sprecon(0, r, a, (const void *)b, mu, sign, index, n, ts.stream),
+           "QLA_F3_D_xpeq_sprecon_M_times_pH");
+}
+void QLA_D3_D_vpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_sprecon(1, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
+           "QLA_D3_D_vpeq_sprecon_M_times_pH");
+}
+void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
+                                      int mu, int sign, int *index, int n)
+{
+    if (n <= 0 || mu < 0 || mu > 4) return;
+    finish(qla_launch_sprecon(1, r, a, (const void *)b, mu, sign, index, n, ts.stream),
+           "QLA_D3_D_xpeq_sprecon_M_times_pH");
+}
+
+} // extern "C"

@@ -1,0 +1,251 @@
+// qla_dev.hpp -- device building blocks for the gfx950 site-loop kernels.
+//
+// Layout facts these helpers rely on (reference include/qla_types.h:29-33):
+// every site record is an array of {re, im} complex numbers, site-major, with
+// no padding, so a field of n records is n*BPS contiguous bytes and a pointer
+// array holds the start address of each record.
+//
+// Data movement model.  A wavefront owns a tile of 64 sites.  Whole records
+// are moved between HBM and LDS with gfx950 direct-to-LDS loads
+// (global_load_lds_dword{x3,x4}): lane l of one instruction moves the chunk
+// that lands at LDS offset base + l*CH, so consecutive lanes read consecutive
+// chunks of the same or the next record -- coalesced for contiguous fields and
+// for pointer gathers whose targets are adjacent.  After the loads the wave
+// reads its own site's record from LDS at lane*BPS (the transpose), computes
+// in registers, writes the result record to LDS and stores it with the same
+// chunk mapping, again coalesced.
+//
+// Arithmetic.  Everything is compiled with -ffp-contract=off so each complex
+// multiply-accumulate rounds exactly like the reference macro it restates.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qla {
+
+constexpr int WAVE = 64;
+
+template <class T> struct cpx { T re, im; };
+
+typedef __attribute__((address_space(3))) void *lds_void_ptr;
+
+// Wave-scope barrier: orders LDS accesses of the lanes of one wavefront
+// (the direct-to-LDS loads are waited for by the compiler's vmcnt tracking).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// 64-bit value of lane `src` (ds_bpermute on both halves).
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
+{
+    int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    lo = __builtin_amdgcn_ds_bpermute(src << 2, lo);
+    hi = __builtin_amdgcn_ds_bpermute(src << 2, hi);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// ---------------------------------------------------------------------------
+// HBM -> LDS staging.  CH is the chunk size in bytes (12 or 16: gfx950 LDS
+// DMA widths); it divides BPS.  Records must be 4-byte aligned for CH = 12
+// and 16-byte aligned for CH = 16.
+// ---------------------------------------------------------------------------
+
+// One direct-to-LDS load of CH bytes per lane: lane l's chunk lands at
+// lds_base + l*CH (the builtin needs a literal width).
+template <int CH> __device__ __forceinline__ void dma_load(const void *g, char *lds_base);
+template <> __device__ __forceinline__ void dma_load<12>(const void *g, char *lds_base)
+{
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 12, 0, 0);
+}
+template <> __device__ __forceinline__ void dma_load<16>(const void *g, char *lds_base)
+{
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 16, 0, 0);
+}
+
+// Contiguous records g[0 .. nsites) -> lds (site s at s*BPS).
+template <int BPS, int CH>
+__device__ __forceinline__ void stage_contig(char *lds, const char *g, int nsites, int lane)
+{
+    static_assert(BPS % CH == 0, "chunk must divide the record");
+    constexpr int NPS = BPS / CH;
+    const int nch = nsites * NPS;
+#pragma unroll
+    for (int k = 0; k < NPS; k++) {
+        const int c = k * WAVE + lane;
+        if (c < nch) dma_load<CH>(g + (size_t)c * CH, lds + k * WAVE * CH);
+    }
+}
+
+// Records at per-site addresses: lane s holds the address of site s's record
+// (sites >= nsites are skipped) -> lds (site s at s*BPS).
+template <int BPS, int CH>
+__device__ __forceinline__ void stage_gather(char *lds, uint64_t my_addr, int nsites, int lane)
+{
+    static_assert(BPS % CH == 0, "chunk must divide the record");
+    constexpr int NPS = BPS / CH;
+#pragma unroll
+    for (int k = 0; k < NPS; k++) {
+        const int c = k * WAVE + lane;
+        const int s = c / NPS;
+        const int p = c - s * NPS;
+        const uint64_t a = shfl64(my_addr, s);
+        if (s < nsites) dma_load<CH>((const void *)(a + (uint64_t)p * CH), lds + k * WAVE * CH);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS -> HBM.  Chunk types for the stores.
+// ---------------------------------------------------------------------------
+struct __attribute__((aligned(4))) chunk12 { uint32_t x, y, z; };
+struct __attribute__((aligned(16))) chunk16 { uint32_t x, y, z, w; };
+template <int CH> struct chunk_of;
+template <> struct chunk_of<12> { typedef chunk12 type; };
+template <> struct chunk_of<16> { typedef chunk16 type; };
+
+template <int BPS, int CH>
+__device__ __forceinline__ void store_contig(char *g, const char *lds, int nsites, int lane)
+{
+    typedef typename chunk_of<CH>::type C;
+    constexpr int NPS = BPS / CH;
+    const int nch = nsites * NPS;
+#pragma unroll
+    for (int k = 0; k < NPS; k++) {
+        const int c = k * WAVE + lane;
+        if (c < nch) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * CH);
+    }
+}
+
+template <int BPS, int CH>
+__device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds, int nsites, int lane)
+{
+    typedef typename chunk_of<CH>::type C;
+    constexpr int NPS = BPS / CH;
+#pragma unroll
+    for (int k = 0; k < NPS; k++) {
+        const int c = k * WAVE + lane;
+        const int s = c / NPS;
+        const int p = c - s * NPS;
+        const uint64_t a = shfl64(my_addr, s);
+        if (s < nsites) *(C *)(a + (uint64_t)p * CH) = *(const C *)(lds + c * CH);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex arithmetic in reference macro order (include/qla_complex.h).
+// ---------------------------------------------------------------------------
+// x += a*b   (QLA_c_peq_c_times_c, qla_complex.h:330-333)
+// x -= a*b   (QLA_c_meq_c_times_c)
+// with ADJ: a is conjugated (QLA_c_peq_ca_times_c, qla_complex.h:386-389)
+template <bool MEQ, bool ADJ, class T>
+__device__ __forceinline__ void acc_mul(cpx<T> &x, cpx<T> a, cpx<T> b)
+{
+    T tr, ti;
+    if (!ADJ) { tr = a.re * b.re - a.im * b.im; ti = a.re * b.im + a.im * b.re; }
+    else      { tr = a.re * b.re + a.im * b.im; ti = a.re * b.im - a.im * b.re; }
+    if (!MEQ) { x.re = x.re + tr; x.im = x.im + ti; }
+    else      { x.re = x.re - tr; x.im = x.im - ti; }
+}
+
+// eqop codes, as in the reference's operatortypes.pl: eq, peq, meq, eqm
+enum { EQ = 0, PEQ = 1, MEQ = 2, EQM = 3 };
+
+// r[ic][is] eqop sum_kc A[ic][kc] t[kc][is]; A = a or a^dagger (ADJ).
+// Accumulator start and sign follow expressions_tensor.pl:544-630:
+// eq: x=0, +=; peq: x=r, +=; meq: x=r, -=; eqm: x=0, -=.
+template <int EQOP, bool ADJ, int NS, class T>
+__device__ __forceinline__ void mat_times_cols(cpx<T> *r, const cpx<T> *a, const cpx<T> *t)
+{
+    constexpr bool SUB = (EQOP == MEQ || EQOP == EQM);
+#pragma unroll
+    for (int ic = 0; ic < 3; ic++) {
+#pragma unroll
+        for (int is = 0; is < NS; is++) {
+            cpx<T> x;
+            if (EQOP == PEQ || EQOP == MEQ) x = r[ic * NS + is];
+            else { x.re = T(0); x.im = T(0); }
+#pragma unroll
+            for (int kc = 0; kc < 3; kc++)
+                acc_mul<SUB, ADJ>(x, ADJ ? a[kc * 3 + ic] : a[ic * 3 + kc], t[kc * NS + is]);
+            r[ic * NS + is] = x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spin projection (expressions_gamma.pl:244-276) for one colour row d[0..3].
+// ---------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ cpx<T> c_plus(cpx<T> a, cpx<T> b)  { return { a.re + b.re, a.im + b.im }; }
+template <class T> __device__ __forceinline__ cpx<T> c_minus(cpx<T> a, cpx<T> b) { return { a.re - b.re, a.im - b.im }; }
+// QLA_c_eq_c_plus_ic: (a.re - b.im, a.im + b.re)
+template <class T> __device__ __forceinline__ cpx<T> c_plus_i(cpx<T> a, cpx<T> b)  { return { a.re - b.im, a.im + b.re }; }
+// QLA_c_eq_c_minus_ic: (a.re + b.im, a.im - b.re)
+template <class T> __device__ __forceinline__ cpx<T> c_minus_i(cpx<T> a, cpx<T> b) { return { a.re + b.im, a.im - b.re }; }
+
+template <int MU, int SIGN, class T>
+__device__ __forceinline__ void spproj_row(cpx<T> &h0, cpx<T> &h1, const cpx<T> *d)
+{
+    if (SIGN == 1) {
+        if (MU == 0) { h0 = c_plus_i(d[0], d[3]);  h1 = c_plus_i(d[1], d[2]); }
+        if (MU == 1) { h0 = c_minus(d[0], d[3]);   h1 = c_plus(d[1], d[2]); }
+        if (MU == 2) { h0 = c_plus_i(d[0], d[2]);  h1 = c_minus_i(d[1], d[3]); }
+        if (MU == 3) { h0 = c_plus(d[0], d[2]);    h1 = c_plus(d[1], d[3]); }
+        if (MU == 4) { h0 = d[0];                  h1 = d[1]; }
+    } else {
+        if (MU == 0) { h0 = c_minus_i(d[0], d[3]); h1 = c_minus_i(d[1], d[2]); }
+        if (MU == 1) { h0 = c_plus(d[0], d[3]);    h1 = c_minus(d[1], d[2]); }
+        if (MU == 2) { h0 = c_minus_i(d[0], d[2]); h1 = c_plus_i(d[1], d[3]); }
+        if (MU == 3) { h0 = c_minus(d[0], d[2]);   h1 = c_minus(d[1], d[3]); }
+        if (MU == 4) { h0 = d[2];                  h1 = d[3]; }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spin reconstruction (expressions_gamma.pl:311-337): spin component sp of
+// the result is op(h[src]) with op in {0, +h, -h, +i h, -i h} = {0,1,2,3,4}.
+// ---------------------------------------------------------------------------
+template <int MU, int SIGN> struct recon_tab;
+#define QLA_RECON(MU_, SIGN_, O0, O1, O2, O3, S0, S1, S2, S3)                          \
+    template <> struct recon_tab<MU_, SIGN_> {                                         \
+        static constexpr int op[4] = { O0, O1, O2, O3 };                               \
+        static constexpr int src[4] = { S0, S1, S2, S3 };                              \
+    };
+QLA_RECON(0, 1, 1, 1, 4, 4, 0, 1, 1, 0)
+QLA_RECON(1, 1, 1, 1, 1, 2, 0, 1, 1, 0)
+QLA_RECON(2, 1, 1, 1, 4, 3, 0, 1, 0, 1)
+QLA_RECON(3, 1, 1, 1, 1, 1, 0, 1, 0, 1)
+QLA_RECON(4, 1, 1, 1, 0, 0, 0, 1, 0, 1)
+QLA_RECON(0, -1, 1, 1, 3, 3, 0, 1, 1, 0)
+QLA_RECON(1, -1, 1, 1, 2, 1, 0, 1, 1, 0)
+QLA_RECON(2, -1, 1, 1, 3, 4, 0, 1, 0, 1)
+QLA_RECON(3, -1, 1, 1, 2, 2, 0, 1, 0, 1)
+QLA_RECON(4, -1, 0, 0, 1, 1, 0, 1, 0, 1)
+#undef QLA_RECON
+
+// r eqop op(h) for one complex component (carith1 table, lib/perl/headers.pl:136-170):
+// '0' under eq/eqm writes zero and under peq/meq leaves r alone; the sign of
+// the eqop folds into the sign of the operand (e.g. peq -ic = QLA_c_meq_ic).
+template <int EQOP, int OP, class T>
+__device__ __forceinline__ void recon_apply(cpx<T> &r, cpx<T> h)
+{
+    if (OP == 0) {
+        if (EQOP == EQ || EQOP == EQM) { r.re = T(0); r.im = T(0); }
+        return;
+    }
+    constexpr bool NEG = (OP == 2 || OP == 4) != (EQOP == MEQ || EQOP == EQM);
+    constexpr bool IMAG = (OP == 3 || OP == 4);
+    cpx<T> v;
+    if (IMAG) { v.re = -h.im; v.im = h.re; } else v = h;
+    if (EQOP == EQ || EQOP == EQM) {
+        if (NEG) { r.re = -v.re; r.im = -v.im; } else r = v;
+    } else {
+        if (NEG) { r.re = r.re - v.re; r.im = r.im - v.im; }
+        else     { r.re = r.re + v.re; r.im = r.im + v.im; }
+    }
+}
+
+} // namespace qla

@@ -1,0 +1,23 @@
+// qla_launch.hpp -- internal launch interface between the C ABI layer
+// (qla_capi.cpp) and the kernel translation units.  prec: 0 = float, 1 = double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+// V = M*V (eqop eq, b contiguous) or V += M*(*pV) (eqop peq, b pointer array);
+// index != NULL selects the gang-indexed x-form.
+hipError_t qla_launch_matvec(int prec, int eqop_peq, int pb, void *r, const void *a, const void *b,
+                             const int *index, int n, hipStream_t st);
+// H = spproj(M * pD)
+hipError_t qla_launch_spproj(int prec, void *r, const void *a, const void *b, int mu, int sign,
+                             const int *index, int n, hipStream_t st);
+// D += sprecon(M * pH)
+hipError_t qla_launch_sprecon(int prec, void *r, const void *a, const void *b, int mu, int sign,
+                              const int *index, int n, hipStream_t st);
+
+// Global reductions over m complex elements.  kind: 0 = norm2 (b unused),
+// 1 = dot conj(a).b.  The result is written to the DEVICE address `out` as
+// out_kind 0 double, 1 float, 2 double complex, 3 float complex.  `scratch`
+// must hold qla_reduce_scratch_doubles() doubles.
+int qla_reduce_scratch_doubles(void);
+hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, long long m, void *out,
+                              int out_kind, double *scratch, hipStream_t st);

@@ -1,0 +1,97 @@
+"""ctypes binding of ``libqla_mi355x.so``, the gfx950 QLA hot-path library.
+
+The binding mirrors the C ABI one for one (names and argument order of the
+reference QLA symbols); it only converts Python integers/pointers.  There is
+no CPU fallback: if the shared library is missing or cannot be loaded this
+module raises, so nothing can silently run a different code path.
+
+torch is imported before the library is opened so that the HIP runtime torch
+already loaded (same SONAME, libamdhip64.so.7) is the one the library binds
+to; device pointers from torch allocations are then valid in library calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libqla_mi355x.so"
+HEADER = HERE.parent / "include" / "qla_mi355x.h"
+
+vp = C.c_void_p
+i32 = C.c_int
+
+# argument kinds of each exported symbol: 'p' pointer, 'i' int, 'r' void return
+_SIGS: dict[str, tuple[type, list]] = {
+    # a1/a2
+    "QLA_F3_V_veq_M_times_V": (None, [vp, vp, vp, i32]),
+    "QLA_D3_V_veq_M_times_V": (None, [vp, vp, vp, i32]),
+    "QLA_F3_V_vpeq_M_times_pV": (None, [vp, vp, vp, i32]),
+    "QLA_D3_V_vpeq_M_times_pV": (None, [vp, vp, vp, i32]),
+    # a3/a4
+    "QLA_F3_r_veq_norm2_V": (None, [vp, vp, i32]),
+    "QLA_D3_r_veq_norm2_V": (None, [vp, vp, i32]),
+    "QLA_F3_r_veq_norm2_D": (None, [vp, vp, i32]),
+    "QLA_D3_r_veq_norm2_D": (None, [vp, vp, i32]),
+    "QLA_DF3_r_veq_norm2_D": (None, [vp, vp, i32]),
+    "QLA_F3_c_veq_V_dot_V": (None, [vp, vp, vp, i32]),
+    "QLA_F3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
+    "QLA_D3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
+    "QLA_DF3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
+    # a5/a6
+    "QLA_F3_H_veq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, i32]),
+    "QLA_F3_H_xeq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, vp, i32]),
+    "QLA_D3_H_veq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, i32]),
+    "QLA_D3_H_xeq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, vp, i32]),
+    # a7/a8
+    "QLA_F3_D_vpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, i32]),
+    "QLA_F3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
+    "QLA_D3_D_vpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, i32]),
+    "QLA_D3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
+    # control
+    "QLA_MI355X_set_stream": (None, [vp]),
+    "QLA_MI355X_get_stream": (vp, []),
+    "QLA_MI355X_set_sync": (None, [i32]),
+    "QLA_MI355X_synchronize": (None, []),
+    "QLA_MI355X_last_error": (i32, []),
+    "QLA_MI355X_last_error_string": (C.c_char_p, []),
+    "QLA_MI355X_version": (C.c_char_p, []),
+}
+
+_lib: C.CDLL | None = None
+
+
+def header_symbols() -> list[str]:
+    """Names of every function declared in include/qla_mi355x.h."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(QLA_\w+)\s*\(", text)) - {"QLA_MI355X_RESTRICT"})
+
+
+def load() -> C.CDLL:
+    """Open the library (once) and attach argument types.  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (bind to the HIP runtime torch loaded)
+
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -m qla_amd.build` (there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_NOW | os.RTLD_GLOBAL)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check() -> None:
+    """Raise if the calling thread saw a HIP error inside the library."""
+    lib = load()
+    e = lib.QLA_MI355X_last_error()
+    if e != 0:
+        raise RuntimeError(f"qla_mi355x HIP error {e}")
