@@ -155,6 +155,39 @@ void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r
                                       int mu, int sign, int *index, int n);
 
 /* ------------------------------------------------------------------------ */
+/* a9-a12: batched matrix functions.  Reference: generated wrappers         */
+/* (expressions_tensor.pl:239-257) around lib/perl/code-templates/          */
+/* M_eq_exp_M.c:300-367 (Pade + scaling and squaring), M_eq_sqrt_M.c:136-173*/
+/* and M_eq_invsqrt_M.c:135-175 (Newton iteration, at most 20 steps).       */
+/* QLA_FN_* take the colour count first (prototype.pl:594-600); this build  */
+/* supports nc = 3..8 (other nc: no-op and an error is recorded).           */
+/* ------------------------------------------------------------------------ */
+void QLA_F3_M_veq_exp_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                        QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+void QLA_F3_M_veq_sqrt_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                         QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+void QLA_F3_M_veq_invsqrt_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+void QLA_D3_M_veq_exp_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                        QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+void QLA_D3_M_veq_sqrt_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                         QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+void QLA_D3_M_veq_invsqrt_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                            QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
+#ifdef __cplusplus
+void QLA_FN_M_veq_exp_M(int nc, void *r, void *a, int n);
+void QLA_FN_M_veq_sqrt_M(int nc, void *r, void *a, int n);
+void QLA_FN_M_veq_invsqrt_M(int nc, void *r, void *a, int n);
+#else
+void QLA_FN_M_veq_exp_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
+                        QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
+void QLA_FN_M_veq_sqrt_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
+                         QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
+void QLA_FN_M_veq_invsqrt_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
+                            QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
+#endif
+
+/* ------------------------------------------------------------------------ */
 /* Library control (no counterpart in the reference, which has no device   */
 /* and no stream; these only select where and how the calls above run).    */
 /* ------------------------------------------------------------------------ */

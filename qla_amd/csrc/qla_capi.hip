@@ -56,6 +56,13 @@ double *scratch_for_device()
     return p;
 }
 
+// one device unsigned int after the reduction partials and the result slot
+unsigned int *counter_for_device()
+{
+    double *sc = scratch_for_device();
+    return sc ? (unsigned int *)(sc + qla_reduce_scratch_doubles() + 2) : nullptr;
+}
+
 bool is_device_pointer(const void *p)
 {
     hipPointerAttribute_t at;
@@ -218,5 +225,24 @@ void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *r, QLA_D3_ColorMatrix
     finish(qla_launch_sprecon(1, r, a, (const void *)b, mu, sign, index, n, ts.stream),
            "QLA_D3_D_xpeq_sprecon_M_times_pH");
 }
+
+// ----------------------------------------------------------------- a9 - a12
+#define QLA_MATFN(NAME, PREC, FN, NC)                                                  \
+    {                                                                                  \
+        if (n <= 0) return;                                                            \
+        unsigned int *ctr = counter_for_device();                                      \
+        if (!ctr) return;                                                              \
+        finish(qla_launch_matfn(PREC, FN, NC, r, a, n, ctr, ts.stream), NAME);         \
+    }
+void QLA_F3_M_veq_exp_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_exp_M", 0, 0, 3)
+void QLA_F3_M_veq_sqrt_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_sqrt_M", 0, 1, 3)
+void QLA_F3_M_veq_invsqrt_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_invsqrt_M", 0, 2, 3)
+void QLA_D3_M_veq_exp_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_exp_M", 1, 0, 3)
+void QLA_D3_M_veq_sqrt_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_sqrt_M", 1, 1, 3)
+void QLA_D3_M_veq_invsqrt_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_invsqrt_M", 1, 2, 3)
+void QLA_FN_M_veq_exp_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_exp_M", 0, 0, nc)
+void QLA_FN_M_veq_sqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_sqrt_M", 0, 1, nc)
+void QLA_FN_M_veq_invsqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_invsqrt_M", 0, 2, nc)
+#undef QLA_MATFN
 
 } // extern "C"

@@ -21,3 +21,9 @@ hipError_t qla_launch_sprecon(int prec, void *r, const void *a, const void *b, i
 int qla_reduce_scratch_doubles(void);
 hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, long long m, void *out,
                               int out_kind, double *scratch, hipStream_t st);
+
+// Matrix functions on n nc x nc matrices.  fn: 0 exp, 1 sqrt, 2 invsqrt.
+// prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned int
+// the launch uses as its work counter (reset on the stream first).
+hipError_t qla_launch_matfn(int prec, int fn, int nc, void *r, const void *a, int n, unsigned int *counter,
+                            hipStream_t st);

@@ -112,3 +112,29 @@ def random_field(kind: str, n: int, seed: int, dtype=np.float64) -> np.ndarray:
     """Complex Gaussian(0,1) field of record kind 'V', 'H' or 'D'."""
     shape = {"V": (n, 3, 2), "H": (n, 3, 2, 2), "D": (n, 3, 4, 2)}[kind]
     return gaussian(shape, seed, dtype=dtype)
+
+
+def random_antihermitian(n: int, nc: int, seed: int, radius: float = 4.0) -> np.ndarray:
+    """A = i*H with H Gaussian Hermitian, traceless, scaled so its spectral
+    radius is radius*u (u uniform in (0, 1]), i.e. eigenvalues of H in
+    [-radius, radius] across the batch.  Returns (n, nc, nc, 2) float64."""
+    rng = _rng(seed)
+    g = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    h = 0.5 * (g + np.conj(np.swapaxes(g, 1, 2)))
+    tr = np.trace(h, axis1=1, axis2=2).real / nc
+    h = h - tr[:, None, None] * np.eye(nc)
+    rad = np.abs(np.linalg.eigvalsh(h)).max(axis=1)
+    u = 1.0 - rng.random(n)
+    h = h * (radius * u / np.where(rad > 0, rad, 1))[:, None, None]
+    a = 1j * h
+    return np.stack([a.real, a.imag], axis=-1)
+
+
+def random_hpd(n: int, nc: int, seed: int, eps_lo: float = 1e-4, eps_hi: float = 1.0) -> np.ndarray:
+    """A = B^dagger B + eps*I, B complex Gaussian, eps log-uniform in
+    [eps_lo, eps_hi].  Returns (n, nc, nc, 2) float64."""
+    rng = _rng(seed)
+    b = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    eps = np.exp(rng.uniform(np.log(eps_lo), np.log(eps_hi), n))
+    a = np.einsum("nki,nkj->nij", np.conj(b), b) + eps[:, None, None] * np.eye(nc)
+    return np.stack([a.real, a.imag], axis=-1)

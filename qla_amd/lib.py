@@ -50,6 +50,9 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_F3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
     "QLA_D3_D_vpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, i32]),
     "QLA_D3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
+    # a9-a12
+    **{f"QLA_{p}3_M_veq_{fn}_M": (None, [vp, vp, i32]) for p in ("F", "D") for fn in ("exp", "sqrt", "invsqrt")},
+    **{f"QLA_FN_M_veq_{fn}_M": (None, [i32, vp, vp, i32]) for fn in ("exp", "sqrt", "invsqrt")},
     # control
     "QLA_MI355X_set_stream": (None, [vp]),
     "QLA_MI355X_get_stream": (vp, []),
@@ -67,7 +70,8 @@ def header_symbols() -> list[str]:
     """Names of every function declared in include/qla_mi355x.h."""
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(QLA_\w+)\s*\(", text)) - {"QLA_MI355X_RESTRICT"})
+    decl = r"^\s*(?:const\s+)?(?:void|int|char)\s*\**\s*(QLA_\w+)\s*\("
+    return sorted(set(re.findall(decl, text, flags=re.M)))
 
 
 def load() -> C.CDLL:
