@@ -7,13 +7,12 @@
 //
 // Data movement model.  A wavefront owns a tile of 64 sites.  Whole records
 // are moved between HBM and LDS with gfx950 direct-to-LDS loads
-// (global_load_lds_dword{x3,x4}): lane l of one instruction moves the chunk
-// that lands at LDS offset base + l*CH, so consecutive lanes read consecutive
-// chunks of the same or the next record -- coalesced for contiguous fields and
-// for pointer gathers whose targets are adjacent.  After the loads the wave
-// reads its own site's record from LDS at lane*BPS (the transpose), computes
-// in registers, writes the result record to LDS and stores it with the same
-// chunk mapping, again coalesced.
+// (global_load_lds_dword{,x3,x4}): consecutive lanes of one instruction read
+// consecutive chunks of the same or the next record -- coalesced for
+// contiguous fields and for pointer gathers whose targets are adjacent.
+// After the loads each lane reads its own site's record out of LDS (the
+// transpose), computes in registers, writes the result record to LDS and the
+// wave stores it with the same chunk mapping, again coalesced.
 //
 // Arithmetic.  Everything is compiled with -ffp-contract=off so each complex
 // multiply-accumulate rounds exactly like the reference macro it restates.
@@ -50,14 +49,23 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
 }
 
 // ---------------------------------------------------------------------------
-// HBM -> LDS staging.  CH is the chunk size in bytes (12 or 16: gfx950 LDS
-// DMA widths); it divides BPS.  Records must be 4-byte aligned for CH = 12
-// and 16-byte aligned for CH = 16.
+// HBM -> LDS staging.  CH is the chunk size in bytes moved per lane by one
+// direct-to-LDS load: 4, 12 or 16 (gfx950 global_load_lds_dword{,x3,x4}).
+// Lane l's chunk lands at base + l*LS(CH) where the LDS lane stride LS is 4
+// for dword and 16 for both dwordx3 and dwordx4 (a 12-byte chunk leaves a
+// 4-byte hole).  A record of BPS bytes = NPS chunks therefore occupies
+// NPS*LS bytes of LDS; chunk c of the tile sits at c*LS.  Records must be
+// 4-byte aligned for CH = 4 and 12 and 16-byte aligned for CH = 16.
 // ---------------------------------------------------------------------------
+template <int CH> struct lds_stride { static constexpr int v = (CH == 4) ? 4 : 16; };
+template <int BPS, int CH> struct rec_lds { static constexpr int v = (BPS / CH) * lds_stride<CH>::v; };
 
-// One direct-to-LDS load of CH bytes per lane: lane l's chunk lands at
-// lds_base + l*CH (the builtin needs a literal width).
+// One direct-to-LDS load of CH bytes per lane (the builtin needs a literal width).
 template <int CH> __device__ __forceinline__ void dma_load(const void *g, char *lds_base);
+template <> __device__ __forceinline__ void dma_load<4>(const void *g, char *lds_base)
+{
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 4, 0, 0);
+}
 template <> __device__ __forceinline__ void dma_load<12>(const void *g, char *lds_base)
 {
     __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 12, 0, 0);
@@ -67,34 +75,86 @@ template <> __device__ __forceinline__ void dma_load<16>(const void *g, char *ld
     __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 16, 0, 0);
 }
 
-// Contiguous records g[0 .. nsites) -> lds (site s at s*BPS).
+// Contiguous records g[0 .. nsites) -> lds (site s at s*rec_lds).
 template <int BPS, int CH>
 __device__ __forceinline__ void stage_contig(char *lds, const char *g, int nsites, int lane)
 {
     static_assert(BPS % CH == 0, "chunk must divide the record");
-    constexpr int NPS = BPS / CH;
+    constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
     const int nch = nsites * NPS;
 #pragma unroll
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
-        if (c < nch) dma_load<CH>(g + (size_t)c * CH, lds + k * WAVE * CH);
+        if (c < nch) dma_load<CH>(g + (size_t)c * CH, lds + k * WAVE * LS);
     }
 }
 
 // Records at per-site addresses: lane s holds the address of site s's record
-// (sites >= nsites are skipped) -> lds (site s at s*BPS).
+// (sites >= nsites are skipped) -> lds (site s at s*rec_lds).
 template <int BPS, int CH>
 __device__ __forceinline__ void stage_gather(char *lds, uint64_t my_addr, int nsites, int lane)
 {
     static_assert(BPS % CH == 0, "chunk must divide the record");
-    constexpr int NPS = BPS / CH;
+    constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
 #pragma unroll
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
         const int s = c / NPS;
         const int p = c - s * NPS;
         const uint64_t a = shfl64(my_addr, s);
-        if (s < nsites) dma_load<CH>((const void *)(a + (uint64_t)p * CH), lds + k * WAVE * CH);
+        if (s < nsites) dma_load<CH>((const void *)(a + (uint64_t)p * CH), lds + k * WAVE * LS);
+    }
+}
+
+// A contiguous byte range g[0 .. nbytes) -> lds[0 .. nbytes) (nbytes % 4 == 0,
+// g aligned to CH, CH = 4 or 16 so the LDS copy is packed; MAXB bounds nbytes
+// at compile time): CH-byte chunks for the bulk, then 4-byte chunks.
+template <int CH, int MAXB>
+__device__ __forceinline__ void stage_bytes(char *lds, const char *g, int nbytes, int lane)
+{
+    static_assert(CH == 4 || CH == 16, "packed LDS layout needs CH = 4 or 16");
+    const int full = nbytes / CH;
+#pragma unroll
+    for (int k = 0; k < (MAXB / CH + WAVE - 1) / WAVE; k++) {
+        const int c = k * WAVE + lane;
+        if (c < full) dma_load<CH>(g + (size_t)c * CH, lds + k * WAVE * CH);
+    }
+    if (CH > 4) {
+        const int done = full * CH;
+        const int tail = (nbytes - done) / 4;     // < CH/4 <= WAVE
+        if (lane < tail) dma_load<4>(g + done + lane * 4, lds + done);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-lane access to its own record in the LDS chunk layout: element e of
+// size ES bytes at record byte b = e*ES lies in chunk b/CH at offset b%CH
+// (ES divides CH, so an element never straddles two chunks).
+// ---------------------------------------------------------------------------
+template <int CH, int ES> struct elem_off {
+    static_assert(CH % ES == 0, "element must not straddle chunks");
+    static __device__ __forceinline__ constexpr int at(int e)
+    {
+        return (e * ES / CH) * lds_stride<CH>::v + (e * ES) % CH;
+    }
+};
+
+template <int CH, class T, int N>
+__device__ __forceinline__ void lds_read_rec(cpx<T> (&v)[N], const char *rec)
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        v[i].re = *(const T *)(rec + elem_off<CH, sizeof(T)>::at(2 * i));
+        v[i].im = *(const T *)(rec + elem_off<CH, sizeof(T)>::at(2 * i + 1));
+    }
+}
+template <int CH, class T, int N>
+__device__ __forceinline__ void lds_write_rec(char *rec, const cpx<T> (&v)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        *(T *)(rec + elem_off<CH, sizeof(T)>::at(2 * i)) = v[i].re;
+        *(T *)(rec + elem_off<CH, sizeof(T)>::at(2 * i + 1)) = v[i].im;
     }
 }
 
@@ -106,17 +166,18 @@ struct __attribute__((aligned(16))) chunk16 { uint32_t x, y, z, w; };
 template <int CH> struct chunk_of;
 template <> struct chunk_of<12> { typedef chunk12 type; };
 template <> struct chunk_of<16> { typedef chunk16 type; };
+template <> struct chunk_of<4> { typedef uint32_t type; };
 
 template <int BPS, int CH>
 __device__ __forceinline__ void store_contig(char *g, const char *lds, int nsites, int lane)
 {
     typedef typename chunk_of<CH>::type C;
-    constexpr int NPS = BPS / CH;
+    constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
     const int nch = nsites * NPS;
 #pragma unroll
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
-        if (c < nch) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * CH);
+        if (c < nch) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * LS);
     }
 }
 
@@ -124,14 +185,33 @@ template <int BPS, int CH>
 __device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds, int nsites, int lane)
 {
     typedef typename chunk_of<CH>::type C;
-    constexpr int NPS = BPS / CH;
+    constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
 #pragma unroll
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
         const int s = c / NPS;
         const int p = c - s * NPS;
         const uint64_t a = shfl64(my_addr, s);
-        if (s < nsites) *(C *)(a + (uint64_t)p * CH) = *(const C *)(lds + c * CH);
+        if (s < nsites) *(C *)(a + (uint64_t)p * CH) = *(const C *)(lds + c * LS);
+    }
+}
+
+// lds[0 .. nbytes) (packed) -> contiguous g[0 .. nbytes), CH = 4 or 16
+template <int CH, int MAXB>
+__device__ __forceinline__ void store_bytes(char *g, const char *lds, int nbytes, int lane)
+{
+    static_assert(CH == 4 || CH == 16, "packed LDS layout needs CH = 4 or 16");
+    typedef typename chunk_of<CH>::type C;
+    const int full = nbytes / CH;
+#pragma unroll
+    for (int k = 0; k < (MAXB / CH + WAVE - 1) / WAVE; k++) {
+        const int c = k * WAVE + lane;
+        if (c < full) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * CH);
+    }
+    if (CH > 4) {
+        const int done = full * CH;
+        const int tail = (nbytes - done) / 4;
+        if (lane < tail) *(uint32_t *)(g + done + lane * 4) = *(const uint32_t *)(lds + done + lane * 4);
     }
 }
 

@@ -21,22 +21,10 @@ template <class T> struct rec {
     // LDS DMA chunk: dwordx3 needs only the 4-byte alignment of float
     // records; double records are 16-byte aligned (qla_complex.h:35).
     static constexpr int CH = sizeof(T) == 4 ? 12 : 16;
+    // LDS footprint of one staged record (12-byte chunks sit at a 16-byte stride)
+    static constexpr int LV = rec_lds<V, CH>::v, LH = rec_lds<H, CH>::v;
+    static constexpr int LD = rec_lds<D, CH>::v, LM = rec_lds<M, CH>::v;
 };
-
-template <class T, int N>
-__device__ __forceinline__ void lds_read(cpx<T> (&v)[N], const char *p)
-{
-    const cpx<T> *q = (const cpx<T> *)p;
-#pragma unroll
-    for (int i = 0; i < N; i++) v[i] = q[i];
-}
-template <class T, int N>
-__device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
-{
-    cpx<T> *q = (cpx<T> *)p;
-#pragma unroll
-    for (int i = 0; i < N; i++) q[i] = v[i];
-}
 
 // Wait for this wave's direct-to-LDS loads, then order LDS across lanes.
 __device__ __forceinline__ void dma_wait()
@@ -74,13 +62,13 @@ k_matvec(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const void *__res
     typedef rec<T> R;
     constexpr int CH = R::CH;
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::M + 2 * R::V);
+    constexpr int BYTES = WAVE * (R::LM + 2 * R::LV);
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
     tile_t t;
     if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *lb = lm + WAVE * R::M;
-    char *lr = lb + WAVE * R::V;
+    char *lb = lm + WAVE * R::LM;
+    char *lr = lb + WAVE * R::LV;
     const bool act = t.lane < t.nsites;
 
     int site = t.base + t.lane;
@@ -100,12 +88,12 @@ k_matvec(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const void *__res
     }
     dma_wait();
     cpx<T> m[9], v[3], x[3];
-    lds_read(m, lm + t.lane * R::M);
-    lds_read(v, lb + t.lane * R::V);
-    if (READ_R) lds_read(x, lr + t.lane * R::V);
+    lds_read_rec<CH>(m, lm + t.lane * R::LM);
+    lds_read_rec<CH>(v, lb + t.lane * R::LV);
+    if (READ_R) lds_read_rec<CH>(x, lr + t.lane * R::LV);
     mat_times_cols<EQOP, ADJ, 1>(x, m, v);
     wave_sync();
-    lds_write(lr + t.lane * R::V, x);
+    lds_write_rec<CH>(lr + t.lane * R::LV, x);
     wave_sync();
     if (!XF) store_contig<R::V, CH>((char *)(r + 3 * (size_t)t.base), lr, t.nsites, t.lane);
     else store_scatter<R::V, CH>((uint64_t)(r + 3 * (size_t)site), lr, t.nsites, t.lane);
@@ -122,13 +110,13 @@ k_spproj(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *con
     typedef rec<T> R;
     constexpr int CH = R::CH;
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::M + R::D + R::H);
+    constexpr int BYTES = WAVE * (R::LM + R::LD + R::LH);
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
     tile_t t;
     if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *ld = lm + WAVE * R::M;
-    char *lh = ld + WAVE * R::D;
+    char *ld = lm + WAVE * R::LM;
+    char *lh = ld + WAVE * R::LD;
     const bool act = t.lane < t.nsites;
 
     int site = t.base + t.lane;
@@ -144,15 +132,15 @@ k_spproj(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *con
     stage_gather<R::D, CH>(ld, pd, t.nsites, t.lane);
     dma_wait();
     cpx<T> m[9], d[12], h[6];
-    lds_read(m, lm + t.lane * R::M);
-    lds_read(d, ld + t.lane * R::D);
-    if (READ_R) lds_read(h, lh + t.lane * R::H);
+    lds_read_rec<CH>(m, lm + t.lane * R::LM);
+    lds_read_rec<CH>(d, ld + t.lane * R::LD);
+    if (READ_R) lds_read_rec<CH>(h, lh + t.lane * R::LH);
     cpx<T> tp[6];
 #pragma unroll
     for (int ic = 0; ic < 3; ic++) spproj_row<MU, SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
     mat_times_cols<EQOP, ADJ, 2>(h, m, tp);
     wave_sync();
-    lds_write(lh + t.lane * R::H, h);
+    lds_write_rec<CH>(lh + t.lane * R::LH, h);
     wave_sync();
     if (!XF) store_contig<R::H, CH>((char *)(r + 6 * (size_t)t.base), lh, t.nsites, t.lane);
     else store_scatter<R::H, CH>((uint64_t)(r + 6 * (size_t)site), lh, t.nsites, t.lane);
@@ -171,13 +159,13 @@ k_sprecon(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *co
     constexpr int CH = R::CH;
     // eq/eqm overwrite every component; peq/meq read the destination first
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::M + R::H + R::D);
+    constexpr int BYTES = WAVE * (R::LM + R::LH + R::LD);
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
     tile_t t;
     if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *lh = lm + WAVE * R::M;
-    char *ld = lh + WAVE * R::H;
+    char *lh = lm + WAVE * R::LM;
+    char *ld = lh + WAVE * R::LH;
     const bool act = t.lane < t.nsites;
 
     int site = t.base + t.lane;
@@ -193,9 +181,9 @@ k_sprecon(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *co
     stage_gather<R::H, CH>(lh, ph, t.nsites, t.lane);
     dma_wait();
     cpx<T> m[9], h[6], d[12];
-    lds_read(m, lm + t.lane * R::M);
-    lds_read(h, lh + t.lane * R::H);
-    if (READ_R) lds_read(d, ld + t.lane * R::D);
+    lds_read_rec<CH>(m, lm + t.lane * R::LM);
+    lds_read_rec<CH>(h, lh + t.lane * R::LH);
+    if (READ_R) lds_read_rec<CH>(d, ld + t.lane * R::LD);
     cpx<T> tv[6];
     mat_times_cols<EQ, ADJ, 2>(tv, m, h);
 #pragma unroll
@@ -206,7 +194,7 @@ k_sprecon(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *co
         recon_apply<EQOP, TAB::op[3]>(d[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
     }
     wave_sync();
-    lds_write(ld + t.lane * R::D, d);
+    lds_write_rec<CH>(ld + t.lane * R::LD, d);
     wave_sync();
     if (!XF) store_contig<R::D, CH>((char *)(r + 12 * (size_t)t.base), ld, t.nsites, t.lane);
     else store_scatter<R::D, CH>((uint64_t)(r + 12 * (size_t)site), ld, t.nsites, t.lane);
@@ -215,7 +203,7 @@ k_sprecon(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *co
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from the C ABI layer).
 // ---------------------------------------------------------------------------
-template <class T> struct wpb_of { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
+template <class T> struct wpb_of { static constexpr int v = 2; };
 
 template <class T, int EQOP, bool ADJ, bool PB, bool XF>
 static hipError_t launch_matvec_t(void *r, const void *a, const void *b, const int *index, int n,
