@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="approximate CPU work for the oracle baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -77,6 +79,20 @@ def main():
         step(it)
     torch.cuda.synchronize()
     qla_amd.check()
+
+    copy_gbs = None
+    if args.copy_mib:
+        src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        for _ in range(5):
+            dst.copy_(src)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        copy_gbs = 2 * 5 * src.numel() * 4 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+        del src, dst
 
     # whole-job timing
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -128,7 +144,8 @@ def main():
         "data": "synthetic (PCG64-seeded SU(3) links, Gaussian fermions)",
         "config": {"workload": "QLA_F3_H_veq_spproj_M_times_pD, 32^4 periodic, mu 0..3 x sign +-1 per step",
                    "sites": n, "bytes_per_site": BYTES_PER_SITE, "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                   "sites_per_s": round(sites_per_s, 1), "wall_s": round(wall, 4)},
+                   "sites_per_s": round(sites_per_s, 1), "wall_s": round(wall, 4),
+                   "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_spproj<float,EQ,...> avg launch ms", "avg_launch_ms": round(avg_ms, 5)},

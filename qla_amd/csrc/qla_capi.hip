@@ -105,6 +105,7 @@ int QLA_MI355X_last_error(void)
 }
 const char *QLA_MI355X_last_error_string(void) { return hipGetErrorString(ts.err); }
 const char *QLA_MI355X_version(void) { return QLA_MI355X_VERSION_STRING; }
+void QLA_MI355X_set_tiles_per_wave(int tpw) { qla_set_tiles_per_wave(tpw); }
 
 // ----------------------------------------------------------------- a1 / a2
 void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *b, int n)

@@ -96,7 +96,9 @@ __device__ __forceinline__ void stage_gather(char *lds, uint64_t my_addr, int ns
 {
     static_assert(BPS % CH == 0, "chunk must divide the record");
     constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
-#pragma unroll
+    // wide chunks: fully unrolled; 4-byte chunks (the misaligned fallback):
+    // a rolled loop, to keep the fast path's register budget
+#pragma unroll(CH == 4 ? 1 : NPS)
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
         const int s = c / NPS;
@@ -114,7 +116,7 @@ __device__ __forceinline__ void stage_bytes(char *lds, const char *g, int nbytes
 {
     static_assert(CH == 4 || CH == 16, "packed LDS layout needs CH = 4 or 16");
     const int full = nbytes / CH;
-#pragma unroll
+#pragma unroll(CH == 4 ? 1 : (MAXB / CH + WAVE - 1) / WAVE)
     for (int k = 0; k < (MAXB / CH + WAVE - 1) / WAVE; k++) {
         const int c = k * WAVE + lane;
         if (c < full) dma_load<CH>(g + (size_t)c * CH, lds + k * WAVE * CH);
@@ -186,7 +188,7 @@ __device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds,
 {
     typedef typename chunk_of<CH>::type C;
     constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
-#pragma unroll
+#pragma unroll(CH == 4 ? 1 : NPS)
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
         const int s = c / NPS;
@@ -203,7 +205,7 @@ __device__ __forceinline__ void store_bytes(char *g, const char *lds, int nbytes
     static_assert(CH == 4 || CH == 16, "packed LDS layout needs CH = 4 or 16");
     typedef typename chunk_of<CH>::type C;
     const int full = nbytes / CH;
-#pragma unroll
+#pragma unroll(CH == 4 ? 1 : (MAXB / CH + WAVE - 1) / WAVE)
     for (int k = 0; k < (MAXB / CH + WAVE - 1) / WAVE; k++) {
         const int c = k * WAVE + lane;
         if (c < full) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * CH);

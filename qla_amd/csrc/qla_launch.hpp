@@ -7,6 +7,8 @@
 // index != NULL selects the gang-indexed x-form.
 hipError_t qla_launch_matvec(int prec, int eqop_peq, int pb, void *r, const void *a, const void *b,
                              const int *index, int n, hipStream_t st);
+// launch-shape knob of the persistent site kernels
+void qla_set_tiles_per_wave(int tpw);
 // H = spproj(M * pD)
 hipError_t qla_launch_spproj(int prec, void *r, const void *a, const void *b, int mu, int sign,
                              const int *index, int n, hipStream_t st);

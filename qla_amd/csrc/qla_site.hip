@@ -18,12 +18,6 @@ namespace qla {
 template <class T> struct rec {
     static constexpr int C = (int)sizeof(cpx<T>);
     static constexpr int V = 3 * C, H = 6 * C, D = 12 * C, M = 9 * C;
-    // LDS DMA chunk: dwordx3 needs only the 4-byte alignment of float
-    // records; double records are 16-byte aligned (qla_complex.h:35).
-    static constexpr int CH = sizeof(T) == 4 ? 12 : 16;
-    // LDS footprint of one staged record (12-byte chunks sit at a 16-byte stride)
-    static constexpr int LV = rec_lds<V, CH>::v, LH = rec_lds<H, CH>::v;
-    static constexpr int LD = rec_lds<D, CH>::v, LM = rec_lds<M, CH>::v;
 };
 
 // Wait for this wave's direct-to-LDS loads, then order LDS across lanes.
@@ -33,23 +27,75 @@ __device__ __forceinline__ void dma_wait()
     wave_sync();
 }
 
-struct tile_t {
-    int base;    // first site (v-form) or first index slot (x-form)
-    int nsites;  // valid sites in this tile (1..64)
-    int lane;
-};
-
-template <int WPB>
-__device__ __forceinline__ bool get_tile(tile_t &t, int n)
+// ---------------------------------------------------------------------------
+// Tile movers.  LDS always holds packed records (site s at s*BPS): 16-byte
+// chunks where the addresses allow, 4-byte chunks otherwise (both land
+// packed in LDS).  `al16` flags of contiguous operands come from the host;
+// gathered addresses are checked per wave.
+// ---------------------------------------------------------------------------
+template <int BPS>
+__device__ __forceinline__ void load_tile(char *lds, const char *g, int nsites, int lane, bool al16)
 {
-    const int wave = threadIdx.x / WAVE;
-    const int tile = blockIdx.x * WPB + wave;
-    t.lane = lane_id();
-    t.base = tile * WAVE;
-    if (t.base >= n) return false;
-    t.nsites = min(WAVE, n - t.base);
-    return true;
+    if (al16) stage_bytes<16, WAVE * BPS>(lds, g, nsites * BPS, lane);
+    else stage_bytes<4, WAVE * BPS>(lds, g, nsites * BPS, lane);
 }
+template <int BPS>
+__device__ __forceinline__ void gather_tile(char *lds, uint64_t addr, int nsites, int lane)
+{
+    if constexpr (BPS % 16 == 0) {
+        if (__all((addr & 15) == 0)) { stage_gather<BPS, 16>(lds, addr, nsites, lane); return; }
+    }
+    stage_gather<BPS, 4>(lds, addr, nsites, lane);
+}
+template <int BPS>
+__device__ __forceinline__ void store_tile(char *g, const char *lds, int nsites, int lane, bool al16)
+{
+    if (al16) store_bytes<16, WAVE * BPS>(g, lds, nsites * BPS, lane);
+    else store_bytes<4, WAVE * BPS>(g, lds, nsites * BPS, lane);
+}
+template <int BPS>
+__device__ __forceinline__ void scatter_tile(uint64_t addr, const char *lds, int nsites, int lane)
+{
+    if constexpr (BPS % 16 == 0) {
+        if (__all((addr & 15) == 0)) { store_scatter<BPS, 16>(addr, lds, nsites, lane); return; }
+    }
+    store_scatter<BPS, 4>(addr, lds, nsites, lane);
+}
+
+template <class T, int N>
+__device__ __forceinline__ void lds_read(cpx<T> (&v)[N], const char *p)
+{
+    const cpx<T> *q = (const cpx<T> *)p;
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] = q[i];
+}
+template <class T, int N>
+__device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
+{
+    cpx<T> *q = (cpx<T> *)p;
+#pragma unroll
+    for (int i = 0; i < N; i++) q[i] = v[i];
+}
+
+// alignment flags of the contiguous operands (bit set: 16-byte aligned)
+enum { AL_A = 1, AL_B = 2, AL_R = 4 };
+
+// Persistent tile loop: wave w of the grid takes tiles w, w + W, w + 2W, ...
+// (W = waves in the grid).  The per-site gather address of the next tile
+// (pointer array entry, or index entry for the x-form) is loaded while the
+// current tile's records are in flight, so a tile costs one memory latency.
+struct tiles {
+    int tile, stride, ntiles, n, lane;
+    __device__ __forceinline__ tiles(int wpb, int n_) : n(n_)
+    {
+        lane = lane_id();
+        tile = blockIdx.x * wpb + (int)(threadIdx.x / WAVE);
+        stride = gridDim.x * wpb;
+        ntiles = (n + WAVE - 1) / WAVE;
+    }
+    __device__ __forceinline__ int base() const { return tile * WAVE; }
+    __device__ __forceinline__ int nsites() const { return min(WAVE, n - tile * WAVE); }
+};
 
 // ---------------------------------------------------------------------------
 // V eqop M * V  (b contiguous: PB = false; b pointer array: PB = true)
@@ -57,163 +103,222 @@ __device__ __forceinline__ bool get_tile(tile_t &t, int n)
 template <class T, int EQOP, bool ADJ, bool PB, bool XF, int WPB>
 __global__ void __launch_bounds__(WPB * WAVE)
 k_matvec(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const void *__restrict__ b,
-         const int *__restrict__ index, int n)
+         const int *__restrict__ index, int n, int al)
 {
     typedef rec<T> R;
-    constexpr int CH = R::CH;
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::LM + 2 * R::LV);
+    constexpr int BYTES = WAVE * (R::M + 2 * R::V);
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
-    tile_t t;
-    if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *lb = lm + WAVE * R::LM;
-    char *lr = lb + WAVE * R::LV;
-    const bool act = t.lane < t.nsites;
-
-    int site = t.base + t.lane;
-    if (XF) site = act ? index[t.base + t.lane] : 0;
-    if (!XF) {
-        stage_contig<R::M, CH>(lm, (const char *)(a + 9 * (size_t)t.base), t.nsites, t.lane);
-        if (!PB) stage_contig<R::V, CH>(lb, (const char *)b + (size_t)t.base * R::V, t.nsites, t.lane);
-        if (READ_R) stage_contig<R::V, CH>(lr, (const char *)(r + 3 * (size_t)t.base), t.nsites, t.lane);
-    } else {
-        stage_gather<R::M, CH>(lm, (uint64_t)(a + 9 * (size_t)site), t.nsites, t.lane);
-        if (!PB) stage_gather<R::V, CH>(lb, (uint64_t)((const char *)b + (size_t)site * R::V), t.nsites, t.lane);
-        if (READ_R) stage_gather<R::V, CH>(lr, (uint64_t)(r + 3 * (size_t)site), t.nsites, t.lane);
+    char *lb = lm + WAVE * R::M;
+    char *lr = lb + WAVE * R::V;
+    tiles t(WPB, n);
+    for (; t.tile < t.ntiles; t.tile += t.stride) {
+        const int base = t.base(), ns = t.nsites();
+        const bool act = t.lane < ns;
+        int site = base + t.lane;
+        if (XF) site = act ? index[base + t.lane] : 0;
+        if (!XF) {
+            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, t.lane, al & AL_A);
+            if (!PB) load_tile<R::V>(lb, (const char *)b + (size_t)base * R::V, ns, t.lane, al & AL_B);
+            if (READ_R) load_tile<R::V>(lr, (const char *)(r + 3 * (size_t)base), ns, t.lane, al & AL_R);
+        } else {
+            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, t.lane);
+            if (!PB) gather_tile<R::V>(lb, act ? (uint64_t)((const char *)b + (size_t)site * R::V) : 0, ns, t.lane);
+            if (READ_R) gather_tile<R::V>(lr, act ? (uint64_t)(r + 3 * (size_t)site) : 0, ns, t.lane);
+        }
+        if (PB) {
+            const uint64_t p = act ? (uint64_t)((const void *const *)b)[site] : 0;
+            gather_tile<R::V>(lb, p, ns, t.lane);
+        }
+        dma_wait();
+        cpx<T> m[9], v[3], x[3];
+        lds_read(m, lm + t.lane * R::M);
+        lds_read(v, lb + t.lane * R::V);
+        if (READ_R) lds_read(x, lr + t.lane * R::V);
+        mat_times_cols<EQOP, ADJ, 1>(x, m, v);
+        wave_sync();
+        lds_write(lr + t.lane * R::V, x);
+        wave_sync();
+        if (!XF) store_tile<R::V>((char *)(r + 3 * (size_t)base), lr, ns, t.lane, al & AL_R);
+        else scatter_tile<R::V>(act ? (uint64_t)(r + 3 * (size_t)site) : 0, lr, ns, t.lane);
+        wave_sync();
     }
-    if (PB) {
-        const uint64_t p = act ? (uint64_t)((const void *const *)b)[site] : 0;
-        stage_gather<R::V, CH>(lb, p, t.nsites, t.lane);
-    }
-    dma_wait();
-    cpx<T> m[9], v[3], x[3];
-    lds_read_rec<CH>(m, lm + t.lane * R::LM);
-    lds_read_rec<CH>(v, lb + t.lane * R::LV);
-    if (READ_R) lds_read_rec<CH>(x, lr + t.lane * R::LV);
-    mat_times_cols<EQOP, ADJ, 1>(x, m, v);
-    wave_sync();
-    lds_write_rec<CH>(lr + t.lane * R::LV, x);
-    wave_sync();
-    if (!XF) store_contig<R::V, CH>((char *)(r + 3 * (size_t)t.base), lr, t.nsites, t.lane);
-    else store_scatter<R::V, CH>((uint64_t)(r + 3 * (size_t)site), lr, t.nsites, t.lane);
 }
 
 // ---------------------------------------------------------------------------
 // H eqop spproj_{MU,SIGN}(M * pD)
 // ---------------------------------------------------------------------------
-template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF, int WPB>
-__global__ void __launch_bounds__(WPB * WAVE)
+template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF, int WPB, int MINB>
+__global__ void __launch_bounds__(WPB * WAVE, MINB)
 k_spproj(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *const *__restrict__ b,
-         const int *__restrict__ index, int n)
+         const int *__restrict__ index, int n, int al)
 {
     typedef rec<T> R;
-    constexpr int CH = R::CH;
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::LM + R::LD + R::LH);
+    // M region is reused for the H result (H <= M)
+    constexpr int BYTES = WAVE * (R::M + R::D + (READ_R ? R::H : 0));
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
-    tile_t t;
-    if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *ld = lm + WAVE * R::LM;
-    char *lh = ld + WAVE * R::LD;
-    const bool act = t.lane < t.nsites;
-
-    int site = t.base + t.lane;
-    if (XF) site = act ? index[t.base + t.lane] : 0;
-    const uint64_t pd = act ? (uint64_t)b[site] : 0;
-    if (!XF) {
-        stage_contig<R::M, CH>(lm, (const char *)(a + 9 * (size_t)t.base), t.nsites, t.lane);
-        if (READ_R) stage_contig<R::H, CH>(lh, (const char *)(r + 6 * (size_t)t.base), t.nsites, t.lane);
-    } else {
-        stage_gather<R::M, CH>(lm, (uint64_t)(a + 9 * (size_t)site), t.nsites, t.lane);
-        if (READ_R) stage_gather<R::H, CH>(lh, (uint64_t)(r + 6 * (size_t)site), t.nsites, t.lane);
-    }
-    stage_gather<R::D, CH>(ld, pd, t.nsites, t.lane);
-    dma_wait();
-    cpx<T> m[9], d[12], h[6];
-    lds_read_rec<CH>(m, lm + t.lane * R::LM);
-    lds_read_rec<CH>(d, ld + t.lane * R::LD);
-    if (READ_R) lds_read_rec<CH>(h, lh + t.lane * R::LH);
-    cpx<T> tp[6];
+    char *ld = lm + WAVE * R::M;
+    char *lh = READ_R ? ld + WAVE * R::D : lm;
+    tiles t(WPB, n);
+    if (t.tile >= t.ntiles) return;
+    // gather address of the first tile
+    int site = t.base() + t.lane;
+    bool act = t.lane < t.nsites();
+    if (XF) site = act ? index[site] : 0;
+    uint64_t pd = act ? (uint64_t)b[site] : 0;
+    for (;;) {
+        const int base = t.base(), ns = t.nsites();
+        if (!XF) {
+            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, t.lane, al & AL_A);
+            if (READ_R) load_tile<R::H>(lh, (const char *)(r + 6 * (size_t)base), ns, t.lane, al & AL_R);
+        } else {
+            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, t.lane);
+            if (READ_R) gather_tile<R::H>(lh, act ? (uint64_t)(r + 6 * (size_t)site) : 0, ns, t.lane);
+        }
+        gather_tile<R::D>(ld, pd, ns, t.lane);
+        // prefetch the next tile's gather address while this tile is in flight
+        const int next = t.tile + t.stride;
+        int nsite = 0;
+        uint64_t npd = 0;
+        bool nact = false;
+        if (next < t.ntiles) {
+            nsite = next * WAVE + t.lane;
+            nact = t.lane < min(WAVE, n - next * WAVE);
+            if (XF) nsite = nact ? index[nsite] : 0;
+            else npd = nact ? (uint64_t)b[nsite] : 0;
+        }
+        dma_wait();
+        cpx<T> m[9], d[12], h[6];
+        lds_read(m, lm + t.lane * R::M);
+        lds_read(d, ld + t.lane * R::D);
+        if (READ_R) lds_read(h, lh + t.lane * R::H);
+        cpx<T> tp[6];
 #pragma unroll
-    for (int ic = 0; ic < 3; ic++) spproj_row<MU, SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
-    mat_times_cols<EQOP, ADJ, 2>(h, m, tp);
-    wave_sync();
-    lds_write_rec<CH>(lh + t.lane * R::LH, h);
-    wave_sync();
-    if (!XF) store_contig<R::H, CH>((char *)(r + 6 * (size_t)t.base), lh, t.nsites, t.lane);
-    else store_scatter<R::H, CH>((uint64_t)(r + 6 * (size_t)site), lh, t.nsites, t.lane);
+        for (int ic = 0; ic < 3; ic++) spproj_row<MU, SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
+        mat_times_cols<EQOP, ADJ, 2>(h, m, tp);
+        wave_sync();
+        lds_write(lh + t.lane * R::H, h);
+        wave_sync();
+        if (!XF) store_tile<R::H>((char *)(r + 6 * (size_t)base), lh, ns, t.lane, al & AL_R);
+        else scatter_tile<R::H>(act ? (uint64_t)(r + 6 * (size_t)site) : 0, lh, ns, t.lane);
+        wave_sync();
+        if (next >= t.ntiles) break;
+        t.tile = next;
+        site = nsite;
+        act = nact;
+        if (XF) npd = nact ? (uint64_t)b[nsite] : 0;
+        pd = npd;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // D eqop sprecon_{MU,SIGN}(M * pH)
 // ---------------------------------------------------------------------------
-template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF, int WPB>
-__global__ void __launch_bounds__(WPB * WAVE)
+template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF, int WPB, int MINB>
+__global__ void __launch_bounds__(WPB * WAVE, MINB)
 k_sprecon(cpx<T> *__restrict__ r, const cpx<T> *__restrict__ a, const cpx<T> *const *__restrict__ b,
-          const int *__restrict__ index, int n)
+          const int *__restrict__ index, int n, int al)
 {
     typedef rec<T> R;
     typedef recon_tab<MU, SIGN> TAB;
-    constexpr int CH = R::CH;
     // eq/eqm overwrite every component; peq/meq read the destination first
     constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    constexpr int BYTES = WAVE * (R::LM + R::LH + R::LD);
+    constexpr int BYTES = WAVE * (R::M + R::H + R::D);
     __shared__ __attribute__((aligned(16))) char smem[WPB * BYTES];
-    tile_t t;
-    if (!get_tile<WPB>(t, n)) return;
     char *lm = smem + (threadIdx.x / WAVE) * BYTES;
-    char *lh = lm + WAVE * R::LM;
-    char *ld = lh + WAVE * R::LH;
-    const bool act = t.lane < t.nsites;
-
-    int site = t.base + t.lane;
-    if (XF) site = act ? index[t.base + t.lane] : 0;
-    const uint64_t ph = act ? (uint64_t)b[site] : 0;
-    if (!XF) {
-        stage_contig<R::M, CH>(lm, (const char *)(a + 9 * (size_t)t.base), t.nsites, t.lane);
-        if (READ_R) stage_contig<R::D, CH>(ld, (const char *)(r + 12 * (size_t)t.base), t.nsites, t.lane);
-    } else {
-        stage_gather<R::M, CH>(lm, (uint64_t)(a + 9 * (size_t)site), t.nsites, t.lane);
-        if (READ_R) stage_gather<R::D, CH>(ld, (uint64_t)(r + 12 * (size_t)site), t.nsites, t.lane);
-    }
-    stage_gather<R::H, CH>(lh, ph, t.nsites, t.lane);
-    dma_wait();
-    cpx<T> m[9], h[6], d[12];
-    lds_read_rec<CH>(m, lm + t.lane * R::LM);
-    lds_read_rec<CH>(h, lh + t.lane * R::LH);
-    if (READ_R) lds_read_rec<CH>(d, ld + t.lane * R::LD);
-    cpx<T> tv[6];
-    mat_times_cols<EQ, ADJ, 2>(tv, m, h);
+    char *lh = lm + WAVE * R::M;
+    char *ld = lh + WAVE * R::H;
+    tiles t(WPB, n);
+    if (t.tile >= t.ntiles) return;
+    int site = t.base() + t.lane;
+    bool act = t.lane < t.nsites();
+    if (XF) site = act ? index[site] : 0;
+    uint64_t ph = act ? (uint64_t)b[site] : 0;
+    for (;;) {
+        const int base = t.base(), ns = t.nsites();
+        if (!XF) {
+            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, t.lane, al & AL_A);
+            if (READ_R) load_tile<R::D>(ld, (const char *)(r + 12 * (size_t)base), ns, t.lane, al & AL_R);
+        } else {
+            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, t.lane);
+            if (READ_R) gather_tile<R::D>(ld, act ? (uint64_t)(r + 12 * (size_t)site) : 0, ns, t.lane);
+        }
+        gather_tile<R::H>(lh, ph, ns, t.lane);
+        const int next = t.tile + t.stride;
+        int nsite = 0;
+        uint64_t nph = 0;
+        bool nact = false;
+        if (next < t.ntiles) {
+            nsite = next * WAVE + t.lane;
+            nact = t.lane < min(WAVE, n - next * WAVE);
+            if (XF) nsite = nact ? index[nsite] : 0;
+            else nph = nact ? (uint64_t)b[nsite] : 0;
+        }
+        dma_wait();
+        cpx<T> m[9], h[6], d[12];
+        lds_read(m, lm + t.lane * R::M);
+        lds_read(h, lh + t.lane * R::H);
+        if (READ_R) lds_read(d, ld + t.lane * R::D);
+        cpx<T> tv[6];
+        mat_times_cols<EQ, ADJ, 2>(tv, m, h);
 #pragma unroll
-    for (int ic = 0; ic < 3; ic++) {
-        recon_apply<EQOP, TAB::op[0]>(d[4 * ic + 0], tv[2 * ic + TAB::src[0]]);
-        recon_apply<EQOP, TAB::op[1]>(d[4 * ic + 1], tv[2 * ic + TAB::src[1]]);
-        recon_apply<EQOP, TAB::op[2]>(d[4 * ic + 2], tv[2 * ic + TAB::src[2]]);
-        recon_apply<EQOP, TAB::op[3]>(d[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
+        for (int ic = 0; ic < 3; ic++) {
+            recon_apply<EQOP, TAB::op[0]>(d[4 * ic + 0], tv[2 * ic + TAB::src[0]]);
+            recon_apply<EQOP, TAB::op[1]>(d[4 * ic + 1], tv[2 * ic + TAB::src[1]]);
+            recon_apply<EQOP, TAB::op[2]>(d[4 * ic + 2], tv[2 * ic + TAB::src[2]]);
+            recon_apply<EQOP, TAB::op[3]>(d[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
+        }
+        wave_sync();
+        lds_write(ld + t.lane * R::D, d);
+        wave_sync();
+        if (!XF) store_tile<R::D>((char *)(r + 12 * (size_t)base), ld, ns, t.lane, al & AL_R);
+        else scatter_tile<R::D>(act ? (uint64_t)(r + 12 * (size_t)site) : 0, ld, ns, t.lane);
+        wave_sync();
+        if (next >= t.ntiles) break;
+        t.tile = next;
+        site = nsite;
+        act = nact;
+        if (XF) nph = nact ? (uint64_t)b[nsite] : 0;
+        ph = nph;
     }
-    wave_sync();
-    lds_write_rec<CH>(ld + t.lane * R::LD, d);
-    wave_sync();
-    if (!XF) store_contig<R::D, CH>((char *)(r + 12 * (size_t)t.base), ld, t.nsites, t.lane);
-    else store_scatter<R::D, CH>((uint64_t)(r + 12 * (size_t)site), ld, t.nsites, t.lane);
 }
 
 // ---------------------------------------------------------------------------
 // Host-side launchers (called from the C ABI layer).
+// Launch shape: WPB waves per block, MINB blocks per CU requested from the
+// register allocator; the grid covers the tiles with about TPW tiles per
+// wave (the persistent loop balances the rest).
 // ---------------------------------------------------------------------------
-template <class T> struct wpb_of { static constexpr int v = 2; };
+template <class T> struct shape_of;
+template <> struct shape_of<float> { static constexpr int WPB = 2, MINB = 5; };
+template <> struct shape_of<double> { static constexpr int WPB = 1, MINB = 5; };
+
+static int g_tiles_per_wave = 2;
+
+static inline int align_flags(const void *a, const void *b, const void *r)
+{
+    return (((uintptr_t)a % 16 == 0) ? AL_A : 0) | (((uintptr_t)b % 16 == 0) ? AL_B : 0) |
+           (((uintptr_t)r % 16 == 0) ? AL_R : 0);
+}
+
+static inline dim3 grid_for(int n, int wpb)
+{
+    const long long ntiles = ((long long)n + WAVE - 1) / WAVE;
+    const long long per_block = (long long)wpb * g_tiles_per_wave;
+    long long g = (ntiles + per_block - 1) / per_block;
+    return dim3((unsigned)(g < 1 ? 1 : g));
+}
 
 template <class T, int EQOP, bool ADJ, bool PB, bool XF>
 static hipError_t launch_matvec_t(void *r, const void *a, const void *b, const int *index, int n,
                                   hipStream_t st)
 {
-    constexpr int W = wpb_of<T>::v;
-    const int ntiles = (n + WAVE - 1) / WAVE;
-    dim3 grid((ntiles + W - 1) / W), block(W * WAVE);
-    hipLaunchKernelGGL((k_matvec<T, EQOP, ADJ, PB, XF, W>), grid, block, 0, st, (cpx<T> *)r,
-                       (const cpx<T> *)a, b, index, n);
+    constexpr int W = shape_of<T>::WPB;
+    hipLaunchKernelGGL((k_matvec<T, EQOP, ADJ, PB, XF, W>), grid_for(n, W), dim3(W * WAVE), 0, st, (cpx<T> *)r,
+                       (const cpx<T> *)a, b, index, n, align_flags(a, b, r));
     return hipGetLastError();
 }
 
@@ -221,11 +326,9 @@ template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF>
 static hipError_t launch_spproj_t(void *r, const void *a, const void *b, const int *index, int n,
                                   hipStream_t st)
 {
-    constexpr int W = wpb_of<T>::v;
-    const int ntiles = (n + WAVE - 1) / WAVE;
-    dim3 grid((ntiles + W - 1) / W), block(W * WAVE);
-    hipLaunchKernelGGL((k_spproj<T, EQOP, ADJ, MU, SIGN, XF, W>), grid, block, 0, st, (cpx<T> *)r,
-                       (const cpx<T> *)a, (const cpx<T> *const *)b, index, n);
+    constexpr int W = shape_of<T>::WPB, MB = shape_of<T>::MINB;
+    hipLaunchKernelGGL((k_spproj<T, EQOP, ADJ, MU, SIGN, XF, W, MB>), grid_for(n, W), dim3(W * WAVE), 0, st,
+                       (cpx<T> *)r, (const cpx<T> *)a, (const cpx<T> *const *)b, index, n, align_flags(a, b, r));
     return hipGetLastError();
 }
 
@@ -233,11 +336,9 @@ template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF>
 static hipError_t launch_sprecon_t(void *r, const void *a, const void *b, const int *index, int n,
                                    hipStream_t st)
 {
-    constexpr int W = wpb_of<T>::v;
-    const int ntiles = (n + WAVE - 1) / WAVE;
-    dim3 grid((ntiles + W - 1) / W), block(W * WAVE);
-    hipLaunchKernelGGL((k_sprecon<T, EQOP, ADJ, MU, SIGN, XF, W>), grid, block, 0, st, (cpx<T> *)r,
-                       (const cpx<T> *)a, (const cpx<T> *const *)b, index, n);
+    constexpr int W = shape_of<T>::WPB, MB = shape_of<T>::MINB;
+    hipLaunchKernelGGL((k_sprecon<T, EQOP, ADJ, MU, SIGN, XF, W, MB>), grid_for(n, W), dim3(W * WAVE), 0, st,
+                       (cpx<T> *)r, (const cpx<T> *)a, (const cpx<T> *const *)b, index, n, align_flags(a, b, r));
     return hipGetLastError();
 }
 
@@ -323,6 +424,9 @@ hipError_t qla_launch_spproj(int prec, void *r, const void *a, const void *b, in
     if (index) return spproj_mu<double, EQ, false, true>(r, a, b, mu, sign, index, n, st);
     return spproj_mu<double, EQ, false, false>(r, a, b, mu, sign, index, n, st);
 }
+
+// Tiles per wave of the persistent site kernels (launch-shape knob).
+void qla_set_tiles_per_wave(int tpw) { g_tiles_per_wave = tpw < 1 ? 1 : tpw; }
 
 hipError_t qla_launch_sprecon(int prec, void *r, const void *a, const void *b, int mu, int sign,
                               const int *index, int n, hipStream_t st)

@@ -61,6 +61,7 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_last_error": (i32, []),
     "QLA_MI355X_last_error_string": (C.c_char_p, []),
     "QLA_MI355X_version": (C.c_char_p, []),
+    "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
 
 _lib: C.CDLL | None = None
