@@ -60,19 +60,26 @@ __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
 template <int CH> struct lds_stride { static constexpr int v = (CH == 4) ? 4 : 16; };
 template <int BPS, int CH> struct rec_lds { static constexpr int v = (BPS / CH) * lds_stride<CH>::v; };
 
+// Cache policy of the streaming loads and stores: every field record is
+// touched once per call, so loads and stores carry the non-temporal hint
+// (the "nt" bit, cache-policy value 2 on gfx950).
+#ifndef QLA_STREAM_CPOL
+#define QLA_STREAM_CPOL 2
+#endif
+
 // One direct-to-LDS load of CH bytes per lane (the builtin needs a literal width).
 template <int CH> __device__ __forceinline__ void dma_load(const void *g, char *lds_base);
 template <> __device__ __forceinline__ void dma_load<4>(const void *g, char *lds_base)
 {
-    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 4, 0, 0);
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 4, 0, QLA_STREAM_CPOL);
 }
 template <> __device__ __forceinline__ void dma_load<12>(const void *g, char *lds_base)
 {
-    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 12, 0, 0);
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 12, 0, QLA_STREAM_CPOL);
 }
 template <> __device__ __forceinline__ void dma_load<16>(const void *g, char *lds_base)
 {
-    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g, (lds_void_ptr)lds_base, 16, 0, QLA_STREAM_CPOL);
 }
 
 // Contiguous records g[0 .. nsites) -> lds (site s at s*rec_lds).
@@ -163,12 +170,22 @@ __device__ __forceinline__ void lds_write_rec(char *rec, const cpx<T> (&v)[N])
 // ---------------------------------------------------------------------------
 // LDS -> HBM.  Chunk types for the stores.
 // ---------------------------------------------------------------------------
-struct __attribute__((aligned(4))) chunk12 { uint32_t x, y, z; };
-struct __attribute__((aligned(16))) chunk16 { uint32_t x, y, z, w; };
+typedef uint32_t chunk12 __attribute__((ext_vector_type(3), aligned(4)));
+typedef uint32_t chunk16 __attribute__((ext_vector_type(4), aligned(16)));
 template <int CH> struct chunk_of;
 template <> struct chunk_of<12> { typedef chunk12 type; };
 template <> struct chunk_of<16> { typedef chunk16 type; };
 template <> struct chunk_of<4> { typedef uint32_t type; };
+
+// store of one chunk with the streaming cache policy
+template <class C> __device__ __forceinline__ void stream_store(C *p, C v)
+{
+#if QLA_STREAM_CPOL == 2
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 
 template <int BPS, int CH>
 __device__ __forceinline__ void store_contig(char *g, const char *lds, int nsites, int lane)
@@ -179,7 +196,7 @@ __device__ __forceinline__ void store_contig(char *g, const char *lds, int nsite
 #pragma unroll
     for (int k = 0; k < NPS; k++) {
         const int c = k * WAVE + lane;
-        if (c < nch) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * LS);
+        if (c < nch) stream_store((C *)(g + (size_t)c * CH), *(const C *)(lds + c * LS));
     }
 }
 
@@ -194,7 +211,7 @@ __device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds,
         const int s = c / NPS;
         const int p = c - s * NPS;
         const uint64_t a = shfl64(my_addr, s);
-        if (s < nsites) *(C *)(a + (uint64_t)p * CH) = *(const C *)(lds + c * LS);
+        if (s < nsites) stream_store((C *)(a + (uint64_t)p * CH), *(const C *)(lds + c * LS));
     }
 }
 
@@ -208,12 +225,12 @@ __device__ __forceinline__ void store_bytes(char *g, const char *lds, int nbytes
 #pragma unroll(CH == 4 ? 1 : (MAXB / CH + WAVE - 1) / WAVE)
     for (int k = 0; k < (MAXB / CH + WAVE - 1) / WAVE; k++) {
         const int c = k * WAVE + lane;
-        if (c < full) *(C *)(g + (size_t)c * CH) = *(const C *)(lds + c * CH);
+        if (c < full) stream_store((C *)(g + (size_t)c * CH), *(const C *)(lds + c * CH));
     }
     if (CH > 4) {
         const int done = full * CH;
         const int tail = (nbytes - done) / 4;
-        if (lane < tail) *(uint32_t *)(g + done + lane * 4) = *(const uint32_t *)(lds + done + lane * 4);
+        if (lane < tail) stream_store((uint32_t *)(g + done + lane * 4), *(const uint32_t *)(lds + done + lane * 4));
     }
 }
 
