@@ -43,6 +43,21 @@ def parse():
     return p.parse_args()
 
 
+def _timed(stream, fn, steps, warmup):
+    """ms for `steps` calls of fn(it) after `warmup` untimed ones (HIP events)."""
+    import torch
+    for it in range(warmup):
+        fn(it)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for it in range(steps):
+        fn(it)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -57,14 +72,20 @@ def main():
     lib = qla_amd.load()
     n = lattice.volume(DIMS)
     fw, bw = lattice.neighbours(DIMS)
-    sets = []
-    for k in range(NSETS):
-        u = torch.from_numpy(lattice.random_su3(n, 1 + 10 * k).astype(np.float32)).cuda()
-        d = torch.from_numpy(lattice.random_field("D", n, 2 + 10 * k).astype(np.float32)).cuda()
-        h = torch.zeros((n, 3, 2, 2), dtype=torch.float32, device="cuda")
-        ptr = {(mu, s): qla_amd.pointer_array(d, (fw if s == 1 else bw)[mu]) for mu in range(4) for s in (1, -1)}
-        sets.append((u, d, h, ptr))
     combos = [(mu, s) for mu in range(4) for s in (1, -1)]
+    nbr = {(mu, s): (fw if s == 1 else bw)[mu] for mu, s in combos}
+    sets = []
+    u64 = d64 = None
+    for k in range(NSETS):
+        u = lattice.random_su3(n, 1 + 10 * k)
+        d = lattice.random_field("D", n, 2 + 10 * k)
+        if k == 0:
+            u64, d64 = u, d
+        du = torch.from_numpy(u.astype(np.float32)).cuda()
+        dd = torch.from_numpy(d.astype(np.float32)).cuda()
+        h = torch.zeros((n, 3, 2, 2), dtype=torch.float32, device="cuda")
+        ptr = {c: qla_amd.pointer_array(dd, nbr[c]) for c in combos}
+        sets.append((du, dd, h, ptr))
     stream = torch.cuda.current_stream()
     qla_amd.set_stream(stream)
     qla_amd.set_sync(False)
@@ -75,48 +96,30 @@ def main():
             u, d, h, ptr = sets[(it * len(combos) + j) % NSETS]
             f(h.data_ptr(), u.data_ptr(), ptr[(mu, s)].data_ptr(), mu, s, n)
 
-    for it in range(args.warmup):
-        step(it)
-    torch.cuda.synchronize()
-    qla_amd.check()
-
     copy_gbs = None
     if args.copy_mib:
         src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
-        dst.copy_(src)
-        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        c0.record(stream)
-        for _ in range(5):
-            dst.copy_(src)
-        c1.record(stream)
-        torch.cuda.synchronize()
-        copy_gbs = 2 * 5 * src.numel() * 4 / (c0.elapsed_time(c1) * 1e-3) / 1e9
+        ms = _timed(stream, lambda it: dst.copy_(src), 5, 1)
+        copy_gbs = 2 * 5 * src.numel() * 4 / (ms * 1e-3) / 1e9
         del src, dst
 
-    # whole-job timing
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    ev0.record(stream)
+    # headline: plain FP32 spproj, whole timed region
     t0 = time.perf_counter()
-    for it in range(args.steps):
-        step(it)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+    ms_total = _timed(stream, step, args.steps, args.warmup)
     wall = time.perf_counter() - t0
     qla_amd.check()
-    ms_total = ev0.elapsed_time(ev1)
     ms_per_step = ms_total / args.steps
     calls = args.steps * len(combos)
-    alg_bytes = float(BYTES_PER_SITE) * n * calls
-    gbs = alg_bytes / (ms_total * 1e-3) / 1e9
+    gbs = float(BYTES_PER_SITE) * n * calls / (ms_total * 1e-3) / 1e9
     sites_per_s = n * calls / (ms_total * 1e-3)
-
     # the timed region holds only back-to-back spproj launches on one stream,
     # so its length over the launch count is the kernel's average duration
     # (rocprofv3 --kernel-trace on the same command agrees, see profiles/)
     avg_ms = ms_total / calls
     achieved = BYTES_PER_SITE * n / (avg_ms * 1e-3) / 1e9
+
+    extra = variants(lib, stream, n, combos, nbr, sets, u64, d64, args, sites_per_s)
 
     cpu = None
     if not args.no_cpu:
@@ -144,8 +147,95 @@ def main():
                      "kernel": "qla::k_spproj<float,...>", "avg_launch_ms": round(avg_ms, 5),
                      "alg_bytes_per_launch": BYTES_PER_SITE * n},
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(line))
+
+
+def variants(lib, stream, n, combos, nbr, sets, u64, d64, args, plain_sites_per_s):
+    """Companion measurements on the same 32^4 workload: sprecon (FP32),
+    spproj in FP64, and the compressed-precision spproj/sprecon with bfloat16
+    and binary16 links; each with its algorithmic GB/s, sites/s and, for the
+    compressed forms, the relative error against the FP64 kernel's result."""
+    import torch
+    import qla_amd
+    steps = max(5, args.steps // 2)
+    out = {}
+    nc = len(combos)
+
+    def rate(ms, bytes_per_site, k_steps):
+        c = k_steps * nc
+        return {"ms_per_call": round(ms / c, 5), "GBs": round(bytes_per_site * n * c / (ms * 1e-3) / 1e9, 1),
+                "sites_per_s": round(n * c / (ms * 1e-3), 1)}
+
+    # sprecon FP32: D += sprecon(M * pH)
+    hsrc = [torch.from_numpy(np.random.default_rng(5 + k).standard_normal((n, 3, 2, 2)).astype(np.float32)).cuda()
+            for k in range(NSETS)]
+    dsts = [torch.zeros((n, 3, 4, 2), dtype=torch.float32, device="cuda") for _ in range(NSETS)]
+    hptr = [{c: qla_amd.pointer_array(hsrc[k], nbr[c]) for c in combos} for k in range(NSETS)]
+
+    def sprecon(it):
+        for j, (mu, s) in enumerate(combos):
+            k = (it * nc + j) % NSETS
+            lib.QLA_F3_D_vpeq_sprecon_M_times_pH(dsts[k].data_ptr(), sets[k][0].data_ptr(),
+                                                 hptr[k][(mu, s)].data_ptr(), mu, s, n)
+    out["sprecon_f3"] = rate(_timed(stream, sprecon, steps, 2), 320, steps)
+    del hsrc, hptr
+
+    # FP64 spproj on set-0 inputs (also the reference for the compressed error)
+    u64d = torch.from_numpy(u64).cuda()
+    d64d = torch.from_numpy(d64).cuda()
+    h64 = torch.zeros((n, 3, 2, 2), dtype=torch.float64, device="cuda")
+    p64 = {c: qla_amd.pointer_array(d64d, nbr[c]) for c in combos}
+
+    def spproj64(it):
+        for mu, s in combos:
+            lib.QLA_D3_H_veq_spproj_M_times_pD(h64.data_ptr(), u64d.data_ptr(), p64[(mu, s)].data_ptr(), mu, s, n)
+    out["spproj_d3"] = rate(_timed(stream, spproj64, steps, 2), 440, steps)
+    lib.QLA_D3_H_veq_spproj_M_times_pD(h64.data_ptr(), u64d.data_ptr(), p64[(0, 1)].data_ptr(), 0, 1, n)
+    torch.cuda.synchronize()
+    ref = h64.clone()
+    del u64d, d64d, p64, h64
+
+    # compressed forms
+    for fmt, name in ((0, "bf16"), (1, "f16")):
+        cs = []
+        for k in range(NSETS):
+            u, d = sets[k][0], sets[k][1]
+            link = torch.empty((n, 12), dtype=torch.int16, device="cuda")
+            lib.QLA_MI355X_F3_M_pack12(link.data_ptr(), fmt, u.data_ptr(), n)
+            drec = torch.empty((n, 52), dtype=torch.uint8, device="cuda")
+            lib.QLA_MI355X_F3_D_packh(drec.data_ptr(), d.data_ptr(), n)
+            hrec = torch.zeros((n, 28), dtype=torch.uint8, device="cuda")
+            dp = {c: qla_amd.pointer_array(drec, nbr[c]) for c in combos}
+            hp = {c: qla_amd.pointer_array(hrec, nbr[c]) for c in combos}
+            cs.append((link, drec, hrec, dp, hp))
+
+        def cspproj(it, fmt=fmt, cs=cs):
+            for j, (mu, s) in enumerate(combos):
+                link, drec, hrec, dp, hp = cs[(it * nc + j) % NSETS]
+                lib.QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh(hrec.data_ptr(), link.data_ptr(), fmt,
+                                                             dp[(mu, s)].data_ptr(), mu, s, n)
+
+        def csprecon(it, fmt=fmt, cs=cs):
+            for j, (mu, s) in enumerate(combos):
+                link, drec, hrec, dp, hp = cs[(it * nc + j) % NSETS]
+                lib.QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh(drec.data_ptr(), link.data_ptr(), fmt,
+                                                               hp[(mu, s)].data_ptr(), mu, s, n)
+        r = rate(_timed(stream, cspproj, steps, 2), 112, steps)
+        r["speedup_vs_plain_sites"] = round(r["sites_per_s"] / plain_sites_per_s, 3)
+        link, drec, hrec, dp, hp = cs[0]
+        lib.QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh(hrec.data_ptr(), link.data_ptr(), fmt,
+                                                     dp[(0, 1)].data_ptr(), 0, 1, n)
+        hdec = torch.empty((n, 3, 2, 2), dtype=torch.float32, device="cuda")
+        lib.QLA_MI355X_F3_H_unpackh(hdec.data_ptr(), hrec.data_ptr(), n)
+        torch.cuda.synchronize()
+        r["rel_err_vs_fp64"] = float(torch.linalg.vector_norm(hdec.double() - ref) / torch.linalg.vector_norm(ref))
+        out[f"spproj_{name}_links"] = r
+        out[f"sprecon_{name}_links"] = rate(_timed(stream, csprecon, steps, 2), 164, steps)
+        del cs
+    qla_amd.check()
+    return out
 
 
 def measured_traffic():

@@ -188,6 +188,32 @@ void QLA_FN_M_veq_invsqrt_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][n
 #endif
 
 /* ------------------------------------------------------------------------ */
+/* Compressed-precision extension (no counterpart in the reference, which    */
+/* stores FP32/FP64 records only).  Record formats, 4-byte aligned:          */
+/*   link 24 B: rows 0,1 of an SU(3) matrix as 12 bfloat16 (fmt 0) or IEEE    */
+/*              binary16 (fmt 1) reals; row 2 = conj(row0 x row1) in FP32.     */
+/*   Dh   52 B: DiracFermion as 24 binary16 then a float scale s (x = v*s).   */
+/*   Hh   28 B: HalfFermion as 12 binary16 then a float scale.                 */
+/* Encoding: s = max |x| of the record, v = binary16(x/s) rounded to nearest  */
+/* even.  Arithmetic after decoding is FP32 in the order of the plain forms.  */
+/* Gathers take pointer arrays to Dh / Hh records, like the QLA pD / pH args. */
+/* ------------------------------------------------------------------------ */
+enum { QLA_MI355X_LINK_BF16 = 0, QLA_MI355X_LINK_F16 = 1 };
+void QLA_MI355X_F3_M_pack12(void *r, int fmt, QLA_F3_ColorMatrix *a, int n);
+void QLA_MI355X_F3_M_unpack12(QLA_F3_ColorMatrix *r, int fmt, void *a, int n);
+void QLA_MI355X_F3_D_packh(void *r, QLA_F3_DiracFermion *a, int n);
+void QLA_MI355X_F3_D_unpackh(QLA_F3_DiracFermion *r, void *a, int n);
+void QLA_MI355X_F3_H_packh(void *r, QLA_F3_HalfFermion *a, int n);
+void QLA_MI355X_F3_H_unpackh(QLA_F3_HalfFermion *r, void *a, int n);
+/* Hh = spproj(M12 * pDh) and Dh += sprecon(M12 * pHh); mu/sign as in QLA. */
+void QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh(void *r, void *a, int fmt, void **b, int mu, int sign, int n);
+void QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh(void *r, void *a, int fmt, void **b, int mu, int sign, int n);
+/* norm2 and dot of decoded Dh fields with the QLA_DF3 semantics (float
+ * component norms / double-promoted products, double sum). */
+void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n);
+void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n);
+
+/* ------------------------------------------------------------------------ */
 /* Library control (no counterpart in the reference, which has no device   */
 /* and no stream; these only select where and how the calls above run).    */
 /* ------------------------------------------------------------------------ */

@@ -88,6 +88,23 @@ void reduce(int prec, int kind, const void *a, const void *b, long long m, void 
     record(e, name);
 }
 
+// compressed-field reductions: result doubles to r (host or device)
+void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
+{
+    double *sc = scratch_for_device();
+    if (!sc) return;
+    if (n < 0) n = 0;
+    if (is_device_pointer(r)) {
+        finish(qla_launch_reduce_dh(a, b, n, (double *)r, sc, ts.stream), name);
+        return;
+    }
+    double *slot = sc + qla_reduce_scratch_doubles();
+    hipError_t e = qla_launch_reduce_dh(a, b, n, slot, sc, ts.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    record(e, name);
+}
+
 } // namespace
 
 extern "C" {
@@ -105,7 +122,11 @@ int QLA_MI355X_last_error(void)
 }
 const char *QLA_MI355X_last_error_string(void) { return hipGetErrorString(ts.err); }
 const char *QLA_MI355X_version(void) { return QLA_MI355X_VERSION_STRING; }
-void QLA_MI355X_set_tiles_per_wave(int tpw) { qla_set_tiles_per_wave(tpw); }
+void QLA_MI355X_set_tiles_per_wave(int tpw)
+{
+    qla_set_tiles_per_wave(tpw);
+    qla_set_cmp_tiles_per_wave(tpw);
+}
 
 // ----------------------------------------------------------------- a1 / a2
 void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *b, int n)
@@ -246,4 +267,35 @@ void QLA_FN_M_veq_sqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_ve
 void QLA_FN_M_veq_invsqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_invsqrt_M", 0, 2, nc)
 #undef QLA_MATFN
 
+// ----------------------------------------------------------------- compressed
+void QLA_MI355X_F3_M_pack12(void *r, int fmt, QLA_F3_ColorMatrix *a, int n)
+{ finish(qla_launch_convert(0, fmt, r, a, n, ts.stream), "QLA_MI355X_F3_M_pack12"); }
+void QLA_MI355X_F3_M_unpack12(QLA_F3_ColorMatrix *r, int fmt, void *a, int n)
+{ finish(qla_launch_convert(1, fmt, r, a, n, ts.stream), "QLA_MI355X_F3_M_unpack12"); }
+void QLA_MI355X_F3_D_packh(void *r, QLA_F3_DiracFermion *a, int n)
+{ finish(qla_launch_convert(2, 0, r, a, n, ts.stream), "QLA_MI355X_F3_D_packh"); }
+void QLA_MI355X_F3_D_unpackh(QLA_F3_DiracFermion *r, void *a, int n)
+{ finish(qla_launch_convert(3, 0, r, a, n, ts.stream), "QLA_MI355X_F3_D_unpackh"); }
+void QLA_MI355X_F3_H_packh(void *r, QLA_F3_HalfFermion *a, int n)
+{ finish(qla_launch_convert(4, 0, r, a, n, ts.stream), "QLA_MI355X_F3_H_packh"); }
+void QLA_MI355X_F3_H_unpackh(QLA_F3_HalfFermion *r, void *a, int n)
+{ finish(qla_launch_convert(5, 0, r, a, n, ts.stream), "QLA_MI355X_F3_H_unpackh"); }
+void QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh(void *r, void *a, int fmt, void **b, int mu, int sign, int n)
+{
+    finish(qla_launch_spproj_c(fmt, r, a, (const void *)b, mu, sign, n, ts.stream),
+           "QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh");
+}
+void QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh(void *r, void *a, int fmt, void **b, int mu, int sign, int n)
+{
+    finish(qla_launch_sprecon_c(fmt, r, a, (const void *)b, mu, sign, n, ts.stream),
+           "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh");
+}
+void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n)
+{
+    reduce_dh(a, nullptr, n, r, sizeof(double), "QLA_MI355X_DF3_r_veq_norm2_Dh");
+}
+void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n)
+{
+    reduce_dh(a, b, n, r, 2 * sizeof(double), "QLA_MI355X_DF3_c_veq_Dh_dot_Dh");
+}
 } // extern "C"

@@ -29,3 +29,15 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
 // the launch uses as its work counter (reset on the stream first).
 hipError_t qla_launch_matfn(int prec, int fn, int nc, void *r, const void *a, int n, unsigned int *counter,
                             hipStream_t st);
+
+// Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
+hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
+                               hipStream_t st);
+hipError_t qla_launch_sprecon_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
+                                hipStream_t st);
+void qla_set_cmp_tiles_per_wave(int tpw);
+// kind: 0 link pack, 1 link unpack, 2 D pack, 3 D unpack, 4 H pack, 5 H unpack
+hipError_t qla_launch_convert(int kind, int fmt, void *r, const void *a, int n, hipStream_t st);
+// norm2 (b == NULL) or dot of decoded Dh fields; doubles to device `out`
+hipError_t qla_launch_reduce_dh(const void *a, const void *b, int n, double *out, double *scratch,
+                                hipStream_t st);

@@ -53,6 +53,17 @@ _SIGS: dict[str, tuple[type, list]] = {
     # a9-a12
     **{f"QLA_{p}3_M_veq_{fn}_M": (None, [vp, vp, i32]) for p in ("F", "D") for fn in ("exp", "sqrt", "invsqrt")},
     **{f"QLA_FN_M_veq_{fn}_M": (None, [i32, vp, vp, i32]) for fn in ("exp", "sqrt", "invsqrt")},
+    # compressed-precision extension
+    "QLA_MI355X_F3_M_pack12": (None, [vp, i32, vp, i32]),
+    "QLA_MI355X_F3_M_unpack12": (None, [vp, i32, vp, i32]),
+    "QLA_MI355X_F3_D_packh": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_D_unpackh": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_H_packh": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_H_unpackh": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh": (None, [vp, vp, i32, vp, i32, i32, i32]),
+    "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh": (None, [vp, vp, i32, vp, i32, i32, i32]),
+    "QLA_MI355X_DF3_r_veq_norm2_Dh": (None, [vp, vp, i32]),
+    "QLA_MI355X_DF3_c_veq_Dh_dot_Dh": (None, [vp, vp, vp, i32]),
     # control
     "QLA_MI355X_set_stream": (None, [vp]),
     "QLA_MI355X_get_stream": (vp, []),

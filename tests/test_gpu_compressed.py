@@ -1,0 +1,145 @@
+"""GPU tests of the compressed-precision extension (no reference counterpart).
+
+Two bars:
+  * bit-exact against the numpy model of the formats and arithmetic
+    (oracle/compressed.py: binary16 / bfloat16 rounding to nearest even,
+    FP32 arithmetic in the plain kernels' order);
+  * accuracy against the FP64 oracle on the same inputs: with binary16 links
+    and binary16 fermions the relative error ||x - x_64|| / ||x_64|| over the
+    field stays <= 1e-3 (the compressed form's target); bfloat16 links carry
+    8-bit mantissas and are held to 5e-3.
+"""
+import numpy as np
+import pytest
+
+import qla_amd
+from oracle import compressed as cm
+from qla_amd import lattice
+from gpu_util import assert_bitwise, dev, host, say
+
+pytestmark = pytest.mark.gpu
+
+FMTS = {"bf16": 0, "f16": 1}
+TOL = {"bf16": 5e-3, "f16": 1e-3}
+
+
+def u8(rec):
+    return dev(np.ascontiguousarray(rec, dtype=np.uint8))
+
+
+def test_pack_unpack_roundtrip():
+    n = 3001
+    u = lattice.random_su3(n, 1).astype(np.float32)
+    d = lattice.random_field("D", n, 2).astype(np.float32)
+    h = lattice.random_field("H", n, 3).astype(np.float32)
+    lib = qla_amd.load()
+    for fmt, code in FMTS.items():
+        say(f"pack12 {fmt}")
+        r = dev(np.zeros((n, 12), dtype=np.uint16))
+        lib.QLA_MI355X_F3_M_pack12(r.data_ptr(), code, dev(u).data_ptr(), n)
+        qla_amd.check()
+        exp = cm.pack_link(u, fmt)
+        assert_bitwise(host(r).view(np.uint16).astype(np.uint32), exp.astype(np.uint32), f"pack12 {fmt}")
+        back = dev(np.zeros_like(u))
+        lib.QLA_MI355X_F3_M_unpack12(back.data_ptr(), code, r.data_ptr(), n)
+        assert_bitwise(host(back), cm.unpack_link(exp, fmt), f"unpack12 {fmt}")
+    for kind, x, k, packf, unpackf in (("D", d, 24, "QLA_MI355X_F3_D_packh", "QLA_MI355X_F3_D_unpackh"),
+                                        ("H", h, 12, "QLA_MI355X_F3_H_packh", "QLA_MI355X_F3_H_unpackh")):
+        say(f"pack {kind}")
+        rec = dev(np.zeros((n, 2 * k + 4), dtype=np.uint8))
+        getattr(lib, packf)(rec.data_ptr(), dev(x).data_ptr(), n)
+        qla_amd.check()
+        exp = cm.to_records(*cm.pack_fermion(x))
+        np.testing.assert_array_equal(host(rec), exp)
+        back = dev(np.zeros_like(x))
+        getattr(lib, unpackf)(back.data_ptr(), rec.data_ptr(), n)
+        v, s = cm.from_records(exp, k)
+        assert_bitwise(host(back), cm.unpack_fermion(v, s, x.shape), f"unpack {kind}")
+
+
+def _inputs(dims, seed=7):
+    n = lattice.volume(dims)
+    u64 = lattice.random_su3(n, seed)
+    d64 = lattice.random_field("D", n, seed + 1)
+    h64 = lattice.random_field("H", n, seed + 2)
+    return n, u64, d64, h64
+
+
+def rel_err(x, ref):
+    return float(np.linalg.norm((x - ref).ravel()) / np.linalg.norm(ref.ravel()))
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_spproj_compressed(orc, fmt):
+    dims = (8, 8, 8, 6)
+    n, u64, d64, _ = _inputs(dims)
+    fw, bw = lattice.neighbours(dims)
+    link = cm.pack_link(u64.astype(np.float32), fmt)
+    drec = cm.to_records(*cm.pack_fermion(d64.astype(np.float32)))
+    dl, dd = dev(link), u8(drec)
+    for mu in range(5):
+        for sign in (1, -1):
+            idx = (fw if sign == 1 else bw)[mu % 4]
+            say(f"Hh spproj {fmt} mu={mu} sign={sign}")
+            r = u8(np.zeros((n, 28)))
+            qla_amd.call("QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh", r, dl, FMTS[fmt],
+                         qla_amd.pointer_array(dd, idx), mu, sign, n)
+            qla_amd.check()
+            got = host(r)
+            exp = cm.spproj(link, fmt, drec, idx, mu, sign)
+            np.testing.assert_array_equal(got, exp, err_msg=f"spproj {fmt} mu={mu} sign={sign}")
+            ref = np.zeros((n, 3, 2, 2))
+            orc.H_eqop_spproj_M_times_pD(orc.EQ, ref, u64, d64, idx, mu, sign)
+            v, s = cm.from_records(got, 12)
+            assert rel_err(cm.unpack_fermion(v, s, ref.shape), ref) <= TOL[fmt]
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "f16"])
+def test_sprecon_compressed(orc, fmt):
+    dims = (8, 8, 6, 6)
+    n, u64, d64, h64 = _inputs(dims, 17)
+    fw, bw = lattice.neighbours(dims)
+    link = cm.pack_link(u64.astype(np.float32), fmt)
+    hrec = cm.to_records(*cm.pack_fermion(h64.astype(np.float32)))
+    drec = cm.to_records(*cm.pack_fermion(d64.astype(np.float32)))
+    dl, dh = dev(link), u8(hrec)
+    for mu in range(5):
+        for sign in (1, -1):
+            idx = (fw if sign == 1 else bw)[mu % 4]
+            say(f"Dh sprecon {fmt} mu={mu} sign={sign}")
+            r = u8(drec)
+            qla_amd.call("QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh", r, dl, FMTS[fmt],
+                         qla_amd.pointer_array(dh, idx), mu, sign, n)
+            qla_amd.check()
+            got = host(r)
+            exp = cm.sprecon(link, fmt, hrec, idx, drec, mu, sign)
+            np.testing.assert_array_equal(got, exp, err_msg=f"sprecon {fmt} mu={mu} sign={sign}")
+            ref = d64.copy()
+            orc.D_eqop_sprecon_M_times_pH(orc.PEQ, ref, u64, h64, idx, mu, sign)
+            v, s = cm.from_records(got, 24)
+            assert rel_err(cm.unpack_fermion(v, s, ref.shape), ref) <= TOL[fmt]
+
+
+def test_compressed_reductions(orc):
+    import ctypes as C
+    n = 40000
+    a = lattice.random_field("D", n, 5).astype(np.float32)
+    b = lattice.random_field("D", n, 6).astype(np.float32)
+    ra, rb = cm.to_records(*cm.pack_fermion(a)), cm.to_records(*cm.pack_fermion(b))
+    va, sa = cm.from_records(ra, 24)
+    vb, sb = cm.from_records(rb, 24)
+    xa, xb = cm.unpack_fermion(va, sa, a.shape), cm.unpack_fermion(vb, sb, b.shape)
+    da, db = u8(ra), u8(rb)
+    say("Dh norm2/dot")
+    out = (C.c_double * 2)()
+    qla_amd.call("QLA_MI355X_DF3_r_veq_norm2_Dh", C.cast(out, C.c_void_p).value, da, n)
+    qla_amd.check()
+    ref = orc.norm2(xa)
+    assert abs(out[0] - ref) <= 1e-12 * ref
+    qla_amd.call("QLA_MI355X_DF3_c_veq_Dh_dot_Dh", C.cast(out, C.c_void_p).value, da, db, n)
+    qla_amd.check()
+    ref = orc.dot(xa, xb)
+    scale = np.sqrt(orc.norm2(xa) * orc.norm2(xb))
+    assert abs(out[0] - ref.real) <= 1e-12 * scale and abs(out[1] - ref.imag) <= 1e-12 * scale
+    # the compressed norm is within the format's precision of the FP64 one
+    assert abs(orc.norm2(xa) - orc.norm2(a.astype(np.float64))) <= 2e-3 * orc.norm2(a.astype(np.float64))

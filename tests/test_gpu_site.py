@@ -207,3 +207,57 @@ def test_full_lattice_32(orc, prec):
     orc.D_eqop_sprecon_M_times_pH(orc.PEQ, exp, u, h, idx, mu, sign, index=even)
     assert_bitwise(host(r), exp, "32^4 x sprecon")
     qla_amd.check()
+
+
+@pytest.mark.parametrize("prec", ["F"])
+@pytest.mark.parametrize("shift", [8, 16, 24])
+def test_misaligned_fields(orc, prec, shift):
+    """QLA only guarantees 8-byte alignment of float records (qla_complex.h:30).
+    Fields placed 8/16/24 bytes past a 16-byte boundary exercise the 4-byte
+    chunk fallbacks of the contiguous and the gathered staging paths."""
+    dims = (6, 4, 4, 5)
+    u, d, h, v = inputs(dims, prec)
+    n = u.shape[0]
+    fw, bw = lattice.neighbours(dims)
+    fl = shift // 4
+
+    def shifted(a):
+        flat = np.zeros(a.size + 16, dtype=a.dtype)
+        t = dev(flat)
+        view = t[fl:fl + a.size].view(a.shape)
+        view.copy_(dev(a))
+        return t, view
+
+    _, du = shifted(u)
+    _, dd = shifted(d)
+    _, dh = shifted(h)
+    for mu, sign in ((0, 1), (3, -1), (4, 1)):
+        idx = (fw if sign == 1 else bw)[mu % 4]
+        say(f"misaligned shift={shift} spproj mu={mu} sign={sign}")
+        _, r = shifted(np.zeros((n, 3, 2, 2), dtype=u.dtype))
+        qla_amd.call("QLA_F3_H_veq_spproj_M_times_pD", r, du, qla_amd.pointer_array(dd, idx), mu, sign, n)
+        qla_amd.check()
+        exp = np.zeros((n, 3, 2, 2), dtype=u.dtype)
+        orc.H_eqop_spproj_M_times_pD(orc.EQ, exp, u, d, idx, mu, sign)
+        assert_bitwise(host(r), exp, f"misaligned spproj shift={shift}")
+        say(f"misaligned shift={shift} sprecon mu={mu} sign={sign}")
+        _, r = shifted(d)
+        qla_amd.call("QLA_F3_D_vpeq_sprecon_M_times_pH", r, du, qla_amd.pointer_array(dh, idx), mu, sign, n)
+        qla_amd.check()
+        exp = d.copy()
+        orc.D_eqop_sprecon_M_times_pH(orc.PEQ, exp, u, h, idx, mu, sign)
+        assert_bitwise(host(r), exp, f"misaligned sprecon shift={shift}")
+    say(f"misaligned shift={shift} matvec")
+    _, dv = shifted(v)
+    _, r = shifted(np.zeros_like(v))
+    qla_amd.call("QLA_F3_V_veq_M_times_V", r, du, dv, n)
+    exp = np.zeros_like(v)
+    orc.V_eqop_M_times_V(orc.EQ, exp, u, v)
+    assert_bitwise(host(r), exp, "misaligned matvec")
+    idx = np.random.default_rng(3).permutation(n)
+    _, r = shifted(v)
+    qla_amd.call("QLA_F3_V_vpeq_M_times_pV", r, du, qla_amd.pointer_array(dv, idx), n)
+    exp = v.copy()
+    orc.V_eqop_M_times_V(orc.PEQ, exp, u, v, b_idx=idx)
+    assert_bitwise(host(r), exp, "misaligned matvec pV")
+    qla_amd.check()
