@@ -214,6 +214,48 @@ void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n);
 void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n);
 
 /* ------------------------------------------------------------------------ */
+/* Ragged batch extension (no counterpart in the reference, where each call  */
+/* is its own site loop).  A plan holds a table of independent QLA_F3 v-form */
+/* calls; running it executes all of them in one persistent launch whose    */
+/* waves claim 64-site tiles of every job from a shared counter, so jobs of  */
+/* very different sizes share the chip.  Each job's result is bit-identical  */
+/* to its plain call.  The plan can be captured into a hipGraph.             */
+/* ------------------------------------------------------------------------ */
+enum {
+    QLA_MI355X_OP_V_VPEQ_M_TIMES_PV = 0,      /* QLA_F3_V_vpeq_M_times_pV         */
+    QLA_MI355X_OP_H_VEQ_SPPROJ_M_TIMES_PD = 1,  /* QLA_F3_H_veq_spproj_M_times_pD   */
+    QLA_MI355X_OP_D_VPEQ_SPRECON_M_TIMES_PH = 2 /* QLA_F3_D_vpeq_sprecon_M_times_pH */
+};
+typedef struct {
+    int op;        /* QLA_MI355X_OP_* */
+    int mu, sign;  /* spin ops only; as in the QLA call */
+    int n;         /* sites */
+    void *r;       /* destination field (device) */
+    void *a;       /* ColorMatrix field (device) */
+    void **b;      /* device pointer array of the gathered records */
+} QLA_MI355X_F3_job;
+typedef struct QLA_MI355X_batch QLA_MI355X_batch;
+/* Copies the job table to the device (the operands are referenced, not
+ * copied); returns NULL and records an error for an unknown op. */
+QLA_MI355X_batch *QLA_MI355X_F3_batch_create(const QLA_MI355X_F3_job *jobs, int njobs);
+/* Runs every job (one persistent launch on the thread's stream). */
+void QLA_MI355X_batch_run(QLA_MI355X_batch *p);
+/* Captures one run into a hipGraph (0 on success); launch it with
+ * QLA_MI355X_batch_launch_graph.  Re-capture after a shape change. */
+int QLA_MI355X_batch_capture(QLA_MI355X_batch *p);
+void QLA_MI355X_batch_launch_graph(QLA_MI355X_batch *p);
+/* Times every launch shape (waves per block, tiles per claim, blocks per
+ * CU) `reps` times on the plan's own operands, keeps the fastest, caches it
+ * for plans of the same size class and returns its time in ms.  The jobs
+ * run (1 + reps) x 48 times: accumulating destinations (vpeq) change. */
+float QLA_MI355X_batch_autotune(QLA_MI355X_batch *p, int reps);
+void QLA_MI355X_batch_get_shape(const QLA_MI355X_batch *p, int *waves_per_block, int *tiles_per_claim,
+                                int *blocks_per_cu);
+int QLA_MI355X_batch_set_shape(QLA_MI355X_batch *p, int waves_per_block, int tiles_per_claim,
+                               int blocks_per_cu);
+void QLA_MI355X_batch_destroy(QLA_MI355X_batch *p);
+
+/* ------------------------------------------------------------------------ */
 /* Library control (no counterpart in the reference, which has no device   */
 /* and no stream; these only select where and how the calls above run).    */
 /* ------------------------------------------------------------------------ */

@@ -10,8 +10,11 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
+#include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 #include "qla_launch.hpp"
 #include "../../include/qla_mi355x.h"
 
@@ -103,6 +106,53 @@ void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
     record(e, name);
+}
+
+// ------------------------------------------------------------- batch plans
+struct batch_shape {
+    int wpb, tpc, bpc;   // waves per block, tiles per claim, blocks per CU
+};
+
+// autotuned shapes, keyed by (log2 of the total tile count, job count bucket)
+std::mutex g_tune_mutex;
+std::map<std::pair<int, int>, batch_shape> g_tune_cache;
+
+int ilog2(long long v)
+{
+    int k = 0;
+    while (v > 1) { v >>= 1; k++; }
+    return k;
+}
+
+} // namespace
+
+struct QLA_MI355X_batch {
+    int device = 0;
+    int njobs = 0;
+    int total_tiles = 0;
+    int cus = 256;
+    qla_batch_job *djobs = nullptr;     // device job table
+    unsigned int *counter = nullptr;    // device claim counter
+    batch_shape shape = { 2, 2, 8 };
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+namespace {
+
+hipError_t batch_launch(QLA_MI355X_batch *p, hipStream_t st)
+{
+    const int blocks = std::max(1, p->cus * p->shape.bpc);
+    return qla_launch_batch(p->djobs, p->njobs, p->total_tiles, p->shape.wpb, p->shape.tpc, blocks, p->counter, st);
+}
+
+void batch_drop_graph(QLA_MI355X_batch *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
 }
 
 } // namespace
@@ -298,4 +348,163 @@ void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n)
 {
     reduce_dh(a, b, n, r, 2 * sizeof(double), "QLA_MI355X_DF3_c_veq_Dh_dot_Dh");
 }
+// ----------------------------------------------------------------- batches
+QLA_MI355X_batch *QLA_MI355X_F3_batch_create(const QLA_MI355X_F3_job *jobs, int njobs)
+{
+    if (njobs <= 0 || !jobs) { record(hipErrorInvalidValue, "QLA_MI355X_F3_batch_create"); return nullptr; }
+    std::vector<qla_batch_job> hj((size_t)njobs);
+    long long tiles = 0;
+    for (int k = 0; k < njobs; k++) {
+        const QLA_MI355X_F3_job &j = jobs[k];
+        if (j.op != QLA_MI355X_OP_V_VPEQ_M_TIMES_PV && j.op != QLA_MI355X_OP_H_VEQ_SPPROJ_M_TIMES_PD &&
+            j.op != QLA_MI355X_OP_D_VPEQ_SPRECON_M_TIMES_PH) {
+            record(hipErrorInvalidValue, "QLA_MI355X_F3_batch_create: unknown op");
+            return nullptr;
+        }
+        qla_batch_job &d = hj[(size_t)k];
+        d.r = j.r;
+        d.a = j.a;
+        d.b = (const void *const *)j.b;
+        d.op = j.op;
+        d.mu = j.mu;
+        d.sign = j.sign == 1 ? 1 : -1;
+        // no-op jobs (n <= 0, or mu outside 0..4 for the spin ops) own no tiles
+        const bool noop = j.n <= 0 || (j.op != QLA_MI355X_OP_V_VPEQ_M_TIMES_PV && (j.mu < 0 || j.mu > 4));
+        d.n = noop ? 0 : j.n;
+        d.tile0 = (int)tiles;
+        d.al = (((uintptr_t)j.a % 16 == 0) ? 1 : 0) | (((uintptr_t)j.r % 16 == 0) ? 4 : 0);
+        tiles += ((long long)d.n + 63) / 64;
+    }
+    if (tiles > 0x7fffffffLL) { record(hipErrorInvalidValue, "QLA_MI355X_F3_batch_create: too many sites"); return nullptr; }
+    QLA_MI355X_batch *p = new QLA_MI355X_batch;
+    p->njobs = njobs;
+    p->total_tiles = (int)tiles;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->djobs, sizeof(qla_batch_job) * (size_t)njobs);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->counter, 256);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->djobs, hj.data(), sizeof(qla_batch_job) * (size_t)njobs, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        record(e, "QLA_MI355X_F3_batch_create");
+        QLA_MI355X_batch_destroy(p);
+        return nullptr;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        auto it = g_tune_cache.find({ ilog2(p->total_tiles), ilog2(njobs) });
+        if (it != g_tune_cache.end()) p->shape = it->second;
+    }
+    return p;
+}
+
+void QLA_MI355X_batch_run(QLA_MI355X_batch *p)
+{
+    if (!p) return;
+    finish(batch_launch(p, ts.stream), "QLA_MI355X_batch_run");
+}
+
+int QLA_MI355X_batch_capture(QLA_MI355X_batch *p)
+{
+    if (!p) return (int)hipErrorInvalidValue;
+    batch_drop_graph(p);
+    hipStream_t st = ts.stream;
+    bool own = false;
+    if (!st) {  // the legacy default stream cannot be captured
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) { record(e, "QLA_MI355X_batch_capture"); return (int)e; }
+        own = true;
+    }
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t le = batch_launch(p, st);
+        e = hipStreamEndCapture(st, &p->graph);
+        if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(st);
+    if (e != hipSuccess) {
+        batch_drop_graph(p);
+        record(e, "QLA_MI355X_batch_capture");
+    }
+    return (int)e;
+}
+
+void QLA_MI355X_batch_launch_graph(QLA_MI355X_batch *p)
+{
+    if (!p) return;
+    if (!p->exec) { record(hipErrorInvalidValue, "QLA_MI355X_batch_launch_graph: not captured"); return; }
+    finish(hipGraphLaunch(p->exec, ts.stream), "QLA_MI355X_batch_launch_graph");
+}
+
+float QLA_MI355X_batch_autotune(QLA_MI355X_batch *p, int reps)
+{
+    if (!p || p->total_tiles == 0) return 0.0f;
+    if (reps < 1) reps = 1;
+    static const int wpbs[] = { 1, 2, 4 };
+    static const int tpcs[] = { 1, 2, 4, 8 };
+    static const int bpcs[] = { 2, 4, 8, 16 };
+    const int lds = qla_batch_lds_bytes_per_wave();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float best_ms = 1e30f;
+    batch_shape best = p->shape;
+    for (int wpb : wpbs) {
+        if (wpb * lds > 64 * 1024) continue;
+        for (int tpc : tpcs) {
+            for (int bpc : bpcs) {
+                if (e != hipSuccess) break;
+                p->shape = { wpb, tpc, bpc };
+                e = batch_launch(p, ts.stream);                       // warm-up
+                if (e == hipSuccess) e = hipEventRecord(e0, ts.stream);
+                for (int r = 0; r < reps && e == hipSuccess; r++) e = batch_launch(p, ts.stream);
+                if (e == hipSuccess) e = hipEventRecord(e1, ts.stream);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e == hipSuccess && ms / reps < best_ms) { best_ms = ms / reps; best = p->shape; }
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    p->shape = best;
+    record(e, "QLA_MI355X_batch_autotune");
+    batch_drop_graph(p);   // a captured graph holds the old shape
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tune_cache[{ ilog2(p->total_tiles), ilog2(p->njobs) }] = best;
+    }
+    return best_ms;
+}
+
+void QLA_MI355X_batch_get_shape(const QLA_MI355X_batch *p, int *waves_per_block, int *tiles_per_claim,
+                                int *blocks_per_cu)
+{
+    if (!p) return;
+    if (waves_per_block) *waves_per_block = p->shape.wpb;
+    if (tiles_per_claim) *tiles_per_claim = p->shape.tpc;
+    if (blocks_per_cu) *blocks_per_cu = p->shape.bpc;
+}
+
+int QLA_MI355X_batch_set_shape(QLA_MI355X_batch *p, int waves_per_block, int tiles_per_claim, int blocks_per_cu)
+{
+    if (!p || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) || tiles_per_claim < 1 ||
+        blocks_per_cu < 1)
+        return (int)hipErrorInvalidValue;
+    p->shape = { waves_per_block, tiles_per_claim, blocks_per_cu };
+    batch_drop_graph(p);
+    return 0;
+}
+
+void QLA_MI355X_batch_destroy(QLA_MI355X_batch *p)
+{
+    if (!p) return;
+    batch_drop_graph(p);
+    if (p->djobs) (void)hipFree(p->djobs);
+    if (p->counter) (void)hipFree(p->counter);
+    delete p;
+}
+
 } // extern "C"

@@ -41,3 +41,18 @@ hipError_t qla_launch_convert(int kind, int fmt, void *r, const void *a, int n, 
 // norm2 (b == NULL) or dot of decoded Dh fields; doubles to device `out`
 hipError_t qla_launch_reduce_dh(const void *a, const void *b, int n, double *out, double *scratch,
                                 hipStream_t st);
+
+// Ragged batch of F3 v-form jobs in one persistent launch (qla_batch.hip).
+enum { QLA_BOP_MATVEC_PV = 0, QLA_BOP_SPPROJ = 1, QLA_BOP_SPRECON = 2 };
+struct qla_batch_job {
+    void *r;                 // destination field (V, H or D records)
+    const void *a;           // ColorMatrix field
+    const void *const *b;    // pointer array (V, D or H records)
+    int n, op, mu, sign;     // as in the QLA call
+    int tile0;               // first global tile of this job
+    int al;                  // alignment flags of a and r (bit 0: a, bit 2: r, 16-byte)
+};
+int qla_batch_lds_bytes_per_wave(void);
+// zero `counter`, then one persistent launch of `blocks` x (wpb waves)
+hipError_t qla_launch_batch(const qla_batch_job *jobs, int njobs, int total_tiles, int wpb, int tpc, int blocks,
+                            unsigned int *counter, hipStream_t st);

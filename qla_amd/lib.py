@@ -64,6 +64,15 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh": (None, [vp, vp, i32, vp, i32, i32, i32]),
     "QLA_MI355X_DF3_r_veq_norm2_Dh": (None, [vp, vp, i32]),
     "QLA_MI355X_DF3_c_veq_Dh_dot_Dh": (None, [vp, vp, vp, i32]),
+    # ragged batches
+    "QLA_MI355X_F3_batch_create": (vp, [vp, i32]),
+    "QLA_MI355X_batch_run": (None, [vp]),
+    "QLA_MI355X_batch_capture": (i32, [vp]),
+    "QLA_MI355X_batch_launch_graph": (None, [vp]),
+    "QLA_MI355X_batch_autotune": (C.c_float, [vp, i32]),
+    "QLA_MI355X_batch_get_shape": (None, [vp, vp, vp, vp]),
+    "QLA_MI355X_batch_set_shape": (i32, [vp, i32, i32, i32]),
+    "QLA_MI355X_batch_destroy": (None, [vp]),
     # control
     "QLA_MI355X_set_stream": (None, [vp]),
     "QLA_MI355X_get_stream": (vp, []),
@@ -75,6 +84,12 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
 
+class F3Job(C.Structure):
+    """QLA_MI355X_F3_job (include/qla_mi355x.h)."""
+    _fields_ = [("op", C.c_int), ("mu", C.c_int), ("sign", C.c_int), ("n", C.c_int),
+                ("r", C.c_void_p), ("a", C.c_void_p), ("b", C.c_void_p)]
+
+
 _lib: C.CDLL | None = None
 
 
@@ -82,7 +97,7 @@ def header_symbols() -> list[str]:
     """Names of every function declared in include/qla_mi355x.h."""
     text = HEADER.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    decl = r"^\s*(?:const\s+)?(?:void|int|char)\s*\**\s*(QLA_\w+)\s*\("
+    decl = r"^\s*(?:const\s+)?(?:void|int|char|float|double|QLA_MI355X_batch)\s*\**\s*(QLA_\w+)\s*\("
     return sorted(set(re.findall(decl, text, flags=re.M)))
 
 
