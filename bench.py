@@ -234,8 +234,88 @@ def variants(lib, stream, n, combos, nbr, sets, u64, d64, args, plain_sites_per_
         out[f"spproj_{name}_links"] = r
         out[f"sprecon_{name}_links"] = rate(_timed(stream, csprecon, steps, 2), 164, steps)
         del cs
+    out["batch_c3"] = batch_config3(lib, stream, args)
+    out["matfn_c4"] = matfn_config4(lib, stream, args)
+    out["c0_16x4"] = config0(lib, stream, args)
     qla_amd.check()
     return out
+
+
+def batch_config3(lib, stream, args):
+    """64 checkerboard sub-lattice jobs (L in {8,...,32}, seed 7) as one
+    autotuned persistent launch, as its captured hipGraph, and as 64
+    sequential plain launches on the same stream."""
+    import torch
+    from qla_amd import batch as qb
+    desc = qb.draw_jobs(64, seed=7)
+    jobs, keep = qb.checkerboard_jobs(desc)
+    sites = sum(kp["n"] for kp in keep)
+    alg = sum(kp["n"] * qb.BYTES[kp["op"]] for kp in keep)
+    b = qb.Batch(jobs)
+    b.autotune(reps=2)
+    reps = max(5, args.steps // 2)
+    ms_batch = _timed(stream, lambda it: b.run(), reps, 2) / reps
+    b.capture()
+    ms_graph = _timed(stream, lambda it: b.launch_graph(), reps, 2) / reps
+    ms_seq = _timed(stream, lambda it: qb.run_sequential(keep), reps, 2) / reps
+    shape = b.shape()
+    b.close()
+    del keep
+    torch.cuda.synchronize()
+    return {"jobs": 64, "sites": sites, "alg_bytes": alg, "shape_wpb_tpc_bpc": list(shape),
+            "batch_ms": round(ms_batch, 4), "graph_ms": round(ms_graph, 4), "sequential_ms": round(ms_seq, 4),
+            "batch_GBs": round(alg / (ms_batch * 1e-3) / 1e9, 1), "graph_GBs": round(alg / (ms_graph * 1e-3) / 1e9, 1),
+            "sequential_GBs": round(alg / (ms_seq * 1e-3) / 1e9, 1),
+            "speedup_graph_vs_sequential": round(ms_seq / ms_graph, 3)}
+
+
+def matfn_config4(lib, stream, args):
+    """M_veq_{exp,sqrt,invsqrt}_M on 2,097,152 Nc=3 matrices (F3) and on
+    262,144 Nc=4 / Nc=8 matrices (FN): matrices per second."""
+    import torch
+    from qla_amd import lattice
+    out = {}
+    n3 = 2097152
+    inputs = {"exp": lattice.random_antihermitian(n3, 3, 41), "sqrt": lattice.random_hpd(n3, 3, 42)}
+    inputs["invsqrt"] = inputs["sqrt"]
+    for fn in ("exp", "sqrt", "invsqrt"):
+        a = torch.from_numpy(inputs[fn].astype(np.float32)).cuda()
+        r = torch.empty_like(a)
+        f = getattr(lib, f"QLA_F3_M_veq_{fn}_M")
+        ms = _timed(stream, lambda it: f(r.data_ptr(), a.data_ptr(), n3), 3, 1) / 3
+        out[f"F3_{fn}"] = {"n": n3, "ms": round(ms, 3), "matrices_per_s": round(n3 / (ms * 1e-3), 1),
+                           "GBs": round(144 * n3 / (ms * 1e-3) / 1e9, 1)}
+    del inputs
+    for nc in (4, 8):
+        nn = 262144
+        ins = {"exp": lattice.random_antihermitian(nn, nc, 43), "sqrt": lattice.random_hpd(nn, nc, 44)}
+        ins["invsqrt"] = ins["sqrt"]
+        for fn in ("exp", "sqrt", "invsqrt"):
+            a = torch.from_numpy(ins[fn].astype(np.float32)).cuda()
+            r = torch.empty_like(a)
+            f = getattr(lib, f"QLA_FN_M_veq_{fn}_M")
+            ms = _timed(stream, lambda it: f(nc, r.data_ptr(), a.data_ptr(), nn), 2, 1) / 2
+            out[f"FN{nc}_{fn}"] = {"n": nn, "ms": round(ms, 3), "matrices_per_s": round(nn / (ms * 1e-3), 1)}
+    torch.cuda.synchronize()
+    return out
+
+
+def config0(lib, stream, args):
+    """V_veq_M_times_V and r_veq_norm2_V on 65,536 sites (16^4), FP32; this
+    size is cache-resident and launch-latency bound (reported as sites/s)."""
+    import torch
+    from qla_amd import lattice
+    n = 65536
+    u = torch.from_numpy(lattice.random_su3(n, 1).astype(np.float32)).cuda()
+    v = torch.from_numpy(lattice.random_field("V", n, 2).astype(np.float32)).cuda()
+    r = torch.empty_like(v)
+    res = torch.empty(2, dtype=torch.float32, device="cuda")
+    reps = 50
+    ms_mv = _timed(stream, lambda it: lib.QLA_F3_V_veq_M_times_V(r.data_ptr(), u.data_ptr(), v.data_ptr(), n),
+                   reps, 5) / reps
+    ms_n2 = _timed(stream, lambda it: lib.QLA_F3_r_veq_norm2_V(res.data_ptr(), v.data_ptr(), n), reps, 5) / reps
+    return {"V_veq_M_times_V_us": round(ms_mv * 1e3, 2), "V_veq_M_times_V_sites_per_s": round(n / (ms_mv * 1e-3), 1),
+            "r_veq_norm2_V_us": round(ms_n2 * 1e3, 2), "r_veq_norm2_V_sites_per_s": round(n / (ms_n2 * 1e-3), 1)}
 
 
 def measured_traffic():

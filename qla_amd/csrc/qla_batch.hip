@@ -26,74 +26,129 @@ template <int MU, int SIGN> using bop_sprecon = sprecon_op<float, PEQ, false, MU
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int BATCH_BYTES = cmax(bop_matvec::BYTES, cmax(bop_spproj<0, 1>::BYTES, bop_sprecon<0, 1>::BYTES));
 
+// stage / finish of the job's op for one tile (the switch is wave-uniform)
 template <class OP>
-__device__ __forceinline__ void run_op(char *lds, const batch_job &j, uint64_t pb, int base, int ns, int lane)
+__device__ __forceinline__ void op_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
+                                        int lane)
 {
-    OP::stage(lds, (cpx<float> *)j.r, (const cpx<float> *)j.a, (const void *)j.b, pb, base, ns, lane, base + lane,
-              lane < ns, j.al);
-    OP::finish(lds, (cpx<float> *)j.r, base, ns, lane, base + lane, lane < ns, j.al);
+    if (!fin)
+        OP::stage(lds, (cpx<float> *)j.r, (const cpx<float> *)j.a, (const void *)j.b, pb, base, ns, lane, base + lane,
+                  lane < ns, j.al);
+    else
+        OP::finish(lds, (cpx<float> *)j.r, base, ns, lane, base + lane, lane < ns, j.al);
 }
 
 template <template <int, int> class OPT>
-__device__ __forceinline__ void run_musign(char *lds, const batch_job &j, uint64_t pb, int base, int ns, int lane)
+__device__ __forceinline__ void musign_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
+                                            int lane)
 {
     if (j.sign == 1) {
         switch (j.mu) {
-        case 0: run_op<OPT<0, 1>>(lds, j, pb, base, ns, lane); break;
-        case 1: run_op<OPT<1, 1>>(lds, j, pb, base, ns, lane); break;
-        case 2: run_op<OPT<2, 1>>(lds, j, pb, base, ns, lane); break;
-        case 3: run_op<OPT<3, 1>>(lds, j, pb, base, ns, lane); break;
-        case 4: run_op<OPT<4, 1>>(lds, j, pb, base, ns, lane); break;
+        case 0: op_step<OPT<0, 1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 1: op_step<OPT<1, 1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 2: op_step<OPT<2, 1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 3: op_step<OPT<3, 1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 4: op_step<OPT<4, 1>>(fin, lds, j, pb, base, ns, lane); break;
         }
     } else {
         switch (j.mu) {
-        case 0: run_op<OPT<0, -1>>(lds, j, pb, base, ns, lane); break;
-        case 1: run_op<OPT<1, -1>>(lds, j, pb, base, ns, lane); break;
-        case 2: run_op<OPT<2, -1>>(lds, j, pb, base, ns, lane); break;
-        case 3: run_op<OPT<3, -1>>(lds, j, pb, base, ns, lane); break;
-        case 4: run_op<OPT<4, -1>>(lds, j, pb, base, ns, lane); break;
+        case 0: op_step<OPT<0, -1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 1: op_step<OPT<1, -1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 2: op_step<OPT<2, -1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 3: op_step<OPT<3, -1>>(fin, lds, j, pb, base, ns, lane); break;
+        case 4: op_step<OPT<4, -1>>(fin, lds, j, pb, base, ns, lane); break;
         }
+    }
+}
+
+__device__ __forceinline__ void job_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
+                                         int lane)
+{
+    switch (j.op) {
+    case QLA_BOP_MATVEC_PV: op_step<bop_matvec>(fin, lds, j, pb, base, ns, lane); break;
+    case QLA_BOP_SPPROJ: musign_step<bop_spproj>(fin, lds, j, pb, base, ns, lane); break;
+    case QLA_BOP_SPRECON: musign_step<bop_sprecon>(fin, lds, j, pb, base, ns, lane); break;
     }
 }
 
 // owning job of `tile`: the last j with tile0_j <= tile (jobs with no tiles
 // have tile0_j == tile0_{j+1} and are never selected)
-__device__ __forceinline__ int find_job(const batch_job *jobs, int njobs, int tile)
+// wave-uniform value in a scalar register (the argument is uniform already;
+// this lets the compiler keep job fields and tile indices in SGPRs)
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// t0[k * stride] is job k's first tile
+__device__ __forceinline__ int find_job(const int *t0, int stride, int njobs, int tile)
 {
     int lo = 0, hi = njobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (jobs[mid].tile0 <= tile) lo = mid;
+        if (uni(t0[mid * stride]) <= tile) lo = mid;
         else hi = mid - 1;
     }
-    return lo;
+    return uni(lo);
 }
 
+struct tile_ref {
+    batch_job j;
+    int base, ns;
+    uint64_t pb;
+};
+
+__device__ __forceinline__ void load_tile_ref(tile_ref &t, const batch_job *jobs, const int *t0, int stride, int njobs,
+                                              int tile, int lane)
+{
+    t.j = jobs[find_job(t0, stride, njobs, tile)];
+    t.base = (tile - t.j.tile0) * WAVE;
+    t.ns = min(WAVE, t.j.n - t.base);
+    t.pb = lane < t.ns ? (uint64_t)t.j.b[t.base + lane] : 0;
+}
+
+// Waves claim `tpc` consecutive tiles at a time.  Each wave holds one claim
+// ahead (issued when it starts on the current one), and the next tile's job
+// entry and pointer-array entries are loaded while the current tile's
+// records are in flight.  A wave stops when a claim starts past the last
+// tile; claims are handed out in increasing order, so every later claim is
+// past it too and no work is dropped.
 template <int WPB>
 __global__ void __launch_bounds__(WPB * WAVE)
 k_batch(const batch_job *__restrict__ jobs, int njobs, int total_tiles, int tpc, unsigned int *__restrict__ counter)
 {
     __shared__ __attribute__((aligned(16))) char smem[WPB * BATCH_BYTES];
+    // job lookup: tile0 fields of the job table, read with wave-uniform
+    // indices (scalar loads through the scalar cache)
+    const int *t0 = &jobs[0].tile0;
+    const int t0_stride = (int)(sizeof(batch_job) / sizeof(int));
     char *lds = smem + (threadIdx.x / WAVE) * BATCH_BYTES;
     const int lane = lane_id();
+
+    unsigned int c0 = 0, pend = 0;
+    if (lane == 0) {
+        c0 = atomicAdd(counter, 1u);
+        pend = atomicAdd(counter, 1u);
+    }
+    const long long first = (long long)uni((int)c0) * tpc;
+    if (first >= total_tiles) return;
+    int tile = (int)first;
+    int last = (int)min((long long)total_tiles, first + tpc);
+    tile_ref cur, nxt;
+    load_tile_ref(cur, jobs, t0, t0_stride, njobs, tile, lane);
     for (;;) {
-        unsigned int claim = 0;
-        if (lane == 0) claim = atomicAdd(counter, 1u);
-        claim = __shfl(claim, 0, WAVE);
-        const long long first = (long long)claim * tpc;
-        if (first >= total_tiles) break;                 // every wave ends here
-        const int last = (int)min((long long)total_tiles, first + tpc);
-        for (int tile = (int)first; tile < last; tile++) {
-            const batch_job j = jobs[find_job(jobs, njobs, tile)];
-            const int base = (tile - j.tile0) * WAVE;
-            const int ns = min(WAVE, j.n - base);
-            const uint64_t pb = lane < ns ? (uint64_t)j.b[base + lane] : 0;
-            switch (j.op) {
-            case QLA_BOP_MATVEC_PV: run_op<bop_matvec>(lds, j, pb, base, ns, lane); break;
-            case QLA_BOP_SPPROJ: run_musign<bop_spproj>(lds, j, pb, base, ns, lane); break;
-            case QLA_BOP_SPRECON: run_musign<bop_sprecon>(lds, j, pb, base, ns, lane); break;
-            }
+        job_step(false, lds, cur.j, cur.pb, cur.base, cur.ns, lane);
+        int ntile = tile + 1, nlast = last;
+        if (ntile >= last) {                       // move on to the claim held ahead
+            const long long nf = (long long)uni((int)pend) * tpc;
+            if (lane == 0) pend = atomicAdd(counter, 1u);
+            ntile = nf < total_tiles ? (int)nf : total_tiles;
+            nlast = (int)min((long long)total_tiles, nf + tpc);
         }
+        const bool more = ntile < total_tiles;
+        if (more) load_tile_ref(nxt, jobs, t0, t0_stride, njobs, ntile, lane);
+        job_step(true, lds, cur.j, cur.pb, cur.base, cur.ns, lane);
+        if (!more) break;
+        cur = nxt;
+        tile = ntile;
+        last = nlast;
     }
 }
 

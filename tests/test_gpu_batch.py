@@ -134,3 +134,33 @@ def test_batch_noop_jobs():
     for kp in keep:
         assert_bitwise(host(kp["dst"]), host(kp["dst0"]), "no-op job")
     b.close()
+
+
+def test_batch_many_jobs_uncached_lookup(orc):
+    """A long job table (2100 jobs, binary-searched per tile): 2100 jobs that each
+    take a slice of one spproj call must reproduce the single plain call."""
+    import torch
+    from qla_amd import lattice
+    from gpu_util import dev
+    dims = (8, 8, 8, 6)
+    n = lattice.volume(dims)
+    fw, _ = lattice.neighbours(dims)
+    u = lattice.random_su3(n, 21).astype(np.float32)
+    d = lattice.random_field("D", n, 22).astype(np.float32)
+    du, dd = dev(u), dev(d)
+    ptr = qla_amd.pointer_array(dd, fw[2])
+    r_plain = torch.zeros((n, 3, 2, 2), dtype=torch.float32, device="cuda")
+    qla_amd.call("QLA_F3_H_veq_spproj_M_times_pD", r_plain, du, ptr, 2, 1, n)
+    r = torch.zeros_like(r_plain)
+    bounds = np.linspace(0, n, 2101).astype(int)
+    jobs = []
+    for k in range(2100):
+        s0, s1 = int(bounds[k]), int(bounds[k + 1])
+        jobs.append(dict(op=qb.OP_H_VEQ_SPPROJ_M_TIMES_PD, mu=2, sign=1, n=s1 - s0,
+                         r=r.data_ptr() + 48 * s0, a=du.data_ptr() + 72 * s0, b=ptr.data_ptr() + 8 * s0))
+    b = qb.Batch(jobs)
+    say("batch 2100 jobs")
+    b.run()
+    qla_amd.check()
+    assert_bitwise(host(r), host(r_plain), "2100-job batch")
+    b.close()
