@@ -234,10 +234,108 @@ def variants(lib, stream, n, combos, nbr, sets, u64, d64, args, plain_sites_per_
         out[f"spproj_{name}_links"] = r
         out[f"sprecon_{name}_links"] = rate(_timed(stream, csprecon, steps, 2), 164, steps)
         del cs
+    out["xform_c1"] = xforms_config1(lib, stream, n, sets, args)
+    out["c2_48x4"] = config2(lib, stream, args)
     out["batch_c3"] = batch_config3(lib, stream, args)
     out["matfn_c4"] = matfn_config4(lib, stream, args)
     out["c0_16x4"] = config0(lib, stream, args)
     qla_amd.check()
+    return out
+
+
+def xforms_config1(lib, stream, n, sets, args):
+    """x-forms on the even checkerboard of 32^4 (524,288 sites): gang index
+    array of even sites, backward/forward neighbour pointers (228 / 324
+    algorithmic B/site: the index adds 4 B)."""
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    dims = DIMS
+    even = torch.from_numpy(lattice.parity_sites(dims, 0)).cuda()
+    ne = even.numel()
+    fw, bw = lattice.neighbours(dims)
+    combos = [(mu, s) for mu in range(4) for s in (1, -1)]
+    hsrc = torch.from_numpy(np.random.default_rng(8).standard_normal((n, 3, 2, 2)).astype(np.float32)).cuda()
+    dd = torch.zeros((n, 3, 4, 2), dtype=torch.float32, device="cuda")
+    out = {}
+    for name, bytes_ in (("H_xeq_spproj_M_times_pD", 228), ("D_xpeq_sprecon_M_times_pH", 324)):
+        ptrs = {}
+        for k in range(NSETS):
+            u, d, h, _ = sets[k]
+            src = d if "spproj" in name else hsrc
+            ptrs[k] = {(mu, s): qla_amd.pointer_array(src, (fw if s == 1 else bw)[mu]) for mu, s in combos}
+        f = getattr(lib, "QLA_F3_" + name)
+
+        def step(it, f=f, ptrs=ptrs, name=name):
+            for j, (mu, s) in enumerate(combos):
+                k = (it * len(combos) + j) % NSETS
+                u, d, h, _ = sets[k]
+                dst = h if "spproj" in name else dd
+                f(dst.data_ptr(), u.data_ptr(), ptrs[k][(mu, s)].data_ptr(), mu, s, even.data_ptr(), ne)
+        steps = max(5, args.steps // 4)
+        ms = _timed(stream, step, steps, 2)
+        c = steps * len(combos)
+        out[name] = {"sites": ne, "ms_per_call": round(ms / c, 5),
+                     "GBs": round(bytes_ * ne * c / (ms * 1e-3) / 1e9, 1), "sites_per_s": round(ne * c / (ms * 1e-3), 1)}
+    return out
+
+
+def config2(lib, stream, args):
+    """48^4 (5,308,416 sites): plain FP32 and compressed (binary16 links,
+    binary16 fermions) spproj for mu = 0..3, sign = +-1, plus r_veq_norm2_D and
+    c_veq_D_dot_D (F3 plain, and the decoded-Dh forms with an FP64 combine)."""
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    dims = (48, 48, 48, 48)
+    n = lattice.volume(dims)
+    fw, bw = lattice.neighbours(dims)
+    combos = [(mu, s) for mu in range(4) for s in (1, -1)]
+    u = torch.from_numpy(lattice.random_su3(n, 51).astype(np.float32)).cuda()
+    d = torch.from_numpy(lattice.random_field("D", n, 52, dtype=np.float32)).cuda()
+    d2 = torch.from_numpy(lattice.random_field("D", n, 53, dtype=np.float32)).cuda()
+    h = torch.empty((n, 3, 2, 2), dtype=torch.float32, device="cuda")
+    ptr = {c: qla_amd.pointer_array(d, (fw if c[1] == 1 else bw)[c[0]]) for c in combos}
+    steps = max(3, args.steps // 10)
+    f = lib.QLA_F3_H_veq_spproj_M_times_pD
+    ms = _timed(stream, lambda it: [f(h.data_ptr(), u.data_ptr(), ptr[c].data_ptr(), c[0], c[1], n) for c in combos],
+                steps, 1)
+    calls = steps * len(combos)
+    out = {"sites": n, "spproj_f3": {"ms_per_call": round(ms / calls, 4),
+                                     "GBs": round(224 * n * calls / (ms * 1e-3) / 1e9, 1),
+                                     "sites_per_s": round(n * calls / (ms * 1e-3), 1)}}
+    plain_sps = n * calls / (ms * 1e-3)
+    link = torch.empty((n, 12), dtype=torch.int16, device="cuda")
+    lib.QLA_MI355X_F3_M_pack12(link.data_ptr(), 1, u.data_ptr(), n)
+    drec = torch.empty((n, 52), dtype=torch.uint8, device="cuda")
+    lib.QLA_MI355X_F3_D_packh(drec.data_ptr(), d.data_ptr(), n)
+    drec2 = torch.empty((n, 52), dtype=torch.uint8, device="cuda")
+    lib.QLA_MI355X_F3_D_packh(drec2.data_ptr(), d2.data_ptr(), n)
+    hrec = torch.empty((n, 28), dtype=torch.uint8, device="cuda")
+    del ptr
+    dptr = {c: qla_amd.pointer_array(drec, (fw if c[1] == 1 else bw)[c[0]]) for c in combos}
+    g = lib.QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh
+    ms = _timed(stream, lambda it: [g(hrec.data_ptr(), link.data_ptr(), 1, dptr[c].data_ptr(), c[0], c[1], n)
+                                    for c in combos], steps, 1)
+    out["spproj_f16_links"] = {"ms_per_call": round(ms / calls, 4), "GBs": round(112 * n * calls / (ms * 1e-3) / 1e9, 1),
+                               "sites_per_s": round(n * calls / (ms * 1e-3), 1),
+                               "speedup_vs_plain_sites": round(n * calls / (ms * 1e-3) / plain_sps, 3)}
+    res = torch.empty(4, dtype=torch.float64, device="cuda")
+    resf = torch.empty(4, dtype=torch.float32, device="cuda")
+    reps = 10
+    red = {
+        "r_veq_norm2_D_f3": (lambda it: lib.QLA_F3_r_veq_norm2_D(resf.data_ptr(), d.data_ptr(), n), 96),
+        "c_veq_D_dot_D_f3": (lambda it: lib.QLA_F3_c_veq_D_dot_D(resf.data_ptr(), d.data_ptr(), d2.data_ptr(), n), 192),
+        "r_veq_norm2_D_df3": (lambda it: lib.QLA_DF3_r_veq_norm2_D(res.data_ptr(), d.data_ptr(), n), 96),
+        "r_veq_norm2_Dh": (lambda it: lib.QLA_MI355X_DF3_r_veq_norm2_Dh(res.data_ptr(), drec.data_ptr(), n), 52),
+        "c_veq_Dh_dot_Dh": (lambda it: lib.QLA_MI355X_DF3_c_veq_Dh_dot_Dh(res.data_ptr(), drec.data_ptr(),
+                                                                         drec2.data_ptr(), n), 104),
+    }
+    for name, (fn, bps) in red.items():
+        ms = _timed(stream, fn, reps, 2) / reps
+        out[name] = {"ms": round(ms, 4), "GBs": round(bps * n / (ms * 1e-3) / 1e9, 1),
+                     "sites_per_s": round(n / (ms * 1e-3), 1)}
+    torch.cuda.synchronize()
     return out
 
 

@@ -95,6 +95,15 @@ k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long 
     }
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// streaming 16-byte load with the non-temporal hint (each element is read once)
+__device__ __forceinline__ float4 ld_stream(const float4 *p)
+{
+    const v4f v = __builtin_nontemporal_load((const v4f *)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // stage 1, two float complexes per 16-byte load (a, b 16-byte aligned, m even)
 template <bool DOT>
 __global__ void __launch_bounds__(RBLOCK)
@@ -109,8 +118,8 @@ k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long lo
         float4 x[4], y[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            x[u] = a[i + u * stride];
-            if (DOT) y[u] = b[i + u * stride];
+            x[u] = ld_stream(a + i + u * stride);
+            if (DOT) y[u] = ld_stream(b + i + u * stride);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
