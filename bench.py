@@ -441,13 +441,14 @@ def cpu_baseline(seconds: float):
     h = np.zeros((n, 3, 2, 2), dtype=np.float32)
     combos = [(mu, s) for mu in range(4) for s in (1, -1)]
     ptrs = {(mu, s): oracle.pointer_array(d, (fw if s == 1 else bw)[mu]) for mu, s in combos}
+    pa = oracle.pointer_array(u, np.arange(n))      # the link field as its own record list
     oracle.H_eqop_spproj_M_times_pD(oracle.EQ, h, u, d, fw[0], 0, 1)   # warm-up
     calls = 0
     t0 = time.perf_counter()
     while True:
         for mu, s in combos:
-            lib.orc_F3_H_eqop_spproj_M_times_pD(0, 0, h.ctypes.data, u.ctypes.data, ptrs[(mu, s)].ctypes.data,
-                                                mu, s, None, n)
+            lib.orc_F3_site(oracle.SPPROJ, oracle.EQ, 0, h.ctypes.data, pa.ctypes.data, ptrs[(mu, s)].ctypes.data,
+                            mu, s, None, n)
             calls += 1
         if time.perf_counter() - t0 > seconds:
             break

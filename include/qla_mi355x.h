@@ -55,26 +55,18 @@ typedef QLA_D_Complex QLA_D3_ColorMatrix[3][3];
 #endif
 
 /* ------------------------------------------------------------------------ */
-/* a1/a2: matrix times colour vector.                                        */
-/* Reference: generated from lib/perl/expressions_tensor.pl:544-630 via     */
-/* build_qla.pl:1587-1622; accumulation x=0 (eq) or x=r (peq), then         */
-/* x += a[ic][kc]*b[kc] for kc = 0,1,2 with QLA_c_peq_c_times_c             */
-/* (include/qla_complex.h:330-333).                                          */
+/* a1/a2, a5-a8 and the rest of the Wilson-hop family: every eqop (eq, peq, */
+/* meq, eqm), M or Ma, field or pointer operands, v- and x-forms, for       */
+/* V = M*V, H = M*spproj(D), D = sprecon(M*H), D = sprecon(M*spproj(D)) and */
+/* the multi-source V = sum_d M_d*V_d.  Reference: generated from           */
+/* lib/perl/expressions_tensor.pl:544-630 (matvec accumulation x=0 or x=r,  */
+/* then x +=/-= a[ic][kc]*b[kc], kc = 0,1,2, include/qla_complex.h:330-341),*/
+/* build_qla.pl:1162-1250 (spin forms), :1693-1725 (multi-source).  The SSE */
+/* variants lib/optimized/sse/src/QLA_F3_V_vpeq_M_times_pV.c,               */
+/* ..._D_vpeq_sprecon_M_times_pH.c, ..._D_vpeq_spproj_M_times_pD.c          */
+/* re-associate; this library keeps the generic C order for every symbol.   */
 /* ------------------------------------------------------------------------ */
-void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
-                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                            QLA_F3_ColorVector *QLA_MI355X_RESTRICT b, int n);
-void QLA_D3_V_veq_M_times_V(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
-                            QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                            QLA_D3_ColorVector *QLA_MI355X_RESTRICT b, int n);
-/* Reference (generic order): expressions_tensor.pl:581-583 (x = r start);
- * SSE variant lib/optimized/sse/src/QLA_F3_V_vpeq_M_times_pV.c:59-103. */
-void QLA_F3_V_vpeq_M_times_pV(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
-                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b, int n);
-void QLA_D3_V_vpeq_M_times_pV(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
-                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                              QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b, int n);
+#include "qla_mi355x_sites.h"
 
 /* ------------------------------------------------------------------------ */
 /* a3/a4: global reductions.  Reference: make_code_unary.pl:318-366         */
@@ -106,53 +98,6 @@ void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
 void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
-
-/* ------------------------------------------------------------------------ */
-/* a5/a6: spin projection of a gathered DiracFermion, then colour matrix    */
-/* times the projected HalfFermion.  Reference: build_qla.pl:1161-1190,     */
-/* make_code_binary.pl:189-222,259-271, tables expressions_gamma.pl:244-276.*/
-/* x-form: every operand at index[i] (gang indexing, indirection.pl:47-58). */
-/* ------------------------------------------------------------------------ */
-void QLA_F3_H_veq_spproj_M_times_pD(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
-                                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int n);
-void QLA_F3_H_xeq_spproj_M_times_pD(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
-                                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int *index, int n);
-void QLA_D3_H_veq_spproj_M_times_pD(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
-                                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int n);
-void QLA_D3_H_xeq_spproj_M_times_pD(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
-                                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int *index, int n);
-
-/* ------------------------------------------------------------------------ */
-/* a7/a8: colour matrix times a gathered HalfFermion, then spin             */
-/* reconstruction added into a DiracFermion.  Reference:                    */
-/* build_qla.pl:1191-1220, make_code_binary.pl:224-257, tables              */
-/* expressions_gamma.pl:311-337; SSE variant                                */
-/* lib/optimized/sse/src/QLA_F3_D_vpeq_sprecon_M_times_pH.c.                */
-/* ------------------------------------------------------------------------ */
-void QLA_F3_D_vpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
-                                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int n);
-void QLA_F3_D_xpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
-                                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int *index, int n);
-void QLA_D3_D_vpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
-                                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int n);
-void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
-                                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
-                                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int *index, int n);
 
 /* ------------------------------------------------------------------------ */
 /* a9-a12: batched matrix functions.  Reference: generated wrappers         */

@@ -41,9 +41,8 @@ def load():
     lib = C.CDLL(str(LIB_PATH))
     vp, i32 = C.c_void_p, C.c_int
     for pre in ("F3", "D3"):
-        getattr(lib, f"orc_{pre}_V_eqop_M_times_V").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
-        getattr(lib, f"orc_{pre}_H_eqop_spproj_M_times_pD").argtypes = [i32, i32, vp, vp, vp, i32, i32, vp, i32]
-        getattr(lib, f"orc_{pre}_D_eqop_sprecon_M_times_pH").argtypes = [i32, i32, vp, vp, vp, i32, i32, vp, i32]
+        getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, i32, i32, vp, i32]
+        getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, i32, vp, i32]
     for p in ("F", "D"):
         f = getattr(lib, f"orc_{p}_norm2")
         f.argtypes, f.restype = [vp, i32, i32], C.c_double
@@ -92,36 +91,75 @@ def threads() -> int:
     return load().orc_get_threads()
 
 
-def V_eqop_M_times_V(eqop, r, a, b=None, b_idx=None, index=None, adj=False, n=None):
-    """r eqop a * b (b contiguous) or a * (*pb) with pb = &b[b_idx[i]]."""
+MATVEC, SPPROJ, SPRECON, WILSON = 0, 1, 2, 3
+REC_REALS = {MATVEC: 6, SPPROJ: 12, SPRECON: 24, WILSON: 24}
+
+
+def _records(x: np.ndarray, idx) -> np.ndarray:
+    """Pointer array to records of x: identity for a field operand (idx None),
+    else record idx[i] (a pointer-array operand)."""
+    return pointer_array(x, np.arange(x.shape[0]) if idx is None else idx)
+
+
+def _index(index):
+    return None if index is None else np.ascontiguousarray(index, dtype=np.int32)
+
+
+def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b_idx=None, n=None):
+    """One QLA site call on host arrays (orc_*_site).
+
+    kind 0 V eqop A*B, 1 H eqop A*spproj(B), 2 D eqop sprecon(A*B),
+    3 D eqop sprecon(A*spproj(B)); A = a or a^dagger (adj).  a_idx / b_idx
+    None: the operand is a field (record i, or index[i] in the x-form); else
+    the operand is a pointer array whose entry s points at record a_idx[s].
+    n defaults to len(index) (x-form) or the number of destination records.
+    """
     lib = load()
-    n = a.shape[0] if n is None else n
-    pb = None if b_idx is None else pointer_array(b, b_idx)
-    ix = None if index is None else np.ascontiguousarray(index, dtype=np.int32)
-    getattr(lib, f"orc_{_prefix(a)}_V_eqop_M_times_V")(
-        eqop, int(adj), _ptr(_need(r)), _ptr(a), None if pb is not None else _ptr(b), _ptr(pb), _ptr(ix),
-        len(ix) if ix is not None else n)
+    pa, pb, ix = _records(a, a_idx), _records(b, b_idx), _index(index)
+    if n is None:
+        n = len(ix) if ix is not None else r.shape[0]
+    getattr(lib, f"orc_{_prefix(a)}_site")(kind, eqop, int(adj), _ptr(_need(r)), _ptr(pa), _ptr(pb),
+                                           mu, sign, _ptr(ix), n)
     return r
+
+
+def multi(eqop, r, a_list, b_list, index=None, adj=False, a_idx=None, b_idx=None, n=None):
+    """Multi-source r eqop sum_d A_d * B_d (orc_*_multi); a_idx / b_idx are
+    per-term index lists (pointer-array operands) or None (fields)."""
+    lib = load()
+    nd = len(a_list)
+    pas = [_records(a, None if a_idx is None else a_idx[d]) for d, a in enumerate(a_list)]
+    pbs = [_records(b, None if b_idx is None else b_idx[d]) for d, b in enumerate(b_list)]
+    pa = np.array([x.ctypes.data for x in pas], dtype=np.uint64)
+    pb = np.array([x.ctypes.data for x in pbs], dtype=np.uint64)
+    ix = _index(index)
+    if n is None:
+        n = len(ix) if ix is not None else r.shape[0]
+    getattr(lib, f"orc_{_prefix(r)}_multi")(eqop, int(adj), _ptr(_need(r)), _ptr(pa), _ptr(pb), nd,
+                                            _ptr(ix), n)
+    return r
+
+
+def V_eqop_M_times_V(eqop, r, a, b=None, b_idx=None, index=None, adj=False, n=None):
+    """r eqop a * b (b a field) or a * (*pb) with pb[s] = &b[b_idx[s]]."""
+    if n is None and index is None:
+        n = a.shape[0]
+    return site(MATVEC, eqop, r, a, b, index=index, adj=adj, b_idx=b_idx, n=n)
 
 
 def H_eqop_spproj_M_times_pD(eqop, r, a, d, d_idx, mu, sign, index=None, adj=False):
-    lib = load()
-    pb = pointer_array(d, d_idx)
-    ix = None if index is None else np.ascontiguousarray(index, dtype=np.int32)
-    n = len(ix) if ix is not None else len(d_idx)
-    getattr(lib, f"orc_{_prefix(a)}_H_eqop_spproj_M_times_pD")(
-        eqop, int(adj), _ptr(_need(r)), _ptr(a), _ptr(pb), mu, sign, _ptr(ix), n)
-    return r
+    n = None if index is not None else len(d_idx)
+    return site(SPPROJ, eqop, r, a, d, mu, sign, index=index, adj=adj, b_idx=d_idx, n=n)
 
 
 def D_eqop_sprecon_M_times_pH(eqop, r, a, h, h_idx, mu, sign, index=None, adj=False):
-    lib = load()
-    pb = pointer_array(h, h_idx)
-    ix = None if index is None else np.ascontiguousarray(index, dtype=np.int32)
-    n = len(ix) if ix is not None else len(h_idx)
-    getattr(lib, f"orc_{_prefix(a)}_D_eqop_sprecon_M_times_pH")(
-        eqop, int(adj), _ptr(_need(r)), _ptr(a), _ptr(pb), mu, sign, _ptr(ix), n)
-    return r
+    n = None if index is not None else len(h_idx)
+    return site(SPRECON, eqop, r, a, h, mu, sign, index=index, adj=adj, b_idx=h_idx, n=n)
+
+
+def D_eqop_spproj_M_times_pD(eqop, r, a, d, d_idx, mu, sign, index=None, adj=False):
+    n = None if index is not None else len(d_idx)
+    return site(WILSON, eqop, r, a, d, mu, sign, index=index, adj=adj, b_idx=d_idx, n=n)
 
 
 def norm2(a: np.ndarray) -> float:

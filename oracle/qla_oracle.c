@@ -29,12 +29,18 @@
 #undef REAL
 #undef P
 
-void orc_F3_V_eqop_M_times_V(int eqop, int adj, float *r, const float *a, const float *b,
-                             const float *const *pb, const int *index, int n)
-{ F_orc_3_V_eqop_M_times_V(eqop, adj, r, a, b, pb, index, n); }
-void orc_D3_V_eqop_M_times_V(int eqop, int adj, double *r, const double *a, const double *b,
-                             const double *const *pb, const int *index, int n)
-{ D_orc_3_V_eqop_M_times_V(eqop, adj, r, a, b, pb, index, n); }
+void orc_F3_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb,
+                 int mu, int sign, const int *index, int n)
+{ F_orc_3_site(kind, eqop, adj, r, pa, pb, mu, sign, index, n); }
+void orc_D3_site(int kind, int eqop, int adj, double *r, const double *const *pa, const double *const *pb,
+                 int mu, int sign, const int *index, int n)
+{ D_orc_3_site(kind, eqop, adj, r, pa, pb, mu, sign, index, n); }
+void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,
+                  int nd, const int *index, int n)
+{ F_orc_3_multi(eqop, adj, r, pa, pb, nd, index, n); }
+void orc_D3_multi(int eqop, int adj, double *r, const double *const *const *pa, const double *const *const *pb,
+                  int nd, const int *index, int n)
+{ D_orc_3_multi(eqop, adj, r, pa, pb, nd, index, n); }
 
 double orc_F_norm2(const float *a, int ncomp, int n) { return F_orc_norm2(a, ncomp, n); }
 double orc_D_norm2(const double *a, int ncomp, int n) { return D_orc_norm2(a, ncomp, n); }
@@ -48,23 +54,6 @@ void orc_F_dot_ld(long double *re, long double *im, const float *a, const float 
 { F_orc_dot_ld(re, im, a, b, ncomp, n); }
 void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n)
 { D_orc_dot_ld(re, im, a, b, ncomp, n); }
-
-void orc_F3_H_eqop_spproj_M_times_pD(int eqop, int adj, float *r, const float *a,
-                                     const float *const *b, int mu, int sign,
-                                     const int *index, int n)
-{ F_orc_3_H_eqop_spproj_M_times_pD(eqop, adj, r, a, b, mu, sign, index, n); }
-void orc_D3_H_eqop_spproj_M_times_pD(int eqop, int adj, double *r, const double *a,
-                                     const double *const *b, int mu, int sign,
-                                     const int *index, int n)
-{ D_orc_3_H_eqop_spproj_M_times_pD(eqop, adj, r, a, b, mu, sign, index, n); }
-void orc_F3_D_eqop_sprecon_M_times_pH(int eqop, int adj, float *r, const float *a,
-                                      const float *const *b, int mu, int sign,
-                                      const int *index, int n)
-{ F_orc_3_D_eqop_sprecon_M_times_pH(eqop, adj, r, a, b, mu, sign, index, n); }
-void orc_D3_D_eqop_sprecon_M_times_pH(int eqop, int adj, double *r, const double *a,
-                                      const double *const *b, int mu, int sign,
-                                      const int *index, int n)
-{ D_orc_3_D_eqop_sprecon_M_times_pH(eqop, adj, r, a, b, mu, sign, index, n); }
 
 void orc_D_spproj(double *h, const double *d, int mu, int sign)
 {

@@ -23,11 +23,25 @@ extern "C" {
 
 enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
 
-/* matrix times colour vector, generic order (expressions_tensor.pl:544-630) */
-void orc_F3_V_eqop_M_times_V(int eqop, int adj, float *r, const float *a, const float *b,
-                             const float *const *pb, const int *index, int n);
-void orc_D3_V_eqop_M_times_V(int eqop, int adj, double *r, const double *a, const double *b,
-                             const double *const *pb, const int *index, int n);
+/* Site operations (qla_oracle_impl.inc): the A and B records of site s are
+ * *pa[s] and *pb[s] (a field operand is passed as the pointer array of its
+ * own records); index NULL = v-form, else the gang index (x-form).
+ * kind 0: V eqop A*B (expressions_tensor.pl:544-630)
+ * kind 1: H eqop A*spproj(B) (make_code_binary.pl:189-222)
+ * kind 2: D eqop sprecon(A*B) (make_code_binary.pl:224-257)
+ * kind 3: D eqop sprecon(A*spproj(B)) (make_code_binary.pl:285-341)
+ * adj selects A = a^dagger (QLA_c_peq_ca_times_c).  mu outside 0..4 is a
+ * no-op for kinds 1-3. */
+void orc_F3_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb,
+                 int mu, int sign, const int *index, int n);
+void orc_D3_site(int kind, int eqop, int adj, double *r, const double *const *pa, const double *const *pb,
+                 int mu, int sign, const int *index, int n);
+/* multi-source r eqop sum_{d<nd} A_d*B_d, A_d = *pa[d][s], B_d = *pb[d][s]
+ * (build_qla.pl:1693-1725, expressions_tensor.pl:983-1002) */
+void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,
+                  int nd, const int *index, int n);
+void orc_D3_multi(int eqop, int adj, double *r, const double *const *const *pa, const double *const *const *pb,
+                  int nd, const int *index, int n);
 
 /* norm2 / dot (make_code_unary.pl:318-366, make_code_binary.pl:110-170).
  * ncomp = complex components per site (3 for V, 12 for D).  Results are the
@@ -43,21 +57,6 @@ long double orc_D_norm2_ld(const double *a, int ncomp, int n);
 void orc_F_dot_ld(long double *re, long double *im, const float *a, const float *b, int ncomp, int n);
 void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n);
 
-/* spin projection + matrix multiply (make_code_binary.pl:189-222), index may be
- * NULL (v-form) or the gang index (x-form). */
-void orc_F3_H_eqop_spproj_M_times_pD(int eqop, int adj, float *r, const float *a,
-                                     const float *const *b, int mu, int sign,
-                                     const int *index, int n);
-void orc_D3_H_eqop_spproj_M_times_pD(int eqop, int adj, double *r, const double *a,
-                                     const double *const *b, int mu, int sign,
-                                     const int *index, int n);
-/* matrix multiply + spin reconstruction (make_code_binary.pl:224-257) */
-void orc_F3_D_eqop_sprecon_M_times_pH(int eqop, int adj, float *r, const float *a,
-                                      const float *const *b, int mu, int sign,
-                                      const int *index, int n);
-void orc_D3_D_eqop_sprecon_M_times_pH(int eqop, int adj, double *r, const double *a,
-                                      const double *const *b, int mu, int sign,
-                                      const int *index, int n);
 /* single-site spin projection / reconstruction (eq), used to pin the tables */
 void orc_D_spproj(double *h, const double *d, int mu, int sign);
 void orc_D_sprecon(double *d, const double *h, int mu, int sign);

@@ -1,0 +1,212 @@
+"""The site-operation part of the C ABI, generated from one variant table.
+
+QLA's generator emits its Wilson-hop building blocks for every combination of
+assignment operator, adjoint, pointer operand and gang index
+(lib/perl/build_qla.pl:1162-1250 and 1693-1725, looping over
+operatortypes.pl:34 ``@eqop_all`` and the indexing list of
+indirection.pl:63-120; names and argument order from prototype.pl:450-688).
+This module enumerates the same combinations for QLA_F3 and QLA_D3:
+
+=============================  ==========================================  =======
+family                         symbol pattern                              kind
+=============================  ==========================================  =======
+matrix times vector            V_{v,x}<eqop>_{M,Ma,pM,pMa}_times_{V,pV}      0
+projection, then matrix        H_{v,x}<eqop>_spproj_<A>_times_{D,pD}         1
+matrix, then reconstruction    D_{v,x}<eqop>_sprecon_<A>_times_{H,pH}        2
+project-multiply-reconstruct   D_{v,x}<eqop>_spproj_<A>_times_{D,pD}         3
+multi-source matrix times vec  V_{v,x}<eqop>_n<A>_times_n{V,pV}  (+ int nd)  multi
+=============================  ==========================================  =======
+
+with <eqop> in eq, peq, meq, eqm.  It writes
+
+* ``include/qla_mi355x_sites.h`` -- the prototypes, and
+* ``qla_amd/csrc/qla_sites_abi.inc`` -- their definitions, one call each into
+  the generic entry helpers of ``qla_capi.hip``.
+
+Both files are committed; ``python -m qla_amd.abi_gen`` rewrites them and a CPU
+test checks that they match this table.
+"""
+from __future__ import annotations
+
+import sys
+from dataclasses import dataclass
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+HEADER_OUT = HERE.parent / "include" / "qla_mi355x_sites.h"
+INC_OUT = HERE / "csrc" / "qla_sites_abi.inc"
+
+PRECS = ("F3", "D3")
+EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use them
+TYPE_NAMES = {"V": "ColorVector", "H": "HalfFermion", "D": "DiracFermion", "M": "ColorMatrix"}
+KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4}
+# (dest record, second-source record, qualifier in the name)
+FAMILY = {
+    "matvec": ("V", "V", ""),
+    "spproj": ("H", "D", "spproj_"),
+    "sprecon": ("D", "H", "sprecon_"),
+    "wilson": ("D", "D", "spproj_"),
+    "multi": ("V", "V", ""),
+}
+
+
+@dataclass(frozen=True)
+class Variant:
+    prec: str       # "F3" / "D3"
+    kind: str       # key of KINDS
+    eqop: int       # 0 eq, 1 peq, 2 meq, 3 eqm
+    adj: bool       # A = M^dagger ("Ma")
+    xf: bool        # gang-indexed x-form
+    pa: bool        # A is a pointer array ("pM")
+    pb: bool        # B is a pointer array ("pV", "pD", "pH")
+
+    @property
+    def multi(self) -> bool:
+        return self.kind == "multi"
+
+    @property
+    def dest(self) -> str:
+        return FAMILY[self.kind][0]
+
+    @property
+    def src2(self) -> str:
+        return FAMILY[self.kind][1]
+
+    @property
+    def spin(self) -> bool:
+        return self.kind in ("spproj", "sprecon", "wilson")
+
+    @property
+    def name(self) -> str:
+        n = "n" if self.multi else ""
+        a = f"{n}{'p' if self.pa else ''}M{'a' if self.adj else ''}"
+        b = f"{n}{'p' if self.pb else ''}{self.src2}"
+        form = "x" if self.xf else "v"
+        return f"QLA_{self.prec}_{self.dest}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{a}_times_{b}"
+
+    def params(self) -> list[tuple[str, str]]:
+        """(kind, C declaration) of every parameter; kind 'p' pointer or 'i' int."""
+        t = lambda s: f"QLA_{self.prec}_{TYPE_NAMES[s]}"                  # noqa: E731
+        stars = lambda ptr: "*" * ((1 if self.multi else 0) + (1 if ptr else 0))  # noqa: E731
+        out = [("p", f"{t(self.dest)} *QLA_MI355X_RESTRICT r"),
+               ("p", f"{t('M')} *QLA_MI355X_RESTRICT {stars(self.pa)}a"),
+               ("p", f"{t(self.src2)} *QLA_MI355X_RESTRICT {stars(self.pb)}b")]
+        if self.spin:
+            out += [("i", "int mu"), ("i", "int sign")]
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        if self.multi:
+            out.append(("i", "int nd"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+
+def variants() -> list[Variant]:
+    out = []
+    for prec in PRECS:
+        for kind in KINDS:
+            for xf in (False, True):
+                for eqop in range(4):
+                    for adj in (False, True):
+                        for pa in (False, True):
+                            for pb in (False, True):
+                                out.append(Variant(prec, kind, eqop, adj, xf, pa, pb))
+    return out
+
+
+def _decl(v: Variant) -> str:
+    ps = [d for _, d in v.params()]
+    head = f"void {v.name}("
+    pad = " " * len(head)
+    return head + (",\n" + pad).join(ps) + ")"
+
+
+_FAMILY_DOC = {
+    "matvec": ("colour matrix times colour vector, r[ic] eqop sum_kc A[ic][kc] b[kc]",
+               "lib/perl/expressions_tensor.pl:544-630 (x=0 or x=r, then += / -= over kc)"),
+    "spproj": ("spin projection of B, then colour matrix times the HalfFermion",
+               "build_qla.pl:1162-1190, make_code_binary.pl:189-222,259-271, expressions_gamma.pl:244-276"),
+    "sprecon": ("colour matrix times B, then spin reconstruction into a DiracFermion",
+                "build_qla.pl:1192-1220, make_code_binary.pl:224-257, expressions_gamma.pl:311-337"),
+    "wilson": ("spin projection of B, colour matrix times it, spin reconstruction",
+               "build_qla.pl:1222-1250, make_code_binary.pl:285-341"),
+    "multi": ("r eqop sum_{d<nd} A_d * B_d (eqop for d = 0, then += / -=; nd <= 0 is a no-op)",
+              "build_qla.pl:1693-1725, expressions_tensor.pl:983-1002, prototype.pl:683-685"),
+}
+
+
+def header_text() -> str:
+    vs = variants()
+    lines = [
+        "/*",
+        " * qla_mi355x_sites.h -- Wilson-hop building blocks of the QLA C ABI on MI355X.",
+        " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
+        " * semantics are documented there).",
+        " *",
+        " * Every symbol keeps the name and argument order of the QLA 1.9.0 symbol",
+        " * (lib/perl/prototype.pl:450-688): dest, src1, src2, `int mu, int sign`",
+        " * for the spin forms, the gang `int *index` for x-forms, `int n`, and",
+        " * `int nd` for the multi-source forms.  `Ma` uses the adjoint of the",
+        " * matrix; a `p` operand is a pointer array (device array of device record",
+        " * addresses); x-forms use every operand at index[i].  Multi-source forms:",
+        " * `a` and `b` are arrays of nd field (or pointer-array) addresses held in",
+        " * host or device memory.",
+        " */",
+        "#ifndef QLA_MI355X_SITES_H",
+        "#define QLA_MI355X_SITES_H",
+    ]
+    last = None
+    for v in vs:
+        key = (v.kind, v.prec)
+        if key != last:
+            what, ref = _FAMILY_DOC[v.kind]
+            lines += ["", f"/* {v.prec}: {what}.", f" * Reference: {ref}. */"]
+            last = key
+        lines.append(_decl(v) + ";")
+    lines += ["", "#endif", ""]
+    return "\n".join(lines)
+
+
+def inc_text() -> str:
+    lines = [
+        "// qla_sites_abi.inc -- definitions of the Wilson-hop building blocks of the C ABI.",
+        "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        "// Included by qla_capi.hip inside extern \"C\"; site_entry / multi_entry are there.",
+    ]
+    for v in variants():
+        prec = 0 if v.prec == "F3" else 1
+        lines.append(_decl(v))
+        idx = "index" if v.xf else "nullptr"
+        if v.multi:
+            body = (f"multi_entry({prec}, {v.eqop}, {int(v.adj)}, {int(v.xf)}, {int(v.pa)}, {int(v.pb)}, r, "
+                    f"(const void *)a, (const void *)b, {idx}, n, nd, \"{v.name}\");")
+        else:
+            mu, sign = ("mu", "sign") if v.spin else ("0", "1")
+            body = (f"site_entry({prec}, {KINDS[v.kind]}, {v.eqop}, {int(v.adj)}, {int(v.xf)}, {int(v.pa)}, "
+                    f"{int(v.pb)}, r, (const void *)a, (const void *)b, {mu}, {sign}, {idx}, n, \"{v.name}\");")
+        lines.append("{")
+        lines.append("    " + body)
+        lines.append("}")
+    lines.append("")
+    return "\n".join(lines)
+
+
+def write() -> None:
+    HEADER_OUT.write_text(header_text())
+    INC_OUT.write_text(inc_text())
+
+
+def current() -> bool:
+    """True when the committed outputs match the table."""
+    return (HEADER_OUT.exists() and HEADER_OUT.read_text() == header_text()
+            and INC_OUT.exists() and INC_OUT.read_text() == inc_text())
+
+
+if __name__ == "__main__":
+    write()
+    print(f"wrote {HEADER_OUT} and {INC_OUT}: {len(variants())} symbols")
+    sys.exit(0)

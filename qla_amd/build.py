@@ -39,14 +39,34 @@ CXXFLAGS = [
 ]
 
 
+# Sources compiled more than once, each time with its own defines: the site
+# kernels once per (precision, kind) and the multi-source kernels once per
+# precision, so the kernel instantiations spread over parallel compiles.
+MULTI_BUILD = {
+    "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in range(4)],
+    "qla_multi.hip": [{"QLA_SITE_PREC": p} for p in (0, 1)],
+}
+
+
 def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
-def _compile(src: Path) -> Path:
+def units() -> list[tuple[Path, dict]]:
+    """(source, defines) of every object of the library."""
+    out = []
+    for src in sources():
+        for defs in MULTI_BUILD.get(src.name, [{}]):
+            out.append((src, defs))
+    return out
+
+
+def _compile(unit: tuple[Path, dict]) -> Path:
+    src, defs = unit
     OBJ.mkdir(exist_ok=True)
-    obj = OBJ / (src.stem + ".o")
-    cmd = [HIPCC, *CXXFLAGS, "-c", str(src), "-o", str(obj)]
+    tag = "".join(f"_{k.rsplit('_', 1)[-1].lower()}{v}" for k, v in defs.items())
+    obj = OBJ / (src.stem + tag + ".o")
+    cmd = [HIPCC, *CXXFLAGS, *[f"-D{k}={v}" for k, v in defs.items()], "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stdout}\n{res.stderr}")
@@ -57,10 +77,12 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every kernel source for gfx950 and link the shared library."""
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    srcs = sources()
-    jobs = jobs or min(len(srcs), 8)
+    us = units()
+    jobs = jobs or min(len(us), os.cpu_count() or 8, 16)
+    # longest compiles first
+    us.sort(key=lambda u: 0 if u[1] else 1)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(_compile, us))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -19,55 +19,45 @@ namespace qla {
 // device job table entry (matches qla_batch_job in qla_launch.hpp)
 typedef qla_batch_job batch_job;
 
-typedef matvec_op<float, PEQ, false, true, false> bop_matvec;
-template <int MU, int SIGN> using bop_spproj = spproj_op<float, EQ, false, MU, SIGN, false>;
-template <int MU, int SIGN> using bop_sprecon = sprecon_op<float, PEQ, false, MU, SIGN, false>;
+typedef site_op<float, K_MATVEC, PEQ, false, false> bop_matvec;
+typedef site_op<float, K_SPPROJ, EQ, false, false> bop_spproj;
+typedef site_op<float, K_SPRECON, PEQ, false, false> bop_sprecon;
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr int BATCH_BYTES = cmax(bop_matvec::BYTES, cmax(bop_spproj<0, 1>::BYTES, bop_sprecon<0, 1>::BYTES));
+constexpr int BATCH_BYTES = cmax(bop_matvec::BYTES, cmax(bop_spproj::BYTES, bop_sprecon::BYTES));
+
+// operands of a job in the form of the per-call kernels (v-form, b a pointer array)
+__device__ __forceinline__ site_args job_args(const batch_job &j)
+{
+    site_args p;
+    p.r = j.r;
+    p.a = j.a;
+    p.b = (const void *)j.b;
+    p.index = nullptr;
+    p.n = j.n;
+    p.al = j.al;
+    p.ms = j.ms;
+    p.pa = 0;
+    p.pb = 1;
+    return p;
+}
 
 // stage / finish of the job's op for one tile (the switch is wave-uniform)
 template <class OP>
-__device__ __forceinline__ void op_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
-                                        int lane)
+__device__ __forceinline__ void op_step(bool fin, char *lds, const site_args &p, const lane_addr &la, int base,
+                                        int ns, int lane)
 {
-    if (!fin)
-        OP::stage(lds, (cpx<float> *)j.r, (const cpx<float> *)j.a, (const void *)j.b, pb, base, ns, lane, base + lane,
-                  lane < ns, j.al);
-    else
-        OP::finish(lds, (cpx<float> *)j.r, base, ns, lane, base + lane, lane < ns, j.al);
+    if (!fin) OP::stage(lds, p, la, base, ns, lane);
+    else OP::finish(lds, p, la, base, ns, lane);
 }
 
-template <template <int, int> class OPT>
-__device__ __forceinline__ void musign_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
-                                            int lane)
+__device__ __forceinline__ void job_step(bool fin, char *lds, int op, const site_args &p, const lane_addr &la,
+                                         int base, int ns, int lane)
 {
-    if (j.sign == 1) {
-        switch (j.mu) {
-        case 0: op_step<OPT<0, 1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 1: op_step<OPT<1, 1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 2: op_step<OPT<2, 1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 3: op_step<OPT<3, 1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 4: op_step<OPT<4, 1>>(fin, lds, j, pb, base, ns, lane); break;
-        }
-    } else {
-        switch (j.mu) {
-        case 0: op_step<OPT<0, -1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 1: op_step<OPT<1, -1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 2: op_step<OPT<2, -1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 3: op_step<OPT<3, -1>>(fin, lds, j, pb, base, ns, lane); break;
-        case 4: op_step<OPT<4, -1>>(fin, lds, j, pb, base, ns, lane); break;
-        }
-    }
-}
-
-__device__ __forceinline__ void job_step(bool fin, char *lds, const batch_job &j, uint64_t pb, int base, int ns,
-                                         int lane)
-{
-    switch (j.op) {
-    case QLA_BOP_MATVEC_PV: op_step<bop_matvec>(fin, lds, j, pb, base, ns, lane); break;
-    case QLA_BOP_SPPROJ: musign_step<bop_spproj>(fin, lds, j, pb, base, ns, lane); break;
-    case QLA_BOP_SPRECON: musign_step<bop_sprecon>(fin, lds, j, pb, base, ns, lane); break;
+    switch (op) {
+    case QLA_BOP_MATVEC_PV: op_step<bop_matvec>(fin, lds, p, la, base, ns, lane); break;
+    case QLA_BOP_SPPROJ: op_step<bop_spproj>(fin, lds, p, la, base, ns, lane); break;
+    case QLA_BOP_SPRECON: op_step<bop_sprecon>(fin, lds, p, la, base, ns, lane); break;
     }
 }
 
@@ -90,18 +80,22 @@ __device__ __forceinline__ int find_job(const int *t0, int stride, int njobs, in
 }
 
 struct tile_ref {
-    batch_job j;
-    int base, ns;
-    uint64_t pb;
+    site_args p;
+    lane_addr la;
+    int op, base, ns;
 };
 
 __device__ __forceinline__ void load_tile_ref(tile_ref &t, const batch_job *jobs, const int *t0, int stride, int njobs,
                                               int tile, int lane)
 {
-    t.j = jobs[find_job(t0, stride, njobs, tile)];
-    t.base = (tile - t.j.tile0) * WAVE;
-    t.ns = min(WAVE, t.j.n - t.base);
-    t.pb = lane < t.ns ? (uint64_t)t.j.b[t.base + lane] : 0;
+    const batch_job j = jobs[find_job(t0, stride, njobs, tile)];
+    t.p = job_args(j);
+    t.op = j.op;
+    t.base = (tile - j.tile0) * WAVE;
+    t.ns = min(WAVE, j.n - t.base);
+    t.la.ga = 0;
+    t.la.gr = 0;
+    t.la.gb = lane < t.ns ? (uint64_t)j.b[t.base + lane] : 0;
 }
 
 // Waves claim `tpc` consecutive tiles at a time.  Each wave holds one claim
@@ -133,8 +127,9 @@ k_batch(const batch_job *__restrict__ jobs, int njobs, int total_tiles, int tpc,
     int last = (int)min((long long)total_tiles, first + tpc);
     tile_ref cur, nxt;
     load_tile_ref(cur, jobs, t0, t0_stride, njobs, tile, lane);
+    vm_wait();      // the loop's only memory wait is the one in finish
     for (;;) {
-        job_step(false, lds, cur.j, cur.pb, cur.base, cur.ns, lane);
+        job_step(false, lds, cur.op, cur.p, cur.la, cur.base, cur.ns, lane);
         int ntile = tile + 1, nlast = last;
         if (ntile >= last) {                       // move on to the claim held ahead
             const long long nf = (long long)uni((int)pend) * tpc;
@@ -144,7 +139,7 @@ k_batch(const batch_job *__restrict__ jobs, int njobs, int total_tiles, int tpc,
         }
         const bool more = ntile < total_tiles;
         if (more) load_tile_ref(nxt, jobs, t0, t0_stride, njobs, ntile, lane);
-        job_step(true, lds, cur.j, cur.pb, cur.base, cur.ns, lane);
+        job_step(true, lds, cur.op, cur.p, cur.la, cur.base, cur.ns, lane);
         if (!more) break;
         cur = nxt;
         tile = ntile;

@@ -11,6 +11,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -108,6 +109,74 @@ void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const
     record(e, name);
 }
 
+
+// ------------------------------------------------------------- site calls
+// tiles per wave of the persistent site kernels (launch shape only)
+std::atomic<int> g_tiles_per_wave{ 2 };
+
+// One QLA site call (qla_sites_abi.inc): the reference's no-op cases, then
+// one launch.  kind 0 matvec, 1 spproj, 2 sprecon, 3 project-multiply-
+// reconstruct; mu/sign only for kinds 1-3.
+void site_entry(int prec, int kind, int eqop, int adj, int xf, int pa, int pb, void *r, const void *a,
+                const void *b, int mu, int sign, const int *index, int n, const char *name)
+{
+    if (n <= 0) return;
+    if (kind != 0 && (mu < 0 || mu > 4)) return;     // switch without default (expressions_gamma.pl:349-381)
+    qla_site_args p;
+    p.r = r;
+    p.a = a;
+    p.b = b;
+    p.index = xf ? index : nullptr;
+    p.n = n;
+    p.al = ((!pa && (uintptr_t)a % 16 == 0) ? 1 : 0) | ((!pb && (uintptr_t)b % 16 == 0) ? 2 : 0) |
+           (((uintptr_t)r % 16 == 0) ? 4 : 0);
+    p.ms = kind == 0 ? 0 : mu + (sign == 1 ? 0 : 5);
+    p.pa = pa;
+    p.pb = pb;
+    finish(qla_launch_site(prec, kind, eqop, adj, xf, p, g_tiles_per_wave.load(), ts.stream), name);
+}
+
+// nd term addresses from host or device memory
+hipError_t term_list(const void **dst, const void *src, int nd)
+{
+    if (is_device_pointer(src)) return hipMemcpy(dst, src, sizeof(void *) * (size_t)nd, hipMemcpyDeviceToHost);
+    memcpy(dst, src, sizeof(void *) * (size_t)nd);
+    return hipSuccess;
+}
+
+// One multi-source call: terms in launches of at most QLA_MULTI_MAXND, the
+// first with the call's eqop and the rest accumulating (peq for eq/peq, meq
+// for meq/eqm) -- the reference's own d >= 1 rule (expressions_tensor.pl:985-1000).
+void multi_entry(int prec, int eqop, int adj, int xf, int pa, int pb, void *r, const void *a, const void *b,
+                 const int *index, int n, int nd, const char *name)
+{
+    if (n <= 0 || nd <= 0) return;
+    std::vector<const void *> av((size_t)nd), bv((size_t)nd);
+    hipError_t e = term_list(av.data(), a, nd);
+    if (e == hipSuccess) e = term_list(bv.data(), b, nd);
+    for (int d0 = 0; e == hipSuccess && d0 < nd; d0 += QLA_MULTI_MAXND) {
+        qla_multi_args p;
+        memset(&p, 0, sizeof(p));
+        p.r = r;
+        p.index = xf ? index : nullptr;
+        p.n = n;
+        p.nd = std::min(QLA_MULTI_MAXND, nd - d0);
+        p.pa = pa;
+        p.pb = pb;
+        int al_a = 1, al_b = 1;
+        for (int d = 0; d < p.nd; d++) {
+            p.a[d] = av[(size_t)(d0 + d)];
+            p.b[d] = bv[(size_t)(d0 + d)];
+            al_a &= (!pa && (uintptr_t)p.a[d] % 16 == 0) ? 1 : 0;
+            al_b &= (!pb && (uintptr_t)p.b[d] % 16 == 0) ? 1 : 0;
+        }
+        p.al = al_a | (al_b << 1) | (((uintptr_t)r % 16 == 0) ? 4 : 0);
+        const int op = d0 == 0 ? eqop : ((eqop == 2 || eqop == 3) ? 2 : 1);
+        e = qla_launch_multi(prec, op, adj, xf, p, g_tiles_per_wave.load(), ts.stream);
+    }
+    finish(e, name);
+}
+
 // ------------------------------------------------------------- batch plans
 struct batch_shape {
     int wpb, tpc, bpc;   // waves per block, tiles per claim, blocks per CU
@@ -157,6 +226,36 @@ void batch_drop_graph(QLA_MI355X_batch *p)
 
 } // namespace
 
+
+// Per-object launchers: qla_site.hip is compiled once per (precision, kind)
+// and qla_multi.hip once per precision.
+#define QLA_SITE_LAUNCHER(P, K) \
+    hipError_t qla_site_launch_p##P##_k##K(int, int, int, const qla_site_args &, int, hipStream_t);
+QLA_SITE_LAUNCHER(0, 0) QLA_SITE_LAUNCHER(0, 1) QLA_SITE_LAUNCHER(0, 2) QLA_SITE_LAUNCHER(0, 3)
+QLA_SITE_LAUNCHER(1, 0) QLA_SITE_LAUNCHER(1, 1) QLA_SITE_LAUNCHER(1, 2) QLA_SITE_LAUNCHER(1, 3)
+#undef QLA_SITE_LAUNCHER
+hipError_t qla_multi_launch_p0(int, int, int, const qla_multi_args &, int, hipStream_t);
+hipError_t qla_multi_launch_p1(int, int, int, const qla_multi_args &, int, hipStream_t);
+
+hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const qla_site_args &p, int tpw,
+                           hipStream_t st)
+{
+    typedef hipError_t (*launcher)(int, int, int, const qla_site_args &, int, hipStream_t);
+    static const launcher tab[2][4] = {
+        { qla_site_launch_p0_k0, qla_site_launch_p0_k1, qla_site_launch_p0_k2, qla_site_launch_p0_k3 },
+        { qla_site_launch_p1_k0, qla_site_launch_p1_k1, qla_site_launch_p1_k2, qla_site_launch_p1_k3 },
+    };
+    if (prec < 0 || prec > 1 || kind < 0 || kind > 3) return hipErrorInvalidValue;
+    return tab[prec][kind](eqop, adj, xf, p, tpw, st);
+}
+
+hipError_t qla_launch_multi(int prec, int eqop, int adj, int xf, const qla_multi_args &p, int tpw, hipStream_t st)
+{
+    if (prec == 0) return qla_multi_launch_p0(eqop, adj, xf, p, tpw, st);
+    if (prec == 1) return qla_multi_launch_p1(eqop, adj, xf, p, tpw, st);
+    return hipErrorInvalidValue;
+}
+
 extern "C" {
 
 // ----------------------------------------------------------------- control
@@ -174,31 +273,12 @@ const char *QLA_MI355X_last_error_string(void) { return hipGetErrorString(ts.err
 const char *QLA_MI355X_version(void) { return QLA_MI355X_VERSION_STRING; }
 void QLA_MI355X_set_tiles_per_wave(int tpw)
 {
-    qla_set_tiles_per_wave(tpw);
+    g_tiles_per_wave.store(tpw < 1 ? 1 : tpw);
     qla_set_cmp_tiles_per_wave(tpw);
 }
 
-// ----------------------------------------------------------------- a1 / a2
-void QLA_F3_V_veq_M_times_V(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *b, int n)
-{
-    if (n <= 0) return;
-    finish(qla_launch_matvec(0, 0, 0, r, a, b, nullptr, n, ts.stream), "QLA_F3_V_veq_M_times_V");
-}
-void QLA_D3_V_veq_M_times_V(QLA_D3_ColorVector *r, QLA_D3_ColorMatrix *a, QLA_D3_ColorVector *b, int n)
-{
-    if (n <= 0) return;
-    finish(qla_launch_matvec(1, 0, 0, r, a, b, nullptr, n, ts.stream), "QLA_D3_V_veq_M_times_V");
-}
-void QLA_F3_V_vpeq_M_times_pV(QLA_F3_ColorVector *r, QLA_F3_ColorMatrix *a, QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b, int n)
-{
-    if (n <= 0) return;
-    finish(qla_launch_matvec(0, 1, 1, r, a, (const void *)b, nullptr, n, ts.stream), "QLA_F3_V_vpeq_M_times_pV");
-}
-void QLA_D3_V_vpeq_M_times_pV(QLA_D3_ColorVector *r, QLA_D3_ColorMatrix *a, QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b, int n)
-{
-    if (n <= 0) return;
-    finish(qla_launch_matvec(1, 1, 1, r, a, (const void *)b, nullptr, n, ts.stream), "QLA_D3_V_vpeq_M_times_pV");
-}
+// ------------------------------------------- a1/a2, a5-a8: the Wilson-hop family
+#include "qla_sites_abi.inc"
 
 // ----------------------------------------------------------------- a3 / a4
 void QLA_F3_r_veq_norm2_V(QLA_F_Real *r, QLA_F3_ColorVector *a, int n)
@@ -236,66 +316,6 @@ void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_D3_DiracFermion *a, QLA_D3_Dirac
 void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_F3_DiracFermion *a, QLA_F3_DiracFermion *b, int n)
 {
     reduce(0, 1, a, b, n > 0 ? 12LL * n : 0, r, 2, 2 * sizeof(double), "QLA_DF3_c_veq_D_dot_D");
-}
-
-// ----------------------------------------------------------------- a5 / a6
-void QLA_F3_H_veq_spproj_M_times_pD(QLA_F3_HalfFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_spproj(0, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
-           "QLA_F3_H_veq_spproj_M_times_pD");
-}
-void QLA_F3_H_xeq_spproj_M_times_pD(QLA_F3_HalfFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int *index, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_spproj(0, r, a, (const void *)b, mu, sign, index, n, ts.stream),
-           "QLA_F3_H_xeq_spproj_M_times_pD");
-}
-void QLA_D3_H_veq_spproj_M_times_pD(QLA_D3_HalfFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_spproj(1, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
-           "QLA_D3_H_veq_spproj_M_times_pD");
-}
-void QLA_D3_H_xeq_spproj_M_times_pD(QLA_D3_HalfFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
-                                    int mu, int sign, int *index, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_spproj(1, r, a, (const void *)b, mu, sign, index, n, ts.stream),
-           "QLA_D3_H_xeq_spproj_M_times_pD");
-}
-
-// ----------------------------------------------------------------- a7 / a8
-void QLA_F3_D_vpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_sprecon(0, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
-           "QLA_F3_D_vpeq_sprecon_M_times_pH");
-}
-void QLA_F3_D_xpeq_sprecon_M_times_pH(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *a, QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int *index, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_sprecon(0, r, a, (const void *)b, mu, sign, index, n, ts.stream),
-           "QLA_F3_D_xpeq_sprecon_M_times_pH");
-}
-void QLA_D3_D_vpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_sprecon(1, r, a, (const void *)b, mu, sign, nullptr, n, ts.stream),
-           "QLA_D3_D_vpeq_sprecon_M_times_pH");
-}
-void QLA_D3_D_xpeq_sprecon_M_times_pH(QLA_D3_DiracFermion *r, QLA_D3_ColorMatrix *a, QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
-                                      int mu, int sign, int *index, int n)
-{
-    if (n <= 0 || mu < 0 || mu > 4) return;
-    finish(qla_launch_sprecon(1, r, a, (const void *)b, mu, sign, index, n, ts.stream),
-           "QLA_D3_D_xpeq_sprecon_M_times_pH");
 }
 
 // ----------------------------------------------------------------- a9 - a12
@@ -366,8 +386,8 @@ QLA_MI355X_batch *QLA_MI355X_F3_batch_create(const QLA_MI355X_F3_job *jobs, int 
         d.a = j.a;
         d.b = (const void *const *)j.b;
         d.op = j.op;
-        d.mu = j.mu;
-        d.sign = j.sign == 1 ? 1 : -1;
+        d.ms = j.op == QLA_MI355X_OP_V_VPEQ_M_TIMES_PV ? 0 : j.mu + (j.sign == 1 ? 0 : 5);
+        d.spare = 0;
         // no-op jobs (n <= 0, or mu outside 0..4 for the spin ops) own no tiles
         const bool noop = j.n <= 0 || (j.op != QLA_MI355X_OP_V_VPEQ_M_TIMES_PV && (j.mu < 0 || j.mu > 4));
         d.n = noop ? 0 : j.n;

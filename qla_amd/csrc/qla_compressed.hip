@@ -99,7 +99,7 @@ __device__ __forceinline__ float fermion_encode(uint32_t (&w)[NW], const cpx<flo
 
 __device__ __forceinline__ void cmp_dma_wait()
 {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    vm_wait();
     wave_sync();
 }
 
@@ -122,6 +122,7 @@ k_spproj_c(char *__restrict__ r, const char *__restrict__ a, const char *const *
     int tile = blockIdx.x * WPB + threadIdx.x / WAVE;
     if (tile >= ntiles) return;
     uint64_t pd = (tile * WAVE + lane < n) ? (uint64_t)b[tile * WAVE + lane] : 0;
+    vm_wait();      // the loop's only memory wait is the one before its LDS reads
     for (;;) {
         const int base = tile * WAVE, ns = min(WAVE, n - base);
         if (al & 1) stage_bytes<16, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);
@@ -177,6 +178,7 @@ k_sprecon_c(char *__restrict__ r, const char *__restrict__ a, const char *const 
     int tile = blockIdx.x * WPB + threadIdx.x / WAVE;
     if (tile >= ntiles) return;
     uint64_t ph = (tile * WAVE + lane < n) ? (uint64_t)b[tile * WAVE + lane] : 0;
+    vm_wait();      // the loop's only memory wait is the one before its LDS reads
     for (;;) {
         const int base = tile * WAVE, ns = min(WAVE, n - base);
         if (al & 1) stage_bytes<16, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);

@@ -39,6 +39,14 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Wait until every vector-memory operation of this wave has completed
+// (s_waitcnt vmcnt(0): simm16 0x0F70 on gfx9).  Issued through the builtin so
+// the compiler's wait-count pass sees it: loads issued before it count as
+// complete afterwards, and the pass inserts no second full wait of its own
+// when their destination registers are read or reused later (for example a
+// loop-carried address register loaded in a prologue).
+__device__ __forceinline__ void vm_wait() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // 64-bit value of lane `src` (ds_bpermute on both halves).
 __device__ __forceinline__ uint64_t shfl64(uint64_t v, int src)
 {

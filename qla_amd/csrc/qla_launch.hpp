@@ -3,18 +3,36 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-// V = M*V (eqop eq, b contiguous) or V += M*(*pV) (eqop peq, b pointer array);
-// index != NULL selects the gang-indexed x-form.
-hipError_t qla_launch_matvec(int prec, int eqop_peq, int pb, void *r, const void *a, const void *b,
-                             const int *index, int n, hipStream_t st);
-// launch-shape knob of the persistent site kernels
-void qla_set_tiles_per_wave(int tpw);
-// H = spproj(M * pD)
-hipError_t qla_launch_spproj(int prec, void *r, const void *a, const void *b, int mu, int sign,
-                             const int *index, int n, hipStream_t st);
-// D += sprecon(M * pH)
-hipError_t qla_launch_sprecon(int prec, void *r, const void *a, const void *b, int mu, int sign,
-                              const int *index, int n, hipStream_t st);
+// ---------------------------------------------------------------------------
+// Site operations (qla_site.hip, qla_multi.hip).  kind: 0 V = M*V, 1 H =
+// M*spproj(D), 2 D = sprecon(M*H), 3 D = sprecon(M*spproj(D)); eqop 0 eq,
+// 1 peq, 2 meq, 3 eqm; adj selects M^dagger; xf the gang-indexed x-form.
+// ---------------------------------------------------------------------------
+struct qla_site_args {
+    void *r;            // destination field
+    const void *a;      // ColorMatrix field, or pointer array when pa
+    const void *b;      // second source field, or pointer array when pb
+    const int *index;   // gang index (x-form)
+    int n;              // sites (v-form) / index entries (x-form)
+    int al;             // 16-byte alignment of the contiguous fields: 1 a, 2 b, 4 r
+    int ms;             // mu + 5 * (sign != 1), 0..9
+    int pa, pb;         // a / b are pointer arrays
+};
+// tpw: tiles per wave of the persistent grid (launch shape only)
+hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const qla_site_args &p, int tpw,
+                           hipStream_t st);
+
+// Multi-source matrix times vector, r eqop sum_d A_d * B_d for d < nd <= QLA_MULTI_MAXND
+// (the C ABI layer splits longer sums into peq/meq launches).
+#define QLA_MULTI_MAXND 16
+struct qla_multi_args {
+    void *r;
+    const int *index;
+    int n, al, nd, pa, pb;
+    const void *a[QLA_MULTI_MAXND];   // per term: ColorMatrix field or pointer array
+    const void *b[QLA_MULTI_MAXND];   // per term: ColorVector field or pointer array
+};
+hipError_t qla_launch_multi(int prec, int eqop, int adj, int xf, const qla_multi_args &p, int tpw, hipStream_t st);
 
 // Global reductions over m complex elements.  kind: 0 = norm2 (b unused),
 // 1 = dot conj(a).b.  The result is written to the DEVICE address `out` as
@@ -48,9 +66,11 @@ struct qla_batch_job {
     void *r;                 // destination field (V, H or D records)
     const void *a;           // ColorMatrix field
     const void *const *b;    // pointer array (V, D or H records)
-    int n, op, mu, sign;     // as in the QLA call
+    int n, op;               // as in the QLA call
+    int ms;                  // mu + 5 * (sign != 1)
     int tile0;               // first global tile of this job
     int al;                  // alignment flags of a and r (bit 0: a, bit 2: r, 16-byte)
+    int spare;
 };
 int qla_batch_lds_bytes_per_wave(void);
 // zero `counter`, then one persistent launch of `blocks` x (wpb waves)

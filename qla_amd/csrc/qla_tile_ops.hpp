@@ -1,16 +1,25 @@
-// qla_tile_ops.hpp -- one 64-site tile of each site operation, split into
-// a stage step (issue the direct-to-LDS loads of the tile's records) and a
+// qla_tile_ops.hpp -- one 64-site tile of a site operation, split into a
+// stage step (issue the direct-to-LDS loads of the tile's records) and a
 // finish step (wait, transpose out of LDS, compute in the reference order,
-// write the result back through LDS).  The plain per-op kernels and the
-// batched persistent kernel are loops over these steps; between the two
-// steps a loop can issue the next tile's gather-address loads so they travel
-// with the current tile's records.
+// write the result back through LDS).  The per-call kernels and the batched
+// persistent kernel are loops over these steps; between the two steps a loop
+// issues the next tile's gather-address loads so they travel with the
+// current tile's records.
 //
-// Per-lane operand addressing: `site` is the lane's site index (v-form:
-// base + lane; x-form: index[base + lane]), `act` whether the lane holds a
-// site of the tile, `pb` the lane's pointer-array entry (gathered record).
+// One generic operation covers the QLA Wilson-hop building blocks
+// (build_qla.pl:1162-1250, make_code_binary.pl:189-350):
+//   K_MATVEC   V  eqop A * B               B: V      (expressions_tensor.pl:544-630)
+//   K_SPPROJ   H  eqop A * spproj(B)       B: D      (make_code_binary.pl:189-222)
+//   K_SPRECON  D  eqop sprecon(A * B)      B: H      (make_code_binary.pl:224-257)
+//   K_WILSON   D  eqop sprecon(A * spproj(B))  B: D  (make_code_binary.pl:285-341)
+// with A = M or M^dagger (the `Ma` forms), A and B either fields or pointer
+// arrays (the `pM` / `pB` forms), and the v-form (site i) or the gang-indexed
+// x-form (site index[i]).  Precision, kind, eqop, adjoint and x-form are
+// compile-time; mu/sign and the pointer flags are wave-uniform run-time
+// values (a scalar branch per tile).
 #pragma once
 #include "qla_dev.hpp"
+#include "qla_launch.hpp"
 
 namespace qla {
 
@@ -22,7 +31,7 @@ template <class T> struct rec {
 // Wait for this wave's direct-to-LDS loads, then order LDS across lanes.
 __device__ __forceinline__ void dma_wait()
 {
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    vm_wait();
     wave_sync();
 }
 
@@ -79,138 +88,204 @@ __device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
 // alignment flags of the contiguous operands (bit set: 16-byte aligned)
 enum { AL_A = 1, AL_B = 2, AL_R = 4 };
 
+// kinds of site operation
+enum { K_MATVEC = 0, K_SPPROJ = 1, K_SPRECON = 2, K_WILSON = 3 };
+
+// record bytes of the second source (B) and of the destination (R)
+template <class T, int KIND> struct kind_rec;
+template <class T> struct kind_rec<T, K_MATVEC>  { static constexpr int B = rec<T>::V, R = rec<T>::V; };
+template <class T> struct kind_rec<T, K_SPPROJ>  { static constexpr int B = rec<T>::D, R = rec<T>::H; };
+template <class T> struct kind_rec<T, K_SPRECON> { static constexpr int B = rec<T>::H, R = rec<T>::D; };
+template <class T> struct kind_rec<T, K_WILSON>  { static constexpr int B = rec<T>::D, R = rec<T>::D; };
+
+// mu/sign code: ms = mu + 5 * (sign != 1), 0..9
+template <int K> struct ms_c {
+    static constexpr int MU = K % 5, SIGN = K < 5 ? 1 : -1;
+};
+// f(ms_c<ms>) with ms compile-time (MS >= 0) or run-time (MS < 0, one
+// scalar branch; codes outside 0..9 never reach a kernel)
+template <int MS, class F>
+__device__ __forceinline__ void with_ms(int ms, F &&f)
+{
+    if constexpr (MS >= 0) {
+        f(ms_c<MS>{});
+    } else {
+        switch (ms) {
+        case 0: f(ms_c<0>{}); break;
+        case 1: f(ms_c<1>{}); break;
+        case 2: f(ms_c<2>{}); break;
+        case 3: f(ms_c<3>{}); break;
+        case 4: f(ms_c<4>{}); break;
+        case 5: f(ms_c<5>{}); break;
+        case 6: f(ms_c<6>{}); break;
+        case 7: f(ms_c<7>{}); break;
+        case 8: f(ms_c<8>{}); break;
+        default: f(ms_c<9>{}); break;
+        }
+    }
+}
+
+// Operands of one site call (a kernel argument: wave-uniform, in SGPRs).
+typedef qla_site_args site_args;
+
+// Per-lane record addresses of one tile: gathered A and B records (pointer
+// operands, or every operand of the x-form) and the x-form destination.
+struct lane_addr {
+    uint64_t ga, gb, gr;
+};
+
+template <class T, int KIND, bool XF>
+__device__ __forceinline__ lane_addr addresses_of(const site_args &p, int site, bool act)
+{
+    typedef kind_rec<T, KIND> K;
+    lane_addr la = { 0, 0, 0 };
+    if (act) {
+        if (p.pa) la.ga = (uint64_t)((const void *const *)p.a)[site];
+        else if (XF) la.ga = (uint64_t)((const char *)p.a + (size_t)site * rec<T>::M);
+        if (p.pb) la.gb = (uint64_t)((const void *const *)p.b)[site];
+        else if (XF) la.gb = (uint64_t)((const char *)p.b + (size_t)site * K::B);
+        if (XF) la.gr = (uint64_t)((char *)p.r + (size_t)site * K::R);
+    }
+    return la;
+}
 
 // ---------------------------------------------------------------------------
-// V eqop M * V : b contiguous (PB = false) or pointer array (PB = true)
+// The tile operation.  LDS per wave: A tile, B tile, and (peq/meq) the
+// destination tile; eq/eqm write their result over the A/B region (the
+// result record is never larger than A + B).
 // ---------------------------------------------------------------------------
-template <class T, int EQOP, bool ADJ, bool PB, bool XF> struct matvec_op {
+template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct site_op {
     typedef rec<T> R;
+    typedef kind_rec<T, KIND> K;
     static constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    static constexpr int BYTES = WAVE * (R::M + 2 * R::V);
-    static __device__ __forceinline__ void stage(char *lds, cpx<T> *r, const cpx<T> *a, const void *b, uint64_t pb,
-                                                 int base, int ns, int lane, int site, bool act, int al)
+    static constexpr int OFF_B = WAVE * R::M;
+    static constexpr int OFF_R = READ_R ? WAVE * (R::M + K::B) : 0;
+    static constexpr int BYTES = WAVE * (R::M + K::B + (READ_R ? K::R : 0));
+    static_assert(K::R <= R::M + K::B, "result must fit over the source tiles");
+
+    static __device__ __forceinline__ void stage(char *lds, const site_args &p, const lane_addr &la, int base,
+                                                 int ns, int lane)
     {
-        char *lm = lds, *lb = lm + WAVE * R::M, *lr = lb + WAVE * R::V;
-        if (!XF) {
-            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, lane, al & AL_A);
-            if (!PB) load_tile<R::V>(lb, (const char *)b + (size_t)base * R::V, ns, lane, al & AL_B);
-            if (READ_R) load_tile<R::V>(lr, (const char *)(r + 3 * (size_t)base), ns, lane, al & AL_R);
-        } else {
-            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, lane);
-            if (!PB) gather_tile<R::V>(lb, act ? (uint64_t)((const char *)b + (size_t)site * R::V) : 0, ns, lane);
-            if (READ_R) gather_tile<R::V>(lr, act ? (uint64_t)(r + 3 * (size_t)site) : 0, ns, lane);
+        if (!XF && !p.pa) load_tile<R::M>(lds, (const char *)p.a + (size_t)base * R::M, ns, lane, p.al & AL_A);
+        else gather_tile<R::M>(lds, la.ga, ns, lane);
+        if (!XF && !p.pb) load_tile<K::B>(lds + OFF_B, (const char *)p.b + (size_t)base * K::B, ns, lane, p.al & AL_B);
+        else gather_tile<K::B>(lds + OFF_B, la.gb, ns, lane);
+        if (READ_R) {
+            if (!XF) load_tile<K::R>(lds + OFF_R, (const char *)p.r + (size_t)base * K::R, ns, lane, p.al & AL_R);
+            else gather_tile<K::R>(lds + OFF_R, la.gr, ns, lane);
         }
-        if (PB) gather_tile<R::V>(lb, pb, ns, lane);
     }
-    static __device__ __forceinline__ void finish(char *lds, cpx<T> *r, int base, int ns, int lane, int site, bool act,
-                                                  int al)
+
+    static __device__ __forceinline__ void finish(char *lds, const site_args &p, const lane_addr &la, int base,
+                                                  int ns, int lane)
     {
-        char *lm = lds, *lb = lm + WAVE * R::M, *lr = lb + WAVE * R::V;
+        constexpr int NR = K::R / R::C;
         dma_wait();
-        cpx<T> m[9], v[3], x[3];
-        lds_read(m, lm + lane * R::M);
-        lds_read(v, lb + lane * R::V);
-        if (READ_R) lds_read(x, lr + lane * R::V);
-        mat_times_cols<EQOP, ADJ, 1>(x, m, v);
+        cpx<T> m[9], o[NR];
+        lds_read(m, lds + lane * R::M);
+        const char *lb = lds + OFF_B + lane * K::B;
+        if (READ_R) lds_read(o, lds + OFF_R + lane * K::R);
+        if constexpr (KIND == K_MATVEC) {
+            cpx<T> v[3];
+            lds_read(v, lb);
+            mat_times_cols<EQOP, ADJ, 1>(o, m, v);
+        } else if constexpr (KIND == K_SPPROJ) {
+            cpx<T> d[12], tp[6];
+            lds_read(d, lb);
+            with_ms<MS>(p.ms, [&](auto c) {
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++)
+                    spproj_row<decltype(c)::MU, decltype(c)::SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
+            });
+            mat_times_cols<EQOP, ADJ, 2>(o, m, tp);
+        } else {
+            cpx<T> tv[6];
+            if constexpr (KIND == K_SPRECON) {
+                cpx<T> h[6];
+                lds_read(h, lb);
+                mat_times_cols<EQ, ADJ, 2>(tv, m, h);
+            } else {
+                cpx<T> d[12], tp[6];
+                lds_read(d, lb);
+                with_ms<MS>(p.ms, [&](auto c) {
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++)
+                        spproj_row<decltype(c)::MU, decltype(c)::SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
+                });
+                mat_times_cols<EQ, ADJ, 2>(tv, m, tp);
+            }
+            with_ms<MS>(p.ms, [&](auto c) {
+                typedef recon_tab<decltype(c)::MU, decltype(c)::SIGN> TAB;
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) {
+                    recon_apply<EQOP, TAB::op[0]>(o[4 * ic + 0], tv[2 * ic + TAB::src[0]]);
+                    recon_apply<EQOP, TAB::op[1]>(o[4 * ic + 1], tv[2 * ic + TAB::src[1]]);
+                    recon_apply<EQOP, TAB::op[2]>(o[4 * ic + 2], tv[2 * ic + TAB::src[2]]);
+                    recon_apply<EQOP, TAB::op[3]>(o[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
+                }
+            });
+        }
+        char *lr = lds + OFF_R;
         wave_sync();
-        lds_write(lr + lane * R::V, x);
+        lds_write(lr + lane * K::R, o);
         wave_sync();
-        if (!XF) store_tile<R::V>((char *)(r + 3 * (size_t)base), lr, ns, lane, al & AL_R);
-        else scatter_tile<R::V>(act ? (uint64_t)(r + 3 * (size_t)site) : 0, lr, ns, lane);
+        if (!XF) store_tile<K::R>((char *)p.r + (size_t)base * K::R, lr, ns, lane, p.al & AL_R);
+        else scatter_tile<K::R>(la.gr, lr, ns, lane);
         wave_sync();
     }
 };
 
 // ---------------------------------------------------------------------------
-// H eqop spproj_{MU,SIGN}(M * pD)
+// Multi-source colour matrix times vector:  r eqop sum_d A_d * B_d
+// (V_eqop_nM_times_nV and its pointer forms; build_qla.pl:1693-1725,
+// expressions_tensor.pl:983-1002).  The reference applies eqop for d = 0
+// and peq (eq) / meq (eqm) for d >= 1, storing r between terms; r holds the
+// same-precision accumulator, so one register accumulator over d and kc
+// gives identical bits.
 // ---------------------------------------------------------------------------
-template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF> struct spproj_op {
-    typedef rec<T> R;
-    static constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    // the M region is reused for the H result (H <= M)
-    static constexpr int BYTES = WAVE * (R::M + R::D + (READ_R ? R::H : 0));
-    static __device__ __forceinline__ char *hreg(char *lds) { return READ_R ? lds + WAVE * (R::M + R::D) : lds; }
-    static __device__ __forceinline__ void stage(char *lds, cpx<T> *r, const cpx<T> *a, const void *, uint64_t pb,
-                                                 int base, int ns, int lane, int site, bool act, int al)
-    {
-        char *lm = lds, *ld = lm + WAVE * R::M, *lh = hreg(lds);
-        if (!XF) {
-            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, lane, al & AL_A);
-            if (READ_R) load_tile<R::H>(lh, (const char *)(r + 6 * (size_t)base), ns, lane, al & AL_R);
-        } else {
-            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, lane);
-            if (READ_R) gather_tile<R::H>(lh, act ? (uint64_t)(r + 6 * (size_t)site) : 0, ns, lane);
-        }
-        gather_tile<R::D>(ld, pb, ns, lane);
-    }
-    static __device__ __forceinline__ void finish(char *lds, cpx<T> *r, int base, int ns, int lane, int site, bool act,
-                                                  int al)
-    {
-        char *lm = lds, *ld = lm + WAVE * R::M, *lh = hreg(lds);
-        dma_wait();
-        cpx<T> m[9], d[12], h[6];
-        lds_read(m, lm + lane * R::M);
-        lds_read(d, ld + lane * R::D);
-        if (READ_R) lds_read(h, lh + lane * R::H);
-        cpx<T> tp[6];
-#pragma unroll
-        for (int ic = 0; ic < 3; ic++) spproj_row<MU, SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
-        mat_times_cols<EQOP, ADJ, 2>(h, m, tp);
-        wave_sync();
-        lds_write(lh + lane * R::H, h);
-        wave_sync();
-        if (!XF) store_tile<R::H>((char *)(r + 6 * (size_t)base), lh, ns, lane, al & AL_R);
-        else scatter_tile<R::H>(act ? (uint64_t)(r + 6 * (size_t)site) : 0, lh, ns, lane);
-        wave_sync();
-    }
-};
+constexpr int MULTI_MAXND = QLA_MULTI_MAXND;
+typedef qla_multi_args multi_args;
 
-// ---------------------------------------------------------------------------
-// D eqop sprecon_{MU,SIGN}(M * pH)
-// ---------------------------------------------------------------------------
-template <class T, int EQOP, bool ADJ, int MU, int SIGN, bool XF> struct sprecon_op {
+template <class T, int EQOP, bool ADJ, bool XF> struct multi_op {
     typedef rec<T> R;
-    typedef recon_tab<MU, SIGN> TAB;
-    // eq/eqm overwrite every component; peq/meq read the destination first
     static constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    static constexpr int BYTES = WAVE * (R::M + R::H + R::D);
-    static __device__ __forceinline__ void stage(char *lds, cpx<T> *r, const cpx<T> *a, const void *, uint64_t pb,
-                                                 int base, int ns, int lane, int site, bool act, int al)
+    static constexpr bool SUB = (EQOP == MEQ || EQOP == EQM);
+    static constexpr int OFF_B = WAVE * R::M;
+    static constexpr int OFF_R = WAVE * (R::M + R::V);
+    static constexpr int BYTES = WAVE * (R::M + R::V + (READ_R ? R::V : 0));
+
+    // gathered addresses of term d's records for one lane
+    static __device__ __forceinline__ void term_addresses(const multi_args &p, int d, int site, bool act,
+                                                          uint64_t &ga, uint64_t &gb)
     {
-        char *lm = lds, *lh = lm + WAVE * R::M, *ld = lh + WAVE * R::H;
-        if (!XF) {
-            load_tile<R::M>(lm, (const char *)(a + 9 * (size_t)base), ns, lane, al & AL_A);
-            if (READ_R) load_tile<R::D>(ld, (const char *)(r + 12 * (size_t)base), ns, lane, al & AL_R);
-        } else {
-            gather_tile<R::M>(lm, act ? (uint64_t)(a + 9 * (size_t)site) : 0, ns, lane);
-            if (READ_R) gather_tile<R::D>(ld, act ? (uint64_t)(r + 12 * (size_t)site) : 0, ns, lane);
-        }
-        gather_tile<R::H>(lh, pb, ns, lane);
+        ga = 0;
+        gb = 0;
+        if (!act) return;
+        if (p.pa) ga = (uint64_t)((const void *const *)p.a[d])[site];
+        else if (XF) ga = (uint64_t)((const char *)p.a[d] + (size_t)site * R::M);
+        if (p.pb) gb = (uint64_t)((const void *const *)p.b[d])[site];
+        else if (XF) gb = (uint64_t)((const char *)p.b[d] + (size_t)site * R::V);
     }
-    static __device__ __forceinline__ void finish(char *lds, cpx<T> *r, int base, int ns, int lane, int site, bool act,
-                                                  int al)
+
+    static __device__ __forceinline__ void stage_term(char *lds, const multi_args &p, int d, uint64_t ga,
+                                                      uint64_t gb, int base, int ns, int lane)
     {
-        char *lm = lds, *lh = lm + WAVE * R::M, *ld = lh + WAVE * R::H;
-        dma_wait();
-        cpx<T> m[9], h[6], d[12];
-        lds_read(m, lm + lane * R::M);
-        lds_read(h, lh + lane * R::H);
-        if (READ_R) lds_read(d, ld + lane * R::D);
-        cpx<T> tv[6];
-        mat_times_cols<EQ, ADJ, 2>(tv, m, h);
+        if (!XF && !p.pa) load_tile<R::M>(lds, (const char *)p.a[d] + (size_t)base * R::M, ns, lane, p.al & AL_A);
+        else gather_tile<R::M>(lds, ga, ns, lane);
+        if (!XF && !p.pb) load_tile<R::V>(lds + OFF_B, (const char *)p.b[d] + (size_t)base * R::V, ns, lane, p.al & AL_B);
+        else gather_tile<R::V>(lds + OFF_B, gb, ns, lane);
+    }
+
+    static __device__ __forceinline__ void accumulate(cpx<T> (&x)[3], const char *lds, int lane)
+    {
+        cpx<T> m[9], v[3];
+        lds_read(m, lds + lane * R::M);
+        lds_read(v, lds + OFF_B + lane * R::V);
 #pragma unroll
-        for (int ic = 0; ic < 3; ic++) {
-            recon_apply<EQOP, TAB::op[0]>(d[4 * ic + 0], tv[2 * ic + TAB::src[0]]);
-            recon_apply<EQOP, TAB::op[1]>(d[4 * ic + 1], tv[2 * ic + TAB::src[1]]);
-            recon_apply<EQOP, TAB::op[2]>(d[4 * ic + 2], tv[2 * ic + TAB::src[2]]);
-            recon_apply<EQOP, TAB::op[3]>(d[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
-        }
-        wave_sync();
-        lds_write(ld + lane * R::D, d);
-        wave_sync();
-        if (!XF) store_tile<R::D>((char *)(r + 12 * (size_t)base), ld, ns, lane, al & AL_R);
-        else scatter_tile<R::D>(act ? (uint64_t)(r + 12 * (size_t)site) : 0, ld, ns, lane);
-        wave_sync();
+        for (int ic = 0; ic < 3; ic++)
+#pragma unroll
+            for (int kc = 0; kc < 3; kc++)
+                acc_mul<SUB, ADJ>(x[ic], ADJ ? m[kc * 3 + ic] : m[ic * 3 + kc], v[kc]);
     }
 };
 

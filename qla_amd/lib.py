@@ -16,20 +16,17 @@ import os
 import re
 from pathlib import Path
 
+from .abi_gen import variants
+
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libqla_mi355x.so"
-HEADER = HERE.parent / "include" / "qla_mi355x.h"
+HEADERS = (HERE.parent / "include" / "qla_mi355x.h", HERE.parent / "include" / "qla_mi355x_sites.h")
 
 vp = C.c_void_p
 i32 = C.c_int
 
 # argument kinds of each exported symbol: 'p' pointer, 'i' int, 'r' void return
 _SIGS: dict[str, tuple[type, list]] = {
-    # a1/a2
-    "QLA_F3_V_veq_M_times_V": (None, [vp, vp, vp, i32]),
-    "QLA_D3_V_veq_M_times_V": (None, [vp, vp, vp, i32]),
-    "QLA_F3_V_vpeq_M_times_pV": (None, [vp, vp, vp, i32]),
-    "QLA_D3_V_vpeq_M_times_pV": (None, [vp, vp, vp, i32]),
     # a3/a4
     "QLA_F3_r_veq_norm2_V": (None, [vp, vp, i32]),
     "QLA_D3_r_veq_norm2_V": (None, [vp, vp, i32]),
@@ -40,16 +37,6 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_F3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
     "QLA_D3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
     "QLA_DF3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
-    # a5/a6
-    "QLA_F3_H_veq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, i32]),
-    "QLA_F3_H_xeq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, vp, i32]),
-    "QLA_D3_H_veq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, i32]),
-    "QLA_D3_H_xeq_spproj_M_times_pD": (None, [vp, vp, vp, i32, i32, vp, i32]),
-    # a7/a8
-    "QLA_F3_D_vpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, i32]),
-    "QLA_F3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
-    "QLA_D3_D_vpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, i32]),
-    "QLA_D3_D_xpeq_sprecon_M_times_pH": (None, [vp, vp, vp, i32, i32, vp, i32]),
     # a9-a12
     **{f"QLA_{p}3_M_veq_{fn}_M": (None, [vp, vp, i32]) for p in ("F", "D") for fn in ("exp", "sqrt", "invsqrt")},
     **{f"QLA_FN_M_veq_{fn}_M": (None, [i32, vp, vp, i32]) for fn in ("exp", "sqrt", "invsqrt")},
@@ -83,6 +70,8 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_version": (C.c_char_p, []),
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
+# a1/a2, a5-a8 and the rest of the Wilson-hop family (qla_mi355x_sites.h)
+_SIGS.update({v.name: (None, [vp if k == "p" else i32 for k in v.ctypes_kinds()]) for v in variants()})
 
 class F3Job(C.Structure):
     """QLA_MI355X_F3_job (include/qla_mi355x.h)."""
@@ -94,11 +83,13 @@ _lib: C.CDLL | None = None
 
 
 def header_symbols() -> list[str]:
-    """Names of every function declared in include/qla_mi355x.h."""
-    text = HEADER.read_text()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    """Names of every function declared in the public headers."""
     decl = r"^\s*(?:const\s+)?(?:void|int|char|float|double|QLA_MI355X_batch)\s*\**\s*(QLA_\w+)\s*\("
-    return sorted(set(re.findall(decl, text, flags=re.M)))
+    out = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        out.update(re.findall(decl, text, flags=re.M))
+    return sorted(out)
 
 
 def load() -> C.CDLL:

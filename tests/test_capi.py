@@ -29,3 +29,28 @@ def test_dynamic_symbol_table():
 def test_version_and_error_state():
     assert "gfx950" in qla_amd.version()
     assert qla_amd.load().QLA_MI355X_last_error() == 0
+
+
+def test_generated_site_abi_is_current():
+    """include/qla_mi355x_sites.h and qla_sites_abi.inc match the variant table."""
+    from qla_amd import abi_gen
+    assert abi_gen.current()
+    names = [v.name for v in abi_gen.variants()]
+    assert len(names) == len(set(names)) == 640
+    # QLA names of the reference's own callers and tests
+    for s in ("QLA_F3_D_vpeq_spproj_M_times_pD", "QLA_D3_V_vpeq_nM_times_npV", "QLA_F3_H_veq_spproj_Ma_times_pD",
+              "QLA_F3_D_xmeq_sprecon_pMa_times_pH", "QLA_D3_V_xeqm_npMa_times_nV"):
+        assert s in names, s
+
+
+def test_header_compiles_as_c_and_cxx(tmp_path):
+    inc = qlib.HEADERS[0].parent
+    src = tmp_path / "use.c"
+    src.write_text('#include "qla_mi355x.h"\n'
+                   'void f(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *u, QLA_F3_DiracFermion **p, int n)\n'
+                   '{ QLA_F3_D_vpeq_spproj_M_times_pD(r, u, p, 0, 1, n); }\n')
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
+        args = [cc, std, "-fsyntax-only", "-Wall", "-Werror", "-I", str(inc)]
+        if cc == "g++":
+            args += ["-x", "c++"]
+        subprocess.run(args + [str(src)], check=True)

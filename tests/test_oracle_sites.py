@@ -1,0 +1,186 @@
+"""Pins the oracle's site operations in every eqop, adjoint and operand form.
+
+Each case compares the oracle (orc_*_site / orc_*_multi) with a long-double
+evaluation built independently from numpy and the gamma basis of the
+reference documentation (doc/qla.tex:1063-1113, typed in gamma_basis.py):
+
+  kind 0   V  eqop A v
+  kind 1   H  eqop A (P psi)            P: rows of (1 + s gamma_mu)
+  kind 2   D  eqop R (A h)              R: (1 + s gamma_mu) = R P
+  kind 3   D  eqop A (1 + s gamma_mu) psi   (mu = 0..3; (1 + s gamma_5)/2 for mu = 4)
+  multi    V  eqop sum_d A_d v_d
+
+with A = U or U^dagger and eqop eq (r = y), peq (r0 + y), meq (r0 - y),
+eqm (-y), per component within a rounding bound that a dropped term, a
+wrong sign, a transposed or unconjugated matrix, or a wrong eqop exceeds by
+orders of magnitude.  Pointer operands (pM, pV/pD/pH) are exercised through
+permuted record lists, the x-form through a parity subset.
+"""
+import numpy as np
+import pytest
+
+from gamma_basis import G, G5
+from qla_amd import lattice
+from test_oracle_pin import U32, U64, cplx, proj_recon_mats
+
+EQOPS = ("eq", "peq", "meq", "eqm")
+DEST = {0: "V", 1: "H", 2: "D", 3: "D"}
+SRC2 = {0: "V", 1: "D", 2: "H", 3: "D"}
+L = np.clongdouble
+
+
+def apply_eqop(eqop, r0, y):
+    return {0: y, 1: r0 + y, 2: r0 - y, 3: -y}[eqop]
+
+
+def exact_site(kind, A, b, mu, sign):
+    """(y, |y| bound) in long double; A (n,3,3), b records (n, 3[, s])."""
+    if kind == 0:
+        return np.einsum("nij,nj->ni", A, b), np.einsum("nij,nj->ni", np.abs(A), np.abs(b))
+    P, R = proj_recon_mats(mu, sign)
+    if kind == 1:
+        t = np.einsum("sk,nck->ncs", P.astype(L), b)
+        tb = np.einsum("sk,nck->ncs", np.abs(P), np.abs(b))
+        return np.einsum("nab,nbs->nas", A, t), np.einsum("nab,nbs->nas", np.abs(A), tb)
+    if kind == 2:
+        w = np.einsum("nab,nbs->nas", A, b)
+        wb = np.einsum("nab,nbs->nas", np.abs(A), np.abs(b))
+        return np.einsum("ks,nas->nak", R.astype(L), w), np.einsum("ks,nas->nak", np.abs(R), wb)
+    # kind 3: the full spin matrix (1 + s gamma), independent of the P/R split
+    g = G[mu] if mu < 4 else G5
+    S = (np.eye(4) + sign * g) if mu < 4 else 0.5 * (np.eye(4) + sign * g)
+    sp = np.einsum("kl,ncl->nck", S.astype(L), b)
+    spb = np.einsum("kl,ncl->nck", np.abs(S), np.abs(b))
+    return np.einsum("nab,nbk->nak", A, sp), 2 * np.einsum("nab,nbk->nak", np.abs(A), spb)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("adj", [False, True])
+def test_site_vs_exact(orc, prec, kind, eqop, adj):
+    n = 160
+    dt = np.float32 if prec == "F" else np.float64
+    u_ = U32 if prec == "F" else U64
+    rng = np.random.default_rng(100 * kind + 10 * eqop + adj)
+    U = lattice.random_su3(n, 31 + kind).astype(dt)
+    b = lattice.random_field(SRC2[kind], n, 32 + kind).astype(dt)
+    r0 = lattice.random_field(DEST[kind], n, 33 + eqop).astype(dt)
+    a_idx = rng.permutation(n)
+    b_idx = rng.permutation(n)
+    for ms in range(10 if kind else 1):
+        mu, sign = ms % 5, (1 if ms < 5 else -1)
+        for pa in (False, True):
+            r = r0.copy()
+            orc.site(kind, eqop, r, U, b, mu, sign, adj=adj, a_idx=a_idx if pa else None, b_idx=b_idx)
+            Uc = cplx(U)[a_idx] if pa else cplx(U)
+            A = np.conj(np.swapaxes(Uc, 1, 2)) if adj else Uc
+            y, yb = exact_site(kind, A, cplx(b)[b_idx], mu, sign)
+            exact = apply_eqop(eqop, cplx(r0), y)
+            bound = 8 * u_ * (np.abs(cplx(r0)) + yb) + 1e-300
+            err = np.abs(cplx(r) - exact)
+            assert (err <= bound).all(), (EQOPS[eqop], mu, sign, pa, float(err.max()))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_site_bad_mu_and_x_form(orc, prec, kind):
+    dims = (4, 4, 2, 2)
+    n = lattice.volume(dims)
+    dt = np.float32 if prec == "F" else np.float64
+    U = lattice.random_su3(n, 3).astype(dt)
+    b = lattice.random_field(SRC2[kind], n, 4).astype(dt)
+    r0 = lattice.random_field(DEST[kind], n, 5).astype(dt)
+    fw, _ = lattice.neighbours(dims)
+    for mu in (-1, 5):
+        r = r0.copy()
+        orc.site(kind, 0, r, U, b, mu, 1, b_idx=fw[0])
+        np.testing.assert_array_equal(r, r0)
+    even, odd = lattice.parity_sites(dims, 0), lattice.parity_sites(dims, 1)
+    for eqop in range(4):
+        r = r0.copy()
+        orc.site(kind, eqop, r, U, b, 2, -1, index=even, b_idx=fw[1])
+        np.testing.assert_array_equal(r[odd], r0[odd])
+        full = r0.copy()
+        orc.site(kind, eqop, full, U, b, 2, -1, b_idx=fw[1])
+        np.testing.assert_array_equal(r[even], full[even])
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_wilson_is_recon_of_projected(orc, prec):
+    """kind 3 = kind 2 applied to the projection of B, bit for bit: the
+    projection into a temporary by eq (make_code_binary.pl:285-341)."""
+    n = 200
+    dt = np.float32 if prec == "F" else np.float64
+    U = lattice.random_su3(n, 41).astype(dt)
+    d = lattice.random_field("D", n, 42).astype(dt)
+    r0 = lattice.random_field("D", n, 43).astype(dt)
+    eye = np.zeros((n, 3, 3, 2), dtype=dt)
+    eye[:, [0, 1, 2], [0, 1, 2], 0] = 1
+    idx = np.random.default_rng(4).permutation(n)
+    for ms in range(10):
+        mu, sign = ms % 5, (1 if ms < 5 else -1)
+        h = np.zeros((n, 3, 2, 2), dtype=dt)
+        orc.site(1, 0, h, eye, d, mu, sign, b_idx=idx)          # h = 1 * spproj(d[idx]) exactly
+        for eqop in range(4):
+            for adj in (False, True):
+                w = r0.copy()
+                orc.site(3, eqop, w, U, d, mu, sign, adj=adj, b_idx=idx)
+                v = r0.copy()
+                orc.site(2, eqop, v, U, h, mu, sign, adj=adj)
+                np.testing.assert_array_equal(w, v)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("nd", [1, 4, 17])
+def test_multi_vs_exact(orc, prec, eqop, adj, nd):
+    n = 120
+    dt = np.float32 if prec == "F" else np.float64
+    u_ = U32 if prec == "F" else U64
+    rng = np.random.default_rng(nd * 10 + eqop)
+    As = [lattice.random_su3(n, 50 + d).astype(dt) for d in range(nd)]
+    Vs = [lattice.random_field("V", n, 80 + d).astype(dt) for d in range(nd)]
+    a_idx = [rng.permutation(n) for _ in range(nd)]
+    b_idx = [rng.permutation(n) for _ in range(nd)]
+    r0 = lattice.random_field("V", n, 7).astype(dt)
+    for pa, pb in ((False, False), (True, True), (False, True)):
+        r = r0.copy()
+        orc.multi(eqop, r, As, Vs, adj=adj, a_idx=a_idx if pa else None, b_idx=b_idx if pb else None)
+        y = np.zeros((n, 3), dtype=L)
+        yb = np.zeros((n, 3), dtype=L)
+        for d in range(nd):
+            Uc = cplx(As[d])[a_idx[d]] if pa else cplx(As[d])
+            A = np.conj(np.swapaxes(Uc, 1, 2)) if adj else Uc
+            v = cplx(Vs[d])[b_idx[d]] if pb else cplx(Vs[d])
+            y += np.einsum("nij,nj->ni", A, v)
+            yb += np.einsum("nij,nj->ni", np.abs(A), np.abs(v))
+        exact = apply_eqop(eqop, cplx(r0), y)
+        bound = 4 * (3 * nd + 2) * u_ * (np.abs(cplx(r0)) + yb)
+        assert (np.abs(cplx(r) - exact) <= bound).all()
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+def test_multi_is_chained_matvec(orc, prec, eqop):
+    """Term d = 0 with the call's eqop, then peq (eq/peq) or meq (meq/eqm),
+    r stored between terms (expressions_tensor.pl:983-1002): bit for bit."""
+    n, nd = 150, 5
+    dt = np.float32 if prec == "F" else np.float64
+    As = [lattice.random_su3(n, 60 + d).astype(dt) for d in range(nd)]
+    Vs = [lattice.random_field("V", n, 70 + d).astype(dt) for d in range(nd)]
+    r0 = lattice.random_field("V", n, 8).astype(dt)
+    idx = np.arange(1, n, 2)
+    for adj in (False, True):
+        for index in (None, idx):
+            got = r0.copy()
+            orc.multi(eqop, got, As, Vs, adj=adj, index=index)
+            exp = r0.copy()
+            later = 2 if eqop in (2, 3) else 1
+            for d in range(nd):
+                orc.site(0, eqop if d == 0 else later, exp, As[d], Vs[d], adj=adj, index=index)
+            np.testing.assert_array_equal(got, exp)
+    nothing = r0.copy()
+    orc.multi(eqop, nothing, [], [])
+    np.testing.assert_array_equal(nothing, r0)
