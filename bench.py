@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="approximate CPU work for the oracle baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-extra", action="store_true",
+                   help="headline only (no companion measurements), e.g. for profiler runs")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     return p.parse_args()
@@ -119,7 +121,7 @@ def main():
     avg_ms = ms_total / calls
     achieved = BYTES_PER_SITE * n / (avg_ms * 1e-3) / 1e9
 
-    extra = variants(lib, stream, n, combos, nbr, sets, u64, d64, args, sites_per_s)
+    extra = None if args.no_extra else variants(lib, stream, n, combos, nbr, sets, u64, d64, args, sites_per_s)
 
     cpu = None
     if not args.no_cpu:
@@ -144,7 +146,7 @@ def main():
                    "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": "qla::k_spproj<float,...>", "avg_launch_ms": round(avg_ms, 5),
+                     "kernel": "qla::k_site<float, 1, 0, false, false, 2> (spproj, eq, v-form)", "avg_launch_ms": round(avg_ms, 5),
                      "alg_bytes_per_launch": BYTES_PER_SITE * n},
         "cpu_baseline": cpu,
         "extra": extra,
@@ -234,11 +236,78 @@ def variants(lib, stream, n, combos, nbr, sets, u64, d64, args, plain_sites_per_
         out[f"spproj_{name}_links"] = r
         out[f"sprecon_{name}_links"] = rate(_timed(stream, csprecon, steps, 2), 164, steps)
         del cs
+    out["wilson_family"] = wilson_family(lib, stream, n, combos, nbr, sets, args)
     out["xform_c1"] = xforms_config1(lib, stream, n, sets, args)
     out["c2_48x4"] = config2(lib, stream, args)
     out["batch_c3"] = batch_config3(lib, stream, args)
     out["matfn_c4"] = matfn_config4(lib, stream, args)
     out["c0_16x4"] = config0(lib, stream, args)
+    qla_amd.check()
+    return out
+
+
+def wilson_family(lib, stream, n, combos, nbr, sets, args):
+    """Other members of the Wilson-hop family on the same 32^4 sets (mu 0..3,
+    sign +-1 rotated over the three input sets): the project-multiply-
+    reconstruct D_vpeq_spproj_M_times_pD (FP32 and FP64), the adjoint and
+    pointer-link forms, and the multi-source V_vpeq_nM_times_npV with 4 and 8
+    terms.  Algorithmic bytes per site: each record read once, the
+    destination read and written for peq, 8 B per pointer."""
+    import torch
+    import qla_amd
+    out = {}
+    nc = len(combos)
+    steps = max(5, args.steps // 4)
+
+    def rate(ms, bytes_per_site, calls):
+        return {"B_per_site": bytes_per_site, "ms_per_call": round(ms / calls, 5),
+                "GBs": round(bytes_per_site * n * calls / (ms * 1e-3) / 1e9, 1),
+                "sites_per_s": round(n * calls / (ms * 1e-3), 1)}
+
+    dsts = [torch.zeros((n, 3, 4, 2), dtype=torch.float32, device="cuda") for _ in range(NSETS)]
+    hsrc = [torch.zeros((n, 3, 2, 2), dtype=torch.float32, device="cuda").normal_() for _ in range(NSETS)]
+    hptr = [{c: qla_amd.pointer_array(hsrc[k], nbr[c]) for c in combos} for k in range(NSETS)]
+    lptr = [{c: qla_amd.pointer_array(sets[k][0], nbr[c]) for c in combos} for k in range(NSETS)]
+
+    def run(fn):
+        def step(it):
+            for j, (mu, s) in enumerate(combos):
+                fn((it * nc + j) % NSETS, mu, s)
+        return _timed(stream, step, steps, 2)
+
+    f = lib.QLA_F3_D_vpeq_spproj_M_times_pD
+    ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), sets[k][0].data_ptr(), sets[k][3][(mu, s)].data_ptr(), mu, s, n))
+    out["D_vpeq_spproj_M_times_pD_f3"] = rate(ms, 72 + 8 + 96 + 96 + 96, steps * nc)
+    f = lib.QLA_F3_H_veq_spproj_Ma_times_pD
+    ms = run(lambda k, mu, s: f(sets[k][2].data_ptr(), sets[k][0].data_ptr(), sets[k][3][(mu, s)].data_ptr(), mu, s, n))
+    out["H_veq_spproj_Ma_times_pD_f3"] = rate(ms, 224, steps * nc)
+    f = lib.QLA_F3_D_vpeq_sprecon_pMa_times_pH
+    ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), lptr[k][(mu, s)].data_ptr(), hptr[k][(mu, s)].data_ptr(), mu, s, n))
+    out["D_vpeq_sprecon_pMa_times_pH_f3"] = rate(ms, 8 + 72 + 8 + 48 + 96 + 96, steps * nc)
+    del dsts, hsrc, hptr, lptr
+
+    # multi-source: nd link fields times nd gathered vectors, accumulated
+    import ctypes
+    for nd in (4, 8):
+        links = [torch.zeros((n, 3, 3, 2), dtype=torch.float32, device="cuda").normal_() for _ in range(nd)]
+        vecs = [torch.zeros((n, 3, 2), dtype=torch.float32, device="cuda").normal_() for _ in range(nd)]
+        vptr = [qla_amd.pointer_array(vecs[d], nbr[combos[d % nc]]) for d in range(nd)]
+        r = torch.zeros((n, 3, 2), dtype=torch.float32, device="cuda")
+        al = (ctypes.c_uint64 * nd)(*[t.data_ptr() for t in links])
+        bl = (ctypes.c_uint64 * nd)(*[t.data_ptr() for t in vptr])
+        f = lib.QLA_F3_V_vpeq_nM_times_npV
+        ms = _timed(stream, lambda it: f(r.data_ptr(), ctypes.addressof(al), ctypes.addressof(bl), n, nd), steps, 2)
+        out[f"V_vpeq_nM_times_npV_nd{nd}_f3"] = rate(ms, 48 + nd * (72 + 8 + 24), steps)
+        del links, vecs, vptr, r
+
+    # FP64 project-multiply-reconstruct (one input set)
+    u64 = sets[0][0].double()
+    d64 = sets[0][1].double()
+    r64 = torch.zeros((n, 3, 4, 2), dtype=torch.float64, device="cuda")
+    p64 = {c: qla_amd.pointer_array(d64, nbr[c]) for c in combos}
+    f = lib.QLA_D3_D_vpeq_spproj_M_times_pD
+    ms = run(lambda k, mu, s: f(r64.data_ptr(), u64.data_ptr(), p64[(mu, s)].data_ptr(), mu, s, n))
+    out["D_vpeq_spproj_M_times_pD_d3"] = rate(ms, 144 + 8 + 192 * 3, steps * nc)
     qla_amd.check()
     return out
 
