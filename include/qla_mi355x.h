@@ -48,10 +48,12 @@ typedef QLA_F_Complex QLA_F3_ColorVector[3];
 typedef QLA_F_Complex QLA_F3_HalfFermion[3][2];
 typedef QLA_F_Complex QLA_F3_DiracFermion[3][4];
 typedef QLA_F_Complex QLA_F3_ColorMatrix[3][3];
+typedef QLA_F_Complex QLA_F3_DiracPropagator[3][4][3][4];
 typedef QLA_D_Complex QLA_D3_ColorVector[3];
 typedef QLA_D_Complex QLA_D3_HalfFermion[3][2];
 typedef QLA_D_Complex QLA_D3_DiracFermion[3][4];
 typedef QLA_D_Complex QLA_D3_ColorMatrix[3][3];
+typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 #endif
 
 /* ------------------------------------------------------------------------ */
@@ -75,29 +77,10 @@ typedef QLA_D_Complex QLA_D3_ColorMatrix[3][3];
 /* make_code_binary.pl:110-170 (dot: sum += conj(a)*b with operands         */
 /* promoted first, QLA_c_peq_ca_times_c, qla_complex.h:386-389).            */
 /* ------------------------------------------------------------------------ */
-void QLA_F3_r_veq_norm2_V(QLA_F_Real *QLA_MI355X_RESTRICT r,
-                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a, int n);
-void QLA_D3_r_veq_norm2_V(QLA_D_Real *QLA_MI355X_RESTRICT r,
-                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT a, int n);
-void QLA_F3_r_veq_norm2_D(QLA_F_Real *QLA_MI355X_RESTRICT r,
-                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
-void QLA_D3_r_veq_norm2_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
-                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
-/* Mixed precision (lib qla_df3; build_qla.pl:2119-2156, 2273-2359). */
-void QLA_DF3_r_veq_norm2_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
-                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a, int n);
-void QLA_F3_c_veq_V_dot_V(QLA_F_Complex *QLA_MI355X_RESTRICT r,
-                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
-                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT b, int n);
-void QLA_F3_c_veq_D_dot_D(QLA_F_Complex *QLA_MI355X_RESTRICT r,
-                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
-                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
-void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
-                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
-                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
-void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
-                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
-                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b, int n);
+/* Every type (V, H, D, M, P), F3 / D3 / DF3, v- and x-forms, field or     */
+/* pointer operands, plus the real part of the inner product and the DF3 /  */
+/* FD3 precision conversions (build_qla.pl:530-585, 2089-2360).             */
+#include "qla_mi355x_fields.h"
 
 /* ------------------------------------------------------------------------ */
 /* a9-a12: batched matrix functions.  Reference: generated wrappers         */

@@ -52,6 +52,8 @@ def load():
         getattr(lib, f"orc_{p}_dot_ld").argtypes = [vp, vp, vp, vp, i32, i32]
         for fn in ("exp", "sqrt", "invsqrt"):
             getattr(lib, f"orc_{p}_M_veq_{fn}_M").argtypes = [i32, vp, vp, i32]
+    lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
+    lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
     lib.orc_D_sprecon.argtypes = [vp, vp, i32, i32]
     lib.orc_set_threads.argtypes = [i32]
@@ -160,6 +162,29 @@ def D_eqop_sprecon_M_times_pH(eqop, r, a, h, h_idx, mu, sign, index=None, adj=Fa
 def D_eqop_spproj_M_times_pD(eqop, r, a, d, d_idx, mu, sign, index=None, adj=False):
     n = None if index is not None else len(d_idx)
     return site(WILSON, eqop, r, a, d, mu, sign, index=index, adj=adj, b_idx=d_idx, n=n)
+
+
+def records(a: np.ndarray, index=None, a_idx=None) -> np.ndarray:
+    """The records a QLA reduction or conversion reads, in site order i:
+    a[s] (field) or a[a_idx[s]] (pointer array to records a_idx), s = index[i]
+    (x-form) or i."""
+    s = np.arange(a.shape[0]) if index is None else np.asarray(index)
+    if a_idx is not None:
+        s = np.asarray(a_idx)[s]
+    return np.ascontiguousarray(a[s])
+
+
+def convert(a: np.ndarray, dest: np.ndarray, index=None, a_idx=None) -> np.ndarray:
+    """QLA_DF3 / QLA_FD3 T_{v,x}eq_[p]T: dest[s] = (precision of dest) a-record
+    of site s, s = index[i] (x-form) or i (orc_convert_DF / orc_convert_FD)."""
+    lib = load()
+    src = records(a, index, a_idx)
+    conv = np.empty(src.shape, dtype=dest.dtype)
+    f = lib.orc_convert_DF if dest.dtype == np.float64 else lib.orc_convert_FD
+    f(_ptr(conv), _ptr(src), src.size)
+    s = np.arange(a.shape[0]) if index is None else np.asarray(index)
+    dest[s] = conv
+    return dest
 
 
 def norm2(a: np.ndarray) -> float:

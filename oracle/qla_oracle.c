@@ -55,6 +55,17 @@ void orc_F_dot_ld(long double *re, long double *im, const float *a, const float 
 void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n)
 { D_orc_dot_ld(re, im, a, b, ncomp, n); }
 
+/* precision conversion (build_qla.pl:530-585: the "identity" unary op with
+ * a cast between the precisions): C's float -> double and double -> float */
+void orc_convert_DF(double *r, const float *a, size_t m)
+{
+    for (size_t i = 0; i < m; i++) r[i] = (double)a[i];
+}
+void orc_convert_FD(float *r, const double *a, size_t m)
+{
+    for (size_t i = 0; i < m; i++) r[i] = (float)a[i];
+}
+
 void orc_D_spproj(double *h, const double *d, int mu, int sign)
 {
     D_cpx *H = (D_cpx *)h;

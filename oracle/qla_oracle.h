@@ -16,6 +16,7 @@
  */
 #ifndef QLA_ORACLE_H
 #define QLA_ORACLE_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -56,6 +57,11 @@ long double orc_F_norm2_ld(const float *a, int ncomp, int n);
 long double orc_D_norm2_ld(const double *a, int ncomp, int n);
 void orc_F_dot_ld(long double *re, long double *im, const float *a, const float *b, int ncomp, int n);
 void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n);
+
+/* precision conversion of m reals: float -> double, double -> float
+ * (build_qla.pl:530-585, the C conversions) */
+void orc_convert_DF(double *r, const float *a, size_t m);
+void orc_convert_FD(float *r, const double *a, size_t m);
 
 /* single-site spin projection / reconstruction (eq), used to pin the tables */
 void orc_D_spproj(double *h, const double *d, int mu, int sign);

@@ -17,11 +17,26 @@ project-multiply-reconstruct   D_{v,x}<eqop>_spproj_<A>_times_{D,pD}         3
 multi-source matrix times vec  V_{v,x}<eqop>_n<A>_times_n{V,pV}  (+ int nd)  multi
 =============================  ==========================================  =======
 
-with <eqop> in eq, peq, meq, eqm.  It writes
+with <eqop> in eq, peq, meq, eqm; and the mixed-precision field layer
+(build_qla.pl:530-585 conversions, :2089-2360 global reductions) for the
+types V, H, D, M, P:
 
-* ``include/qla_mi355x_sites.h`` -- the prototypes, and
-* ``qla_amd/csrc/qla_sites_abi.inc`` -- their definitions, one call each into
-  the generic entry helpers of ``qla_capi.hip``.
+=============================  ==========================================  =======
+family                         symbol pattern                              libs
+=============================  ==========================================  =======
+global squared norm            r_{v,x}eq_norm2_{T,pT}                      F3 D3 DF3
+global inner product           c_{v,x}eq_[p]T_dot_[p]T                     F3 D3 DF3
+real part of inner product     r_{v,x}eq_re_[p]T_dot_[p]T                  F3 D3 DF3
+precision conversion           T_{v,x}eq_[p]T  (float -> double, back)     DF3 FD3
+=============================  ==========================================  =======
+
+It writes
+
+* ``include/qla_mi355x_sites.h`` and ``include/qla_mi355x_fields.h`` -- the
+  prototypes, and
+* ``qla_amd/csrc/qla_sites_abi.inc`` and ``qla_amd/csrc/qla_fields_abi.inc`` --
+  their definitions, one call each into the generic entry helpers of
+  ``qla_capi.hip``.
 
 Both files are committed; ``python -m qla_amd.abi_gen`` rewrites them and a CPU
 test checks that they match this table.
@@ -35,10 +50,16 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 HEADER_OUT = HERE.parent / "include" / "qla_mi355x_sites.h"
 INC_OUT = HERE / "csrc" / "qla_sites_abi.inc"
+FIELDS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_fields.h"
+FIELDS_INC_OUT = HERE / "csrc" / "qla_fields_abi.inc"
 
 PRECS = ("F3", "D3")
 EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use them
-TYPE_NAMES = {"V": "ColorVector", "H": "HalfFermion", "D": "DiracFermion", "M": "ColorMatrix"}
+TYPE_NAMES = {"V": "ColorVector", "H": "HalfFermion", "D": "DiracFermion", "M": "ColorMatrix",
+              "P": "DiracPropagator"}
+# complex numbers per record (include/qla_types.h:29-42)
+NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144}
+FIELD_TYPES = ("V", "H", "D", "M", "P")
 KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4}
 # (dest record, second-source record, qualifier in the name)
 FAMILY = {
@@ -102,6 +123,111 @@ class Variant:
 
     def ctypes_kinds(self) -> str:
         return "".join(k for k, _ in self.params())
+
+
+@dataclass(frozen=True)
+class RedVariant:
+    """A global reduction: norm2 of a, or conj(a).b summed (dot / re dot)."""
+    prec: str       # "F3", "D3" or "DF3" (float fields, double result)
+    kind: str       # "norm2", "dot", "redot"
+    t: str          # record type letter
+    xf: bool
+    pa: bool
+    pb: bool        # dot / redot only
+
+    @property
+    def name(self) -> str:
+        form = "x" if self.xf else "v"
+        a = ("p" if self.pa else "") + self.t
+        if self.kind == "norm2":
+            return f"QLA_{self.prec}_r_{form}eq_norm2_{a}"
+        b = ("p" if self.pb else "") + self.t
+        lead = "c" if self.kind == "dot" else "r"
+        qual = "re_" if self.kind == "redot" else ""
+        return f"QLA_{self.prec}_{lead}_{form}eq_{qual}{a}_dot_{b}"
+
+    def params(self) -> list[tuple[str, str]]:
+        rp = "F" if self.prec == "F3" else "D"
+        fp = "D3" if self.prec == "D3" else "F3"
+        rt = f"QLA_{rp}_Complex" if self.kind == "dot" else f"QLA_{rp}_Real"
+        ft = f"QLA_{fp}_{TYPE_NAMES[self.t]}"
+        out = [("p", f"{rt} *QLA_MI355X_RESTRICT r"),
+               ("p", f"{ft} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a")]
+        if self.kind != "norm2":
+            out.append(("p", f"{ft} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        # result kinds of the device combine: 0 double, 1 float, 2 double complex,
+        # 3 float complex, 4 double real part, 5 float real part
+        out = {("norm2", True): 1, ("norm2", False): 0, ("dot", True): 3, ("dot", False): 2,
+               ("redot", True): 5, ("redot", False): 4}[(self.kind, self.prec == "F3")]
+        nbytes = {0: 8, 1: 4, 2: 16, 3: 8, 4: 8, 5: 4}[out]
+        b = "(const void *)b" if self.kind != "norm2" else "nullptr"
+        idx = "index" if self.xf else "nullptr"
+        return (f"reduce_entry({prec}, {int(self.kind != 'norm2')}, {NCOMP[self.t]}, {int(self.xf)}, "
+                f"{int(self.pa)}, {int(self.pb)}, r, (const void *)a, {b}, {idx}, n, {out}, {nbytes}, "
+                f"\"{self.name}\");")
+
+
+@dataclass(frozen=True)
+class ConvVariant:
+    """Precision conversion of a field: DF3 float -> double, FD3 double -> float."""
+    lib: str        # "DF3" or "FD3"
+    t: str
+    xf: bool
+    pa: bool
+
+    @property
+    def name(self) -> str:
+        form = "x" if self.xf else "v"
+        return f"QLA_{self.lib}_{self.t}_{form}eq_{'p' if self.pa else ''}{self.t}"
+
+    def params(self) -> list[tuple[str, str]]:
+        dst, src = ("D3", "F3") if self.lib == "DF3" else ("F3", "D3")
+        out = [("p", f"QLA_{dst}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT r"),
+               ("p", f"QLA_{src}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a")]
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        idx = "index" if self.xf else "nullptr"
+        return (f"convert_entry({int(self.lib == 'FD3')}, {NCOMP[self.t]}, {int(self.xf)}, {int(self.pa)}, r, "
+                f"(const void *)a, {idx}, n, \"{self.name}\");")
+
+
+def reductions() -> list[RedVariant]:
+    out = []
+    for prec in ("F3", "D3", "DF3"):
+        for kind in ("norm2", "dot", "redot"):
+            for t in FIELD_TYPES:
+                for xf in (False, True):
+                    for pa in (False, True):
+                        for pb in ((False,) if kind == "norm2" else (False, True)):
+                            out.append(RedVariant(prec, kind, t, xf, pa, pb))
+    return out
+
+
+def conversions() -> list[ConvVariant]:
+    return [ConvVariant(lib, t, xf, pa) for lib in ("DF3", "FD3") for t in FIELD_TYPES
+            for xf in (False, True) for pa in (False, True)]
+
+
+def all_symbols() -> list:
+    """Every generated symbol (site family, reductions, conversions)."""
+    return variants() + reductions() + conversions()
 
 
 def variants() -> list[Variant]:
@@ -195,18 +321,78 @@ def inc_text() -> str:
     return "\n".join(lines)
 
 
+_FIELDS_DOC = {
+    "norm2": ("global squared norm, sum over sites and components of |a|^2",
+              "build_qla.pl:2089-2156, make_code_unary.pl:318-366; per component the norm is formed "
+              "in the field precision and added to a double sum"),
+    "dot": ("global inner product, sum of conj(a) b over sites and components",
+            "build_qla.pl:2196-2360, make_code_binary.pl:110-170; operands promoted to double"),
+    "redot": ("real part of the global inner product",
+              "build_qla.pl:2244-2260 (qualifier re), make_code_binary.pl:110-170"),
+    "conv": ("precision conversion of a field (DF3: float to double, exact; FD3: double to float, "
+             "rounded to nearest)", "build_qla.pl:530-585, make_code_unary.pl (identity with a cast)"),
+}
+
+
+def fields_header_text() -> str:
+    lines = [
+        "/*",
+        " * qla_mi355x_fields.h -- global reductions and precision conversions of the QLA",
+        " * C ABI on MI355X (types V, H, D, M, P).",
+        " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
+        " * semantics are documented there).",
+        " *",
+        " * Names and argument order are QLA's (lib/perl/prototype.pl:450-688).  A `p`",
+        " * operand is a pointer array; x-forms use the records at index[i].  The result",
+        " * pointer of a reduction may be a host or a device address; n <= 0 gives 0.",
+        " * QLA_DF3_* reductions take float fields and return double results; QLA_DF3_*",
+        " * / QLA_FD3_* conversions write double / float fields.",
+        " */",
+        "#ifndef QLA_MI355X_FIELDS_H",
+        "#define QLA_MI355X_FIELDS_H",
+    ]
+    last = None
+    for v in reductions() + conversions():
+        kind = v.kind if isinstance(v, RedVariant) else "conv"
+        lib = v.prec if isinstance(v, RedVariant) else v.lib
+        key = (kind, lib)
+        if key != last:
+            what, ref = _FIELDS_DOC[kind]
+            lines += ["", f"/* {lib}: {what}.", f" * Reference: {ref}. */"]
+            last = key
+        lines.append(_decl(v) + ";")
+    lines += ["", "#endif", ""]
+    return "\n".join(lines)
+
+
+def fields_inc_text() -> str:
+    lines = [
+        "// qla_fields_abi.inc -- definitions of the global reductions and precision conversions.",
+        "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        "// Included by qla_capi.hip inside extern \"C\"; reduce_entry / convert_entry are there.",
+    ]
+    for v in reductions() + conversions():
+        lines += [_decl(v), "{", "    " + v.body(), "}"]
+    lines.append("")
+    return "\n".join(lines)
+
+
+_OUTPUTS = ((HEADER_OUT, header_text), (INC_OUT, inc_text), (FIELDS_HEADER_OUT, fields_header_text),
+            (FIELDS_INC_OUT, fields_inc_text))
+
+
 def write() -> None:
-    HEADER_OUT.write_text(header_text())
-    INC_OUT.write_text(inc_text())
+    for path, text in _OUTPUTS:
+        path.write_text(text())
 
 
 def current() -> bool:
     """True when the committed outputs match the table."""
-    return (HEADER_OUT.exists() and HEADER_OUT.read_text() == header_text()
-            and INC_OUT.exists() and INC_OUT.read_text() == inc_text())
+    return all(path.exists() and path.read_text() == text() for path, text in _OUTPUTS)
 
 
 if __name__ == "__main__":
     write()
-    print(f"wrote {HEADER_OUT} and {INC_OUT}: {len(variants())} symbols")
+    print(f"wrote {', '.join(str(p) for p, _ in _OUTPUTS)}: {len(all_symbols())} symbols")
     sys.exit(0)

@@ -75,21 +75,50 @@ bool is_device_pointer(const void *p)
     return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
 }
 
-// run a reduction and deliver `bytes` of result to r (host or device)
-void reduce(int prec, int kind, const void *a, const void *b, long long m, void *r, int out_kind,
-            size_t bytes, const char *name)
+// run a reduction and deliver `bytes` of result to r (host or device);
+// launch(out, scratch) enqueues the kernels that write the result to out
+template <class L>
+void reduce_to(L launch, void *r, size_t bytes, const char *name)
 {
     double *sc = scratch_for_device();
     if (!sc) return;
     if (is_device_pointer(r)) {
-        finish(qla_launch_reduce2(prec, kind, a, b, m, r, out_kind, sc, ts.stream), name);
+        finish(launch(r, sc), name);
         return;
     }
     double *slot = sc + qla_reduce_scratch_doubles();
-    hipError_t e = qla_launch_reduce2(prec, kind, a, b, m, slot, out_kind, sc, ts.stream);
+    hipError_t e = launch((void *)slot, sc);
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
     record(e, name);
+}
+
+// One QLA global reduction (qla_fields_abi.inc): kind 0 norm2, 1 dot (the
+// real-part forms take out_kind 4 / 5); ncomp complexes per record.  The
+// v-form over fields streams the arrays as one run of n*ncomp complexes;
+// x-forms and pointer operands gather records.
+void reduce_entry(int prec, int kind, int ncomp, int xf, int pa, int pb, void *r, const void *a, const void *b,
+                  const int *index, int n, int out_kind, size_t bytes, const char *name)
+{
+    if (!xf && !pa && !pb) {
+        const long long m = n > 0 ? (long long)ncomp * n : 0;
+        reduce_to([&](void *out, double *sc) {
+            return qla_launch_reduce2(prec, kind, a, b, m, out, out_kind, sc, ts.stream);
+        }, r, bytes, name);
+        return;
+    }
+    reduce_to([&](void *out, double *sc) {
+        return qla_launch_reduce_gather(prec, kind, ncomp, a, b, xf ? index : nullptr, n, pa, pb, out, out_kind, sc,
+                                        ts.stream);
+    }, r, bytes, name);
+}
+
+// One QLA precision conversion: to_float 0 DF3 (float -> double), 1 FD3.
+void convert_entry(int to_float, int ncomp, int xf, int pa, void *r, const void *a, const int *index, int n,
+                   const char *name)
+{
+    if (n <= 0) return;
+    finish(qla_launch_precision(to_float, ncomp, r, a, xf ? index : nullptr, n, pa, ts.stream), name);
 }
 
 // compressed-field reductions: result doubles to r (host or device)
@@ -280,43 +309,8 @@ void QLA_MI355X_set_tiles_per_wave(int tpw)
 // ------------------------------------------- a1/a2, a5-a8: the Wilson-hop family
 #include "qla_sites_abi.inc"
 
-// ----------------------------------------------------------------- a3 / a4
-void QLA_F3_r_veq_norm2_V(QLA_F_Real *r, QLA_F3_ColorVector *a, int n)
-{
-    reduce(0, 0, a, nullptr, n > 0 ? 3LL * n : 0, r, 1, sizeof(float), "QLA_F3_r_veq_norm2_V");
-}
-void QLA_D3_r_veq_norm2_V(QLA_D_Real *r, QLA_D3_ColorVector *a, int n)
-{
-    reduce(1, 0, a, nullptr, n > 0 ? 3LL * n : 0, r, 0, sizeof(double), "QLA_D3_r_veq_norm2_V");
-}
-void QLA_F3_r_veq_norm2_D(QLA_F_Real *r, QLA_F3_DiracFermion *a, int n)
-{
-    reduce(0, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 1, sizeof(float), "QLA_F3_r_veq_norm2_D");
-}
-void QLA_D3_r_veq_norm2_D(QLA_D_Real *r, QLA_D3_DiracFermion *a, int n)
-{
-    reduce(1, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 0, sizeof(double), "QLA_D3_r_veq_norm2_D");
-}
-void QLA_DF3_r_veq_norm2_D(QLA_D_Real *r, QLA_F3_DiracFermion *a, int n)
-{
-    reduce(0, 0, a, nullptr, n > 0 ? 12LL * n : 0, r, 0, sizeof(double), "QLA_DF3_r_veq_norm2_D");
-}
-void QLA_F3_c_veq_V_dot_V(QLA_F_Complex *r, QLA_F3_ColorVector *a, QLA_F3_ColorVector *b, int n)
-{
-    reduce(0, 1, a, b, n > 0 ? 3LL * n : 0, r, 3, 2 * sizeof(float), "QLA_F3_c_veq_V_dot_V");
-}
-void QLA_F3_c_veq_D_dot_D(QLA_F_Complex *r, QLA_F3_DiracFermion *a, QLA_F3_DiracFermion *b, int n)
-{
-    reduce(0, 1, a, b, n > 0 ? 12LL * n : 0, r, 3, 2 * sizeof(float), "QLA_F3_c_veq_D_dot_D");
-}
-void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_D3_DiracFermion *a, QLA_D3_DiracFermion *b, int n)
-{
-    reduce(1, 1, a, b, n > 0 ? 12LL * n : 0, r, 2, 2 * sizeof(double), "QLA_D3_c_veq_D_dot_D");
-}
-void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *r, QLA_F3_DiracFermion *a, QLA_F3_DiracFermion *b, int n)
-{
-    reduce(0, 1, a, b, n > 0 ? 12LL * n : 0, r, 2, 2 * sizeof(double), "QLA_DF3_c_veq_D_dot_D");
-}
+// ------------------------------- a3/a4 and the rest: reductions, conversions
+#include "qla_fields_abi.inc"
 
 // ----------------------------------------------------------------- a9 - a12
 #define QLA_MATFN(NAME, PREC, FN, NC)                                                  \

@@ -42,6 +42,19 @@ int qla_reduce_scratch_doubles(void);
 hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, long long m, void *out,
                               int out_kind, double *scratch, hipStream_t st);
 
+// The same reductions over gathered records: record of site i at *a[s]
+// (pa) or a + s*ncomp complexes, s = index[i] (index != NULL) or i; ncomp one
+// of 3, 6, 9, 12, 144.  out_kind 4 / 5: real part of a dot as double / float.
+hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a, const void *b, const int *index,
+                                    int n, int pa, int pb, void *out, int out_kind, double *scratch, hipStream_t st);
+
+// Precision conversion of n records of ncomp complexes: to_float 0 float ->
+// double, 1 double -> float (round to nearest).  Destination record of site
+// i at index[i] (x-form) or i; source record at the same site, or *a[site]
+// when pa.
+hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, int n, int pa,
+                                hipStream_t st);
+
 // Matrix functions on n nc x nc matrices.  fn: 0 exp, 1 sqrt, 2 invsqrt.
 // prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned int
 // the launch uses as its work counter (reset on the stream first).

@@ -151,8 +151,61 @@ k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long lo
     }
 }
 
+// stage 1 over gathered records: the record of site i is *pa[s] (pointer
+// array) or a[s] (field), s = index[i] (x-form) or i, NC complexes each.
+// The lanes of a wave walk the 64 records of a tile one complex at a time
+// (coalesced where the records are adjacent); the shuffles that hand each
+// lane its record's address run with the whole wave active.
+template <class T, bool DOT, int NC>
+__global__ void __launch_bounds__(RBLOCK)
+k_reduce1_gather(const void *__restrict__ a, const void *__restrict__ b, const int *__restrict__ index, int n,
+                 int pa, int pb, double *__restrict__ part)
+{
+    __shared__ double sh[RBLOCK / WAVE];
+    double sr = 0, si = 0;
+    const int lane = threadIdx.x % WAVE;
+    const int nwaves = gridDim.x * (RBLOCK / WAVE);
+    const int ntiles = (n + WAVE - 1) / WAVE;
+    for (int tile = blockIdx.x * (RBLOCK / WAVE) + (int)(threadIdx.x / WAVE); tile < ntiles; tile += nwaves) {
+        const int i = tile * WAVE + lane;
+        const int ns = min(WAVE, n - tile * WAVE);
+        uint64_t ga = 0, gb = 0;
+        if (i < n) {
+            const int s = index ? index[i] : i;
+            ga = pa ? (uint64_t)((const void *const *)a)[s] : (uint64_t)((const cpx<T> *)a + (size_t)s * NC);
+            if (DOT) gb = pb ? (uint64_t)((const void *const *)b)[s] : (uint64_t)((const cpx<T> *)b + (size_t)s * NC);
+        }
+        const int tot = ns * NC;
+        const int iters = (tot + WAVE - 1) / WAVE;
+        for (int k = 0; k < iters; k++) {
+            const int c = k * WAVE + lane;
+            const int sl = c / NC;
+            const int comp = c - sl * NC;
+            const int src = sl < WAVE ? sl : 0;
+            const uint64_t xa = shfl64(ga, src);
+            const uint64_t xb = DOT ? shfl64(gb, src) : 0;
+            if (c < tot) {
+                const cpx<T> x = ((const cpx<T> *)xa)[comp];
+                if (DOT) {
+                    const cpx<T> y = ((const cpx<T> *)xb)[comp];
+                    acc_dot(sr, si, x.re, x.im, y.re, y.im);
+                } else {
+                    acc_norm2(sr, x.re, x.im);
+                }
+            }
+        }
+    }
+    double tr = block_sum(sr, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = tr;
+    if (DOT) {
+        double ti = block_sum(si, sh);
+        if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = ti;
+    }
+}
+
 // stage 2: fixed-order combine of g partials; out_kind: 0 double, 1 float,
-// 2 double complex, 3 float complex
+// 2 double complex, 3 float complex, 4 double real part of a dot, 5 float
+// real part of a dot
 __global__ void __launch_bounds__(RBLOCK)
 k_reduce2(const double *__restrict__ part, int g, int dot, void *out, int out_kind)
 {
@@ -170,6 +223,8 @@ k_reduce2(const double *__restrict__ part, int g, int dot, void *out, int out_ki
         case 1: ((float *)out)[0] = (float)tr; break;
         case 2: ((double *)out)[0] = tr; ((double *)out)[1] = ti; break;
         case 3: ((float *)out)[0] = (float)tr; ((float *)out)[1] = (float)ti; break;
+        case 4: ((double *)out)[0] = tr; break;
+        case 5: ((float *)out)[0] = (float)tr; break;
         }
     }
 }
@@ -231,5 +286,48 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(RBLOCK), 0, st, (const double *)scratch, g, (int)dot, out,
                        out_kind);
+    return hipGetLastError();
+}
+
+// Gathered records (x-forms and pointer operands): ncomp complexes per
+// record, one of 3, 6, 9, 12, 144 (V, H, M, D, P).
+template <class T, bool DOT>
+static hipError_t launch_gather_t(int ncomp, const void *a, const void *b, const int *index, int n, int pa, int pb,
+                                  int g, double *scratch, hipStream_t st)
+{
+#define QLA_GATHER_CASE(NC)                                                                                     \
+    case NC:                                                                                                   \
+        hipLaunchKernelGGL((k_reduce1_gather<T, DOT, NC>), dim3(g), dim3(RBLOCK), 0, st, a, b, index, n, pa, pb, \
+                           scratch);                                                                           \
+        return hipGetLastError();
+    switch (ncomp) {
+        QLA_GATHER_CASE(3)
+        QLA_GATHER_CASE(6)
+        QLA_GATHER_CASE(9)
+        QLA_GATHER_CASE(12)
+        QLA_GATHER_CASE(144)
+    }
+#undef QLA_GATHER_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a, const void *b, const int *index,
+                                    int n, int pa, int pb, void *out, int out_kind, double *scratch, hipStream_t st)
+{
+    const bool dot = (kind == 1);
+    int g = 0;
+    if (n > 0) {
+        // about two tiles of 64 records per wave
+        const long long ntiles = ((long long)n + WAVE - 1) / WAVE;
+        const long long want = (ntiles + 2 * (RBLOCK / WAVE) - 1) / (2 * (RBLOCK / WAVE));
+        g = (int)(want < 1 ? 1 : (want > RMAXGRID ? RMAXGRID : want));
+        hipError_t e;
+        if (prec == 0) e = dot ? launch_gather_t<float, true>(ncomp, a, b, index, n, pa, pb, g, scratch, st)
+                               : launch_gather_t<float, false>(ncomp, a, b, index, n, pa, pb, g, scratch, st);
+        else e = dot ? launch_gather_t<double, true>(ncomp, a, b, index, n, pa, pb, g, scratch, st)
+                     : launch_gather_t<double, false>(ncomp, a, b, index, n, pa, pb, g, scratch, st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(RBLOCK), 0, st, (const double *)scratch, g, (int)dot, out, out_kind);
     return hipGetLastError();
 }

@@ -109,8 +109,10 @@ def random_su3(n: int, seed: int, dtype=np.float64, chunk: int = 1 << 20) -> np.
 
 
 def random_field(kind: str, n: int, seed: int, dtype=np.float64) -> np.ndarray:
-    """Complex Gaussian(0,1) field of record kind 'V', 'H' or 'D'."""
-    shape = {"V": (n, 3, 2), "H": (n, 3, 2, 2), "D": (n, 3, 4, 2)}[kind]
+    """Complex Gaussian(0,1) field of record kind 'V', 'H', 'D', 'M' or 'P'
+    (ColorVector, HalfFermion, DiracFermion, ColorMatrix, DiracPropagator)."""
+    shape = {"V": (n, 3, 2), "H": (n, 3, 2, 2), "D": (n, 3, 4, 2), "M": (n, 3, 3, 2),
+             "P": (n, 3, 4, 3, 4, 2)}[kind]
     return gaussian(shape, seed, dtype=dtype)
 
 

@@ -16,27 +16,17 @@ import os
 import re
 from pathlib import Path
 
-from .abi_gen import variants
+from .abi_gen import all_symbols
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libqla_mi355x.so"
-HEADERS = (HERE.parent / "include" / "qla_mi355x.h", HERE.parent / "include" / "qla_mi355x_sites.h")
+HEADERS = tuple(HERE.parent / "include" / h for h in ("qla_mi355x.h", "qla_mi355x_sites.h", "qla_mi355x_fields.h"))
 
 vp = C.c_void_p
 i32 = C.c_int
 
 # argument kinds of each exported symbol: 'p' pointer, 'i' int, 'r' void return
 _SIGS: dict[str, tuple[type, list]] = {
-    # a3/a4
-    "QLA_F3_r_veq_norm2_V": (None, [vp, vp, i32]),
-    "QLA_D3_r_veq_norm2_V": (None, [vp, vp, i32]),
-    "QLA_F3_r_veq_norm2_D": (None, [vp, vp, i32]),
-    "QLA_D3_r_veq_norm2_D": (None, [vp, vp, i32]),
-    "QLA_DF3_r_veq_norm2_D": (None, [vp, vp, i32]),
-    "QLA_F3_c_veq_V_dot_V": (None, [vp, vp, vp, i32]),
-    "QLA_F3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
-    "QLA_D3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
-    "QLA_DF3_c_veq_D_dot_D": (None, [vp, vp, vp, i32]),
     # a9-a12
     **{f"QLA_{p}3_M_veq_{fn}_M": (None, [vp, vp, i32]) for p in ("F", "D") for fn in ("exp", "sqrt", "invsqrt")},
     **{f"QLA_FN_M_veq_{fn}_M": (None, [i32, vp, vp, i32]) for fn in ("exp", "sqrt", "invsqrt")},
@@ -70,8 +60,10 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_version": (C.c_char_p, []),
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
-# a1/a2, a5-a8 and the rest of the Wilson-hop family (qla_mi355x_sites.h)
-_SIGS.update({v.name: (None, [vp if k == "p" else i32 for k in v.ctypes_kinds()]) for v in variants()})
+# the generated families: a1/a2, a5-a8 and the rest of the Wilson-hop family
+# (qla_mi355x_sites.h), a3/a4 and the rest of the reductions, conversions
+# (qla_mi355x_fields.h)
+_SIGS.update({v.name: (None, [vp if k == "p" else i32 for k in v.ctypes_kinds()]) for v in all_symbols()})
 
 class F3Job(C.Structure):
     """QLA_MI355X_F3_job (include/qla_mi355x.h)."""
