@@ -404,7 +404,44 @@ def config2(lib, stream, args):
         ms = _timed(stream, fn, reps, 2) / reps
         out[name] = {"ms": round(ms, 4), "GBs": round(bps * n / (ms * 1e-3) / 1e9, 1),
                      "sites_per_s": round(n / (ms * 1e-3), 1)}
+    del link, drec, drec2, hrec, dptr
+    out["fields"] = fields_layer(lib, stream, n, d, d2, fw, args)
     torch.cuda.synchronize()
+    return out
+
+
+def fields_layer(lib, stream, n, d, d2, fw, args):
+    """The mixed-precision layer on the 48^4 D fields: DF3/FD3 conversions
+    (96 + 192 B/site), the x-form norm2 over the even sites (96 + 4 B/site),
+    the DF3 dot through a neighbour pointer array (96 + 8 + 96 B/site) and the
+    DiracPropagator norm2 on a 2^20-site field (1152 B/site)."""
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    out = {}
+    reps = 10
+    d64 = torch.empty((n, 3, 4, 2), dtype=torch.float64, device="cuda")
+    back = torch.empty_like(d)
+    res = torch.empty(4, dtype=torch.float64, device="cuda")
+    resf = torch.empty(4, dtype=torch.float32, device="cuda")
+    even = torch.from_numpy(lattice.parity_sites((48, 48, 48, 48), 0)).cuda()
+    ptr = qla_amd.pointer_array(d2, fw[0])
+    p_n = 1 << 20
+    prop = torch.zeros((p_n, 3, 4, 3, 4, 2), dtype=torch.float32, device="cuda").normal_()
+    cases = {
+        "DF3_D_veq_D": (lambda it: lib.QLA_DF3_D_veq_D(d64.data_ptr(), d.data_ptr(), n), 288, n),
+        "FD3_D_veq_D": (lambda it: lib.QLA_FD3_D_veq_D(back.data_ptr(), d64.data_ptr(), n), 288, n),
+        "F3_r_xeq_norm2_D_even": (lambda it: lib.QLA_F3_r_xeq_norm2_D(resf.data_ptr(), d.data_ptr(), even.data_ptr(),
+                                                                      even.numel()), 100, even.numel()),
+        "DF3_c_veq_D_dot_pD": (lambda it: lib.QLA_DF3_c_veq_D_dot_pD(res.data_ptr(), d.data_ptr(), ptr.data_ptr(), n),
+                               200, n),
+        "F3_r_veq_norm2_P": (lambda it: lib.QLA_F3_r_veq_norm2_P(resf.data_ptr(), prop.data_ptr(), p_n), 1152, p_n),
+    }
+    for name, (fn, bps, sites) in cases.items():
+        ms = _timed(stream, fn, reps, 2) / reps
+        out[name] = {"B_per_site": bps, "ms": round(ms, 4), "GBs": round(bps * sites / (ms * 1e-3) / 1e9, 1),
+                     "sites_per_s": round(sites / (ms * 1e-3), 1)}
+    qla_amd.check()
     return out
 
 
