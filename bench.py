@@ -553,8 +553,8 @@ def cpu_baseline(seconds: float):
     t0 = time.perf_counter()
     while True:
         for mu, s in combos:
-            lib.orc_F3_site(oracle.SPPROJ, oracle.EQ, 0, h.ctypes.data, pa.ctypes.data, ptrs[(mu, s)].ctypes.data,
-                            mu, s, None, n)
+            lib.orc_F3_site(oracle.SPPROJ, oracle.EQ, 0, h.ctypes.data, None, pa.ctypes.data,
+                            ptrs[(mu, s)].ctypes.data, mu, s, None, n)
             calls += 1
         if time.perf_counter() - t0 > seconds:
             break

@@ -125,6 +125,22 @@ void QLA_F3_c_xeq_pV_dot_pV(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                             QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_F3_c_eq_xV_dot_V(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_F3_c_eq_V_dot_xV(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                          QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_F3_c_eq_xV_dot_xV(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_F3_c_veq_H_dot_H(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -161,6 +177,22 @@ void QLA_F3_c_xeq_pH_dot_pH(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                             QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_F3_c_eq_xH_dot_H(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_F3_c_eq_H_dot_xH(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                          QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_F3_c_eq_xH_dot_xH(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_F3_c_veq_D_dot_D(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -197,6 +229,22 @@ void QLA_F3_c_xeq_pD_dot_pD(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                             QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_F3_c_eq_xD_dot_D(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_F3_c_eq_D_dot_xD(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_F3_c_eq_xD_dot_xD(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_F3_c_veq_M_dot_M(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -233,6 +281,22 @@ void QLA_F3_c_xeq_pM_dot_pM(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                             QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_F3_c_eq_xM_dot_M(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_F3_c_eq_M_dot_xM(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                          QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_F3_c_eq_xM_dot_xM(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_F3_c_veq_P_dot_P(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -269,6 +333,22 @@ void QLA_F3_c_xeq_pP_dot_pP(QLA_F_Complex *QLA_MI355X_RESTRICT r,
                             QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_F3_c_eq_xP_dot_P(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_F3_c_eq_P_dot_xP(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                          QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_F3_c_eq_xP_dot_xP(QLA_F_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 
 /* F3: real part of the global inner product.
  * Reference: build_qla.pl:2244-2260 (qualifier re), make_code_binary.pl:110-170. */
@@ -308,6 +388,22 @@ void QLA_F3_r_xeq_re_pV_dot_pV(QLA_F_Real *QLA_MI355X_RESTRICT r,
                                QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_F3_r_eq_re_xV_dot_V(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_F3_r_eq_re_V_dot_xV(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                             QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_F3_r_eq_re_xV_dot_xV(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_F3_r_veq_re_H_dot_H(QLA_F_Real *QLA_MI355X_RESTRICT r,
                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -344,6 +440,22 @@ void QLA_F3_r_xeq_re_pH_dot_pH(QLA_F_Real *QLA_MI355X_RESTRICT r,
                                QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_F3_r_eq_re_xH_dot_H(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_F3_r_eq_re_H_dot_xH(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                             QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_F3_r_eq_re_xH_dot_xH(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_F3_r_veq_re_D_dot_D(QLA_F_Real *QLA_MI355X_RESTRICT r,
                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -380,6 +492,22 @@ void QLA_F3_r_xeq_re_pD_dot_pD(QLA_F_Real *QLA_MI355X_RESTRICT r,
                                QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_F3_r_eq_re_xD_dot_D(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_F3_r_eq_re_D_dot_xD(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                             QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_F3_r_eq_re_xD_dot_xD(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_F3_r_veq_re_M_dot_M(QLA_F_Real *QLA_MI355X_RESTRICT r,
                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -416,6 +544,22 @@ void QLA_F3_r_xeq_re_pM_dot_pM(QLA_F_Real *QLA_MI355X_RESTRICT r,
                                QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_F3_r_eq_re_xM_dot_M(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_F3_r_eq_re_M_dot_xM(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                             QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_F3_r_eq_re_xM_dot_xM(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_F3_r_veq_re_P_dot_P(QLA_F_Real *QLA_MI355X_RESTRICT r,
                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -452,6 +596,22 @@ void QLA_F3_r_xeq_re_pP_dot_pP(QLA_F_Real *QLA_MI355X_RESTRICT r,
                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_F3_r_eq_re_xP_dot_P(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_F3_r_eq_re_P_dot_xP(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                             QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                             QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_F3_r_eq_re_xP_dot_xP(QLA_F_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 
 /* D3: global squared norm, sum over sites and components of |a|^2.
  * Reference: build_qla.pl:2089-2156, make_code_unary.pl:318-366; per component the norm is formed in the field precision and added to a double sum. */
@@ -564,6 +724,22 @@ void QLA_D3_c_xeq_pV_dot_pV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                             QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_D3_c_eq_xV_dot_V(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_D3_c_eq_V_dot_xV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                          QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_D3_c_eq_xV_dot_xV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_D3_c_veq_H_dot_H(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                           QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
                           QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -600,6 +776,22 @@ void QLA_D3_c_xeq_pH_dot_pH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                             QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_D3_c_eq_xH_dot_H(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_D3_c_eq_H_dot_xH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                          QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_D3_c_eq_xH_dot_xH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_D3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                           QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
                           QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -636,6 +828,22 @@ void QLA_D3_c_xeq_pD_dot_pD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                             QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_D3_c_eq_xD_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_D3_c_eq_D_dot_xD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                          QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_D3_c_eq_xD_dot_xD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_D3_c_veq_M_dot_M(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                           QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
                           QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -672,6 +880,22 @@ void QLA_D3_c_xeq_pM_dot_pM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                             QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_D3_c_eq_xM_dot_M(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_D3_c_eq_M_dot_xM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                          QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_D3_c_eq_xM_dot_xM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 void QLA_D3_c_veq_P_dot_P(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                           QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
                           QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -708,6 +932,22 @@ void QLA_D3_c_xeq_pP_dot_pP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                             QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                             int *index,
                             int n);
+void QLA_D3_c_eq_xP_dot_P(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                          int *indexa,
+                          QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                          int n);
+void QLA_D3_c_eq_P_dot_xP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                          QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                          QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                          int *indexb,
+                          int n);
+void QLA_D3_c_eq_xP_dot_xP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
 
 /* D3: real part of the global inner product.
  * Reference: build_qla.pl:2244-2260 (qualifier re), make_code_binary.pl:110-170. */
@@ -747,6 +987,22 @@ void QLA_D3_r_xeq_re_pV_dot_pV(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                QLA_D3_ColorVector *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_D3_r_eq_re_xV_dot_V(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_D3_r_eq_re_V_dot_xV(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                             QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_D3_r_eq_re_xV_dot_xV(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_D3_ColorVector *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_D3_r_veq_re_H_dot_H(QLA_D_Real *QLA_MI355X_RESTRICT r,
                              QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
                              QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -783,6 +1039,22 @@ void QLA_D3_r_xeq_re_pH_dot_pH(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_D3_r_eq_re_xH_dot_H(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_D3_r_eq_re_H_dot_xH(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                             QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_D3_r_eq_re_xH_dot_xH(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_D3_HalfFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_D3_r_veq_re_D_dot_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
                              QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
                              QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -819,6 +1091,22 @@ void QLA_D3_r_xeq_re_pD_dot_pD(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_D3_r_eq_re_xD_dot_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_D3_r_eq_re_D_dot_xD(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                             QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_D3_r_eq_re_xD_dot_xD(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_D3_DiracFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_D3_r_veq_re_M_dot_M(QLA_D_Real *QLA_MI355X_RESTRICT r,
                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -855,6 +1143,22 @@ void QLA_D3_r_xeq_re_pM_dot_pM(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_D3_r_eq_re_xM_dot_M(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_D3_r_eq_re_M_dot_xM(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                             QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_D3_r_eq_re_xM_dot_xM(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 void QLA_D3_r_veq_re_P_dot_P(QLA_D_Real *QLA_MI355X_RESTRICT r,
                              QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
                              QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -891,6 +1195,22 @@ void QLA_D3_r_xeq_re_pP_dot_pP(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                                int *index,
                                int n);
+void QLA_D3_r_eq_re_xP_dot_P(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                             int *indexa,
+                             QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                             int n);
+void QLA_D3_r_eq_re_P_dot_xP(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                             QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                             QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                             int *indexb,
+                             int n);
+void QLA_D3_r_eq_re_xP_dot_xP(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
 
 /* DF3: global squared norm, sum over sites and components of |a|^2.
  * Reference: build_qla.pl:2089-2156, make_code_unary.pl:318-366; per component the norm is formed in the field precision and added to a double sum. */
@@ -1003,6 +1323,22 @@ void QLA_DF3_c_xeq_pV_dot_pV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b,
                              int *index,
                              int n);
+void QLA_DF3_c_eq_xV_dot_V(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                           int n);
+void QLA_DF3_c_eq_V_dot_xV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                           QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
+void QLA_DF3_c_eq_xV_dot_xV(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                            QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                            int *indexa,
+                            QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                            int *indexb,
+                            int n);
 void QLA_DF3_c_veq_H_dot_H(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                            QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
                            QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -1039,6 +1375,22 @@ void QLA_DF3_c_xeq_pH_dot_pH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
                              int *index,
                              int n);
+void QLA_DF3_c_eq_xH_dot_H(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                           int n);
+void QLA_DF3_c_eq_H_dot_xH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                           QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
+void QLA_DF3_c_eq_xH_dot_xH(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                            QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                            int *indexa,
+                            QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                            int *indexb,
+                            int n);
 void QLA_DF3_c_veq_D_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -1075,6 +1427,22 @@ void QLA_DF3_c_xeq_pD_dot_pD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
                              int *index,
                              int n);
+void QLA_DF3_c_eq_xD_dot_D(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                           int n);
+void QLA_DF3_c_eq_D_dot_xD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                           QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
+void QLA_DF3_c_eq_xD_dot_xD(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                            int *indexa,
+                            QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                            int *indexb,
+                            int n);
 void QLA_DF3_c_veq_M_dot_M(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -1111,6 +1479,22 @@ void QLA_DF3_c_xeq_pM_dot_pM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                              int *index,
                              int n);
+void QLA_DF3_c_eq_xM_dot_M(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                           int n);
+void QLA_DF3_c_eq_M_dot_xM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                           QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
+void QLA_DF3_c_eq_xM_dot_xM(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                            int *indexa,
+                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                            int *indexb,
+                            int n);
 void QLA_DF3_c_veq_P_dot_P(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                            QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
                            QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -1147,6 +1531,22 @@ void QLA_DF3_c_xeq_pP_dot_pP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                              int *index,
                              int n);
+void QLA_DF3_c_eq_xP_dot_P(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                           int *indexa,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                           int n);
+void QLA_DF3_c_eq_P_dot_xP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                           QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                           int *indexb,
+                           int n);
+void QLA_DF3_c_eq_xP_dot_xP(QLA_D_Complex *QLA_MI355X_RESTRICT r,
+                            QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                            int *indexa,
+                            QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                            int *indexb,
+                            int n);
 
 /* DF3: real part of the global inner product.
  * Reference: build_qla.pl:2244-2260 (qualifier re), make_code_binary.pl:110-170. */
@@ -1186,6 +1586,22 @@ void QLA_DF3_r_xeq_re_pV_dot_pV(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                 QLA_F3_ColorVector *QLA_MI355X_RESTRICT *b,
                                 int *index,
                                 int n);
+void QLA_DF3_r_eq_re_xV_dot_V(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                              int n);
+void QLA_DF3_r_eq_re_V_dot_xV(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                              QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
+void QLA_DF3_r_eq_re_xV_dot_xV(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                               QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                               int *indexa,
+                               QLA_F3_ColorVector *QLA_MI355X_RESTRICT b,
+                               int *indexb,
+                               int n);
 void QLA_DF3_r_veq_re_H_dot_H(QLA_D_Real *QLA_MI355X_RESTRICT r,
                               QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
                               QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
@@ -1222,6 +1638,22 @@ void QLA_DF3_r_xeq_re_pH_dot_pH(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                 QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *b,
                                 int *index,
                                 int n);
+void QLA_DF3_r_eq_re_xH_dot_H(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                              int n);
+void QLA_DF3_r_eq_re_H_dot_xH(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                              QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
+void QLA_DF3_r_eq_re_xH_dot_xH(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                               QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                               int *indexa,
+                               QLA_F3_HalfFermion *QLA_MI355X_RESTRICT b,
+                               int *indexb,
+                               int n);
 void QLA_DF3_r_veq_re_D_dot_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
                               QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
                               QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
@@ -1258,6 +1690,22 @@ void QLA_DF3_r_xeq_re_pD_dot_pD(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                 QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *b,
                                 int *index,
                                 int n);
+void QLA_DF3_r_eq_re_xD_dot_D(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                              int n);
+void QLA_DF3_r_eq_re_D_dot_xD(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                              QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
+void QLA_DF3_r_eq_re_xD_dot_xD(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                               QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                               int *indexa,
+                               QLA_F3_DiracFermion *QLA_MI355X_RESTRICT b,
+                               int *indexb,
+                               int n);
 void QLA_DF3_r_veq_re_M_dot_M(QLA_D_Real *QLA_MI355X_RESTRICT r,
                               QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                               QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
@@ -1294,6 +1742,22 @@ void QLA_DF3_r_xeq_re_pM_dot_pM(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                 QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT *b,
                                 int *index,
                                 int n);
+void QLA_DF3_r_eq_re_xM_dot_M(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                              int n);
+void QLA_DF3_r_eq_re_M_dot_xM(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                              QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
+void QLA_DF3_r_eq_re_xM_dot_xM(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                               QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                               int *indexa,
+                               QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT b,
+                               int *indexb,
+                               int n);
 void QLA_DF3_r_veq_re_P_dot_P(QLA_D_Real *QLA_MI355X_RESTRICT r,
                               QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
                               QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
@@ -1330,6 +1794,22 @@ void QLA_DF3_r_xeq_re_pP_dot_pP(QLA_D_Real *QLA_MI355X_RESTRICT r,
                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *b,
                                 int *index,
                                 int n);
+void QLA_DF3_r_eq_re_xP_dot_P(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                              int *indexa,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                              int n);
+void QLA_DF3_r_eq_re_P_dot_xP(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                              QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                              int *indexb,
+                              int n);
+void QLA_DF3_r_eq_re_xP_dot_xP(QLA_D_Real *QLA_MI355X_RESTRICT r,
+                               QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                               int *indexa,
+                               QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT b,
+                               int *indexb,
+                               int n);
 
 /* DF3: precision conversion of a field (DF3: float to double, exact; FD3: double to float, rounded to nearest).
  * Reference: build_qla.pl:530-585, make_code_unary.pl (identity with a cast). */
@@ -1347,6 +1827,19 @@ void QLA_DF3_V_xeq_pV(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
                       QLA_F3_ColorVector *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_DF3_xV_eq_V(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_DF3_V_eq_xV(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_DF3_xV_eq_xV(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_DF3_H_veq_H(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1360,6 +1853,19 @@ void QLA_DF3_H_xeq_H(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
 void QLA_DF3_H_xeq_pH(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
                       QLA_F3_HalfFermion *QLA_MI355X_RESTRICT *a,
                       int *index,
+                      int n);
+void QLA_DF3_xH_eq_H(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_DF3_H_eq_xH(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_DF3_xH_eq_xH(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                      int *indexa,
                       int n);
 void QLA_DF3_D_veq_D(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
                      QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
@@ -1375,6 +1881,19 @@ void QLA_DF3_D_xeq_pD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
                       QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_DF3_xD_eq_D(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_DF3_D_eq_xD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_DF3_xD_eq_xD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_DF3_M_veq_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1389,6 +1908,19 @@ void QLA_DF3_M_xeq_pM(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
                       QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_DF3_xM_eq_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_DF3_M_eq_xM(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_DF3_xM_eq_xM(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_DF3_P_veq_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
                      QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1402,6 +1934,19 @@ void QLA_DF3_P_xeq_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
 void QLA_DF3_P_xeq_pP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
                       QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *a,
                       int *index,
+                      int n);
+void QLA_DF3_xP_eq_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_DF3_P_eq_xP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_DF3_xP_eq_xP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                      int *indexa,
                       int n);
 
 /* FD3: precision conversion of a field (DF3: float to double, exact; FD3: double to float, rounded to nearest).
@@ -1420,6 +1965,19 @@ void QLA_FD3_V_xeq_pV(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
                       QLA_D3_ColorVector *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_FD3_xV_eq_V(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_FD3_V_eq_xV(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_FD3_xV_eq_xV(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_FD3_H_veq_H(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1433,6 +1991,19 @@ void QLA_FD3_H_xeq_H(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
 void QLA_FD3_H_xeq_pH(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
                       QLA_D3_HalfFermion *QLA_MI355X_RESTRICT *a,
                       int *index,
+                      int n);
+void QLA_FD3_xH_eq_H(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_FD3_H_eq_xH(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_FD3_xH_eq_xH(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                      int *indexa,
                       int n);
 void QLA_FD3_D_veq_D(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
                      QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
@@ -1448,6 +2019,19 @@ void QLA_FD3_D_xeq_pD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
                       QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_FD3_xD_eq_D(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_FD3_D_eq_xD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_FD3_xD_eq_xD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_FD3_M_veq_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1462,6 +2046,19 @@ void QLA_FD3_M_xeq_pM(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
                       QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT *a,
                       int *index,
                       int n);
+void QLA_FD3_xM_eq_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_FD3_M_eq_xM(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_FD3_xM_eq_xM(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                      int *indexa,
+                      int n);
 void QLA_FD3_P_veq_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
                      QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
                      int n);
@@ -1475,6 +2072,19 @@ void QLA_FD3_P_xeq_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
 void QLA_FD3_P_xeq_pP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
                       QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *a,
                       int *index,
+                      int n);
+void QLA_FD3_xP_eq_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     int *indexr,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_FD3_P_eq_xP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *indexa,
+                     int n);
+void QLA_FD3_xP_eq_xP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                      int *indexr,
+                      QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                      int *indexa,
                       int n);
 
 #endif

@@ -41,7 +41,7 @@ def load():
     lib = C.CDLL(str(LIB_PATH))
     vp, i32 = C.c_void_p, C.c_int
     for pre in ("F3", "D3"):
-        getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, i32, i32, vp, i32]
+        getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32, i32, vp, i32]
         getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, i32, vp, i32]
     for p in ("F", "D"):
         f = getattr(lib, f"orc_{p}_norm2")
@@ -107,20 +107,27 @@ def _index(index):
     return None if index is None else np.ascontiguousarray(index, dtype=np.int32)
 
 
-def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b_idx=None, n=None):
+def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b_idx=None, n=None,
+         ir=None, ia=None, ib=None):
     """One QLA site call on host arrays (orc_*_site).
 
     kind 0 V eqop A*B, 1 H eqop A*spproj(B), 2 D eqop sprecon(A*B),
     3 D eqop sprecon(A*spproj(B)); A = a or a^dagger (adj).  a_idx / b_idx
     None: the operand is a field (record i, or index[i] in the x-form); else
     the operand is a pointer array whose entry s points at record a_idx[s].
-    n defaults to len(index) (x-form) or the number of destination records.
+    ir / ia / ib: per-operand index arrays of the QLA_INDEX_ALL forms (step i
+    uses r[ir[i]], a[ia[i]], b[ib[i]]); an indexed field operand is the
+    pointer array of the records it names.  n defaults to len(index)
+    (x-form) or the number of destination records.
     """
     lib = load()
-    pa, pb, ix = _records(a, a_idx), _records(b, b_idx), _index(index)
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
     if n is None:
-        n = len(ix) if ix is not None else r.shape[0]
-    getattr(lib, f"orc_{_prefix(a)}_site")(kind, eqop, int(adj), _ptr(_need(r)), _ptr(pa), _ptr(pb),
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    getattr(lib, f"orc_{_prefix(a)}_site")(kind, eqop, int(adj), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
                                            mu, sign, _ptr(ix), n)
     return r
 
@@ -164,25 +171,31 @@ def D_eqop_spproj_M_times_pD(eqop, r, a, d, d_idx, mu, sign, index=None, adj=Fal
     return site(WILSON, eqop, r, a, d, mu, sign, index=index, adj=adj, b_idx=d_idx, n=n)
 
 
-def records(a: np.ndarray, index=None, a_idx=None) -> np.ndarray:
-    """The records a QLA reduction or conversion reads, in site order i:
+def records(a: np.ndarray, index=None, a_idx=None, own=None) -> np.ndarray:
+    """The records a QLA reduction or conversion reads, in step order i:
     a[s] (field) or a[a_idx[s]] (pointer array to records a_idx), s = index[i]
-    (x-form) or i."""
+    (x-form) or i; with a per-operand index (QLA_INDEX_ALL forms) a[own[i]]."""
+    if own is not None:
+        return np.ascontiguousarray(a[np.asarray(own)])
     s = np.arange(a.shape[0]) if index is None else np.asarray(index)
     if a_idx is not None:
         s = np.asarray(a_idx)[s]
     return np.ascontiguousarray(a[s])
 
 
-def convert(a: np.ndarray, dest: np.ndarray, index=None, a_idx=None) -> np.ndarray:
-    """QLA_DF3 / QLA_FD3 T_{v,x}eq_[p]T: dest[s] = (precision of dest) a-record
-    of site s, s = index[i] (x-form) or i (orc_convert_DF / orc_convert_FD)."""
+def convert(a: np.ndarray, dest: np.ndarray, index=None, a_idx=None, ir=None, ia=None) -> np.ndarray:
+    """QLA_DF3 / QLA_FD3 T_{v,x}eq_[p]T: the destination record of step i
+    (r[ir[i]], r[index[i]] or r[i]) = the source record of step i converted
+    to the destination precision (orc_convert_DF / orc_convert_FD)."""
     lib = load()
-    src = records(a, index, a_idx)
+    src = records(a, index, a_idx, own=ia)
     conv = np.empty(src.shape, dtype=dest.dtype)
     f = lib.orc_convert_DF if dest.dtype == np.float64 else lib.orc_convert_FD
     f(_ptr(conv), _ptr(src), src.size)
-    s = np.arange(a.shape[0]) if index is None else np.asarray(index)
+    if ir is not None:
+        s = np.asarray(ir)
+    else:
+        s = np.arange(src.shape[0]) if index is None else np.asarray(index)
     dest[s] = conv
     return dest
 

@@ -29,12 +29,12 @@
 #undef REAL
 #undef P
 
-void orc_F3_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb,
-                 int mu, int sign, const int *index, int n)
-{ F_orc_3_site(kind, eqop, adj, r, pa, pb, mu, sign, index, n); }
-void orc_D3_site(int kind, int eqop, int adj, double *r, const double *const *pa, const double *const *pb,
-                 int mu, int sign, const int *index, int n)
-{ D_orc_3_site(kind, eqop, adj, r, pa, pb, mu, sign, index, n); }
+void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const float *const *pa,
+                 const float *const *pb, int mu, int sign, const int *index, int n)
+{ F_orc_3_site(kind, eqop, adj, r, rindex, pa, pb, mu, sign, index, n); }
+void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,
+                 const double *const *pb, int mu, int sign, const int *index, int n)
+{ D_orc_3_site(kind, eqop, adj, r, rindex, pa, pb, mu, sign, index, n); }
 void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,
                   int nd, const int *index, int n)
 { F_orc_3_multi(eqop, adj, r, pa, pb, nd, index, n); }

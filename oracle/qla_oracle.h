@@ -31,12 +31,14 @@ enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
  * kind 1: H eqop A*spproj(B) (make_code_binary.pl:189-222)
  * kind 2: D eqop sprecon(A*B) (make_code_binary.pl:224-257)
  * kind 3: D eqop sprecon(A*spproj(B)) (make_code_binary.pl:285-341)
- * adj selects A = a^dagger (QLA_c_peq_ca_times_c).  mu outside 0..4 is a
+ * adj selects A = a^dagger (QLA_c_peq_ca_times_c).  rindex != NULL puts the
+ * result of step i at r[rindex[i]] (the per-operand destination index of the
+ * QLA_INDEX_ALL forms, indirection_all.pl:66-84).  mu outside 0..4 is a
  * no-op for kinds 1-3. */
-void orc_F3_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb,
-                 int mu, int sign, const int *index, int n);
-void orc_D3_site(int kind, int eqop, int adj, double *r, const double *const *pa, const double *const *pb,
-                 int mu, int sign, const int *index, int n);
+void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const float *const *pa,
+                 const float *const *pb, int mu, int sign, const int *index, int n);
+void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,
+                 const double *const *pb, int mu, int sign, const int *index, int n);
 /* multi-source r eqop sum_{d<nd} A_d*B_d, A_d = *pa[d][s], B_d = *pb[d][s]
  * (build_qla.pl:1693-1725, expressions_tensor.pl:983-1002) */
 void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,

@@ -60,6 +60,8 @@ TYPE_NAMES = {"V": "ColorVector", "H": "HalfFermion", "D": "DiracFermion", "M": 
 # complex numbers per record (include/qla_types.h:29-42)
 NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144}
 FIELD_TYPES = ("V", "H", "D", "M", "P")
+# (dest, src1, src2) index flags of the per-operand index forms of a binary op
+OWN_INDEX_FORMS = [(r, a, b) for r in (False, True) for a in (False, True) for b in (False, True) if r or a or b]
 KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4}
 # (dest record, second-source record, qualifier in the name)
 FAMILY = {
@@ -80,6 +82,13 @@ class Variant:
     xf: bool        # gang-indexed x-form
     pa: bool        # A is a pointer array ("pM")
     pb: bool        # B is a pointer array ("pV", "pD", "pH")
+    ir: bool = False    # per-operand index forms of QLA_INDEX_ALL (indirection_all.pl:66-84):
+    ia: bool = False    # r[indexr[i]], a[indexa[i]], b[indexb[i]] ("xH", "xM", "xD")
+    ib: bool = False
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ia or self.ib
 
     @property
     def multi(self) -> bool:
@@ -100,18 +109,25 @@ class Variant:
     @property
     def name(self) -> str:
         n = "n" if self.multi else ""
-        a = f"{n}{'p' if self.pa else ''}M{'a' if self.adj else ''}"
-        b = f"{n}{'p' if self.pb else ''}{self.src2}"
-        form = "x" if self.xf else "v"
-        return f"QLA_{self.prec}_{self.dest}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{a}_times_{b}"
+        a = f"{n}{'p' if self.pa else ''}{'x' if self.ia else ''}M{'a' if self.adj else ''}"
+        b = f"{n}{'p' if self.pb else ''}{'x' if self.ib else ''}{self.src2}"
+        r = f"{'x' if self.ir else ''}{self.dest}"
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        return f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{a}_times_{b}"
 
     def params(self) -> list[tuple[str, str]]:
         """(kind, C declaration) of every parameter; kind 'p' pointer or 'i' int."""
         t = lambda s: f"QLA_{self.prec}_{TYPE_NAMES[s]}"                  # noqa: E731
         stars = lambda ptr: "*" * ((1 if self.multi else 0) + (1 if ptr else 0))  # noqa: E731
-        out = [("p", f"{t(self.dest)} *QLA_MI355X_RESTRICT r"),
-               ("p", f"{t('M')} *QLA_MI355X_RESTRICT {stars(self.pa)}a"),
-               ("p", f"{t(self.src2)} *QLA_MI355X_RESTRICT {stars(self.pb)}b")]
+        out = [("p", f"{t(self.dest)} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{t('M')} *QLA_MI355X_RESTRICT {stars(self.pa)}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out.append(("p", f"{t(self.src2)} *QLA_MI355X_RESTRICT {stars(self.pb)}b"))
+        if self.ib:
+            out.append(("p", "int *indexb"))
         if self.spin:
             out += [("i", "int mu"), ("i", "int sign")]
         if self.xf:
@@ -134,14 +150,16 @@ class RedVariant:
     xf: bool
     pa: bool
     pb: bool        # dot / redot only
+    ia: bool = False    # per-operand index forms of the binary reductions
+    ib: bool = False    # (QLA_INDEX_ALL): a[indexa[i]], b[indexb[i]]
 
     @property
     def name(self) -> str:
-        form = "x" if self.xf else "v"
-        a = ("p" if self.pa else "") + self.t
+        form = "" if (self.ia or self.ib) else ("x" if self.xf else "v")
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + self.t
         if self.kind == "norm2":
             return f"QLA_{self.prec}_r_{form}eq_norm2_{a}"
-        b = ("p" if self.pb else "") + self.t
+        b = ("p" if self.pb else "") + ("x" if self.ib else "") + self.t
         lead = "c" if self.kind == "dot" else "r"
         qual = "re_" if self.kind == "redot" else ""
         return f"QLA_{self.prec}_{lead}_{form}eq_{qual}{a}_dot_{b}"
@@ -153,8 +171,12 @@ class RedVariant:
         ft = f"QLA_{fp}_{TYPE_NAMES[self.t]}"
         out = [("p", f"{rt} *QLA_MI355X_RESTRICT r"),
                ("p", f"{ft} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a")]
+        if self.ia:
+            out.append(("p", "int *indexa"))
         if self.kind != "norm2":
             out.append(("p", f"{ft} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+        if self.ib:
+            out.append(("p", "int *indexb"))
         if self.xf:
             out.append(("p", "int *index"))
         out.append(("i", "int n"))
@@ -172,8 +194,10 @@ class RedVariant:
         nbytes = {0: 8, 1: 4, 2: 16, 3: 8, 4: 8, 5: 4}[out]
         b = "(const void *)b" if self.kind != "norm2" else "nullptr"
         idx = "index" if self.xf else "nullptr"
-        return (f"reduce_entry({prec}, {int(self.kind != 'norm2')}, {NCOMP[self.t]}, {int(self.xf)}, "
-                f"{int(self.pa)}, {int(self.pb)}, r, (const void *)a, {b}, {idx}, n, {out}, {nbytes}, "
+        ia = "indexa" if self.ia else "nullptr"
+        ib = "indexb" if self.ib else "nullptr"
+        return (f"reduce_entry({prec}, {int(self.kind != 'norm2')}, {NCOMP[self.t]}, {int(self.pa)}, "
+                f"{int(self.pb)}, r, (const void *)a, {b}, {idx}, {ia}, {ib}, n, {out}, {nbytes}, "
                 f"\"{self.name}\");")
 
 
@@ -184,16 +208,24 @@ class ConvVariant:
     t: str
     xf: bool
     pa: bool
+    ir: bool = False    # per-operand index forms (QLA_INDEX_ALL): r[indexr[i]],
+    ia: bool = False    # a[indexa[i]]
 
     @property
     def name(self) -> str:
-        form = "x" if self.xf else "v"
-        return f"QLA_{self.lib}_{self.t}_{form}eq_{'p' if self.pa else ''}{self.t}"
+        form = "" if (self.ir or self.ia) else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + self.t
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + self.t
+        return f"QLA_{self.lib}_{r}_{form}eq_{a}"
 
     def params(self) -> list[tuple[str, str]]:
         dst, src = ("D3", "F3") if self.lib == "DF3" else ("F3", "D3")
-        out = [("p", f"QLA_{dst}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT r"),
-               ("p", f"QLA_{src}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a")]
+        out = [("p", f"QLA_{dst}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"QLA_{src}_{TYPE_NAMES[self.t]} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
         if self.xf:
             out.append(("p", "int *index"))
         out.append(("i", "int n"))
@@ -204,8 +236,10 @@ class ConvVariant:
 
     def body(self) -> str:
         idx = "index" if self.xf else "nullptr"
-        return (f"convert_entry({int(self.lib == 'FD3')}, {NCOMP[self.t]}, {int(self.xf)}, {int(self.pa)}, r, "
-                f"(const void *)a, {idx}, n, \"{self.name}\");")
+        ir = "indexr" if self.ir else "nullptr"
+        ia = "indexa" if self.ia else "nullptr"
+        return (f"convert_entry({int(self.lib == 'FD3')}, {NCOMP[self.t]}, {int(self.pa)}, r, (const void *)a, "
+                f"{idx}, {ir}, {ia}, n, \"{self.name}\");")
 
 
 def reductions() -> list[RedVariant]:
@@ -217,12 +251,19 @@ def reductions() -> list[RedVariant]:
                     for pa in (False, True):
                         for pb in ((False,) if kind == "norm2" else (False, True)):
                             out.append(RedVariant(prec, kind, t, xf, pa, pb))
+                if kind != "norm2":
+                    for ia, ib in ((True, False), (False, True), (True, True)):
+                        out.append(RedVariant(prec, kind, t, False, False, False, ia, ib))
     return out
 
 
 def conversions() -> list[ConvVariant]:
-    return [ConvVariant(lib, t, xf, pa) for lib in ("DF3", "FD3") for t in FIELD_TYPES
-            for xf in (False, True) for pa in (False, True)]
+    out = []
+    for lib in ("DF3", "FD3"):
+        for t in FIELD_TYPES:
+            out += [ConvVariant(lib, t, xf, pa) for xf in (False, True) for pa in (False, True)]
+            out += [ConvVariant(lib, t, False, False, ir, ia) for ir, ia in ((True, False), (False, True), (True, True))]
+    return out
 
 
 def all_symbols() -> list:
@@ -240,6 +281,13 @@ def variants() -> list[Variant]:
                         for pa in (False, True):
                             for pb in (False, True):
                                 out.append(Variant(prec, kind, eqop, adj, xf, pa, pb))
+            if kind == "multi":
+                continue
+            # per-operand index forms (QLA_INDEX_ALL): fields only, no gang index
+            for ir, ia, ib in OWN_INDEX_FORMS:
+                for eqop in range(4):
+                    for adj in (False, True):
+                        out.append(Variant(prec, kind, eqop, adj, False, False, False, ir, ia, ib))
     return out
 
 
@@ -312,8 +360,9 @@ def inc_text() -> str:
                     f"(const void *)a, (const void *)b, {idx}, n, nd, \"{v.name}\");")
         else:
             mu, sign = ("mu", "sign") if v.spin else ("0", "1")
-            body = (f"site_entry({prec}, {KINDS[v.kind]}, {v.eqop}, {int(v.adj)}, {int(v.xf)}, {int(v.pa)}, "
-                    f"{int(v.pb)}, r, (const void *)a, (const void *)b, {mu}, {sign}, {idx}, n, \"{v.name}\");")
+            own = ", ".join(x if f else "nullptr" for x, f in (("indexr", v.ir), ("indexa", v.ia), ("indexb", v.ib)))
+            body = (f"site_entry({prec}, {KINDS[v.kind]}, {v.eqop}, {int(v.adj)}, {int(v.pa)}, {int(v.pb)}, r, "
+                    f"(const void *)a, (const void *)b, {mu}, {sign}, {idx}, {own}, n, \"{v.name}\");")
         lines.append("{")
         lines.append("    " + body)
         lines.append("}")

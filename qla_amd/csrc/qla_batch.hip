@@ -34,6 +34,7 @@ __device__ __forceinline__ site_args job_args(const batch_job &j)
     p.a = j.a;
     p.b = (const void *)j.b;
     p.index = nullptr;
+    p.ir = p.ia = p.ib = nullptr;
     p.n = j.n;
     p.al = j.al;
     p.ms = j.ms;

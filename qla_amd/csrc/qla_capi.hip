@@ -96,11 +96,12 @@ void reduce_to(L launch, void *r, size_t bytes, const char *name)
 // One QLA global reduction (qla_fields_abi.inc): kind 0 norm2, 1 dot (the
 // real-part forms take out_kind 4 / 5); ncomp complexes per record.  The
 // v-form over fields streams the arrays as one run of n*ncomp complexes;
-// x-forms and pointer operands gather records.
-void reduce_entry(int prec, int kind, int ncomp, int xf, int pa, int pb, void *r, const void *a, const void *b,
-                  const int *index, int n, int out_kind, size_t bytes, const char *name)
+// x-forms, per-operand index forms and pointer operands gather records.
+void reduce_entry(int prec, int kind, int ncomp, int pa, int pb, void *r, const void *a, const void *b,
+                  const int *index, const int *ia, const int *ib, int n, int out_kind, size_t bytes,
+                  const char *name)
 {
-    if (!xf && !pa && !pb) {
+    if (!index && !ia && !ib && !pa && !pb) {
         const long long m = n > 0 ? (long long)ncomp * n : 0;
         reduce_to([&](void *out, double *sc) {
             return qla_launch_reduce2(prec, kind, a, b, m, out, out_kind, sc, ts.stream);
@@ -108,17 +109,17 @@ void reduce_entry(int prec, int kind, int ncomp, int xf, int pa, int pb, void *r
         return;
     }
     reduce_to([&](void *out, double *sc) {
-        return qla_launch_reduce_gather(prec, kind, ncomp, a, b, xf ? index : nullptr, n, pa, pb, out, out_kind, sc,
+        return qla_launch_reduce_gather(prec, kind, ncomp, a, b, index, ia, ib, n, pa, pb, out, out_kind, sc,
                                         ts.stream);
     }, r, bytes, name);
 }
 
 // One QLA precision conversion: to_float 0 DF3 (float -> double), 1 FD3.
-void convert_entry(int to_float, int ncomp, int xf, int pa, void *r, const void *a, const int *index, int n,
-                   const char *name)
+void convert_entry(int to_float, int ncomp, int pa, void *r, const void *a, const int *index, const int *ir,
+                   const int *ia, int n, const char *name)
 {
     if (n <= 0) return;
-    finish(qla_launch_precision(to_float, ncomp, r, a, xf ? index : nullptr, n, pa, ts.stream), name);
+    finish(qla_launch_precision(to_float, ncomp, r, a, index, ir, ia, n, pa, ts.stream), name);
 }
 
 // compressed-field reductions: result doubles to r (host or device)
@@ -145,9 +146,11 @@ std::atomic<int> g_tiles_per_wave{ 2 };
 
 // One QLA site call (qla_sites_abi.inc): the reference's no-op cases, then
 // one launch.  kind 0 matvec, 1 spproj, 2 sprecon, 3 project-multiply-
-// reconstruct; mu/sign only for kinds 1-3.
-void site_entry(int prec, int kind, int eqop, int adj, int xf, int pa, int pb, void *r, const void *a,
-                const void *b, int mu, int sign, const int *index, int n, const char *name)
+// reconstruct; mu/sign only for kinds 1-3.  index is the gang index of an
+// x-form, ir / ia / ib the per-operand indices of the QLA_INDEX_ALL forms.
+void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, const void *a, const void *b,
+                int mu, int sign, const int *index, const int *ir, const int *ia, const int *ib, int n,
+                const char *name)
 {
     if (n <= 0) return;
     if (kind != 0 && (mu < 0 || mu > 4)) return;     // switch without default (expressions_gamma.pl:349-381)
@@ -155,13 +158,17 @@ void site_entry(int prec, int kind, int eqop, int adj, int xf, int pa, int pb, v
     p.r = r;
     p.a = a;
     p.b = b;
-    p.index = xf ? index : nullptr;
+    p.index = index;
+    p.ir = ir;
+    p.ia = ia;
+    p.ib = ib;
     p.n = n;
     p.al = ((!pa && (uintptr_t)a % 16 == 0) ? 1 : 0) | ((!pb && (uintptr_t)b % 16 == 0) ? 2 : 0) |
            (((uintptr_t)r % 16 == 0) ? 4 : 0);
     p.ms = kind == 0 ? 0 : mu + (sign == 1 ? 0 : 5);
     p.pa = pa;
     p.pb = pb;
+    const int xf = (index || ir || ia || ib) ? 1 : 0;
     finish(qla_launch_site(prec, kind, eqop, adj, xf, p, g_tiles_per_wave.load(), ts.stream), name);
 }
 

@@ -13,11 +13,15 @@ struct qla_site_args {
     const void *a;      // ColorMatrix field, or pointer array when pa
     const void *b;      // second source field, or pointer array when pb
     const int *index;   // gang index (x-form)
+    const int *ir;      // per-operand index arrays (QLA_INDEX_ALL forms): the
+    const int *ia;      // record of site i is r[ir[i]], a[ia[i]], b[ib[i]]; an
+    const int *ib;      // operand without one uses the gang index or i
     int n;              // sites (v-form) / index entries (x-form)
     int al;             // 16-byte alignment of the contiguous fields: 1 a, 2 b, 4 r
     int ms;             // mu + 5 * (sign != 1), 0..9
     int pa, pb;         // a / b are pointer arrays
 };
+// xf: per-lane addressing (a gang index or any per-operand index);
 // tpw: tiles per wave of the persistent grid (launch shape only)
 hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const qla_site_args &p, int tpw,
                            hipStream_t st);
@@ -43,17 +47,18 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
                               int out_kind, double *scratch, hipStream_t st);
 
 // The same reductions over gathered records: record of site i at *a[s]
-// (pa) or a + s*ncomp complexes, s = index[i] (index != NULL) or i; ncomp one
-// of 3, 6, 9, 12, 144.  out_kind 4 / 5: real part of a dot as double / float.
+// (pa) or a + s*ncomp complexes, s = ia[i] (per-operand index), index[i]
+// (gang index) or i, likewise for b with ib; ncomp one of 3, 6, 9, 12, 144.  out_kind 4 / 5: real part of a dot as double / float.
 hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a, const void *b, const int *index,
-                                    int n, int pa, int pb, void *out, int out_kind, double *scratch, hipStream_t st);
+                                    const int *ia, const int *ib, int n, int pa, int pb, void *out, int out_kind,
+                                    double *scratch, hipStream_t st);
 
 // Precision conversion of n records of ncomp complexes: to_float 0 float ->
 // double, 1 double -> float (round to nearest).  Destination record of site
-// i at index[i] (x-form) or i; source record at the same site, or *a[site]
-// when pa.
-hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, int n, int pa,
-                                hipStream_t st);
+// i at ir[i], index[i] (x-form) or i; source record at ia[i], index[i] or i,
+// read through the pointer array when pa.
+hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, const int *ir,
+                                const int *ia, int n, int pa, hipStream_t st);
 
 // Matrix functions on n nc x nc matrices.  fn: 0 exp, 1 sqrt, 2 invsqrt.
 // prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned int

@@ -5,9 +5,10 @@
 //
 // v-form with a field source: one complex per thread over the contiguous
 // arrays, non-temporal 8/16-byte loads and stores (each element is touched
-// once).  x-forms and pointer sources: the lanes of a wave walk the 64 records
-// of a tile one complex at a time, reading the source record through its
-// gathered address and writing the destination record of the same site.
+// once).  x-forms, per-operand index forms and pointer sources: the lanes of
+// a wave walk the 64 records of a tile one complex at a time, reading the
+// source record through its gathered address and writing the destination
+// record (r[indexr[i]], r[index[i]] or r[i]).
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -34,9 +35,10 @@ __global__ void __launch_bounds__(PBLOCK) k_precision_contig(TO *__restrict__ r,
     }
 }
 
-template <class TI, class TO, int NC, bool XF>
+template <class TI, class TO, int NC>
 __global__ void __launch_bounds__(PBLOCK)
-k_precision_gather(TO *__restrict__ r, const void *__restrict__ a, const int *__restrict__ index, int n, int pa)
+k_precision_gather(TO *__restrict__ r, const void *__restrict__ a, const int *__restrict__ index,
+                   const int *__restrict__ ir, const int *__restrict__ ia, int n, int pa)
 {
     const int lane = threadIdx.x % WAVE;
     const int nwaves = gridDim.x * (PBLOCK / WAVE);
@@ -46,9 +48,11 @@ k_precision_gather(TO *__restrict__ r, const void *__restrict__ a, const int *__
         const int ns = min(WAVE, n - tile * WAVE);
         uint64_t ga = 0, gr = 0;
         if (i < n) {
-            const int s = XF ? index[i] : i;
-            ga = pa ? (uint64_t)((const void *const *)a)[s] : (uint64_t)((const cpx<TI> *)a + (size_t)s * NC);
-            gr = (uint64_t)(r + (size_t)s * NC * 2);
+            const int g = index ? index[i] : i;
+            const int sa = ia ? ia[i] : g;
+            const int sr = ir ? ir[i] : g;
+            ga = pa ? (uint64_t)((const void *const *)a)[sa] : (uint64_t)((const cpx<TI> *)a + (size_t)sa * NC);
+            gr = (uint64_t)(r + (size_t)sr * NC * 2);
         }
         const int tot = ns * NC;
         const int iters = (tot + WAVE - 1) / WAVE;
@@ -70,16 +74,17 @@ k_precision_gather(TO *__restrict__ r, const void *__restrict__ a, const int *__
     }
 }
 
-template <class TI, class TO, bool XF>
-static hipError_t launch_gather(int ncomp, void *r, const void *a, const int *index, int n, int pa, hipStream_t st)
+template <class TI, class TO>
+static hipError_t launch_gather(int ncomp, void *r, const void *a, const int *index, const int *ir, const int *ia,
+                                int n, int pa, hipStream_t st)
 {
     const long long ntiles = ((long long)n + WAVE - 1) / WAVE;
     const long long want = (ntiles + 2 * (PBLOCK / WAVE) - 1) / (2 * (PBLOCK / WAVE));
     const int g = (int)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
 #define QLA_PREC_CASE(NC)                                                                                    \
     case NC:                                                                                                \
-        hipLaunchKernelGGL((k_precision_gather<TI, TO, NC, XF>), dim3(g), dim3(PBLOCK), 0, st, (TO *)r, a, \
-                           index, n, pa);                                                                   \
+        hipLaunchKernelGGL((k_precision_gather<TI, TO, NC>), dim3(g), dim3(PBLOCK), 0, st, (TO *)r, a, \
+                           index, ir, ia, n, pa);                                                       \
         return hipGetLastError();
     switch (ncomp) {
         QLA_PREC_CASE(3)
@@ -93,27 +98,27 @@ static hipError_t launch_gather(int ncomp, void *r, const void *a, const int *in
 }
 
 template <class TI, class TO>
-static hipError_t launch_t(int ncomp, void *r, const void *a, const int *index, int n, int pa, hipStream_t st)
+static hipError_t launch_t(int ncomp, void *r, const void *a, const int *index, const int *ir, const int *ia, int n,
+                           int pa, hipStream_t st)
 {
-    if (!index && !pa) {
+    if (!index && !ir && !ia && !pa) {
         const long long m = (long long)n * ncomp;
         const long long want = (m + 4LL * PBLOCK - 1) / (4LL * PBLOCK);
         const int g = (int)(want < 1 ? 1 : (want > 65535 ? 65535 : want));
         hipLaunchKernelGGL((k_precision_contig<TI, TO>), dim3(g), dim3(PBLOCK), 0, st, (TO *)r, (const TI *)a, m);
         return hipGetLastError();
     }
-    if (index) return launch_gather<TI, TO, true>(ncomp, r, a, index, n, pa, st);
-    return launch_gather<TI, TO, false>(ncomp, r, a, index, n, pa, st);
+    return launch_gather<TI, TO>(ncomp, r, a, index, ir, ia, n, pa, st);
 }
 
 } // namespace qla
 
 using namespace qla;
 
-hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, int n, int pa,
-                                hipStream_t st)
+hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, const int *ir,
+                                const int *ia, int n, int pa, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    if (to_float) return launch_t<double, float>(ncomp, r, a, index, n, pa, st);
-    return launch_t<float, double>(ncomp, r, a, index, n, pa, st);
+    if (to_float) return launch_t<double, float>(ncomp, r, a, index, ir, ia, n, pa, st);
+    return launch_t<float, double>(ncomp, r, a, index, ir, ia, n, pa, st);
 }

@@ -152,14 +152,16 @@ k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long lo
 }
 
 // stage 1 over gathered records: the record of site i is *pa[s] (pointer
-// array) or a[s] (field), s = index[i] (x-form) or i, NC complexes each.
+// array) or a[s] (field), s = ia[i] (per-operand index), index[i] (x-form)
+// or i, NC complexes each; likewise for b with ib.
 // The lanes of a wave walk the 64 records of a tile one complex at a time
 // (coalesced where the records are adjacent); the shuffles that hand each
 // lane its record's address run with the whole wave active.
 template <class T, bool DOT, int NC>
 __global__ void __launch_bounds__(RBLOCK)
-k_reduce1_gather(const void *__restrict__ a, const void *__restrict__ b, const int *__restrict__ index, int n,
-                 int pa, int pb, double *__restrict__ part)
+k_reduce1_gather(const void *__restrict__ a, const void *__restrict__ b, const int *__restrict__ index,
+                 const int *__restrict__ ia, const int *__restrict__ ib, int n, int pa, int pb,
+                 double *__restrict__ part)
 {
     __shared__ double sh[RBLOCK / WAVE];
     double sr = 0, si = 0;
@@ -171,9 +173,13 @@ k_reduce1_gather(const void *__restrict__ a, const void *__restrict__ b, const i
         const int ns = min(WAVE, n - tile * WAVE);
         uint64_t ga = 0, gb = 0;
         if (i < n) {
-            const int s = index ? index[i] : i;
-            ga = pa ? (uint64_t)((const void *const *)a)[s] : (uint64_t)((const cpx<T> *)a + (size_t)s * NC);
-            if (DOT) gb = pb ? (uint64_t)((const void *const *)b)[s] : (uint64_t)((const cpx<T> *)b + (size_t)s * NC);
+            const int g = index ? index[i] : i;
+            const int sa = ia ? ia[i] : g;
+            ga = pa ? (uint64_t)((const void *const *)a)[sa] : (uint64_t)((const cpx<T> *)a + (size_t)sa * NC);
+            if (DOT) {
+                const int sb = ib ? ib[i] : g;
+                gb = pb ? (uint64_t)((const void *const *)b)[sb] : (uint64_t)((const cpx<T> *)b + (size_t)sb * NC);
+            }
         }
         const int tot = ns * NC;
         const int iters = (tot + WAVE - 1) / WAVE;
@@ -292,13 +298,13 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
 // Gathered records (x-forms and pointer operands): ncomp complexes per
 // record, one of 3, 6, 9, 12, 144 (V, H, M, D, P).
 template <class T, bool DOT>
-static hipError_t launch_gather_t(int ncomp, const void *a, const void *b, const int *index, int n, int pa, int pb,
-                                  int g, double *scratch, hipStream_t st)
+static hipError_t launch_gather_t(int ncomp, const void *a, const void *b, const int *index, const int *ia,
+                                  const int *ib, int n, int pa, int pb, int g, double *scratch, hipStream_t st)
 {
 #define QLA_GATHER_CASE(NC)                                                                                     \
     case NC:                                                                                                   \
-        hipLaunchKernelGGL((k_reduce1_gather<T, DOT, NC>), dim3(g), dim3(RBLOCK), 0, st, a, b, index, n, pa, pb, \
-                           scratch);                                                                           \
+        hipLaunchKernelGGL((k_reduce1_gather<T, DOT, NC>), dim3(g), dim3(RBLOCK), 0, st, a, b, index, ia, ib, n, \
+                           pa, pb, scratch);                                                                   \
         return hipGetLastError();
     switch (ncomp) {
         QLA_GATHER_CASE(3)
@@ -312,7 +318,8 @@ static hipError_t launch_gather_t(int ncomp, const void *a, const void *b, const
 }
 
 hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a, const void *b, const int *index,
-                                    int n, int pa, int pb, void *out, int out_kind, double *scratch, hipStream_t st)
+                                    const int *ia, const int *ib, int n, int pa, int pb, void *out, int out_kind,
+                                    double *scratch, hipStream_t st)
 {
     const bool dot = (kind == 1);
     int g = 0;
@@ -322,10 +329,10 @@ hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a
         const long long want = (ntiles + 2 * (RBLOCK / WAVE) - 1) / (2 * (RBLOCK / WAVE));
         g = (int)(want < 1 ? 1 : (want > RMAXGRID ? RMAXGRID : want));
         hipError_t e;
-        if (prec == 0) e = dot ? launch_gather_t<float, true>(ncomp, a, b, index, n, pa, pb, g, scratch, st)
-                               : launch_gather_t<float, false>(ncomp, a, b, index, n, pa, pb, g, scratch, st);
-        else e = dot ? launch_gather_t<double, true>(ncomp, a, b, index, n, pa, pb, g, scratch, st)
-                     : launch_gather_t<double, false>(ncomp, a, b, index, n, pa, pb, g, scratch, st);
+        if (prec == 0) e = dot ? launch_gather_t<float, true>(ncomp, a, b, index, ia, ib, n, pa, pb, g, scratch, st)
+                               : launch_gather_t<float, false>(ncomp, a, b, index, ia, ib, n, pa, pb, g, scratch, st);
+        else e = dot ? launch_gather_t<double, true>(ncomp, a, b, index, ia, ib, n, pa, pb, g, scratch, st)
+                     : launch_gather_t<double, false>(ncomp, a, b, index, ia, ib, n, pa, pb, g, scratch, st);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(RBLOCK), 0, st, (const double *)scratch, g, (int)dot, out, out_kind);

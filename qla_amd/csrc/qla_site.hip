@@ -4,8 +4,9 @@
 //   D_{eqop}_sprecon_M[a]_times_H  (multiply, then spin-reconstruct)
 //   D_{eqop}_spproj_M[a]_times_D   (project, multiply, reconstruct)
 // in every eqop (eq, peq, meq, eqm), with M or its adjoint, with either
-// source a field or a pointer array (pM, pV/pD/pH), in the v-form (site i)
-// and the gang-indexed x-form (site index[i]).
+// source a field or a pointer array (pM, pV/pD/pH), in the v-form (site i),
+// the gang-indexed x-form (site index[i]) and the per-operand index forms
+// of QLA_INDEX_ALL (r[indexr[i]], a[indexa[i]], b[indexb[i]]).
 //
 // This file is compiled once per (precision, kind): QLA_SITE_PREC 0 float /
 // 1 double, QLA_SITE_KIND 0..3 (K_MATVEC, K_SPPROJ, K_SPRECON, K_WILSON); each
@@ -39,8 +40,9 @@ constexpr int SITE_KIND = QLA_SITE_KIND;
 // Persistent tile loop: wave w of the grid takes tiles w, w + W, w + 2W, ...
 // (W = waves in the grid).  The next tile's gather addresses (pointer-array
 // entries) are loaded between the current tile's stage and finish steps so
-// they travel with its records; in the x-form the index entries run one
-// tile further ahead, so the pointer loads never wait on an index load.
+// they travel with its records; in the XF forms (gang index, per-operand
+// indices) the index entries run one tile further ahead, so the pointer
+// loads never wait on an index load.
 template <class OP, class T, int KIND, bool XF, int WPB>
 __device__ __forceinline__ void run_tiles(const site_args &p)
 {
@@ -54,14 +56,14 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
 
     const int i0 = tile * WAVE + lane;
     const bool act0 = i0 < p.n;
-    lane_addr cur = addresses_of<T, KIND, XF>(p, XF ? (act0 ? p.index[i0] : 0) : i0, act0);
+    lane_addr cur = addresses_of<T, KIND, XF>(p, sites_of<XF>(p, i0, act0), act0);
     int next = tile + stride;
-    int nsite = 0;          // x-form: site of this lane in the next tile
+    lane_sites nsite = { 0, 0, 0 };     // XF forms: this lane's records in the next tile
     bool nact = false;
     if (XF && next < ntiles) {
         const int i1 = next * WAVE + lane;
         nact = i1 < p.n;
-        nsite = nact ? p.index[i1] : 0;
+        nsite = sites_of<XF>(p, i1, nact);
     }
     // the loop body then never waits on a prologue load: its only memory
     // wait is the one in finish
@@ -70,19 +72,20 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
         const int base = tile * WAVE, ns = min(WAVE, p.n - base);
         OP::stage(lds, p, cur, base, ns, lane);
         lane_addr nxt = { 0, 0, 0 };
-        int nnsite = 0;
+        lane_sites nnsite = { 0, 0, 0 };
         bool nnact = false;
         if (next < ntiles) {
             if (!XF) {
-                nsite = next * WAVE + lane;
-                nact = nsite < p.n;
+                const int i1 = next * WAVE + lane;
+                nact = i1 < p.n;
+                nsite = sites_of<XF>(p, i1, nact);
             }
             nxt = addresses_of<T, KIND, XF>(p, nsite, nact);
             const int nn = next + stride;
             if (XF && nn < ntiles) {
                 const int i2 = nn * WAVE + lane;
                 nnact = i2 < p.n;
-                nnsite = nnact ? p.index[i2] : 0;
+                nnsite = sites_of<XF>(p, i2, nnact);
             }
         }
         OP::finish(lds, p, cur, base, ns, lane);

@@ -129,22 +129,42 @@ __device__ __forceinline__ void with_ms(int ms, F &&f)
 typedef qla_site_args site_args;
 
 // Per-lane record addresses of one tile: gathered A and B records (pointer
-// operands, or every operand of the x-form) and the x-form destination.
+// operands, or every operand of the per-lane-addressed forms) and the
+// destination of those forms.
 struct lane_addr {
     uint64_t ga, gb, gr;
 };
 
+// Per-lane record indices of the three operands (XF forms): the operand's
+// own index entry (QLA_INDEX_ALL forms), else the gang index entry, else i.
+struct lane_sites {
+    int r, a, b;
+};
+
+template <bool XF>
+__device__ __forceinline__ lane_sites sites_of(const site_args &p, int i, bool act)
+{
+    lane_sites s = { i, i, i };
+    if (XF && act) {
+        const int g = p.index ? p.index[i] : i;
+        s.r = p.ir ? p.ir[i] : g;
+        s.a = p.ia ? p.ia[i] : g;
+        s.b = p.ib ? p.ib[i] : g;
+    }
+    return s;
+}
+
 template <class T, int KIND, bool XF>
-__device__ __forceinline__ lane_addr addresses_of(const site_args &p, int site, bool act)
+__device__ __forceinline__ lane_addr addresses_of(const site_args &p, const lane_sites &s, bool act)
 {
     typedef kind_rec<T, KIND> K;
     lane_addr la = { 0, 0, 0 };
     if (act) {
-        if (p.pa) la.ga = (uint64_t)((const void *const *)p.a)[site];
-        else if (XF) la.ga = (uint64_t)((const char *)p.a + (size_t)site * rec<T>::M);
-        if (p.pb) la.gb = (uint64_t)((const void *const *)p.b)[site];
-        else if (XF) la.gb = (uint64_t)((const char *)p.b + (size_t)site * K::B);
-        if (XF) la.gr = (uint64_t)((char *)p.r + (size_t)site * K::R);
+        if (p.pa) la.ga = (uint64_t)((const void *const *)p.a)[s.a];
+        else if (XF) la.ga = (uint64_t)((const char *)p.a + (size_t)s.a * rec<T>::M);
+        if (p.pb) la.gb = (uint64_t)((const void *const *)p.b)[s.b];
+        else if (XF) la.gb = (uint64_t)((const char *)p.b + (size_t)s.b * K::B);
+        if (XF) la.gr = (uint64_t)((char *)p.r + (size_t)s.r * K::R);
     }
     return la;
 }

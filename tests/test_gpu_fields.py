@@ -36,21 +36,35 @@ def setup(t, prec):
     rng = np.random.default_rng(len(t) * 7 + n)
     s = dict(n=n, a=lattice.random_field(t, n, 11 + n).astype(dt), b=lattice.random_field(t, n, 12 + n).astype(dt),
              a_idx=rng.permutation(n), b_idx=rng.permutation(n),
-             index=np.sort(rng.choice(n, n // 2, replace=False)).astype(np.int32))
+             index=np.sort(rng.choice(n, n // 2, replace=False)).astype(np.int32),
+             own={k: rng.choice(n, n // 3, replace=False).astype(np.int32) for k in "rab"})
     s["da"], s["db"] = dev(s["a"]), dev(s["b"])
     s["pa"] = qla_amd.pointer_array(s["da"], s["a_idx"])
     s["pb"] = qla_amd.pointer_array(s["db"], s["b_idx"])
     s["dindex"] = dev(s["index"])
+    s["down"] = {k: dev(v) for k, v in s["own"].items()}
     _cache[key] = s
     return s
 
 
+def steps(v, s):
+    if v.xf:
+        return len(s["index"])
+    if v.ia or v.ib:
+        return len(s["own"]["a"])
+    return s["n"]
+
+
 def expected(orc, v, s):
     index = s["index"] if v.xf else None
-    ga = orc.records(s["a"], index, s["a_idx"] if v.pa else None)
+    n = steps(v, s)
+    # v-form records with a per-operand index on the other operand: the first n
+    ga = orc.records(s["a"], index, s["a_idx"] if v.pa else None,
+                     own=s["own"]["a"] if v.ia else (np.arange(n) if v.ib else None))
     if v.kind == "norm2":
         return orc.norm2(ga), orc.norm2_ld(ga)
-    gb = orc.records(s["b"], index, s["b_idx"] if v.pb else None)
+    gb = orc.records(s["b"], index, s["b_idx"] if v.pb else None,
+                     own=s["own"]["b"] if v.ib else (np.arange(n) if v.ia else None))
     d = orc.dot(ga, gb)
     scale = float(np.sqrt(orc.norm2_ld(ga) * orc.norm2_ld(gb)))
     return (d if v.kind == "dot" else d.real), scale
@@ -58,11 +72,15 @@ def expected(orc, v, s):
 
 def call(v, s, out_ptr, n=None):
     args = [out_ptr, s["pa"] if v.pa else s["da"]]
+    if v.ia:
+        args.append(s["down"]["a"])
     if v.kind != "norm2":
         args.append(s["pb"] if v.pb else s["db"])
+    if v.ib:
+        args.append(s["down"]["b"])
     if v.xf:
         args.append(s["dindex"])
-    args.append((len(s["index"]) if v.xf else s["n"]) if n is None else n)
+    args.append(steps(v, s) if n is None else n)
     qla_amd.call(v.name, *args)
     qla_amd.check()
 
@@ -98,7 +116,7 @@ def test_reduction_symbol(orc, v):
 def test_reduction_device_result_and_empty(orc, prec, kind):
     import torch
     for v in [x for x in reductions() if x.prec == prec and x.kind == kind and x.t in ("D", "P")
-              and x.xf and x.pa and (x.pb or kind == "norm2")]:
+              and x.xf and x.pa and (x.pb or kind == "norm2") and not (x.ia or x.ib)]:
         s = setup(v.t, "D3" if prec == "D3" else "F3")
         exp, scale = expected(orc, v, s)
         tt = torch.float32 if prec == "F3" else torch.float64
@@ -122,13 +140,17 @@ def test_conversion_symbol(orc, v):
     dest_dt = np.float64 if v.lib == "DF3" else np.float32
     r0 = lattice.random_field(v.t, s["n"], 99).astype(dest_dt)
     r = dev(r0)
-    args = [r, s["pa"] if v.pa else s["da"]] + ([s["dindex"]] if v.xf else [])
-    n = len(s["index"]) if v.xf else s["n"]
+    own, down = s["own"], s["down"]
+    args = ([r] + ([down["r"]] if v.ir else []) + [s["pa"] if v.pa else s["da"]] + ([down["a"]] if v.ia else [])
+            + ([s["dindex"]] if v.xf else []))
+    n = len(s["index"]) if v.xf else (len(own["r"]) if (v.ir or v.ia) else s["n"])
     say(f"{v.name} n={n}")
     qla_amd.call(v.name, *args, n)
     qla_amd.check()
     exp = r0.copy()
-    orc.convert(s["a"], exp, index=s["index"] if v.xf else None, a_idx=s["a_idx"] if v.pa else None)
+    orc.convert(s["a"], exp, index=s["index"] if v.xf else None, a_idx=s["a_idx"] if v.pa else None,
+                ir=own["r"] if v.ir else (np.arange(n) if v.ia else None),
+                ia=own["a"] if v.ia else (np.arange(n) if v.ir else None))
     assert_bitwise(host(r), exp, v.name)
     say(f"{v.name} n=0")
     qla_amd.call(v.name, *args, 0)

@@ -7,7 +7,9 @@ multi-source V = sum_d M_d*V_d) is called through the C ABI and compared
 with the oracle bit for bit.  Each call uses a ragged site count (the last
 tile is partial), permuted pointer targets and, for the x-forms, a parity
 subset; the spin forms run two mu/sign pairs per symbol and a full
-mu/sign sweep per (kind, eqop, adjoint).  Also covered: misaligned fields
+mu/sign sweep per (kind, eqop, adjoint).  The per-operand index forms of
+QLA_INDEX_ALL (r[indexr[i]], a[indexa[i]], b[indexb[i]]) use distinct random
+subsets per operand.  Also covered: misaligned fields
 (the 4-byte load path), mu out of range and n = 0 (no-ops), multi-source
 sums longer than one launch (nd > 16), nd = 0, and term lists in device
 memory.
@@ -28,6 +30,7 @@ pytestmark = pytest.mark.gpu
 KIND = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3}
 DIMS = (4, 4, 5, 6)             # 480 sites
 N = 453                          # v-form site count: 7 full tiles and a ragged one
+N_OWN = 333                      # steps of the per-operand index forms
 ND = 3
 
 _cache = {}
@@ -49,12 +52,14 @@ def setup(prec, src2, dest, nd=ND):
         a_idx=[rng.permutation(nsites) for _ in range(nd)],
         b_idx=[rng.permutation(nsites) for _ in range(nd)],
         index=lattice.parity_sites(DIMS, 1)[:200].astype(np.int32),
+        own={k: rng.choice(nsites, N_OWN, replace=False).astype(np.int32) for k in "rab"},
     )
     s["du"] = [dev(x) for x in s["u"]]
     s["db"] = [dev(x) for x in s["b"]]
     s["pa"] = [qla_amd.pointer_array(du, ix) for du, ix in zip(s["du"], s["a_idx"])]
     s["pb"] = [qla_amd.pointer_array(db, ix) for db, ix in zip(s["db"], s["b_idx"])]
     s["dindex"] = dev(s["index"])
+    s["down"] = {k: dev(v) for k, v in s["own"].items()}
     _cache[key] = s
     return s
 
@@ -66,18 +71,22 @@ def ms_pairs(name):
 
 def run_site(orc, v, s, mu, sign, n=None):
     kind = KIND[v.kind]
-    n = (len(s["index"]) if v.xf else N) if n is None else n
+    if n is None:
+        n = len(s["index"]) if v.xf else (N_OWN if v.own_index else N)
     r = dev(s["r0"])
     a = s["pa"][0] if v.pa else s["du"][0]
     b = s["pb"][0] if v.pb else s["db"][0]
-    args = [r, a, b] + ([mu, sign] if v.spin else []) + ([s["dindex"]] if v.xf else []) + [n]
+    own = s["down"]
+    args = ([r] + ([own["r"]] if v.ir else []) + [a] + ([own["a"]] if v.ia else []) + [b]
+            + ([own["b"]] if v.ib else []) + ([mu, sign] if v.spin else []) + ([s["dindex"]] if v.xf else []) + [n])
     say(f"{v.name} mu={mu} sign={sign} n={n}")
     qla_amd.call(v.name, *args)
     qla_amd.check()
     exp = s["r0"].copy()
+    o = s["own"]
     orc.site(kind, v.eqop, exp, s["u"][0], s["b"][0], mu, sign, index=s["index"] if v.xf else None,
              adj=v.adj, a_idx=s["a_idx"][0] if v.pa else None, b_idx=s["b_idx"][0] if v.pb else None,
-             n=n)
+             n=n, ir=o["r"] if v.ir else None, ia=o["a"] if v.ia else None, ib=o["b"] if v.ib else None)
     assert_bitwise(host(r), exp, f"{v.name} mu={mu} sign={sign}")
 
 
@@ -121,7 +130,7 @@ def test_symbol(orc, v):
         run_site(orc, v, s, 0, 1)
 
 
-SWEEP = [v for v in variants() if v.spin and not v.xf and not v.pa and v.pb]
+SWEEP = [v for v in variants() if v.spin and not v.xf and not v.pa and v.pb and not v.own_index]
 
 
 @pytest.mark.parametrize("v", SWEEP, ids=lambda v: v.name)
@@ -136,7 +145,7 @@ def test_all_mu_sign(orc, v):
 def test_noop_cases(orc, prec, kind):
     """mu outside 0..4 and n = 0 leave the destination untouched."""
     v = [x for x in variants() if x.prec == prec and x.kind == kind and x.eqop == 0 and x.pb
-         and not x.pa and not x.adj and not x.xf][0]
+         and not x.pa and not x.adj and not x.xf and not x.own_index][0]
     s = setup(v.prec, v.src2, v.dest)
     for mu, n in ((5, N), (-1, N), (2, 0)):
         r = dev(s["r0"])
@@ -153,7 +162,7 @@ def test_misaligned_fields(orc, prec, kind, eqop):
     """Fields at an 8-byte (F3) offset from 16-byte alignment: the 4-byte path."""
     import torch
     v = [x for x in variants() if x.prec == prec and x.kind == kind and x.eqop == eqop and not x.pa
-         and not x.pb and not x.adj and not x.xf][0]
+         and not x.pb and not x.adj and not x.xf and not x.own_index][0]
     s = setup(v.prec, v.src2, v.dest)
     dt = s["u"][0].dtype
     off = 1 if prec == "F3" else 0        # one complex: 8 B (F3) / 16 B (D3, always 16-aligned)

@@ -184,3 +184,32 @@ def test_multi_is_chained_matvec(orc, prec, eqop):
     nothing = r0.copy()
     orc.multi(eqop, nothing, [], [])
     np.testing.assert_array_equal(nothing, r0)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_per_operand_index_forms(orc, prec, kind):
+    """QLA_INDEX_ALL forms (indirection_all.pl:66-84): step i uses
+    r[indexr[i]], a[indexa[i]], b[indexb[i]]; equal bit for bit to the v-form
+    on the gathered operands, scattered back (the reference's own check,
+    test/src/protocol_idx.m4:2110-2160, loops single-site calls the same way)."""
+    n_sites, n = 90, 57
+    dt = np.float32 if prec == "F" else np.float64
+    rng = np.random.default_rng(kind)
+    U = lattice.random_su3(n_sites, 91).astype(dt)
+    b = lattice.random_field(SRC2[kind], n_sites, 92).astype(dt)
+    r0 = lattice.random_field(DEST[kind], n_sites, 93).astype(dt)
+    ir, ia, ib = (rng.choice(n_sites, n, replace=False) for _ in range(3))
+    for eqop in range(4):
+        for adj in (False, True):
+            for use in ((1, 0, 0), (0, 1, 1), (1, 1, 1)):
+                got = r0.copy()
+                orc.site(kind, eqop, got, U, b, 3, -1, adj=adj, n=n, ir=ir if use[0] else None,
+                         ia=ia if use[1] else None, ib=ib if use[2] else None)
+                rs = ir if use[0] else np.arange(n)
+                sub = np.ascontiguousarray(r0[rs])
+                orc.site(kind, eqop, sub, np.ascontiguousarray(U[ia if use[1] else np.arange(n)]),
+                         np.ascontiguousarray(b[ib if use[2] else np.arange(n)]), 3, -1, adj=adj)
+                exp = r0.copy()
+                exp[rs] = sub
+                np.testing.assert_array_equal(got, exp)
