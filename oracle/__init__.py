@@ -42,7 +42,7 @@ def load():
     vp, i32 = C.c_void_p, C.c_int
     for pre in ("F3", "D3"):
         getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32, i32, vp, i32]
-        getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, i32, vp, i32]
+        getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, vp, i32, vp, i32]
     for p in ("F", "D"):
         f = getattr(lib, f"orc_{p}_norm2")
         f.argtypes, f.restype = [vp, i32, i32], C.c_double
@@ -132,19 +132,25 @@ def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b
     return r
 
 
-def multi(eqop, r, a_list, b_list, index=None, adj=False, a_idx=None, b_idx=None, n=None):
+def multi(eqop, r, a_list, b_list, index=None, adj=False, a_idx=None, b_idx=None, n=None,
+          ir=None, ia=None, ib=None):
     """Multi-source r eqop sum_d A_d * B_d (orc_*_multi); a_idx / b_idx are
-    per-term index lists (pointer-array operands) or None (fields)."""
+    per-term index lists (pointer-array operands) or None (fields); ir / ia /
+    ib the per-operand index arrays of the QLA_INDEX_ALL forms (shared by
+    the terms)."""
     lib = load()
     nd = len(a_list)
-    pas = [_records(a, None if a_idx is None else a_idx[d]) for d, a in enumerate(a_list)]
-    pbs = [_records(b, None if b_idx is None else b_idx[d]) for d, b in enumerate(b_list)]
+    pas = [_records(a, ia if ia is not None else (None if a_idx is None else a_idx[d]))
+           for d, a in enumerate(a_list)]
+    pbs = [_records(b, ib if ib is not None else (None if b_idx is None else b_idx[d]))
+           for d, b in enumerate(b_list)]
     pa = np.array([x.ctypes.data for x in pas], dtype=np.uint64)
     pb = np.array([x.ctypes.data for x in pbs], dtype=np.uint64)
     ix = _index(index)
+    rix = _index(ir)
     if n is None:
-        n = len(ix) if ix is not None else r.shape[0]
-    getattr(lib, f"orc_{_prefix(r)}_multi")(eqop, int(adj), _ptr(_need(r)), _ptr(pa), _ptr(pb), nd,
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    getattr(lib, f"orc_{_prefix(r)}_multi")(eqop, int(adj), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), nd,
                                             _ptr(ix), n)
     return r
 

@@ -35,12 +35,12 @@ void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const
 void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,
                  const double *const *pb, int mu, int sign, const int *index, int n)
 { D_orc_3_site(kind, eqop, adj, r, rindex, pa, pb, mu, sign, index, n); }
-void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,
-                  int nd, const int *index, int n)
-{ F_orc_3_multi(eqop, adj, r, pa, pb, nd, index, n); }
-void orc_D3_multi(int eqop, int adj, double *r, const double *const *const *pa, const double *const *const *pb,
-                  int nd, const int *index, int n)
-{ D_orc_3_multi(eqop, adj, r, pa, pb, nd, index, n); }
+void orc_F3_multi(int eqop, int adj, float *r, const int *rindex, const float *const *const *pa,
+                  const float *const *const *pb, int nd, const int *index, int n)
+{ F_orc_3_multi(eqop, adj, r, rindex, pa, pb, nd, index, n); }
+void orc_D3_multi(int eqop, int adj, double *r, const int *rindex, const double *const *const *pa,
+                  const double *const *const *pb, int nd, const int *index, int n)
+{ D_orc_3_multi(eqop, adj, r, rindex, pa, pb, nd, index, n); }
 
 double orc_F_norm2(const float *a, int ncomp, int n) { return F_orc_norm2(a, ncomp, n); }
 double orc_D_norm2(const double *a, int ncomp, int n) { return D_orc_norm2(a, ncomp, n); }

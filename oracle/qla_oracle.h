@@ -40,11 +40,12 @@ void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const
 void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,
                  const double *const *pb, int mu, int sign, const int *index, int n);
 /* multi-source r eqop sum_{d<nd} A_d*B_d, A_d = *pa[d][s], B_d = *pb[d][s]
- * (build_qla.pl:1693-1725, expressions_tensor.pl:983-1002) */
-void orc_F3_multi(int eqop, int adj, float *r, const float *const *const *pa, const float *const *const *pb,
-                  int nd, const int *index, int n);
-void orc_D3_multi(int eqop, int adj, double *r, const double *const *const *pa, const double *const *const *pb,
-                  int nd, const int *index, int n);
+ * (build_qla.pl:1693-1725, expressions_tensor.pl:983-1002); destination
+ * r[rindex[i]] when rindex != NULL, else r[s] */
+void orc_F3_multi(int eqop, int adj, float *r, const int *rindex, const float *const *const *pa,
+                  const float *const *const *pb, int nd, const int *index, int n);
+void orc_D3_multi(int eqop, int adj, double *r, const int *rindex, const double *const *const *pa,
+                  const double *const *const *pb, int nd, const int *index, int n);
 
 /* norm2 / dot (make_code_unary.pl:318-366, make_code_binary.pl:110-170).
  * ncomp = complex components per site (3 for V, 12 for D).  Results are the

@@ -281,8 +281,6 @@ def variants() -> list[Variant]:
                         for pa in (False, True):
                             for pb in (False, True):
                                 out.append(Variant(prec, kind, eqop, adj, xf, pa, pb))
-            if kind == "multi":
-                continue
             # per-operand index forms (QLA_INDEX_ALL): fields only, no gang index
             for ir, ia, ib in OWN_INDEX_FORMS:
                 for eqop in range(4):
@@ -355,12 +353,12 @@ def inc_text() -> str:
         prec = 0 if v.prec == "F3" else 1
         lines.append(_decl(v))
         idx = "index" if v.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", v.ir), ("indexa", v.ia), ("indexb", v.ib)))
         if v.multi:
-            body = (f"multi_entry({prec}, {v.eqop}, {int(v.adj)}, {int(v.xf)}, {int(v.pa)}, {int(v.pb)}, r, "
-                    f"(const void *)a, (const void *)b, {idx}, n, nd, \"{v.name}\");")
+            body = (f"multi_entry({prec}, {v.eqop}, {int(v.adj)}, {int(v.pa)}, {int(v.pb)}, r, "
+                    f"(const void *)a, (const void *)b, {idx}, {own}, n, nd, \"{v.name}\");")
         else:
             mu, sign = ("mu", "sign") if v.spin else ("0", "1")
-            own = ", ".join(x if f else "nullptr" for x, f in (("indexr", v.ir), ("indexa", v.ia), ("indexb", v.ib)))
             body = (f"site_entry({prec}, {KINDS[v.kind]}, {v.eqop}, {int(v.adj)}, {int(v.pa)}, {int(v.pb)}, r, "
                     f"(const void *)a, (const void *)b, {mu}, {sign}, {idx}, {own}, n, \"{v.name}\");")
         lines.append("{")

@@ -183,9 +183,10 @@ hipError_t term_list(const void **dst, const void *src, int nd)
 // One multi-source call: terms in launches of at most QLA_MULTI_MAXND, the
 // first with the call's eqop and the rest accumulating (peq for eq/peq, meq
 // for meq/eqm) -- the reference's own d >= 1 rule (expressions_tensor.pl:985-1000).
-void multi_entry(int prec, int eqop, int adj, int xf, int pa, int pb, void *r, const void *a, const void *b,
-                 const int *index, int n, int nd, const char *name)
+void multi_entry(int prec, int eqop, int adj, int pa, int pb, void *r, const void *a, const void *b,
+                 const int *index, const int *ir, const int *ia, const int *ib, int n, int nd, const char *name)
 {
+    const int xf = (index || ir || ia || ib) ? 1 : 0;
     if (n <= 0 || nd <= 0) return;
     std::vector<const void *> av((size_t)nd), bv((size_t)nd);
     hipError_t e = term_list(av.data(), a, nd);
@@ -194,7 +195,10 @@ void multi_entry(int prec, int eqop, int adj, int xf, int pa, int pb, void *r, c
         qla_multi_args p;
         memset(&p, 0, sizeof(p));
         p.r = r;
-        p.index = xf ? index : nullptr;
+        p.index = index;
+        p.ir = ir;
+        p.ia = ia;
+        p.ib = ib;
         p.n = n;
         p.nd = std::min(QLA_MULTI_MAXND, nd - d0);
         p.pa = pa;

@@ -31,7 +31,8 @@ hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const 
 #define QLA_MULTI_MAXND 16
 struct qla_multi_args {
     void *r;
-    const int *index;
+    const int *index;                 // gang index (x-form)
+    const int *ir, *ia, *ib;          // per-operand indices (QLA_INDEX_ALL forms), shared by the terms
     int n, al, nd, pa, pb;
     const void *a[QLA_MULTI_MAXND];   // per term: ColorMatrix field or pointer array
     const void *b[QLA_MULTI_MAXND];   // per term: ColorVector field or pointer array

@@ -4,6 +4,10 @@
 // D3 override lib/optimized/c/src/QLA_D3_V_vpeq_nM_times_npV.c keeps the same
 // per-site order: d ascending, kc ascending inside each term).
 //
+// The x-forms (gang index) and the per-operand index forms of QLA_INDEX_ALL
+// (one indexr / indexa / indexb array shared by all terms) address every
+// record per lane.
+//
 // Compiled once per precision (QLA_SITE_PREC 0 float / 1 double); exports
 // qla_multi_launch_p<PREC>.  A tile of 64 sites runs nd phases: each phase
 // stages term d's matrix and vector records into LDS (direct-to-LDS loads),
@@ -37,9 +41,9 @@ __global__ void __launch_bounds__(WPB * WAVE) k_multi(multi_args p)
     int tile = blockIdx.x * WPB + (int)(threadIdx.x / WAVE);
     if (tile >= ntiles) return;
 
-    int i = tile * WAVE + lane;
-    bool act = i < p.n;
-    int site = XF ? (act ? p.index[i] : 0) : i;
+    const int i0 = tile * WAVE + lane;
+    bool act = i0 < p.n;
+    lane_sites site = OP::sites(p, i0, act);
     uint64_t ga, gb;
     OP::term_addresses(p, 0, site, act, ga, gb);
     vm_wait();      // the loop's only memory wait is the one before each phase's LDS reads
@@ -47,8 +51,8 @@ __global__ void __launch_bounds__(WPB * WAVE) k_multi(multi_args p)
         const int base = tile * WAVE, ns = min(WAVE, p.n - base);
         const int next = tile + stride;
         const bool more = next < ntiles;
-        const uint64_t gr = (XF && act) ? (uint64_t)((char *)p.r + (size_t)site * R::V) : 0;
-        int nsite = 0;
+        const uint64_t gr = (XF && act) ? (uint64_t)((char *)p.r + (size_t)site.r * R::V) : 0;
+        lane_sites nsite = { 0, 0, 0 };
         bool nact = false;
         cpx<T> x[3];
         for (int d = 0; d < p.nd; d++) {
@@ -60,7 +64,7 @@ __global__ void __launch_bounds__(WPB * WAVE) k_multi(multi_args p)
             if (d == 0 && more) {
                 const int i1 = next * WAVE + lane;
                 nact = i1 < p.n;
-                nsite = XF ? (nact ? p.index[i1] : 0) : i1;
+                nsite = OP::sites(p, i1, nact);
             }
             // gather addresses of the next phase: term d + 1, or term 0 of the next tile
             if (d + 1 < p.nd) OP::term_addresses(p, d + 1, site, act, ga, gb);

@@ -274,17 +274,31 @@ template <class T, int EQOP, bool ADJ, bool XF> struct multi_op {
     static constexpr int OFF_R = WAVE * (R::M + R::V);
     static constexpr int BYTES = WAVE * (R::M + R::V + (READ_R ? R::V : 0));
 
+    // this lane's record indices of step i (XF forms: per-operand index,
+    // else gang index, else i)
+    static __device__ __forceinline__ lane_sites sites(const multi_args &p, int i, bool act)
+    {
+        lane_sites s = { i, i, i };
+        if (XF && act) {
+            const int g = p.index ? p.index[i] : i;
+            s.r = p.ir ? p.ir[i] : g;
+            s.a = p.ia ? p.ia[i] : g;
+            s.b = p.ib ? p.ib[i] : g;
+        }
+        return s;
+    }
+
     // gathered addresses of term d's records for one lane
-    static __device__ __forceinline__ void term_addresses(const multi_args &p, int d, int site, bool act,
+    static __device__ __forceinline__ void term_addresses(const multi_args &p, int d, const lane_sites &s, bool act,
                                                           uint64_t &ga, uint64_t &gb)
     {
         ga = 0;
         gb = 0;
         if (!act) return;
-        if (p.pa) ga = (uint64_t)((const void *const *)p.a[d])[site];
-        else if (XF) ga = (uint64_t)((const char *)p.a[d] + (size_t)site * R::M);
-        if (p.pb) gb = (uint64_t)((const void *const *)p.b[d])[site];
-        else if (XF) gb = (uint64_t)((const char *)p.b[d] + (size_t)site * R::V);
+        if (p.pa) ga = (uint64_t)((const void *const *)p.a[d])[s.a];
+        else if (XF) ga = (uint64_t)((const char *)p.a[d] + (size_t)s.a * R::M);
+        if (p.pb) gb = (uint64_t)((const void *const *)p.b[d])[s.b];
+        else if (XF) gb = (uint64_t)((const char *)p.b[d] + (size_t)s.b * R::V);
     }
 
     static __device__ __forceinline__ void stage_term(char *lds, const multi_args &p, int d, uint64_t ga,

@@ -36,9 +36,9 @@ def test_generated_site_abi_is_current():
     from qla_amd import abi_gen
     assert abi_gen.current()
     names = [v.name for v in abi_gen.variants()]
-    assert len(names) == len(set(names)) == 1088
+    assert len(names) == len(set(names)) == 1200
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 1548
+    assert len(every) == len(set(every)) == 1660
     # QLA names of the reference's own callers and tests
     for s in ("QLA_F3_D_vpeq_spproj_M_times_pD", "QLA_D3_V_vpeq_nM_times_npV", "QLA_F3_H_veq_spproj_Ma_times_pD",
               "QLA_F3_D_xmeq_sprecon_pMa_times_pH", "QLA_D3_V_xeqm_npMa_times_nV",

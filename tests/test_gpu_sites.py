@@ -96,7 +96,7 @@ def term_list(ts):
 
 
 def run_multi(orc, v, s, nd, device_lists=False):
-    n = len(s["index"]) if v.xf else N
+    n = len(s["index"]) if v.xf else (N_OWN if v.own_index else N)
     r = dev(s["r0"])
     a_terms = s["pa"][:nd] if v.pa else s["du"][:nd]
     b_terms = s["pb"][:nd] if v.pb else s["db"][:nd]
@@ -108,13 +108,16 @@ def run_multi(orc, v, s, nd, device_lists=False):
     else:
         al, bl = term_list(a_terms), term_list(b_terms)
         a_arg, b_arg = ctypes.addressof(al), ctypes.addressof(bl)
-    args = [r, a_arg, b_arg] + ([s["dindex"]] if v.xf else []) + [n, nd]
+    own, o = s["down"], s["own"]
+    args = ([r] + ([own["r"]] if v.ir else []) + [a_arg] + ([own["a"]] if v.ia else []) + [b_arg]
+            + ([own["b"]] if v.ib else []) + ([s["dindex"]] if v.xf else []) + [n, nd])
     say(f"{v.name} nd={nd} device_lists={device_lists}")
     qla_amd.call(v.name, *args)
     qla_amd.check()
     exp = s["r0"].copy()
     orc.multi(v.eqop, exp, s["u"][:nd], s["b"][:nd], index=s["index"] if v.xf else None, adj=v.adj,
-              a_idx=s["a_idx"][:nd] if v.pa else None, b_idx=s["b_idx"][:nd] if v.pb else None, n=n)
+              a_idx=s["a_idx"][:nd] if v.pa else None, b_idx=s["b_idx"][:nd] if v.pb else None, n=n,
+              ir=o["r"] if v.ir else None, ia=o["a"] if v.ia else None, ib=o["b"] if v.ib else None)
     assert_bitwise(host(r), exp, f"{v.name} nd={nd}")
 
 
@@ -195,7 +198,7 @@ def test_multi_long_and_empty(orc, prec):
     term lists in device memory."""
     for eqop in range(4):
         v = [x for x in variants() if x.prec == prec and x.multi and x.eqop == eqop and x.pb and not x.pa
-             and not x.xf and not x.adj][0]
+             and not x.xf and not x.adj and not x.own_index][0]
         s = setup(v.prec, v.src2, v.dest, nd=20)
         run_multi(orc, v, s, 20)
         run_multi(orc, v, s, 5, device_lists=True)

@@ -213,3 +213,28 @@ def test_per_operand_index_forms(orc, prec, kind):
                 exp = r0.copy()
                 exp[rs] = sub
                 np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_multi_per_operand_index_forms(orc, prec):
+    """V_eq_nxM_times_nV and relatives: one indexr / indexa / indexb shared by
+    all terms; bit for bit the v-form on gathered terms, scattered back."""
+    n_sites, n, nd = 70, 41, 3
+    dt = np.float32 if prec == "F" else np.float64
+    rng = np.random.default_rng(5)
+    As = [lattice.random_su3(n_sites, 110 + d).astype(dt) for d in range(nd)]
+    Vs = [lattice.random_field("V", n_sites, 120 + d).astype(dt) for d in range(nd)]
+    r0 = lattice.random_field("V", n_sites, 130).astype(dt)
+    ir, ia, ib = (rng.choice(n_sites, n, replace=False) for _ in range(3))
+    for eqop in range(4):
+        for use in ((1, 0, 0), (0, 1, 0), (1, 1, 1)):
+            got = r0.copy()
+            orc.multi(eqop, got, As, Vs, n=n, ir=ir if use[0] else None, ia=ia if use[1] else None,
+                      ib=ib if use[2] else None)
+            rs = ir if use[0] else np.arange(n)
+            sub = np.ascontiguousarray(r0[rs])
+            orc.multi(eqop, sub, [np.ascontiguousarray(a[ia if use[1] else np.arange(n)]) for a in As],
+                      [np.ascontiguousarray(v[ib if use[2] else np.arange(n)]) for v in Vs])
+            exp = r0.copy()
+            exp[rs] = sub
+            np.testing.assert_array_equal(got, exp)
