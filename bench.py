@@ -475,7 +475,10 @@ def batch_config3(lib, stream, args):
 
 def matfn_config4(lib, stream, args):
     """M_veq_{exp,sqrt,invsqrt}_M on 2,097,152 Nc=3 matrices (F3) and on
-    262,144 Nc=4 / Nc=8 matrices (FN): matrices per second."""
+    262,144 Nc=4 / Nc=8 matrices (FN), then the other F3 colour-matrix
+    functions (expA, expTA, sqrtPH, invsqrtPH, log, det, inverse,
+    eigenvalsH, M_inverse_V) on the 2,097,152 Nc=3 matrices: matrices per
+    second and algorithmic GB/s (one record read, one written)."""
     import torch
     from qla_amd import lattice
     out = {}
@@ -500,6 +503,33 @@ def matfn_config4(lib, stream, args):
             f = getattr(lib, f"QLA_FN_M_veq_{fn}_M")
             ms = _timed(stream, lambda it: f(nc, r.data_ptr(), a.data_ptr(), nn), 2, 1) / 2
             out[f"FN{nc}_{fn}"] = {"n": nn, "ms": round(ms, 3), "matrices_per_s": round(nn / (ms * 1e-3), 1)}
+    # the other colour-matrix functions on the same 2,097,152 Nc=3 matrices
+    rng = np.random.default_rng(45)
+    q = lattice.random_antihermitian(n3, 3, 46)
+    hpd = lattice.random_hpd(n3, 3, 47, eps_lo=1e-2)
+    gauss = rng.standard_normal((n3, 3, 3, 2))
+    herm = np.stack([q[..., 1], -q[..., 0]], axis=-1)      # -i * (i H) = H
+    ins = {"expA": q, "expTA": q, "sqrtPH": hpd, "invsqrtPH": hpd, "log": hpd, "det": gauss, "inverse": gauss,
+           "eigenvalsH": herm, "inverse_V": gauss}
+    vb = torch.from_numpy(rng.standard_normal((n3, 3, 2)).astype(np.float32)).cuda()
+    for fn, host_a in ins.items():
+        a = torch.from_numpy(host_a.astype(np.float32)).cuda()
+        dest = {"det": (n3, 2), "eigenvalsH": (n3, 3, 2), "inverse_V": (n3, 3, 2)}.get(fn, (n3, 3, 3, 2))
+        r = torch.empty(dest, dtype=torch.float32, device="cuda")
+        rbytes = r[0].numel() * 4
+        if fn == "inverse_V":
+            f = lib.QLA_F3_V_veq_M_inverse_V
+            call = lambda it: f(r.data_ptr(), a.data_ptr(), vb.data_ptr(), n3)      # noqa: E731
+            nbytes = 72 + 24 + rbytes
+        else:
+            kind = {"det": "C", "eigenvalsH": "V"}.get(fn, "M")
+            f = getattr(lib, f"QLA_F3_{kind}_veq_{fn}_M")
+            call = lambda it: f(r.data_ptr(), a.data_ptr(), n3)       # noqa: E731
+            nbytes = 72 + rbytes
+        ms = _timed(stream, call, 3, 1) / 3
+        out[f"F3_{fn}"] = {"n": n3, "ms": round(ms, 3), "matrices_per_s": round(n3 / (ms * 1e-3), 1),
+                           "GBs": round(nbytes * n3 / (ms * 1e-3) / 1e9, 1)}
+        del a, r
     torch.cuda.synchronize()
     return out
 

@@ -57,7 +57,7 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 #endif
 
 /* ------------------------------------------------------------------------ */
-/* a1/a2, a5-a8 and the rest of the Wilson-hop family: every eqop (eq, peq, */
+/* The Wilson-hop building blocks: every eqop (eq, peq,                    */
 /* meq, eqm), M or Ma, field or pointer operands, v- and x-forms, for       */
 /* V = M*V, H = M*spproj(D), D = sprecon(M*H), D = sprecon(M*spproj(D)) and */
 /* the multi-source V = sum_d M_d*V_d.  Reference: generated from           */
@@ -71,7 +71,7 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 #include "qla_mi355x_sites.h"
 
 /* ------------------------------------------------------------------------ */
-/* a3/a4: global reductions.  Reference: make_code_unary.pl:318-366         */
+/* Global reductions.  Reference: make_code_unary.pl:318-366               */
 /* (norm2: per component the square is formed in the operand precision and */
 /* added to a sum promoted per datatypes.pl:355-359, F -> D, D -> D) and    */
 /* make_code_binary.pl:110-170 (dot: sum += conj(a)*b with operands         */
@@ -83,37 +83,15 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 #include "qla_mi355x_fields.h"
 
 /* ------------------------------------------------------------------------ */
-/* a9-a12: batched matrix functions.  Reference: generated wrappers         */
-/* (expressions_tensor.pl:239-257) around lib/perl/code-templates/          */
-/* M_eq_exp_M.c:300-367 (Pade + scaling and squaring), M_eq_sqrt_M.c:136-173*/
-/* and M_eq_invsqrt_M.c:135-175 (Newton iteration, at most 20 steps).       */
-/* QLA_FN_* take the colour count first (prototype.pl:594-600); this build  */
-/* supports nc = 3..8 (other nc: no-op and an error is recorded).           */
+/* Colour-matrix functions: det, inverse, a^-1 b for vector / matrix b,     */
+/* exp, expA, expTA, sqrt, sqrtPH, invsqrt, invsqrtPH, log and the          */
+/* Hermitian eigenvalues, QLA_F3 / QLA_D3 / QLA_FN, every indexing form.    */
+/* Reference: build_qla.pl:1002-1066 and 1663-1690 around the algorithms of */
+/* the lib/perl/code-templates sources.  QLA_FN_* take the colour count    */
+/* first (prototype.pl:594-600); this build supports nc = 3..8 (other nc:   */
+/* no-op and an error is recorded; eigenvalsH: nc = 3 only).                */
 /* ------------------------------------------------------------------------ */
-void QLA_F3_M_veq_exp_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                        QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-void QLA_F3_M_veq_sqrt_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                         QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-void QLA_F3_M_veq_invsqrt_M(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                            QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-void QLA_D3_M_veq_exp_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                        QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-void QLA_D3_M_veq_sqrt_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                         QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-void QLA_D3_M_veq_invsqrt_M(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
-                            QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a, int n);
-#ifdef __cplusplus
-void QLA_FN_M_veq_exp_M(int nc, void *r, void *a, int n);
-void QLA_FN_M_veq_sqrt_M(int nc, void *r, void *a, int n);
-void QLA_FN_M_veq_invsqrt_M(int nc, void *r, void *a, int n);
-#else
-void QLA_FN_M_veq_exp_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
-                        QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
-void QLA_FN_M_veq_sqrt_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
-                         QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
-void QLA_FN_M_veq_invsqrt_M(int nc, QLA_F_Complex (*QLA_MI355X_RESTRICT r)[nc][nc],
-                            QLA_F_Complex (*QLA_MI355X_RESTRICT a)[nc][nc], int n);
-#endif
+#include "qla_mi355x_matfuncs.h"
 
 /* ------------------------------------------------------------------------ */
 /* Compressed-precision extension (no counterpart in the reference, which    */

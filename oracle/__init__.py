@@ -52,6 +52,7 @@ def load():
         getattr(lib, f"orc_{p}_dot_ld").argtypes = [vp, vp, vp, vp, i32, i32]
         for fn in ("exp", "sqrt", "invsqrt"):
             getattr(lib, f"orc_{p}_M_veq_{fn}_M").argtypes = [i32, vp, vp, i32]
+        getattr(lib, f"orc_{p}_matfunc").argtypes = [i32, i32, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -259,4 +260,34 @@ def matfn(fn: str, a: np.ndarray) -> np.ndarray:
     p = "F" if a.dtype == np.float32 else "D"
     r = np.empty_like(a)
     getattr(lib, f"orc_{p}_M_veq_{fn}_M")(nc, _ptr(r), _ptr(_need(a)), a.shape[0])
+    return r
+
+
+# function codes of orc_{F,D}_matfunc (qla_oracle.h)
+MATFUNC = {"exp": 0, "sqrt": 1, "invsqrt": 2, "det": 3, "inverse": 4, "inverse_V": 5, "inverse_M": 6,
+           "expA": 7, "expTA": 8, "sqrtPH": 9, "invsqrtPH": 10, "log": 11, "eigenvalsH": 12}
+
+
+def matfunc(fn: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    """One of the matrix functions on (n, nc, nc, 2) matrices ``a``.
+
+    det -> (n, 2); inverse_V (b: (n, nc, 2) vectors) and eigenvalsH ->
+    (n, nc, 2); the others (inverse_M with b: (n, nc, nc, 2)) -> matrices."""
+    lib = load()
+    n, nc = a.shape[0], a.shape[1]
+    p = "F" if a.dtype == np.float32 else "D"
+    if fn == "det":
+        shape = (n, 2)
+    elif fn in ("inverse_V", "eigenvalsH"):
+        shape = (n, nc, 2)
+    else:
+        shape = a.shape
+    if fn in ("inverse_V", "inverse_M"):
+        if b is None or b.dtype != a.dtype or b.shape[0] != n:
+            raise ValueError(f"{fn} needs a second operand of {n} records")
+    if fn == "eigenvalsH" and nc != 3:
+        raise ValueError("eigenvalsH: nc = 3 only")
+    r = np.empty(shape, dtype=a.dtype)
+    getattr(lib, f"orc_{p}_matfunc")(MATFUNC[fn], nc, _ptr(r), _ptr(_need(a)),
+                                     _ptr(_need(b)) if b is not None else None, n)
     return r

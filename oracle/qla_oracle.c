@@ -4,6 +4,7 @@
  * CPU-baseline leg of the benchmark; never by the product library.
  * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off).
  */
+#define _GNU_SOURCE /* sincosf / sincos, as lib/cmath/QLA_cexpi.c:4 selects */
 #include <math.h>
 #include <float.h>
 #include <string.h>
@@ -17,15 +18,32 @@
 #define REAL float
 #define P(x) F_##x
 #define F_EPS FLT_EPSILON
+#define SQRTP sqrtf
+#define FABSP fabsf
+#define ACOSP acosf
+#define CBRTP cbrtf
+#define SINCOSP sincosf
 #include "qla_oracle_impl.inc"
+#include "qla_oracle_matfunc.inc"
 #undef REAL
 #undef P
+#undef SQRTP
+#undef FABSP
+#undef ACOSP
+#undef CBRTP
+#undef SINCOSP
 
 /* double precision instantiation */
 #define REAL double
 #define P(x) D_##x
 #define D_EPS DBL_EPSILON
+#define SQRTP sqrt
+#define FABSP fabs
+#define ACOSP acos
+#define CBRTP cbrt
+#define SINCOSP sincos
 #include "qla_oracle_impl.inc"
+#include "qla_oracle_matfunc.inc"
 #undef REAL
 #undef P
 
@@ -89,6 +107,11 @@ void orc_F_M_veq_sqrt_M(int nc, float *r, const float *a, int n) { F_orc_M_veq_s
 void orc_D_M_veq_sqrt_M(int nc, double *r, const double *a, int n) { D_orc_M_veq_sqrt_M(nc, r, a, n); }
 void orc_F_M_veq_invsqrt_M(int nc, float *r, const float *a, int n) { F_orc_M_veq_invsqrt_M(nc, r, a, n); }
 void orc_D_M_veq_invsqrt_M(int nc, double *r, const double *a, int n) { D_orc_M_veq_invsqrt_M(nc, r, a, n); }
+
+void orc_F_matfunc(int fn, int nc, float *r, const float *a, const float *b, int n)
+{ F_orc_matfunc(fn, nc, r, a, b, n); }
+void orc_D_matfunc(int fn, int nc, double *r, const double *a, const double *b, int n)
+{ D_orc_matfunc(fn, nc, r, a, b, n); }
 
 void orc_set_threads(int nthreads)
 {

@@ -78,6 +78,17 @@ void orc_F_M_veq_sqrt_M(int nc, float *r, const float *a, int n);
 void orc_D_M_veq_sqrt_M(int nc, double *r, const double *a, int n);
 void orc_F_M_veq_invsqrt_M(int nc, float *r, const float *a, int n);
 void orc_D_M_veq_invsqrt_M(int nc, double *r, const double *a, int n);
+/* the other matrix functions (qla_oracle_matfunc.inc), n records each:
+ * fn 0 exp, 1 sqrt, 2 invsqrt (as above), 3 det (C_eq_det_M.c), 4 inverse
+ * (M_eq_inverse_M.c), 5 a^-1 b for a vector b (V_eq_M_inverse_V.c), 6 a^-1 b
+ * for a matrix b (M_eq_M_inverse_M.c), 7 expA, 8 expTA, 9 sqrtPH,
+ * 10 invsqrtPH, 11 log, 12 Hermitian eigenvalues (nc = 3 only).  Results:
+ * one complex for det, nc complexes for 5 and 12, an nc x nc matrix else.
+ * b is read by fn 5 and 6 only. */
+enum { OF_EXP = 0, OF_SQRT, OF_INVSQRT, OF_DET, OF_INVERSE, OF_INV_V, OF_INV_M, OF_EXPA, OF_EXPTA, OF_SQRTPH,
+       OF_INVSQRTPH, OF_LOG, OF_EIGH };
+void orc_F_matfunc(int fn, int nc, float *r, const float *a, const float *b, int n);
+void orc_D_matfunc(int fn, int nc, double *r, const double *a, const double *b, int n);
 
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);

@@ -30,13 +30,27 @@ real part of inner product     r_{v,x}eq_re_[p]T_dot_[p]T                  F3 D3
 precision conversion           T_{v,x}eq_[p]T  (float -> double, back)     DF3 FD3
 =============================  ==========================================  =======
 
+and the colour-matrix functions (build_qla.pl:1002-1066, 1663-1690) for
+QLA_F3, QLA_D3 and QLA_FN (leading ``int nc``, nc = 3..8):
+
+=============================  ==========================================
+family                         symbol pattern
+=============================  ==========================================
+determinant                    C_{v,x}eq_det_[p]M
+Hermitian eigenvalues (nc=3)   V_{v,x}eq_eigenvalsH_[p]M
+matrix function                M_{v,x}eq_<f>_[p]M, f in inverse, sqrt,
+                               sqrtPH, invsqrt, invsqrtPH, exp, expA,
+                               expTA, log
+inverse times vector / matrix  {V,M}_{v,x}eq_[p]M_inverse_[p]{V,M}
+=============================  ==========================================
+
+each also in the per-operand index forms (``xM_eq_expA_xM`` ...).
+
 It writes
 
-* ``include/qla_mi355x_sites.h`` and ``include/qla_mi355x_fields.h`` -- the
-  prototypes, and
-* ``qla_amd/csrc/qla_sites_abi.inc`` and ``qla_amd/csrc/qla_fields_abi.inc`` --
-  their definitions, one call each into the generic entry helpers of
-  ``qla_capi.hip``.
+* ``include/qla_mi355x_{sites,fields,matfuncs}.h`` -- the prototypes, and
+* ``qla_amd/csrc/qla_{sites,fields,matfuncs}_abi.inc`` -- their definitions,
+  one call each into the generic entry helpers of ``qla_capi.hip``.
 
 Both files are committed; ``python -m qla_amd.abi_gen`` rewrites them and a CPU
 test checks that they match this table.
@@ -52,6 +66,8 @@ HEADER_OUT = HERE.parent / "include" / "qla_mi355x_sites.h"
 INC_OUT = HERE / "csrc" / "qla_sites_abi.inc"
 FIELDS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_fields.h"
 FIELDS_INC_OUT = HERE / "csrc" / "qla_fields_abi.inc"
+MATFUNCS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_matfuncs.h"
+MATFUNCS_INC_OUT = HERE / "csrc" / "qla_matfuncs_abi.inc"
 
 PRECS = ("F3", "D3")
 EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use them
@@ -242,6 +258,112 @@ class ConvVariant:
                 f"{idx}, {ir}, {ia}, n, \"{self.name}\");")
 
 
+# colour-matrix functions: name -> (kernel code, result type, second operand type)
+MATFUNCS = {
+    "exp": (0, "M", None), "sqrt": (1, "M", None), "invsqrt": (2, "M", None), "det": (3, "C", None),
+    "inverse": (4, "M", None), "inverse_V": (5, "V", "V"), "inverse_M": (6, "M", "M"),
+    "expA": (7, "M", None), "expTA": (8, "M", None), "sqrtPH": (9, "M", None), "invsqrtPH": (10, "M", None),
+    "log": (11, "M", None), "eigenvalsH": (12, "V", None),
+}
+MATFUNC_LIBS = ("F3", "D3", "FN")
+UNARY_OWN_FORMS = [(True, False), (False, True), (True, True)]
+
+
+@dataclass(frozen=True)
+class MatVariant:
+    """A colour-matrix function r = f(a), or r = a^-1 b (inverse_V / inverse_M)."""
+    lib: str        # "F3", "D3" or "FN" (float, leading int nc)
+    fn: str         # key of MATFUNCS
+    xf: bool
+    pa: bool
+    pb: bool = False
+    ir: bool = False    # per-operand index forms (QLA_INDEX_ALL)
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def code(self) -> int:
+        return MATFUNCS[self.fn][0]
+
+    @property
+    def dest(self) -> str:
+        return MATFUNCS[self.fn][1]
+
+    @property
+    def src2(self):
+        return MATFUNCS[self.fn][2]
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ia or self.ib
+
+    @property
+    def name(self) -> str:
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + self.dest
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + "M"
+        if self.src2:
+            b = ("p" if self.pb else "") + ("x" if self.ib else "") + self.src2
+            return f"QLA_{self.lib}_{r}_{form}eq_{a}_inverse_{b}"
+        return f"QLA_{self.lib}_{r}_{form}eq_{self.fn}_{a}"
+
+    def params(self) -> list[tuple[str, str]]:
+        fn_lib = self.lib == "FN"
+        real = "D" if self.lib == "D3" else "F"
+
+        def rec(t: str, var: str, ptr: bool) -> str:
+            if t == "C":
+                return f"QLA_{real}_Complex *QLA_MI355X_RESTRICT {var}"
+            if fn_lib:
+                return f"void *{var}"
+            return f"QLA_{self.lib}_{TYPE_NAMES[t]} *QLA_MI355X_RESTRICT {'*' if ptr else ''}{var}"
+        out = [("i", "int nc")] if fn_lib else []
+        out.append(("p", rec(self.dest, "r", False)))
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", rec("M", "a", self.pa)))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        if self.src2:
+            out.append(("p", rec(self.src2, "b", self.pb)))
+            if self.ib:
+                out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.lib == "D3" else 0
+        nc = "nc" if self.lib == "FN" else "3"
+        b = "(const void *)b" if self.src2 else "nullptr"
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"matfunc_entry({prec}, {self.code}, {nc}, {int(self.pa)}, {int(self.pb)}, (void *)r, "
+                f"(const void *)a, {b}, {idx}, {own}, n, \"{self.name}\");")
+
+
+def matfuncs() -> list[MatVariant]:
+    out = []
+    for lib in MATFUNC_LIBS:
+        for fn in MATFUNCS:
+            if MATFUNCS[fn][2]:
+                for xf in (False, True):
+                    for pa in (False, True):
+                        for pb in (False, True):
+                            out.append(MatVariant(lib, fn, xf, pa, pb))
+                for ir, ia, ib in OWN_INDEX_FORMS:
+                    out.append(MatVariant(lib, fn, False, False, False, ir, ia, ib))
+            else:
+                out += [MatVariant(lib, fn, xf, pa) for xf in (False, True) for pa in (False, True)]
+                out += [MatVariant(lib, fn, False, False, False, ir, ia) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def reductions() -> list[RedVariant]:
     out = []
     for prec in ("F3", "D3", "DF3"):
@@ -267,8 +389,8 @@ def conversions() -> list[ConvVariant]:
 
 
 def all_symbols() -> list:
-    """Every generated symbol (site family, reductions, conversions)."""
-    return variants() + reductions() + conversions()
+    """Every generated symbol (site family, reductions, conversions, matrix functions)."""
+    return variants() + reductions() + conversions() + matfuncs()
 
 
 def variants() -> list[Variant]:
@@ -425,8 +547,74 @@ def fields_inc_text() -> str:
     return "\n".join(lines)
 
 
+_MATFUNC_DOC = {
+    "det": ("determinant", "code-templates/C_eq_det_M.c (closed form for nc = 3, pivoted LU otherwise)"),
+    "inverse": ("matrix inverse", "code-templates/M_eq_inverse_M.c (adjugate for nc = 3, Gauss-Jordan otherwise)"),
+    "inverse_V": ("a^-1 b for a colour vector b",
+                  "code-templates/V_eq_M_inverse_V.c (adjugate for nc = 3, Gauss-Jordan otherwise)"),
+    "inverse_M": ("a^-1 b for a colour matrix b", "code-templates/M_eq_M_inverse_M.c (Gauss-Jordan)"),
+    "exp": ("matrix exponential", "code-templates/M_eq_exp_M.c (scaled Pade(3,3), squaring)"),
+    "expA": ("exponential of an anti-Hermitian matrix",
+             "code-templates/M_eq_expA_M.c (Cayley-Hamilton for nc = 3, the exp branch otherwise)"),
+    "expTA": ("exponential of a traceless anti-Hermitian matrix",
+              "code-templates/M_eq_expTA_M.c (Cayley-Hamilton for nc = 3, the exp branch otherwise)"),
+    "sqrt": ("matrix square root", "code-templates/M_eq_sqrt_M.c (Newton iteration)"),
+    "sqrtPH": ("square root of a positive Hermitian matrix",
+               "code-templates/M_eq_sqrtPH_M.c (eigenvalue closed form for nc = 3, the sqrt branch otherwise)"),
+    "invsqrt": ("inverse square root", "code-templates/M_eq_invsqrt_M.c (Newton iteration)"),
+    "invsqrtPH": ("inverse square root of a positive Hermitian matrix",
+                  "code-templates/M_eq_invsqrtPH_M.c (closed form for nc = 3, the invsqrt branch otherwise)"),
+    "log": ("matrix logarithm", "code-templates/M_eq_log_M.c (4 square roots, Pade, rescale)"),
+    "eigenvalsH": ("eigenvalues of a Hermitian matrix, nc = 3 only (template order, imaginary parts 0)",
+                   "code-templates/V_eq_eigenvalsH_M.c:103-161"),
+}
+
+
+def matfuncs_header_text() -> str:
+    lines = [
+        "/*",
+        " * qla_mi355x_matfuncs.h -- colour-matrix functions of the QLA C ABI on MI355X.",
+        " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
+        " * semantics are documented there).",
+        " *",
+        " * Names and argument order are QLA's (lib/perl/build_qla.pl:1002-1066 and",
+        " * 1663-1690, prototype.pl:450-688).  QLA_FN_* take the number of colours",
+        " * first (nc = 3..8 here; other values record an error and write nothing)",
+        " * and their record arguments are untyped: nc x nc complex floats per matrix,",
+        " * nc per vector, or a pointer array of such records for a `p` operand.  A `p`",
+        " * operand is a pointer array; x-forms use every operand at index[i].",
+        " */",
+        "#ifndef QLA_MI355X_MATFUNCS_H",
+        "#define QLA_MI355X_MATFUNCS_H",
+    ]
+    last = None
+    for v in matfuncs():
+        key = (v.lib, v.fn)
+        if key != last:
+            what, ref = _MATFUNC_DOC[v.fn]
+            lines += ["", f"/* {v.lib}: {what}.", f" * Reference: {ref}. */"]
+            last = key
+        lines.append(_decl(v) + ";")
+    lines += ["", "#endif", ""]
+    return "\n".join(lines)
+
+
+def matfuncs_inc_text() -> str:
+    lines = [
+        "// qla_matfuncs_abi.inc -- definitions of the colour-matrix functions of the C ABI.",
+        "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        "// Included by qla_capi.hip inside extern \"C\"; matfunc_entry is there.",
+    ]
+    for v in matfuncs():
+        lines += [_decl(v), "{", "    " + v.body(), "}"]
+    lines.append("")
+    return "\n".join(lines)
+
+
 _OUTPUTS = ((HEADER_OUT, header_text), (INC_OUT, inc_text), (FIELDS_HEADER_OUT, fields_header_text),
-            (FIELDS_INC_OUT, fields_inc_text))
+            (FIELDS_INC_OUT, fields_inc_text), (MATFUNCS_HEADER_OUT, matfuncs_header_text),
+            (MATFUNCS_INC_OUT, matfuncs_inc_text))
 
 
 def write() -> None:

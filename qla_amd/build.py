@@ -40,11 +40,13 @@ CXXFLAGS = [
 
 
 # Sources compiled more than once, each time with its own defines: the site
-# kernels once per (precision, kind) and the multi-source kernels once per
-# precision, so the kernel instantiations spread over parallel compiles.
+# kernels once per (precision, kind), the multi-source kernels once per
+# precision and the matrix functions once per (precision, nc), so the kernel
+# instantiations spread over parallel compiles.
 MULTI_BUILD = {
     "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in range(4)],
     "qla_multi.hip": [{"QLA_SITE_PREC": p} for p in (0, 1)],
+    "qla_matfn.hip": [{"QLA_MATFN_UNIT": u} for u in range(7)],
 }
 
 

@@ -122,6 +122,24 @@ void convert_entry(int to_float, int ncomp, int pa, void *r, const void *a, cons
     finish(qla_launch_precision(to_float, ncomp, r, a, index, ir, ia, n, pa, ts.stream), name);
 }
 
+// One colour-matrix function (qla_matfuncs_abi.inc): fn codes of
+// qla_launch_matfunc; prec 0 takes nc = 3..8 (QLA_F3 passes 3), prec 1 nc = 3.
+// An unsupported nc or function records hipErrorInvalidValue and writes
+// nothing.
+void matfunc_entry(int prec, int fn, int nc, int pa, int pb, void *r, const void *a, const void *b,
+                   const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    if (nc < 3 || nc > 8 || (prec == 1 && nc != 3) || (fn == 12 && nc != 3)) {
+        record(hipErrorInvalidValue, name);
+        return;
+    }
+    unsigned int *ctr = counter_for_device();
+    if (!ctr) return;
+    const qla_matfunc_args p = { r, a, b, index, ir, ia, ib, n, pa, pb };
+    finish(qla_launch_matfunc(prec, fn, nc, p, ctr, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -289,6 +307,21 @@ hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const 
     return tab[prec][kind](eqop, adj, xf, p, tpw, st);
 }
 
+hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, unsigned int *counter,
+                              hipStream_t st)
+{
+    if (prec == 1) return nc == 3 ? qla_matfunc_launch_u1(fn, p, counter, st) : hipErrorInvalidValue;
+    switch (nc) {
+    case 3: return qla_matfunc_launch_u0(fn, p, counter, st);
+    case 4: return qla_matfunc_launch_u2(fn, p, counter, st);
+    case 5: return qla_matfunc_launch_u3(fn, p, counter, st);
+    case 6: return qla_matfunc_launch_u4(fn, p, counter, st);
+    case 7: return qla_matfunc_launch_u5(fn, p, counter, st);
+    case 8: return qla_matfunc_launch_u6(fn, p, counter, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t qla_launch_multi(int prec, int eqop, int adj, int xf, const qla_multi_args &p, int tpw, hipStream_t st)
 {
     if (prec == 0) return qla_multi_launch_p0(eqop, adj, xf, p, tpw, st);
@@ -317,30 +350,14 @@ void QLA_MI355X_set_tiles_per_wave(int tpw)
     qla_set_cmp_tiles_per_wave(tpw);
 }
 
-// ------------------------------------------- a1/a2, a5-a8: the Wilson-hop family
+// ------------------------------------------------ the Wilson-hop building blocks
 #include "qla_sites_abi.inc"
 
-// ------------------------------- a3/a4 and the rest: reductions, conversions
+// ---------------------------------------------- reductions and conversions
 #include "qla_fields_abi.inc"
 
-// ----------------------------------------------------------------- a9 - a12
-#define QLA_MATFN(NAME, PREC, FN, NC)                                                  \
-    {                                                                                  \
-        if (n <= 0) return;                                                            \
-        unsigned int *ctr = counter_for_device();                                      \
-        if (!ctr) return;                                                              \
-        finish(qla_launch_matfn(PREC, FN, NC, r, a, n, ctr, ts.stream), NAME);         \
-    }
-void QLA_F3_M_veq_exp_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_exp_M", 0, 0, 3)
-void QLA_F3_M_veq_sqrt_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_sqrt_M", 0, 1, 3)
-void QLA_F3_M_veq_invsqrt_M(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n) QLA_MATFN("QLA_F3_M_veq_invsqrt_M", 0, 2, 3)
-void QLA_D3_M_veq_exp_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_exp_M", 1, 0, 3)
-void QLA_D3_M_veq_sqrt_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_sqrt_M", 1, 1, 3)
-void QLA_D3_M_veq_invsqrt_M(QLA_D3_ColorMatrix *r, QLA_D3_ColorMatrix *a, int n) QLA_MATFN("QLA_D3_M_veq_invsqrt_M", 1, 2, 3)
-void QLA_FN_M_veq_exp_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_exp_M", 0, 0, nc)
-void QLA_FN_M_veq_sqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_sqrt_M", 0, 1, nc)
-void QLA_FN_M_veq_invsqrt_M(int nc, void *r, void *a, int n) QLA_MATFN("QLA_FN_M_veq_invsqrt_M", 0, 2, nc)
-#undef QLA_MATFN
+// ------------------------------------------------------ colour-matrix functions
+#include "qla_matfuncs_abi.inc"
 
 // ----------------------------------------------------------------- compressed
 void QLA_MI355X_F3_M_pack12(void *r, int fmt, QLA_F3_ColorMatrix *a, int n)

@@ -61,11 +61,28 @@ hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a
 hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a, const int *index, const int *ir,
                                 const int *ia, int n, int pa, hipStream_t st);
 
-// Matrix functions on n nc x nc matrices.  fn: 0 exp, 1 sqrt, 2 invsqrt.
-// prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned int
-// the launch uses as its work counter (reset on the stream first).
-hipError_t qla_launch_matfn(int prec, int fn, int nc, void *r, const void *a, int n, unsigned int *counter,
-                            hipStream_t st);
+// Colour-matrix functions on n nc x nc matrices.  fn: 0 exp, 1 sqrt,
+// 2 invsqrt, 3 det, 4 inverse, 5 a^-1 b (vector b), 6 a^-1 b (matrix b),
+// 7 expA, 8 expTA, 9 sqrtPH, 10 invsqrtPH, 11 log, 12 Hermitian eigenvalues
+// (nc = 3 only).  Step i reads a at ia[i], index[i] (x-form) or i, through
+// the pointer array when pa (likewise b), and writes r at ir[i], index[i] or
+// i.  prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned
+// int the launch uses as its work counter (reset on the stream first).
+struct qla_matfunc_args {
+    void *r;
+    const void *a, *b;
+    const int *index, *ir, *ia, *ib;
+    int n, pa, pb;
+};
+hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, unsigned int *counter,
+                              hipStream_t st);
+hipError_t qla_matfunc_launch_u0(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u1(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u2(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
 
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,

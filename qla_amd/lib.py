@@ -20,16 +20,14 @@ from .abi_gen import all_symbols
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libqla_mi355x.so"
-HEADERS = tuple(HERE.parent / "include" / h for h in ("qla_mi355x.h", "qla_mi355x_sites.h", "qla_mi355x_fields.h"))
+HEADERS = tuple(HERE.parent / "include" / h for h in ("qla_mi355x.h", "qla_mi355x_sites.h", "qla_mi355x_fields.h",
+                                                     "qla_mi355x_matfuncs.h"))
 
 vp = C.c_void_p
 i32 = C.c_int
 
 # argument kinds of each exported symbol: 'p' pointer, 'i' int, 'r' void return
 _SIGS: dict[str, tuple[type, list]] = {
-    # a9-a12
-    **{f"QLA_{p}3_M_veq_{fn}_M": (None, [vp, vp, i32]) for p in ("F", "D") for fn in ("exp", "sqrt", "invsqrt")},
-    **{f"QLA_FN_M_veq_{fn}_M": (None, [i32, vp, vp, i32]) for fn in ("exp", "sqrt", "invsqrt")},
     # compressed-precision extension
     "QLA_MI355X_F3_M_pack12": (None, [vp, i32, vp, i32]),
     "QLA_MI355X_F3_M_unpack12": (None, [vp, i32, vp, i32]),
@@ -60,9 +58,9 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_version": (C.c_char_p, []),
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
-# the generated families: a1/a2, a5-a8 and the rest of the Wilson-hop family
-# (qla_mi355x_sites.h), a3/a4 and the rest of the reductions, conversions
-# (qla_mi355x_fields.h)
+# the generated families: the Wilson-hop building blocks (qla_mi355x_sites.h),
+# the reductions and conversions (qla_mi355x_fields.h) and the colour-matrix
+# functions (qla_mi355x_matfuncs.h)
 _SIGS.update({v.name: (None, [vp if k == "p" else i32 for k in v.ctypes_kinds()]) for v in all_symbols()})
 
 class F3Job(C.Structure):

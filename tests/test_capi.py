@@ -38,7 +38,12 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 1200
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 1660
+    assert len(every) == len(set(every)) == 1981
+    mf = [v.name for v in abi_gen.matfuncs()]
+    assert len(mf) == 321
+    for s in ("QLA_F3_M_veq_expA_M", "QLA_D3_C_xeq_det_pM", "QLA_FN_M_veq_log_M", "QLA_F3_V_veq_eigenvalsH_M",
+              "QLA_D3_V_veq_pM_inverse_pV", "QLA_F3_xM_eq_xM_inverse_M", "QLA_FN_M_veq_exp_M"):
+        assert s in mf, s
     # QLA names of the reference's own callers and tests
     for s in ("QLA_F3_D_vpeq_spproj_M_times_pD", "QLA_D3_V_vpeq_nM_times_npV", "QLA_F3_H_veq_spproj_Ma_times_pD",
               "QLA_F3_D_xmeq_sprecon_pMa_times_pH", "QLA_D3_V_xeqm_npMa_times_nV",
@@ -51,7 +56,11 @@ def test_header_compiles_as_c_and_cxx(tmp_path):
     src = tmp_path / "use.c"
     src.write_text('#include "qla_mi355x.h"\n'
                    'void f(QLA_F3_DiracFermion *r, QLA_F3_ColorMatrix *u, QLA_F3_DiracFermion **p, int n)\n'
-                   '{ QLA_F3_D_vpeq_spproj_M_times_pD(r, u, p, 0, 1, n); }\n')
+                   '{ QLA_F3_D_vpeq_spproj_M_times_pD(r, u, p, 0, 1, n); }\n'
+                   'void g(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix **a, QLA_F_Complex *d, int *ix, int n)\n'
+                   '{ QLA_F3_M_veq_expTA_pM(r, a, n); QLA_F3_C_xeq_det_M(d, r, ix, n); }\n'
+                   'void h(int nc, QLA_F_Complex (*r)[4][4], QLA_F_Complex (*a)[4][4], int n)\n'
+                   '{ QLA_FN_M_veq_log_M(nc, r, a, n); }\n')
     for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
         args = [cc, std, "-fsyntax-only", "-Wall", "-Werror", "-I", str(inc)]
         if cc == "g++":
