@@ -189,4 +189,9 @@ const char *QLA_MI355X_version(void);
 #ifdef __cplusplus
 }
 #endif
+
+/* QLA's generic names (QLA_V_veq_M_times_V, QLA_F_..., QLA_3_..., generic
+ * types), selected by QLA_Precision / QLA_Colors as in the reference. */
+#include "qla_mi355x_generic.h"
+
 #endif /* QLA_MI355X_H */

@@ -48,7 +48,8 @@ each also in the per-operand index forms (``xM_eq_expA_xM`` ...).
 
 It writes
 
-* ``include/qla_mi355x_{sites,fields,matfuncs}.h`` -- the prototypes, and
+* ``include/qla_mi355x_{sites,fields,matfuncs}.h`` -- the prototypes,
+* ``include/qla_mi355x_generic.h`` -- QLA's generic names for them, and
 * ``qla_amd/csrc/qla_{sites,fields,matfuncs}_abi.inc`` -- their definitions,
   one call each into the generic entry helpers of ``qla_capi.hip``.
 
@@ -68,6 +69,7 @@ FIELDS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_fields.h"
 FIELDS_INC_OUT = HERE / "csrc" / "qla_fields_abi.inc"
 MATFUNCS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_matfuncs.h"
 MATFUNCS_INC_OUT = HERE / "csrc" / "qla_matfuncs_abi.inc"
+GENERIC_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_generic.h"
 
 PRECS = ("F3", "D3")
 EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use them
@@ -612,9 +614,115 @@ def matfuncs_inc_text() -> str:
     return "\n".join(lines)
 
 
+# generic names (prototype.pl:556-575, make_define_map :423-441, selection
+# :216-260): for a library QLA_<P><C>, the fully generic QLA_<rest> when
+# QLA_Precision == P and QLA_Colors == C, the colour-generic QLA_<P>_<rest>
+# when QLA_Colors == C and the precision-generic QLA_<C>_<rest> when
+# QLA_Precision == P; the mixed-precision libraries only get colour-generic
+# names; N-colour names inject QLA_Nc as the first argument.
+_GENERIC_LIBS = {
+    "F3": ("'F'", "3"), "D3": ("'D'", "3"), "FN": ("'F'", "'N'"), "DF3": (None, "3"), "FD3": (None, "3"),
+}
+
+
+def _split(name: str) -> tuple[str, str]:
+    body = name[len("QLA_"):]
+    lib, rest = body.split("_", 1)
+    return lib, rest
+
+
+def generic_header_text() -> str:
+    names = [v.name for v in all_symbols()]
+    groups: dict[tuple[str, str], list[str]] = {}
+    for nm in names:
+        lib, rest = _split(nm)
+        prec, col = _GENERIC_LIBS[lib]
+        p_letter, c_letter = lib[:-1], lib[-1]
+        nc = lib == "FN"
+
+        def mapping(generic: str) -> str:
+            if nc:
+                return f"#define {generic}(...) {nm}(QLA_Nc, __VA_ARGS__)"
+            return f"#define {generic} {nm}"
+        if prec is not None:
+            groups.setdefault((f"QLA_Precision == {prec} && QLA_Colors == {col}", "generic"), []).append(
+                mapping(f"QLA_{rest}"))
+            groups.setdefault((f"QLA_Precision == {prec}", "precision"), []).append(
+                mapping(f"QLA_{c_letter}_{rest}"))
+        groups.setdefault((f"QLA_Colors == {col}", "colour"), []).append(mapping(f"QLA_{p_letter}_{rest}"))
+    lines = [
+        "/*",
+        " * qla_mi355x_generic.h -- QLA's generic function and type names for the",
+        " * specific symbols of this library.",
+        " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        " * Included by qla_mi355x.h.",
+        " *",
+        " * As in the reference (include/qla.h:8-100, lib/perl/prototype.pl:160-260,",
+        " * 423-441, 556-575): QLA_Precision ('F' or 'D', default 'F'; or",
+        " * QLA_PrecisionInt 1 / 2) and QLA_Colors (3 or 'N'; or QLA_Nc, default 3)",
+        " * select the library behind the fully generic names (QLA_V_veq_M_times_V),",
+        " * the colour-generic names (QLA_F_V_veq_M_times_V, QLA_DF_r_veq_norm2_D)",
+        " * and the precision-generic names (QLA_3_V_veq_M_times_V).  N-colour names",
+        " * pass QLA_Nc as the leading nc argument.",
+        " */",
+        "#ifndef QLA_MI355X_GENERIC_H",
+        "#define QLA_MI355X_GENERIC_H",
+        "",
+        "#ifndef QLA_Precision",
+        "#  if !defined(QLA_PrecisionInt) || QLA_PrecisionInt == 1",
+        "#    define QLA_Precision 'F'",
+        "#  elif QLA_PrecisionInt == 2",
+        "#    define QLA_Precision 'D'",
+        "#  else",
+        "#    error \"qla_mi355x: QLA_PrecisionInt must be 1 (F) or 2 (D)\"",
+        "#  endif",
+        "#endif",
+        "#ifndef QLA_Colors",
+        "#  ifndef QLA_Nc",
+        "#    define QLA_Nc 3",
+        "#  endif",
+        "#  if QLA_Nc == 1 || QLA_Nc == 2 || QLA_Nc == 3",
+        "#    define QLA_Colors QLA_Nc",
+        "#  elif QLA_Nc > 0",
+        "#    define QLA_Colors 'N'",
+        "#  else",
+        "#    error \"qla_mi355x: bad QLA_Nc\"",
+        "#  endif",
+        "#elif !defined(QLA_Nc)",
+        "#  if QLA_Colors == 'N'",
+        "#    define QLA_Nc 3",
+        "#  else",
+        "#    define QLA_Nc QLA_Colors",
+        "#  endif",
+        "#endif",
+        "",
+        "/* generic types (include/qla_types.h:380-420 maps them the same way) */",
+        "#ifndef _QLA_TYPES_H",
+        "#if QLA_Precision == 'F'",
+        "typedef QLA_F_Real QLA_Real;",
+        "typedef QLA_F_Complex QLA_Complex;",
+        "#elif QLA_Precision == 'D'",
+        "typedef QLA_D_Real QLA_Real;",
+        "typedef QLA_D_Complex QLA_Complex;",
+        "#endif",
+        "#if QLA_Colors == 3",
+    ]
+    for t in TYPE_NAMES.values():
+        lines += [f"typedef QLA_F3_{t} QLA_F_{t};", f"typedef QLA_D3_{t} QLA_D_{t};"]
+    lines += ["#if QLA_Precision == 'F'"]
+    lines += [f"typedef QLA_F3_{t} QLA_{t};" for t in TYPE_NAMES.values()]
+    lines += ["#elif QLA_Precision == 'D'"]
+    lines += [f"typedef QLA_D3_{t} QLA_{t};" for t in TYPE_NAMES.values()]
+    lines += ["#endif", "#endif", "#endif /* _QLA_TYPES_H */"]
+    for (cond, _), defs in groups.items():
+        lines += ["", f"#if {cond}"] + defs + ["#endif"]
+    lines += ["", "#endif", ""]
+    return "\n".join(lines)
+
+
 _OUTPUTS = ((HEADER_OUT, header_text), (INC_OUT, inc_text), (FIELDS_HEADER_OUT, fields_header_text),
             (FIELDS_INC_OUT, fields_inc_text), (MATFUNCS_HEADER_OUT, matfuncs_header_text),
-            (MATFUNCS_INC_OUT, matfuncs_inc_text))
+            (MATFUNCS_INC_OUT, matfuncs_inc_text), (GENERIC_HEADER_OUT, generic_header_text))
 
 
 def write() -> None:

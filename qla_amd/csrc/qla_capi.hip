@@ -134,10 +134,8 @@ void matfunc_entry(int prec, int fn, int nc, int pa, int pb, void *r, const void
         record(hipErrorInvalidValue, name);
         return;
     }
-    unsigned int *ctr = counter_for_device();
-    if (!ctr) return;
     const qla_matfunc_args p = { r, a, b, index, ir, ia, ib, n, pa, pb };
-    finish(qla_launch_matfunc(prec, fn, nc, p, ctr, ts.stream), name);
+    finish(qla_launch_matfunc(prec, fn, nc, p, ts.stream), name);
 }
 
 // compressed-field reductions: result doubles to r (host or device)
@@ -307,17 +305,16 @@ hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const 
     return tab[prec][kind](eqop, adj, xf, p, tpw, st);
 }
 
-hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, unsigned int *counter,
-                              hipStream_t st)
+hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, hipStream_t st)
 {
-    if (prec == 1) return nc == 3 ? qla_matfunc_launch_u1(fn, p, counter, st) : hipErrorInvalidValue;
+    if (prec == 1) return nc == 3 ? qla_matfunc_launch_u1(fn, p, st) : hipErrorInvalidValue;
     switch (nc) {
-    case 3: return qla_matfunc_launch_u0(fn, p, counter, st);
-    case 4: return qla_matfunc_launch_u2(fn, p, counter, st);
-    case 5: return qla_matfunc_launch_u3(fn, p, counter, st);
-    case 6: return qla_matfunc_launch_u4(fn, p, counter, st);
-    case 7: return qla_matfunc_launch_u5(fn, p, counter, st);
-    case 8: return qla_matfunc_launch_u6(fn, p, counter, st);
+    case 3: return qla_matfunc_launch_u0(fn, p, st);
+    case 4: return qla_matfunc_launch_u2(fn, p, st);
+    case 5: return qla_matfunc_launch_u3(fn, p, st);
+    case 6: return qla_matfunc_launch_u4(fn, p, st);
+    case 7: return qla_matfunc_launch_u5(fn, p, st);
+    case 8: return qla_matfunc_launch_u6(fn, p, st);
     default: return hipErrorInvalidValue;
     }
 }

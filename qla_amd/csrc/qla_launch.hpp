@@ -66,23 +66,21 @@ hipError_t qla_launch_precision(int to_float, int ncomp, void *r, const void *a,
 // 7 expA, 8 expTA, 9 sqrtPH, 10 invsqrtPH, 11 log, 12 Hermitian eigenvalues
 // (nc = 3 only).  Step i reads a at ia[i], index[i] (x-form) or i, through
 // the pointer array when pa (likewise b), and writes r at ir[i], index[i] or
-// i.  prec 0: nc = 3..8; prec 1: nc = 3.  `counter` is one device unsigned
-// int the launch uses as its work counter (reset on the stream first).
+// i.  prec 0: nc = 3..8; prec 1: nc = 3.  n < 2^31 (one thread per matrix).
 struct qla_matfunc_args {
     void *r;
     const void *a, *b;
     const int *index, *ir, *ia, *ib;
     int n, pa, pb;
 };
-hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, unsigned int *counter,
-                              hipStream_t st);
-hipError_t qla_matfunc_launch_u0(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u1(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u2(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
-hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st);
+hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u0(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u1(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u2(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t st);
 
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,

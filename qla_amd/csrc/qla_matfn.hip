@@ -11,11 +11,12 @@
 // the general kernels; the Hermitian eigenvalues exist for nc = 3 only (the
 // template's QR iteration for larger nc is not provided).
 //
-// One thread owns one matrix.  The per-matrix work of exp, sqrt, invsqrt and
-// log is data dependent (squarings, Newton steps), so matrices are handed out
-// in wave-sized batches from a global counter: a wavefront that finishes a
-// cheap batch takes the next one instead of idling until a fixed grid slice
-// is done.  Every wave leaves the loop once its claim is past the end.
+// One thread owns one matrix; the grid has one thread per matrix, so the
+// hardware dispatcher balances the data-dependent work of exp, sqrt,
+// invsqrt and log (squarings, Newton steps) by starting a new workgroup
+// wherever one retires.  (A global work counter claimed per wave costs one
+// same-address atomic per 64 matrices, which serialises at the L2 and capped
+// every function at about 4.4e9 matrices/s.)
 #include "qla_matfn.hpp"
 #include "qla_launch.hpp"
 
@@ -75,80 +76,61 @@ __device__ __forceinline__ const cpx<T> *operand(const void *base, int pointer, 
     return (const cpx<T> *)base + (size_t)site * ncomp;
 }
 
+constexpr int MF_BLOCK = 256;
+
 template <class T, int NC, int FN>
-__global__ void __launch_bounds__(256) k_matfunc(qla_matfunc_args p, unsigned int *__restrict__ counter)
+__global__ void __launch_bounds__(MF_BLOCK) k_matfunc(qla_matfunc_args p)
 {
     typedef fn_shape<FN, NC> S;
-    const int lane = lane_id();
-    for (;;) {
-        unsigned int batch = 0;
-        if (lane == 0) batch = atomicAdd(counter, 1u);
-        batch = __shfl(batch, 0, WAVE);
-        if ((long long)batch * WAVE >= p.n) break;
-        const long long i = (long long)batch * WAVE + lane;
-        if (i < p.n) {
-            const long long site = p.index ? (long long)p.index[i] : i;
-            const long long rs = p.ir ? (long long)p.ir[i] : site;
-            const long long as = p.ia ? (long long)p.ia[i] : site;
-            mat<T, NC> m;
-            const cpx<T> *src = operand<T>(p.a, p.pa, as, NC * NC);
-            LOOP(k, NC * NC) m.e[k] = src[k];
-            cpx<T> bv[S::B > 0 ? S::B : 1];
-            if constexpr (S::B > 0) {
-                const long long bs = p.ib ? (long long)p.ib[i] : site;
-                const cpx<T> *bsrc = operand<T>(p.b, p.pb, bs, S::B);
-                LOOP(k, S::B) bv[k] = bsrc[k];
-            }
-            cpx<T> out[S::OUT];
-            evaluate<T, NC, FN>(out, m, bv);
-            cpx<T> *dst = (cpx<T> *)p.r + (size_t)rs * S::OUT;
-            LOOP(k, S::OUT) dst[k] = out[k];
-        }
+    const long long i = (long long)blockIdx.x * MF_BLOCK + threadIdx.x;
+    if (i >= p.n) return;
+    const long long site = p.index ? (long long)p.index[i] : i;
+    const long long rs = p.ir ? (long long)p.ir[i] : site;
+    const long long as = p.ia ? (long long)p.ia[i] : site;
+    mat<T, NC> m;
+    const cpx<T> *src = operand<T>(p.a, p.pa, as, NC * NC);
+    LOOP(k, NC * NC) m.e[k] = src[k];
+    cpx<T> bv[S::B > 0 ? S::B : 1];
+    if constexpr (S::B > 0) {
+        const long long bs = p.ib ? (long long)p.ib[i] : site;
+        const cpx<T> *bsrc = operand<T>(p.b, p.pb, bs, S::B);
+        LOOP(k, S::B) bv[k] = bsrc[k];
     }
+    cpx<T> out[S::OUT];
+    evaluate<T, NC, FN>(out, m, bv);
+    cpx<T> *dst = (cpx<T> *)p.r + (size_t)rs * S::OUT;
+    LOOP(k, S::OUT) dst[k] = out[k];
 }
 
-namespace {
-__global__ void k_zero_counter(unsigned int *c) { *c = 0u; }
-} // namespace
-
 template <class T, int NC, int FN>
-static hipError_t launch_t(const qla_matfunc_args &p, unsigned int *counter, hipStream_t st)
+static hipError_t launch_t(const qla_matfunc_args &p, hipStream_t st)
 {
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    // enough resident waves to cover the chip; the counter balances the rest
-    const long long waves_needed = ((long long)p.n + WAVE - 1) / WAVE;
-    long long blocks = (waves_needed + 3) / 4;
-    const long long cap = (long long)cus * 8;
-    if (blocks > cap) blocks = cap;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, st, counter);
-    hipLaunchKernelGGL((k_matfunc<T, NC, FN>), dim3((unsigned)blocks), dim3(256), 0, st, p, counter);
+    const long long blocks = ((long long)p.n + MF_BLOCK - 1) / MF_BLOCK;
+    hipLaunchKernelGGL((k_matfunc<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
     return hipGetLastError();
 }
 
 template <class T, int NC>
-static hipError_t launch_nc(int fn, const qla_matfunc_args &p, unsigned int *counter, hipStream_t st)
+static hipError_t launch_nc(int fn, const qla_matfunc_args &p, hipStream_t st)
 {
     switch (fn) {
-    case MF_EXP: return launch_t<T, NC, MF_EXP>(p, counter, st);
-    case MF_SQRT: return launch_t<T, NC, MF_SQRT>(p, counter, st);
-    case MF_INVSQRT: return launch_t<T, NC, MF_INVSQRT>(p, counter, st);
-    case MF_DET: return launch_t<T, NC, MF_DET>(p, counter, st);
-    case MF_INVERSE: return launch_t<T, NC, MF_INVERSE>(p, counter, st);
-    case MF_INV_V: return launch_t<T, NC, MF_INV_V>(p, counter, st);
-    case MF_INV_M: return launch_t<T, NC, MF_INV_M>(p, counter, st);
-    case MF_LOG: return launch_t<T, NC, MF_LOG>(p, counter, st);
+    case MF_EXP: return launch_t<T, NC, MF_EXP>(p, st);
+    case MF_SQRT: return launch_t<T, NC, MF_SQRT>(p, st);
+    case MF_INVSQRT: return launch_t<T, NC, MF_INVSQRT>(p, st);
+    case MF_DET: return launch_t<T, NC, MF_DET>(p, st);
+    case MF_INVERSE: return launch_t<T, NC, MF_INVERSE>(p, st);
+    case MF_INV_V: return launch_t<T, NC, MF_INV_V>(p, st);
+    case MF_INV_M: return launch_t<T, NC, MF_INV_M>(p, st);
+    case MF_LOG: return launch_t<T, NC, MF_LOG>(p, st);
     default: break;
     }
     if constexpr (NC == 3) {
         switch (fn) {
-        case MF_EXPA: return launch_t<T, 3, MF_EXPA>(p, counter, st);
-        case MF_EXPTA: return launch_t<T, 3, MF_EXPTA>(p, counter, st);
-        case MF_SQRTPH: return launch_t<T, 3, MF_SQRTPH>(p, counter, st);
-        case MF_INVSQRTPH: return launch_t<T, 3, MF_INVSQRTPH>(p, counter, st);
-        case MF_EIGH: return launch_t<T, 3, MF_EIGH>(p, counter, st);
+        case MF_EXPA: return launch_t<T, 3, MF_EXPA>(p, st);
+        case MF_EXPTA: return launch_t<T, 3, MF_EXPTA>(p, st);
+        case MF_SQRTPH: return launch_t<T, 3, MF_SQRTPH>(p, st);
+        case MF_INVSQRTPH: return launch_t<T, 3, MF_INVSQRTPH>(p, st);
+        case MF_EIGH: return launch_t<T, 3, MF_EIGH>(p, st);
         default: return hipErrorInvalidValue;
         }
     } else {
@@ -156,9 +138,9 @@ static hipError_t launch_nc(int fn, const qla_matfunc_args &p, unsigned int *cou
         // M_eq_sqrtPH_M.c:270-310 are M_eq_exp_M / M_eq_sqrt_M's)
         switch (fn) {
         case MF_EXPA:
-        case MF_EXPTA: return launch_t<T, NC, MF_EXP>(p, counter, st);
-        case MF_SQRTPH: return launch_t<T, NC, MF_SQRT>(p, counter, st);
-        case MF_INVSQRTPH: return launch_t<T, NC, MF_INVSQRT>(p, counter, st);
+        case MF_EXPTA: return launch_t<T, NC, MF_EXP>(p, st);
+        case MF_SQRTPH: return launch_t<T, NC, MF_SQRT>(p, st);
+        case MF_INVSQRTPH: return launch_t<T, NC, MF_INVSQRT>(p, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -169,24 +151,24 @@ static hipError_t launch_nc(int fn, const qla_matfunc_args &p, unsigned int *cou
 using namespace qla;
 
 #if QLA_MATFN_UNIT == 0
-hipError_t qla_matfunc_launch_u0(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 3>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u0(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 3>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 1
-hipError_t qla_matfunc_launch_u1(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<double, 3>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u1(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<double, 3>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 2
-hipError_t qla_matfunc_launch_u2(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 4>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u2(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 4>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 3
-hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 5>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 5>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 4
-hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 6>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 6>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 5
-hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 7>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 7>(fn, p, st); }
 #elif QLA_MATFN_UNIT == 6
-hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, unsigned int *c, hipStream_t st)
-{ return launch_nc<float, 8>(fn, p, c, st); }
+hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 8>(fn, p, st); }
 #endif

@@ -66,3 +66,38 @@ def test_header_compiles_as_c_and_cxx(tmp_path):
         if cc == "g++":
             args += ["-x", "c++"]
         subprocess.run(args + [str(src)], check=True)
+
+
+def _expand(tmp_path, defines, body):
+    """Preprocess `body` after including the header with `defines`; returns the
+    expansion of the marked line."""
+    inc = qlib.HEADERS[0].parent
+    src = tmp_path / "gen.c"
+    src.write_text('#include "qla_mi355x.h"\nMARK ' + body + "\n")
+    out = subprocess.run(["gcc", "-std=c99", "-E", "-P", "-I", str(inc), *defines, str(src)],
+                         capture_output=True, text=True, check=True).stdout
+    return [ln for ln in out.splitlines() if ln.startswith("MARK")][0]
+
+
+def test_generic_names_select_the_library(tmp_path):
+    """QLA_Precision / QLA_Colors pick the library behind QLA's generic names
+    (prototype.pl:160-260, 556-575): fully generic, colour-generic (QLA_F_,
+    QLA_D_, QLA_DF_) and precision-generic (QLA_3_, QLA_N_) forms."""
+    body = "QLA_V_veq_M_times_V(r, a, b, n); QLA_F_M_veq_expA_M(r, a, n); QLA_DF_r_veq_norm2_D(r, a, n);"
+    e = _expand(tmp_path, [], body)
+    assert "QLA_F3_V_veq_M_times_V(r, a, b, n)" in e and "QLA_F3_M_veq_expA_M" in e and "QLA_DF3_r_veq_norm2_D" in e
+    e = _expand(tmp_path, ["-DQLA_Precision='D'"], "QLA_V_veq_M_times_V(r, a, b, n); QLA_3_C_veq_det_M(r, a, n);")
+    assert "QLA_D3_V_veq_M_times_V(r, a, b, n)" in e and "QLA_D3_C_veq_det_M(r, a, n)" in e
+    e = _expand(tmp_path, ["-DQLA_PrecisionInt=2"], "QLA_D_r_veq_norm2_V(r, a, n); QLA_F_r_veq_norm2_V(r, a, n);")
+    assert "QLA_D3_r_veq_norm2_V(r, a, n)" in e and "QLA_F3_r_veq_norm2_V(r, a, n)" in e
+    e = _expand(tmp_path, ["-DQLA_Nc=5"], "QLA_M_veq_log_M(r, a, n); QLA_N_M_veq_exp_M(r, a, n); QLA_F_C_veq_det_M(r, a, n);")
+    assert ("QLA_FN_M_veq_log_M(5, r, a, n)" in e and "QLA_FN_M_veq_exp_M(5, r, a, n)" in e
+            and "QLA_FN_C_veq_det_M(5, r, a, n)" in e)
+    # generic types follow the same selection; the header still compiles
+    src = tmp_path / "types.c"
+    src.write_text('#include "qla_mi355x.h"\n'
+                   'void f(QLA_ColorMatrix *r, QLA_ColorMatrix **a, QLA_Complex *d, int n)\n'
+                   '{ QLA_M_veq_expTA_pM(r, a, n); QLA_C_veq_det_M(d, r, n); }\n')
+    for defs in ([], ["-DQLA_Precision='D'"]):
+        subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Werror", "-I", str(qlib.HEADERS[0].parent),
+                        *defs, str(src)], check=True)
