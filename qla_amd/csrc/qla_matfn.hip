@@ -78,6 +78,97 @@ __device__ __forceinline__ const cpx<T> *operand(const void *base, int pointer, 
 
 constexpr int MF_BLOCK = 256;
 
+// Block-wide copies of `bytes` (a multiple of 8, at most PER * 8 * MF_BLOCK)
+// with consecutive threads on consecutive 8-byte words, so each wave
+// instruction moves 512 contiguous bytes.  All of a thread's loads are issued
+// before the first is waited for.  HBM is touched once per call, so the
+// global side carries the non-temporal hint.
+template <int PER> __device__ __forceinline__ void block_load8(char *lds, const char *src, int bytes)
+{
+    const int words = bytes >> 3;
+    const unsigned long long *g = (const unsigned long long *)src;
+    unsigned long long v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int w = threadIdx.x + k * MF_BLOCK;
+        if (w < words) v[k] = __builtin_nontemporal_load(g + w);
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int w = threadIdx.x + k * MF_BLOCK;
+        if (w < words) ((unsigned long long *)lds)[w] = v[k];
+    }
+}
+template <int PER> __device__ __forceinline__ void block_store8(char *dst, const char *lds, int bytes)
+{
+    const int words = bytes >> 3;
+    unsigned long long *g = (unsigned long long *)dst;
+    unsigned long long v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int w = threadIdx.x + k * MF_BLOCK;
+        if (w < words) v[k] = ((const unsigned long long *)lds)[w];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int w = threadIdx.x + k * MF_BLOCK;
+        if (w < words) __builtin_nontemporal_store(v[k], g + w);
+    }
+}
+
+// Records of at most this many bytes are staged through LDS by the
+// contiguous v-form kernel (F3, D3 and FN with nc = 4).
+constexpr int MF_STAGE_MAX = 144;
+
+template <class T, int NC, int FN> struct stage_of {
+    typedef fn_shape<FN, NC> S;
+    static constexpr int RA = NC * NC * (int)sizeof(cpx<T>);
+    static constexpr int RB = S::B * (int)sizeof(cpx<T>);
+    static constexpr int RO = S::OUT * (int)sizeof(cpx<T>);
+    static constexpr bool OK = RA <= MF_STAGE_MAX;
+};
+
+// Contiguous v-form (fields, no index): the block's 256 input records are
+// copied into LDS with coalesced loads, each thread evaluates its matrix from
+// LDS, and the results leave through LDS the same way.  Reading one strided
+// record per lane straight from HBM touches ~37 cache lines per wave
+// instruction instead of 4.
+template <class T, int NC, int FN>
+__global__ void __launch_bounds__(MF_BLOCK) k_matfunc_stage(qla_matfunc_args p)
+{
+    typedef fn_shape<FN, NC> S;
+    typedef stage_of<T, NC, FN> G;
+    constexpr int SLOT = G::RA > G::RO ? G::RA : G::RO;
+    __shared__ __attribute__((aligned(16))) char lds_a[MF_BLOCK * SLOT];
+    __shared__ __attribute__((aligned(16))) char lds_b[G::RB > 0 ? MF_BLOCK * G::RB : 16];
+    const long long base = (long long)blockIdx.x * MF_BLOCK;
+    const long long left = (long long)p.n - base;
+    const int cnt = left < MF_BLOCK ? (int)left : MF_BLOCK;
+    block_load8<G::RA / 8>(lds_a, (const char *)p.a + base * G::RA, cnt * G::RA);
+    if constexpr (G::RB > 0) block_load8<G::RB / 8>(lds_b, (const char *)p.b + base * G::RB, cnt * G::RB);
+    __syncthreads();
+    const int t = threadIdx.x;
+    mat<T, NC> m;
+    cpx<T> bv[S::B > 0 ? S::B : 1];
+    if (t < cnt) {
+        const cpx<T> *src = (const cpx<T> *)(lds_a + t * G::RA);
+        LOOP(k, NC * NC) m.e[k] = src[k];
+        if constexpr (S::B > 0) {
+            const cpx<T> *bsrc = (const cpx<T> *)(lds_b + t * G::RB);
+            LOOP(k, S::B) bv[k] = bsrc[k];
+        }
+    }
+    __syncthreads();            // every input is in registers before results overwrite lds_a
+    if (t < cnt) {
+        cpx<T> out[S::OUT];
+        evaluate<T, NC, FN>(out, m, bv);
+        cpx<T> *dst = (cpx<T> *)(lds_a + t * G::RO);
+        LOOP(k, S::OUT) dst[k] = out[k];
+    }
+    __syncthreads();
+    block_store8<G::RO / 8>((char *)p.r + base * G::RO, lds_a, cnt * G::RO);
+}
+
 template <class T, int NC, int FN>
 __global__ void __launch_bounds__(MF_BLOCK) k_matfunc(qla_matfunc_args p)
 {
@@ -106,6 +197,13 @@ template <class T, int NC, int FN>
 static hipError_t launch_t(const qla_matfunc_args &p, hipStream_t st)
 {
     const long long blocks = ((long long)p.n + MF_BLOCK - 1) / MF_BLOCK;
+    const bool contiguous = !p.index && !p.ir && !p.ia && !p.ib && !p.pa && !p.pb;
+    if constexpr (stage_of<T, NC, FN>::OK) {
+        if (contiguous) {
+            hipLaunchKernelGGL((k_matfunc_stage<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_matfunc<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
     return hipGetLastError();
 }
