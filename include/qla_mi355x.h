@@ -180,7 +180,7 @@ void QLA_MI355X_synchronize(void);
 int QLA_MI355X_last_error(void);
 const char *QLA_MI355X_last_error_string(void);
 /* Launch-shape knob of the persistent site kernels (spproj, sprecon,
- * matvec): tiles of 64 sites per wavefront in the grid (default 2).  Set by
+ * matvec): tiles of 64 sites per wavefront in the grid (default 1).  Set by
  * the autotuner; results do not depend on it. */
 void QLA_MI355X_set_tiles_per_wave(int tpw);
 /* Library version string. */

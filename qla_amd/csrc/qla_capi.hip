@@ -158,7 +158,7 @@ void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const
 
 // ------------------------------------------------------------- site calls
 // tiles per wave of the persistent site kernels (launch shape only)
-std::atomic<int> g_tiles_per_wave{ 2 };
+std::atomic<int> g_tiles_per_wave{ 1 };
 
 // One QLA site call (qla_sites_abi.inc): the reference's no-op cases, then
 // one launch.  kind 0 matvec, 1 spproj, 2 sprecon, 3 project-multiply-
