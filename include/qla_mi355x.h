@@ -144,16 +144,20 @@ typedef struct QLA_MI355X_batch QLA_MI355X_batch;
 /* Copies the job table to the device (the operands are referenced, not
  * copied); returns NULL and records an error for an unknown op. */
 QLA_MI355X_batch *QLA_MI355X_F3_batch_create(const QLA_MI355X_F3_job *jobs, int njobs);
-/* Runs every job (one persistent launch on the thread's stream). */
+/* Runs every job (one launch on the thread's stream). */
 void QLA_MI355X_batch_run(QLA_MI355X_batch *p);
 /* Captures one run into a hipGraph (0 on success); launch it with
  * QLA_MI355X_batch_launch_graph.  Re-capture after a shape change. */
 int QLA_MI355X_batch_capture(QLA_MI355X_batch *p);
 void QLA_MI355X_batch_launch_graph(QLA_MI355X_batch *p);
-/* Times every launch shape (waves per block, tiles per claim, blocks per
- * CU) `reps` times on the plan's own operands, keeps the fastest, caches it
- * for plans of the same size class and returns its time in ms.  The jobs
- * run (1 + reps) x 48 times: accumulating destinations (vpeq) change. */
+/* Launch shape: waves per block (1, 2, 4), tiles of 64 sites per claim
+ * (>= 1) and blocks per CU of the persistent grid whose waves claim tiles
+ * from a counter (>= 1), or 0 for the static schedule, whose grid covers
+ * every tile with no counter.  Results do not depend on the shape.
+ * QLA_MI355X_batch_autotune times every shape `reps` times on the plan's own
+ * operands, keeps the fastest, caches it for plans of the same size class
+ * and returns its time in ms.  The jobs run (1 + reps) x 60 times:
+ * accumulating destinations (vpeq) change. */
 float QLA_MI355X_batch_autotune(QLA_MI355X_batch *p, int reps);
 void QLA_MI355X_batch_get_shape(const QLA_MI355X_batch *p, int *waves_per_block, int *tiles_per_claim,
                                 int *blocks_per_cu);

@@ -4,13 +4,14 @@
 //   V_vpeq_M_times_pV, H_veq_spproj_M_times_pD or D_vpeq_sprecon_M_times_pH
 // (QLA_F3 semantics, v-form) with its own n, mu, sign and operands.  Every
 // job is cut into 64-site tiles; the tiles of all jobs form one index space
-// (job j owns tiles [tile0_j, tile0_{j+1})).  Wavefronts of a persistent
-// grid claim `tpc` consecutive tiles at a time from a global counter, find
-// the owning job by binary search over tile0, and run the same per-tile
-// stage/finish steps as the single-op kernels (qla_tile_ops.hpp), so each
-// job's result is bit-identical to its plain call.  Dynamic claiming balances
-// jobs of uneven size; the launch shape (waves per block, tiles per claim,
-// blocks per CU) is chosen by the autotuner in qla_capi.hip.
+// (job j owns tiles [tile0_j, tile0_{j+1})).  Each wavefront takes `tpc`
+// consecutive tiles at a time -- from a global claim counter in a persistent
+// grid (dynamic schedule), or by its position in a grid that covers all
+// tiles (static schedule) -- finds the owning job by binary search over
+// tile0, and runs the same per-tile stage/finish steps as the single-op
+// kernels (qla_tile_ops.hpp), so each job's result is bit-identical to its
+// plain call.  The launch shape (waves per block, tiles per claim, blocks per
+// CU, 0 for the static schedule) is chosen by the autotuner in qla_capi.hip.
 #include "qla_tile_ops.hpp"
 #include "qla_launch.hpp"
 
@@ -148,6 +149,40 @@ k_batch(const batch_job *__restrict__ jobs, int njobs, int total_tiles, int tpc,
     }
 }
 
+// Static schedule (no counter): the grid covers every tile, wave w runs the
+// `tpc` consecutive tiles starting at w * tpc, and the hardware dispatcher
+// balances uneven jobs by starting a block wherever one retires.  A claim
+// counter costs one same-address atomic per claim; those serialise across the
+// XCDs at roughly 70 million per second, which bounds the dynamic schedule
+// on large batches.
+template <int WPB>
+__global__ void __launch_bounds__(WPB * WAVE)
+k_batch_static(const batch_job *__restrict__ jobs, int njobs, int total_tiles, int tpc)
+{
+    __shared__ __attribute__((aligned(16))) char smem[WPB * BATCH_BYTES];
+    const int *t0 = &jobs[0].tile0;
+    const int t0_stride = (int)(sizeof(batch_job) / sizeof(int));
+    char *lds = smem + (threadIdx.x / WAVE) * BATCH_BYTES;
+    const int lane = lane_id();
+    const long long first = ((long long)blockIdx.x * WPB + (long long)(threadIdx.x / WAVE)) * tpc;
+    if (first >= total_tiles) return;
+    int tile = uni((int)first);
+    const int last = (int)min((long long)total_tiles, first + tpc);
+    tile_ref cur, nxt;
+    load_tile_ref(cur, jobs, t0, t0_stride, njobs, tile, lane);
+    vm_wait();
+    for (;;) {
+        job_step(false, lds, cur.op, cur.p, cur.la, cur.base, cur.ns, lane);
+        const int ntile = tile + 1;
+        const bool more = ntile < last;
+        if (more) load_tile_ref(nxt, jobs, t0, t0_stride, njobs, ntile, lane);
+        job_step(true, lds, cur.op, cur.p, cur.la, cur.base, cur.ns, lane);
+        if (!more) break;
+        cur = nxt;
+        tile = ntile;
+    }
+}
+
 __global__ void k_batch_zero(unsigned int *c) { *c = 0u; }
 
 } // namespace qla
@@ -160,7 +195,18 @@ hipError_t qla_launch_batch(const qla_batch_job *jobs, int njobs, int total_tile
                             unsigned int *counter, hipStream_t st)
 {
     if (total_tiles <= 0) return hipSuccess;
-    if (tpc < 1 || blocks < 1) return hipErrorInvalidValue;
+    if (tpc < 1 || blocks < 0) return hipErrorInvalidValue;
+    if (blocks == 0) {          // static schedule
+        const long long per_block = (long long)wpb * tpc;
+        const unsigned g = (unsigned)(((long long)total_tiles + per_block - 1) / per_block);
+        switch (wpb) {
+        case 1: hipLaunchKernelGGL(k_batch_static<1>, dim3(g), dim3(1 * WAVE), 0, st, jobs, njobs, total_tiles, tpc); break;
+        case 2: hipLaunchKernelGGL(k_batch_static<2>, dim3(g), dim3(2 * WAVE), 0, st, jobs, njobs, total_tiles, tpc); break;
+        case 4: hipLaunchKernelGGL(k_batch_static<4>, dim3(g), dim3(4 * WAVE), 0, st, jobs, njobs, total_tiles, tpc); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(1), 0, st, counter);
     switch (wpb) {
     case 1: hipLaunchKernelGGL(k_batch<1>, dim3(blocks), dim3(1 * WAVE), 0, st, jobs, njobs, total_tiles, tpc, counter); break;

@@ -258,7 +258,7 @@ struct QLA_MI355X_batch {
     int cus = 256;
     qla_batch_job *djobs = nullptr;     // device job table
     unsigned int *counter = nullptr;    // device claim counter
-    batch_shape shape = { 2, 2, 8 };
+    batch_shape shape = { 1, 1, 0 };   // static schedule, one tile per wave
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
@@ -268,7 +268,7 @@ namespace {
 
 hipError_t batch_launch(QLA_MI355X_batch *p, hipStream_t st)
 {
-    const int blocks = std::max(1, p->cus * p->shape.bpc);
+    const int blocks = p->shape.bpc == 0 ? 0 : std::max(1, p->cus * p->shape.bpc);   // 0: static schedule
     return qla_launch_batch(p->djobs, p->njobs, p->total_tiles, p->shape.wpb, p->shape.tpc, blocks, p->counter, st);
 }
 
@@ -482,7 +482,7 @@ float QLA_MI355X_batch_autotune(QLA_MI355X_batch *p, int reps)
     if (reps < 1) reps = 1;
     static const int wpbs[] = { 1, 2, 4 };
     static const int tpcs[] = { 1, 2, 4, 8 };
-    static const int bpcs[] = { 2, 4, 8, 16 };
+    static const int bpcs[] = { 0, 2, 4, 8, 16 };     // 0: static schedule
     const int lds = qla_batch_lds_bytes_per_wave();
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
@@ -530,7 +530,7 @@ void QLA_MI355X_batch_get_shape(const QLA_MI355X_batch *p, int *waves_per_block,
 int QLA_MI355X_batch_set_shape(QLA_MI355X_batch *p, int waves_per_block, int tiles_per_claim, int blocks_per_cu)
 {
     if (!p || (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4) || tiles_per_claim < 1 ||
-        blocks_per_cu < 1)
+        blocks_per_cu < 0)
         return (int)hipErrorInvalidValue;
     p->shape = { waves_per_block, tiles_per_claim, blocks_per_cu };
     batch_drop_graph(p);

@@ -368,7 +368,7 @@ k_reduce_dh2(const double *__restrict__ part, int g, int dot, double *__restrict
 // launchers
 // ---------------------------------------------------------------------------
 constexpr int CWPB = 4;
-static int g_cmp_tiles_per_wave = 4;
+static int g_cmp_tiles_per_wave = 1;
 
 static dim3 cgrid(int n)
 {
