@@ -95,7 +95,8 @@ def threads() -> int:
 
 
 MATVEC, SPPROJ, SPRECON, WILSON = 0, 1, 2, 3
-REC_REALS = {MATVEC: 6, SPPROJ: 12, SPRECON: 24, WILSON: 24}
+PROJ, RECON, PROJRECON = 5, 6, 7        # matrix-free: H = spproj(b), D = sprecon(b), D = sprecon(spproj(b))
+REC_REALS = {MATVEC: 6, SPPROJ: 12, SPRECON: 24, WILSON: 24, PROJ: 12, RECON: 24, PROJRECON: 24}
 
 
 def _records(x: np.ndarray, idx) -> np.ndarray:
@@ -113,7 +114,9 @@ def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b
     """One QLA site call on host arrays (orc_*_site).
 
     kind 0 V eqop A*B, 1 H eqop A*spproj(B), 2 D eqop sprecon(A*B),
-    3 D eqop sprecon(A*spproj(B)); A = a or a^dagger (adj).  a_idx / b_idx
+    3 D eqop sprecon(A*spproj(B)); A = a or a^dagger (adj); the matrix-free
+    kinds 5 H eqop spproj(B), 6 D eqop sprecon(B), 7 D eqop
+    sprecon(spproj(B)) take a = None.  a_idx / b_idx
     None: the operand is a field (record i, or index[i] in the x-form); else
     the operand is a pointer array whose entry s points at record a_idx[s].
     ir / ia / ib: per-operand index arrays of the QLA_INDEX_ALL forms (step i
@@ -122,13 +125,13 @@ def site(kind, eqop, r, a, b, mu=0, sign=1, index=None, adj=False, a_idx=None, b
     (x-form) or the number of destination records.
     """
     lib = load()
-    pa = _records(a, ia if ia is not None else a_idx)
+    pa = None if a is None else _records(a, ia if ia is not None else a_idx)
     pb = _records(b, ib if ib is not None else b_idx)
     ix = _index(index)
     rix = _index(ir)
     if n is None:
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
-    getattr(lib, f"orc_{_prefix(a)}_site")(kind, eqop, int(adj), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
+    getattr(lib, f"orc_{_prefix(b)}_site")(kind, eqop, int(adj), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
                                            mu, sign, _ptr(ix), n)
     return r
 

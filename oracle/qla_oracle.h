@@ -31,6 +31,9 @@ enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
  * kind 1: H eqop A*spproj(B) (make_code_binary.pl:189-222)
  * kind 2: D eqop sprecon(A*B) (make_code_binary.pl:224-257)
  * kind 3: D eqop sprecon(A*spproj(B)) (make_code_binary.pl:285-341)
+ * kind 5: H eqop spproj(B) (make_code_unary.pl:183-202; pa unused, may be NULL)
+ * kind 6: D eqop sprecon(B) (make_code_unary.pl:204-223)
+ * kind 7: D eqop sprecon(spproj(B)) (make_code_unary.pl:243-283)
  * adj selects A = a^dagger (QLA_c_peq_ca_times_c).  rindex != NULL puts the
  * result of step i at r[rindex[i]] (the per-operand destination index of the
  * QLA_INDEX_ALL forms, indirection_all.pl:66-84).  mu outside 0..4 is a

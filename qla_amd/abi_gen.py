@@ -2,7 +2,7 @@
 
 QLA's generator emits its Wilson-hop building blocks for every combination of
 assignment operator, adjoint, pointer operand and gang index
-(lib/perl/build_qla.pl:1162-1250 and 1693-1725, looping over
+(lib/perl/build_qla.pl:1087-1250 and 1693-1725, looping over
 operatortypes.pl:34 ``@eqop_all`` and the indexing list of
 indirection.pl:63-120; names and argument order from prototype.pl:450-688).
 This module enumerates the same combinations for QLA_F3 and QLA_D3:
@@ -15,6 +15,9 @@ projection, then matrix        H_{v,x}<eqop>_spproj_<A>_times_{D,pD}         1
 matrix, then reconstruction    D_{v,x}<eqop>_sprecon_<A>_times_{H,pH}        2
 project-multiply-reconstruct   D_{v,x}<eqop>_spproj_<A>_times_{D,pD}         3
 multi-source matrix times vec  V_{v,x}<eqop>_n<A>_times_n{V,pV}  (+ int nd)  multi
+spin projection                H_{v,x}<eqop>_spproj_{D,pD}                   5
+spin reconstruction            D_{v,x}<eqop>_sprecon_{H,pH}                  6
+projection with reconstruction D_{v,x}<eqop>_spproj_{D,pD}                   7
 =============================  ==========================================  =======
 
 with <eqop> in eq, peq, meq, eqm; and the mixed-precision field layer
@@ -80,7 +83,9 @@ NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144}
 FIELD_TYPES = ("V", "H", "D", "M", "P")
 # (dest, src1, src2) index flags of the per-operand index forms of a binary op
 OWN_INDEX_FORMS = [(r, a, b) for r in (False, True) for a in (False, True) for b in (False, True) if r or a or b]
-KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4}
+KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4, "proj": 5, "recon": 6, "projrecon": 7}
+# the matrix-free spin operations (one source, `a` in the QLA name)
+UNARY_KINDS = ("proj", "recon", "projrecon")
 # (dest record, second-source record, qualifier in the name)
 FAMILY = {
     "matvec": ("V", "V", ""),
@@ -88,6 +93,9 @@ FAMILY = {
     "sprecon": ("D", "H", "sprecon_"),
     "wilson": ("D", "D", "spproj_"),
     "multi": ("V", "V", ""),
+    "proj": ("H", "D", "spproj_"),
+    "recon": ("D", "H", "sprecon_"),
+    "projrecon": ("D", "D", "spproj_"),
 }
 
 
@@ -122,15 +130,23 @@ class Variant:
 
     @property
     def spin(self) -> bool:
-        return self.kind in ("spproj", "sprecon", "wilson")
+        return self.kind in ("spproj", "sprecon", "wilson") + UNARY_KINDS
+
+    @property
+    def unary(self) -> bool:
+        return self.kind in UNARY_KINDS
 
     @property
     def name(self) -> str:
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        if self.unary:
+            src = f"{'p' if self.pa else ''}{'x' if self.ia else ''}{self.src2}"
+            r = f"{'x' if self.ir else ''}{self.dest}"
+            return f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{src}"
         n = "n" if self.multi else ""
         a = f"{n}{'p' if self.pa else ''}{'x' if self.ia else ''}M{'a' if self.adj else ''}"
         b = f"{n}{'p' if self.pb else ''}{'x' if self.ib else ''}{self.src2}"
         r = f"{'x' if self.ir else ''}{self.dest}"
-        form = "" if self.own_index else ("x" if self.xf else "v")
         return f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{a}_times_{b}"
 
     def params(self) -> list[tuple[str, str]]:
@@ -138,6 +154,17 @@ class Variant:
         t = lambda s: f"QLA_{self.prec}_{TYPE_NAMES[s]}"                  # noqa: E731
         stars = lambda ptr: "*" * ((1 if self.multi else 0) + (1 if ptr else 0))  # noqa: E731
         out = [("p", f"{t(self.dest)} *QLA_MI355X_RESTRICT r")]
+        if self.unary:
+            if self.ir:
+                out.append(("p", "int *indexr"))
+            out.append(("p", f"{t(self.src2)} *QLA_MI355X_RESTRICT {stars(self.pa)}a"))
+            if self.ia:
+                out.append(("p", "int *indexa"))
+            out += [("i", "int mu"), ("i", "int sign")]
+            if self.xf:
+                out.append(("p", "int *index"))
+            out.append(("i", "int n"))
+            return out
         if self.ir:
             out.append(("p", "int *indexr"))
         out.append(("p", f"{t('M')} *QLA_MI355X_RESTRICT {stars(self.pa)}a"))
@@ -399,6 +426,15 @@ def variants() -> list[Variant]:
     out = []
     for prec in PRECS:
         for kind in KINDS:
+            if kind in UNARY_KINDS:
+                for xf in (False, True):
+                    for eqop in range(4):
+                        for pa in (False, True):
+                            out.append(Variant(prec, kind, eqop, False, xf, pa, False))
+                for ir, ia in UNARY_OWN_FORMS:
+                    for eqop in range(4):
+                        out.append(Variant(prec, kind, eqop, False, False, False, False, ir, ia))
+                continue
             for xf in (False, True):
                 for eqop in range(4):
                     for adj in (False, True):
@@ -431,6 +467,12 @@ _FAMILY_DOC = {
                "build_qla.pl:1222-1250, make_code_binary.pl:285-341"),
     "multi": ("r eqop sum_{d<nd} A_d * B_d (eqop for d = 0, then += / -=; nd <= 0 is a no-op)",
               "build_qla.pl:1693-1725, expressions_tensor.pl:983-1002, prototype.pl:683-685"),
+    "proj": ("spin projection of a DiracFermion, r eqop spproj(a)",
+             "build_qla.pl:1087-1110, make_code_unary.pl:183-202, expressions_gamma.pl:205-276"),
+    "recon": ("spin reconstruction of a HalfFermion, r eqop sprecon(a)",
+              "build_qla.pl:1112-1135, make_code_unary.pl:204-223, expressions_gamma.pl:278-337"),
+    "projrecon": ("spin projection with reconstruction, r eqop sprecon(spproj(a))",
+                  "build_qla.pl:1137-1160, make_code_unary.pl:243-283"),
 }
 
 
@@ -478,7 +520,11 @@ def inc_text() -> str:
         lines.append(_decl(v))
         idx = "index" if v.xf else "nullptr"
         own = ", ".join(x if f else "nullptr" for x, f in (("indexr", v.ir), ("indexa", v.ia), ("indexb", v.ib)))
-        if v.multi:
+        if v.unary:
+            own_u = ", ".join(x if f else "nullptr" for x, f in (("indexr", v.ir), ("nullptr", False), ("indexa", v.ia)))
+            body = (f"site_entry({prec}, {KINDS[v.kind]}, {v.eqop}, 0, 0, {int(v.pa)}, r, nullptr, "
+                    f"(const void *)a, mu, sign, {idx}, {own_u}, n, \"{v.name}\");")
+        elif v.multi:
             body = (f"multi_entry({prec}, {v.eqop}, {int(v.adj)}, {int(v.pa)}, {int(v.pb)}, r, "
                     f"(const void *)a, (const void *)b, {idx}, {own}, n, nd, \"{v.name}\");")
         else:

@@ -162,8 +162,10 @@ std::atomic<int> g_tiles_per_wave{ 1 };
 
 // One QLA site call (qla_sites_abi.inc): the reference's no-op cases, then
 // one launch.  kind 0 matvec, 1 spproj, 2 sprecon, 3 project-multiply-
-// reconstruct; mu/sign only for kinds 1-3.  index is the gang index of an
-// x-form, ir / ia / ib the per-operand indices of the QLA_INDEX_ALL forms.
+// reconstruct, 5-7 the matrix-free projection, reconstruction and
+// projection-with-reconstruction (their source in b, a unused); mu/sign for
+// every kind but 0.  index is the gang index of an x-form, ir / ia / ib the
+// per-operand indices of the QLA_INDEX_ALL forms.
 void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, const void *a, const void *b,
                 int mu, int sign, const int *index, const int *ir, const int *ia, const int *ib, int n,
                 const char *name)
@@ -289,6 +291,8 @@ void batch_drop_graph(QLA_MI355X_batch *p)
     hipError_t qla_site_launch_p##P##_k##K(int, int, int, const qla_site_args &, int, hipStream_t);
 QLA_SITE_LAUNCHER(0, 0) QLA_SITE_LAUNCHER(0, 1) QLA_SITE_LAUNCHER(0, 2) QLA_SITE_LAUNCHER(0, 3)
 QLA_SITE_LAUNCHER(1, 0) QLA_SITE_LAUNCHER(1, 1) QLA_SITE_LAUNCHER(1, 2) QLA_SITE_LAUNCHER(1, 3)
+QLA_SITE_LAUNCHER(0, 5) QLA_SITE_LAUNCHER(0, 6) QLA_SITE_LAUNCHER(0, 7)
+QLA_SITE_LAUNCHER(1, 5) QLA_SITE_LAUNCHER(1, 6) QLA_SITE_LAUNCHER(1, 7)
 #undef QLA_SITE_LAUNCHER
 hipError_t qla_multi_launch_p0(int, int, int, const qla_multi_args &, int, hipStream_t);
 hipError_t qla_multi_launch_p1(int, int, int, const qla_multi_args &, int, hipStream_t);
@@ -297,11 +301,14 @@ hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const 
                            hipStream_t st)
 {
     typedef hipError_t (*launcher)(int, int, int, const qla_site_args &, int, hipStream_t);
-    static const launcher tab[2][4] = {
-        { qla_site_launch_p0_k0, qla_site_launch_p0_k1, qla_site_launch_p0_k2, qla_site_launch_p0_k3 },
-        { qla_site_launch_p1_k0, qla_site_launch_p1_k1, qla_site_launch_p1_k2, qla_site_launch_p1_k3 },
+    // kinds 0-3 with a matrix, 5-7 matrix-free (4 is the multi-source sum)
+    static const launcher tab[2][8] = {
+        { qla_site_launch_p0_k0, qla_site_launch_p0_k1, qla_site_launch_p0_k2, qla_site_launch_p0_k3, nullptr,
+          qla_site_launch_p0_k5, qla_site_launch_p0_k6, qla_site_launch_p0_k7 },
+        { qla_site_launch_p1_k0, qla_site_launch_p1_k1, qla_site_launch_p1_k2, qla_site_launch_p1_k3, nullptr,
+          qla_site_launch_p1_k5, qla_site_launch_p1_k6, qla_site_launch_p1_k7 },
     };
-    if (prec < 0 || prec > 1 || kind < 0 || kind > 3) return hipErrorInvalidValue;
+    if (prec < 0 || prec > 1 || kind < 0 || kind > 7 || !tab[prec][kind]) return hipErrorInvalidValue;
     return tab[prec][kind](eqop, adj, xf, p, tpw, st);
 }
 

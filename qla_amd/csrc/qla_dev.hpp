@@ -311,6 +311,59 @@ __device__ __forceinline__ void spproj_row(cpx<T> &h0, cpx<T> &h1, const cpx<T> 
     }
 }
 
+// r eqop (a op b) for the projection's complex pairs, each eqop/op pair
+// written as the carith2 macro it names (lib/perl/headers.pl:175-200,
+// include/qla_complex.h:287-324): op 0 = c+c, 1 = c-c, 2 = c+ic, 3 = c-ic.
+// eqm is formed as in its macro (e.g. re = -a.re + b.im), not as a negated
+// eq, so zeros keep the reference's sign.
+template <int EQOP, int OP, class T>
+__device__ __forceinline__ void proj_pair(cpx<T> &c, cpx<T> a, cpx<T> b)
+{
+    T xr, xi;       // the eq value, for eq / peq / meq
+    if (OP == 0) { xr = a.re + b.re; xi = a.im + b.im; }
+    if (OP == 1) { xr = a.re - b.re; xi = a.im - b.im; }
+    if (OP == 2) { xr = a.re - b.im; xi = a.im + b.re; }
+    if (OP == 3) { xr = a.re + b.im; xi = a.im - b.re; }
+    if (EQOP == EQ) { c.re = xr; c.im = xi; }
+    if (EQOP == PEQ) { c.re = c.re + xr; c.im = c.im + xi; }
+    if (EQOP == MEQ) { c.re = c.re - xr; c.im = c.im - xi; }
+    if (EQOP == EQM) {
+        if (OP == 0) { c.re = -a.re - b.re; c.im = -a.im - b.im; }
+        if (OP == 1) { c.re = -a.re + b.re; c.im = -a.im + b.im; }
+        if (OP == 2) { c.re = -a.re + b.im; c.im = -a.im - b.re; }
+        if (OP == 3) { c.re = -a.re - b.im; c.im = -a.im + b.re; }
+    }
+}
+// r eqop a (carith1 eqc / peqc / meqc / eqmc)
+template <int EQOP, class T> __device__ __forceinline__ void copy_eqop(cpx<T> &c, cpx<T> a)
+{
+    if (EQOP == EQ) c = a;
+    if (EQOP == PEQ) { c.re = c.re + a.re; c.im = c.im + a.im; }
+    if (EQOP == MEQ) { c.re = c.re - a.re; c.im = c.im - a.im; }
+    if (EQOP == EQM) { c.re = -a.re; c.im = -a.im; }
+}
+
+// H eqop spproj(D) for one colour row (the unary projection,
+// make_code_unary.pl:183-202 with expressions_gamma.pl:244-276): the same
+// pairs as spproj_row, each combined into h with the call's eqop.
+template <int EQOP, int MU, int SIGN, class T>
+__device__ __forceinline__ void spproj_row_eqop(cpx<T> &h0, cpx<T> &h1, const cpx<T> *d)
+{
+    if (SIGN == 1) {
+        if (MU == 0) { proj_pair<EQOP, 2>(h0, d[0], d[3]); proj_pair<EQOP, 2>(h1, d[1], d[2]); }
+        if (MU == 1) { proj_pair<EQOP, 1>(h0, d[0], d[3]); proj_pair<EQOP, 0>(h1, d[1], d[2]); }
+        if (MU == 2) { proj_pair<EQOP, 2>(h0, d[0], d[2]); proj_pair<EQOP, 3>(h1, d[1], d[3]); }
+        if (MU == 3) { proj_pair<EQOP, 0>(h0, d[0], d[2]); proj_pair<EQOP, 0>(h1, d[1], d[3]); }
+        if (MU == 4) { copy_eqop<EQOP>(h0, d[0]); copy_eqop<EQOP>(h1, d[1]); }
+    } else {
+        if (MU == 0) { proj_pair<EQOP, 3>(h0, d[0], d[3]); proj_pair<EQOP, 3>(h1, d[1], d[2]); }
+        if (MU == 1) { proj_pair<EQOP, 0>(h0, d[0], d[3]); proj_pair<EQOP, 1>(h1, d[1], d[2]); }
+        if (MU == 2) { proj_pair<EQOP, 3>(h0, d[0], d[2]); proj_pair<EQOP, 2>(h1, d[1], d[3]); }
+        if (MU == 3) { proj_pair<EQOP, 1>(h0, d[0], d[2]); proj_pair<EQOP, 1>(h1, d[1], d[3]); }
+        if (MU == 4) { copy_eqop<EQOP>(h0, d[2]); copy_eqop<EQOP>(h1, d[3]); }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Spin reconstruction (expressions_gamma.pl:311-337): spin component sp of
 // the result is op(h[src]) with op in {0, +h, -h, +i h, -i h} = {0,1,2,3,4}.

@@ -8,10 +8,14 @@
 // the gang-indexed x-form (site index[i]) and the per-operand index forms
 // of QLA_INDEX_ALL (r[indexr[i]], a[indexa[i]], b[indexb[i]]).
 //
+// and the matrix-free spin operations H_{eqop}_spproj_D,
+// D_{eqop}_sprecon_H and D_{eqop}_spproj_D (build_qla.pl:1087-1160).
+//
 // This file is compiled once per (precision, kind): QLA_SITE_PREC 0 float /
-// 1 double, QLA_SITE_KIND 0..3 (K_MATVEC, K_SPPROJ, K_SPRECON, K_WILSON); each
-// object holds the 16 eqop x adjoint x form kernels of its pair and exports
-// qla_site_launch_p<PREC>_k<KIND>.
+// 1 double, QLA_SITE_KIND 0..3 (K_MATVEC, K_SPPROJ, K_SPRECON, K_WILSON) or
+// 5..7 (K_PROJ, K_RECON, K_PROJRECON); each object holds the eqop x adjoint
+// x form kernels of its pair (no adjoint for the matrix-free kinds) and
+// exports qla_site_launch_p<PREC>_k<KIND>.
 //
 // One wavefront owns one tile of 64 sites.  Records move HBM -> LDS with
 // direct-to-LDS loads (coalesced for contiguous fields and for adjacent
@@ -128,7 +132,11 @@ static hipError_t launch_one(const site_args &p, int tpw, hipStream_t st)
 template <int EQOP>
 static hipError_t launch_eqop(int adj, int xf, const site_args &p, int tpw, hipStream_t st)
 {
-    if (adj) return xf ? launch_one<EQOP, true, true>(p, tpw, st) : launch_one<EQOP, true, false>(p, tpw, st);
+    if constexpr (kind_unary<SITE_KIND>::v) {
+        if (adj) return hipErrorInvalidValue;
+    } else {
+        if (adj) return xf ? launch_one<EQOP, true, true>(p, tpw, st) : launch_one<EQOP, true, false>(p, tpw, st);
+    }
     return xf ? launch_one<EQOP, false, true>(p, tpw, st) : launch_one<EQOP, false, false>(p, tpw, st);
 }
 

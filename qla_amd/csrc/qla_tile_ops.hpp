@@ -12,6 +12,11 @@
 //   K_SPPROJ   H  eqop A * spproj(B)       B: D      (make_code_binary.pl:189-222)
 //   K_SPRECON  D  eqop sprecon(A * B)      B: H      (make_code_binary.pl:224-257)
 //   K_WILSON   D  eqop sprecon(A * spproj(B))  B: D  (make_code_binary.pl:285-341)
+// and the matrix-free spin operations (build_qla.pl:1087-1160), whose one
+// source takes the B slot:
+//   K_PROJ      H  eqop spproj(B)          B: D      (make_code_unary.pl:183-202)
+//   K_RECON     D  eqop sprecon(B)         B: H      (make_code_unary.pl:204-223)
+//   K_PROJRECON D  eqop sprecon(spproj(B)) B: D      (make_code_unary.pl:243-283)
 // with A = M or M^dagger (the `Ma` forms), A and B either fields or pointer
 // arrays (the `pM` / `pB` forms), and the v-form (site i) or the gang-indexed
 // x-form (site index[i]).  Precision, kind, eqop, adjoint and x-form are
@@ -89,7 +94,8 @@ __device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
 enum { AL_A = 1, AL_B = 2, AL_R = 4 };
 
 // kinds of site operation
-enum { K_MATVEC = 0, K_SPPROJ = 1, K_SPRECON = 2, K_WILSON = 3 };
+enum { K_MATVEC = 0, K_SPPROJ = 1, K_SPRECON = 2, K_WILSON = 3, K_PROJ = 5, K_RECON = 6, K_PROJRECON = 7 };
+template <int KIND> struct kind_unary { static constexpr bool v = KIND >= K_PROJ; };
 
 // record bytes of the second source (B) and of the destination (R)
 template <class T, int KIND> struct kind_rec;
@@ -97,6 +103,9 @@ template <class T> struct kind_rec<T, K_MATVEC>  { static constexpr int B = rec<
 template <class T> struct kind_rec<T, K_SPPROJ>  { static constexpr int B = rec<T>::D, R = rec<T>::H; };
 template <class T> struct kind_rec<T, K_SPRECON> { static constexpr int B = rec<T>::H, R = rec<T>::D; };
 template <class T> struct kind_rec<T, K_WILSON>  { static constexpr int B = rec<T>::D, R = rec<T>::D; };
+template <class T> struct kind_rec<T, K_PROJ>    { static constexpr int B = rec<T>::D, R = rec<T>::H; };
+template <class T> struct kind_rec<T, K_RECON>   { static constexpr int B = rec<T>::H, R = rec<T>::D; };
+template <class T> struct kind_rec<T, K_PROJRECON> { static constexpr int B = rec<T>::D, R = rec<T>::D; };
 
 // mu/sign code: ms = mu + 5 * (sign != 1), 0..9
 template <int K> struct ms_c {
@@ -160,8 +169,10 @@ __device__ __forceinline__ lane_addr addresses_of(const site_args &p, const lane
     typedef kind_rec<T, KIND> K;
     lane_addr la = { 0, 0, 0 };
     if (act) {
-        if (p.pa) la.ga = (uint64_t)((const void *const *)p.a)[s.a];
-        else if (XF) la.ga = (uint64_t)((const char *)p.a + (size_t)s.a * rec<T>::M);
+        if constexpr (!kind_unary<KIND>::v) {
+            if (p.pa) la.ga = (uint64_t)((const void *const *)p.a)[s.a];
+            else if (XF) la.ga = (uint64_t)((const char *)p.a + (size_t)s.a * rec<T>::M);
+        }
         if (p.pb) la.gb = (uint64_t)((const void *const *)p.b)[s.b];
         else if (XF) la.gb = (uint64_t)((const char *)p.b + (size_t)s.b * K::B);
         if (XF) la.gr = (uint64_t)((char *)p.r + (size_t)s.r * K::R);
@@ -170,24 +181,28 @@ __device__ __forceinline__ lane_addr addresses_of(const site_args &p, const lane
 }
 
 // ---------------------------------------------------------------------------
-// The tile operation.  LDS per wave: A tile, B tile, and (peq/meq) the
-// destination tile; eq/eqm write their result over the A/B region (the
-// result record is never larger than A + B).
+// The tile operation.  LDS per wave: A tile (none for the matrix-free
+// kinds), B tile, and (peq/meq) the destination tile; eq/eqm write their
+// result over the A/B region when it fits there, else after it.
 // ---------------------------------------------------------------------------
 template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct site_op {
     typedef rec<T> R;
     typedef kind_rec<T, KIND> K;
+    static constexpr bool UNARY = kind_unary<KIND>::v;
+    static constexpr int A_B = UNARY ? 0 : R::M;          // A record bytes held in LDS
     static constexpr bool READ_R = (EQOP == PEQ || EQOP == MEQ);
-    static constexpr int OFF_B = WAVE * R::M;
-    static constexpr int OFF_R = READ_R ? WAVE * (R::M + K::B) : 0;
-    static constexpr int BYTES = WAVE * (R::M + K::B + (READ_R ? K::R : 0));
-    static_assert(K::R <= R::M + K::B, "result must fit over the source tiles");
+    static constexpr bool R_APART = READ_R || K::R > A_B + K::B;
+    static constexpr int OFF_B = WAVE * A_B;
+    static constexpr int OFF_R = R_APART ? WAVE * (A_B + K::B) : 0;
+    static constexpr int BYTES = WAVE * (A_B + K::B + (R_APART ? K::R : 0));
 
     static __device__ __forceinline__ void stage(char *lds, const site_args &p, const lane_addr &la, int base,
                                                  int ns, int lane)
     {
-        if (!XF && !p.pa) load_tile<R::M>(lds, (const char *)p.a + (size_t)base * R::M, ns, lane, p.al & AL_A);
-        else gather_tile<R::M>(lds, la.ga, ns, lane);
+        if constexpr (!UNARY) {
+            if (!XF && !p.pa) load_tile<R::M>(lds, (const char *)p.a + (size_t)base * R::M, ns, lane, p.al & AL_A);
+            else gather_tile<R::M>(lds, la.ga, ns, lane);
+        }
         if (!XF && !p.pb) load_tile<K::B>(lds + OFF_B, (const char *)p.b + (size_t)base * K::B, ns, lane, p.al & AL_B);
         else gather_tile<K::B>(lds + OFF_B, la.gb, ns, lane);
         if (READ_R) {
@@ -202,10 +217,41 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct si
         constexpr int NR = K::R / R::C;
         dma_wait();
         cpx<T> m[9], o[NR];
-        lds_read(m, lds + lane * R::M);
+        if constexpr (!UNARY) lds_read(m, lds + lane * R::M);
         const char *lb = lds + OFF_B + lane * K::B;
         if (READ_R) lds_read(o, lds + OFF_R + lane * K::R);
-        if constexpr (KIND == K_MATVEC) {
+        if constexpr (KIND == K_PROJ) {
+            cpx<T> d[12];
+            lds_read(d, lb);
+            with_ms<MS>(p.ms, [&](auto c) {
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++)
+                    spproj_row_eqop<EQOP, decltype(c)::MU, decltype(c)::SIGN>(o[2 * ic], o[2 * ic + 1], d + 4 * ic);
+            });
+        } else if constexpr (KIND == K_RECON || KIND == K_PROJRECON) {
+            cpx<T> tv[6];
+            if constexpr (KIND == K_RECON) {
+                lds_read(tv, lb);
+            } else {
+                cpx<T> d[12];
+                lds_read(d, lb);
+                with_ms<MS>(p.ms, [&](auto c) {
+#pragma unroll
+                    for (int ic = 0; ic < 3; ic++)
+                        spproj_row<decltype(c)::MU, decltype(c)::SIGN>(tv[2 * ic], tv[2 * ic + 1], d + 4 * ic);
+                });
+            }
+            with_ms<MS>(p.ms, [&](auto c) {
+                typedef recon_tab<decltype(c)::MU, decltype(c)::SIGN> TAB;
+#pragma unroll
+                for (int ic = 0; ic < 3; ic++) {
+                    recon_apply<EQOP, TAB::op[0]>(o[4 * ic + 0], tv[2 * ic + TAB::src[0]]);
+                    recon_apply<EQOP, TAB::op[1]>(o[4 * ic + 1], tv[2 * ic + TAB::src[1]]);
+                    recon_apply<EQOP, TAB::op[2]>(o[4 * ic + 2], tv[2 * ic + TAB::src[2]]);
+                    recon_apply<EQOP, TAB::op[3]>(o[4 * ic + 3], tv[2 * ic + TAB::src[3]]);
+                }
+            });
+        } else if constexpr (KIND == K_MATVEC) {
             cpx<T> v[3];
             lds_read(v, lb);
             mat_times_cols<EQOP, ADJ, 1>(o, m, v);

@@ -2,8 +2,9 @@
 
 Every symbol of include/qla_mi355x_sites.h (QLA_F3_* and QLA_D3_*: the four
 eqops, M and Ma, field and pointer operands, v- and x-forms, for V = M*V,
-H = M*spproj(D), D = sprecon(M*H), D = sprecon(M*spproj(D)) and the
-multi-source V = sum_d M_d*V_d) is called through the C ABI and compared
+H = M*spproj(D), D = sprecon(M*H), D = sprecon(M*spproj(D)), the
+multi-source V = sum_d M_d*V_d, and the matrix-free H = spproj(D),
+D = sprecon(H), D = sprecon(spproj(D))) is called through the C ABI and compared
 with the oracle bit for bit.  Each call uses a ragged site count (the last
 tile is partial), permuted pointer targets and, for the x-forms, a parity
 subset; the spin forms run two mu/sign pairs per symbol and a full
@@ -27,7 +28,7 @@ from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
 
-KIND = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3}
+KIND = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "proj": 5, "recon": 6, "projrecon": 7}
 DIMS = (4, 4, 5, 6)             # 480 sites
 N = 453                          # v-form site count: 7 full tiles and a ragged one
 N_OWN = 333                      # steps of the per-operand index forms
@@ -69,10 +70,31 @@ def ms_pairs(name):
     return [(k % 5, 1 if k < 5 else -1), ((k + 3) % 5, -1 if k < 5 else 1)]
 
 
+def run_unary(orc, v, s, mu, sign, n):
+    """Matrix-free spin symbols: the one source (`a` of the QLA name) is the
+    B data of the set-up; its own index list is own["a"]."""
+    r = dev(s["r0"])
+    src = s["pb"][0] if v.pa else s["db"][0]
+    own, o = s["down"], s["own"]
+    args = ([r] + ([own["r"]] if v.ir else []) + [src] + ([own["a"]] if v.ia else []) + [mu, sign]
+            + ([s["dindex"]] if v.xf else []) + [n])
+    say(f"{v.name} mu={mu} sign={sign} n={n}")
+    qla_amd.call(v.name, *args)
+    qla_amd.check()
+    exp = s["r0"].copy()
+    orc.site(KIND[v.kind], v.eqop, exp, None, s["b"][0], mu, sign, index=s["index"] if v.xf else None,
+             b_idx=s["b_idx"][0] if v.pa else None, n=n, ir=o["r"] if v.ir else None,
+             ib=o["a"] if v.ia else None)
+    assert_bitwise(host(r), exp, f"{v.name} mu={mu} sign={sign}")
+
+
 def run_site(orc, v, s, mu, sign, n=None):
     kind = KIND[v.kind]
     if n is None:
         n = len(s["index"]) if v.xf else (N_OWN if v.own_index else N)
+    if v.unary:
+        run_unary(orc, v, s, mu, sign, n)
+        return
     r = dev(s["r0"])
     a = s["pa"][0] if v.pa else s["du"][0]
     b = s["pb"][0] if v.pb else s["db"][0]
@@ -133,7 +155,8 @@ def test_symbol(orc, v):
         run_site(orc, v, s, 0, 1)
 
 
-SWEEP = [v for v in variants() if v.spin and not v.xf and not v.pa and v.pb and not v.own_index]
+SWEEP = [v for v in variants() if v.spin and not v.xf and not v.own_index
+         and ((not v.unary and not v.pa and v.pb) or (v.unary and v.pa))]
 
 
 @pytest.mark.parametrize("v", SWEEP, ids=lambda v: v.name)
@@ -144,22 +167,26 @@ def test_all_mu_sign(orc, v):
 
 
 @pytest.mark.parametrize("prec", ["F3", "D3"])
-@pytest.mark.parametrize("kind", ["spproj", "sprecon", "wilson"])
+@pytest.mark.parametrize("kind", ["spproj", "sprecon", "wilson", "proj", "recon", "projrecon"])
 def test_noop_cases(orc, prec, kind):
     """mu outside 0..4 and n = 0 leave the destination untouched."""
-    v = [x for x in variants() if x.prec == prec and x.kind == kind and x.eqop == 0 and x.pb
-         and not x.pa and not x.adj and not x.xf and not x.own_index][0]
+    unary = kind in ("proj", "recon", "projrecon")
+    v = [x for x in variants() if x.prec == prec and x.kind == kind and x.eqop == 0 and (x.pa if unary else x.pb)
+         and (unary or not x.pa) and not x.adj and not x.xf and not x.own_index][0]
     s = setup(v.prec, v.src2, v.dest)
     for mu, n in ((5, N), (-1, N), (2, 0)):
         r = dev(s["r0"])
         say(f"{v.name} mu={mu} n={n}")
-        qla_amd.call(v.name, r, s["du"][0], s["pb"][0], mu, 1, n)
+        if unary:
+            qla_amd.call(v.name, r, s["pb"][0], mu, 1, n)
+        else:
+            qla_amd.call(v.name, r, s["du"][0], s["pb"][0], mu, 1, n)
         qla_amd.check()
         assert_bitwise(host(r), s["r0"], f"{v.name} no-op mu={mu} n={n}")
 
 
 @pytest.mark.parametrize("prec", ["F3", "D3"])
-@pytest.mark.parametrize("kind", ["matvec", "spproj", "sprecon", "wilson"])
+@pytest.mark.parametrize("kind", ["matvec", "spproj", "sprecon", "wilson", "proj", "recon", "projrecon"])
 @pytest.mark.parametrize("eqop", [0, 1])
 def test_misaligned_fields(orc, prec, kind, eqop):
     """Fields at an 8-byte (F3) offset from 16-byte alignment: the 4-byte path."""
@@ -182,13 +209,13 @@ def test_misaligned_fields(orc, prec, kind, eqop):
     tr, ar = shifted(s["r0"])
     assert au % 16 == 8 and ab % 16 == 8 and ar % 16 == 8
     mu, sign = 3, -1
-    args = [ar, au, ab] + ([mu, sign] if v.spin else []) + [N]
+    args = ([ar] if v.unary else [ar, au]) + [ab] + ([mu, sign] if v.spin else []) + [N]
     say(f"{v.name} misaligned")
     qla_amd.call(v.name, *args)
     qla_amd.check()
     got = host(tr)[2:].reshape(s["r0"].shape)
     exp = s["r0"].copy()
-    orc.site(KIND[kind], eqop, exp, s["u"][0], s["b"][0], mu, sign, n=N)
+    orc.site(KIND[kind], eqop, exp, None if v.unary else s["u"][0], s["b"][0], mu, sign, n=N)
     assert_bitwise(got, exp, f"{v.name} misaligned")
 
 
@@ -237,3 +264,43 @@ def test_wilson_full_lattice(orc, prec, dims):
         orc.site(3, 1, exp, u[mu], psi, mu, -1, b_idx=fw[mu])
         orc.site(3, 1, exp, u[mu], psi, mu, 1, adj=True, a_idx=bw[mu], b_idx=bw[mu])
     assert_bitwise(host(r), exp, f"{prec} wilson hop")
+
+
+@pytest.mark.parametrize("prec", ["F3", "D3"])
+def test_split_wilson_hop_equals_fused(orc, prec):
+    """The Wilson hopping term as a QDP-style caller builds it from the
+    matrix-free forms -- forward: h = spproj(psi) on every site, then
+    D += sprecon(U_mu(x) h(x + mu)); backward: h = U_mu^dagger spproj(psi),
+    then D += sprecon(h(x - mu)) -- does the same arithmetic as the fused
+    D_vpeq_spproj_M_times_pD / _pMa_times_pD, so the fields agree bit for bit
+    (and with the oracle)."""
+    dims = (16, 16, 8, 8)
+    n = lattice.volume(dims)
+    dt = np.float32 if prec == "F3" else np.float64
+    fw, bw = lattice.neighbours(dims)
+    u = [lattice.random_su3(n, 400 + m).astype(dt) for m in range(4)]
+    psi = lattice.random_field("D", n, 500).astype(dt)
+    du = [dev(x) for x in u]
+    dpsi = dev(psi)
+    split = dev(np.zeros((n, 3, 4, 2), dtype=dt))
+    fused = dev(np.zeros((n, 3, 4, 2), dtype=dt))
+    h = dev(np.zeros((n, 3, 2, 2), dtype=dt))
+    for mu in range(4):
+        say(f"{prec} split hop mu={mu}")
+        qla_amd.call(f"QLA_{prec}_H_veq_spproj_D", h, dpsi, mu, -1, n)
+        qla_amd.call(f"QLA_{prec}_D_vpeq_sprecon_M_times_pH", split, du[mu], qla_amd.pointer_array(h, fw[mu]),
+                     mu, -1, n)
+        qla_amd.call(f"QLA_{prec}_H_veq_spproj_Ma_times_D", h, du[mu], dpsi, mu, 1, n)
+        qla_amd.call(f"QLA_{prec}_D_vpeq_sprecon_pH", split, qla_amd.pointer_array(h, bw[mu]), mu, 1, n)
+        qla_amd.call(f"QLA_{prec}_D_vpeq_spproj_M_times_pD", fused, du[mu], qla_amd.pointer_array(dpsi, fw[mu]),
+                     mu, -1, n)
+        qla_amd.call(f"QLA_{prec}_D_vpeq_spproj_pMa_times_pD", fused, qla_amd.pointer_array(du[mu], bw[mu]),
+                     qla_amd.pointer_array(dpsi, bw[mu]), mu, 1, n)
+        qla_amd.check()
+    got = host(split)
+    assert_bitwise(got, host(fused), f"{prec} split hop vs fused")
+    exp = np.zeros((n, 3, 4, 2), dtype=dt)
+    for mu in range(4):
+        orc.site(3, 1, exp, u[mu], psi, mu, -1, b_idx=fw[mu])
+        orc.site(3, 1, exp, u[mu], psi, mu, 1, adj=True, a_idx=bw[mu], b_idx=bw[mu])
+    assert_bitwise(got, exp, f"{prec} split hop vs oracle")

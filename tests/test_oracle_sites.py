@@ -238,3 +238,87 @@ def test_multi_per_operand_index_forms(orc, prec):
             exp = r0.copy()
             exp[rs] = sub
             np.testing.assert_array_equal(got, exp)
+
+
+# ---- matrix-free spin operations (kinds 5, 6, 7) ---------------------------
+UDEST = {5: "H", 6: "D", 7: "D"}
+USRC = {5: "D", 6: "H", 7: "D"}
+
+
+def exact_unary(kind, b, mu, sign):
+    """(y, |y| bound) in long double from the gamma basis."""
+    P, R = proj_recon_mats(mu, sign)
+    if kind == 5:
+        return np.einsum("sk,nck->ncs", P.astype(L), b), np.einsum("sk,nck->ncs", np.abs(P), np.abs(b))
+    if kind == 6:
+        return np.einsum("ks,ncs->nck", R.astype(L), b), np.einsum("ks,ncs->nck", np.abs(R), np.abs(b))
+    g = G[mu] if mu < 4 else G5
+    S = (np.eye(4) + sign * g) if mu < 4 else 0.5 * (np.eye(4) + sign * g)
+    return np.einsum("kl,ncl->nck", S.astype(L), b), 2 * np.einsum("kl,ncl->nck", np.abs(S), np.abs(b))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [5, 6, 7])
+@pytest.mark.parametrize("eqop", range(4))
+def test_unary_vs_exact(orc, prec, kind, eqop):
+    """H eqop P psi, D eqop R h, D eqop (1 + s gamma_mu) psi against the gamma
+    basis, all ten (mu, sign), field and pointer sources."""
+    n = 160
+    dt = np.float32 if prec == "F" else np.float64
+    u_ = U32 if prec == "F" else U64
+    b = lattice.random_field(USRC[kind], n, 60 + kind).astype(dt)
+    r0 = lattice.random_field(UDEST[kind], n, 70 + eqop).astype(dt)
+    b_idx = np.random.default_rng(kind + 10 * eqop).permutation(n)
+    for ms in range(10):
+        mu, sign = ms % 5, (1 if ms < 5 else -1)
+        for pb in (False, True):
+            r = r0.copy()
+            orc.site(kind, eqop, r, None, b, mu, sign, b_idx=b_idx if pb else None)
+            y, yb = exact_unary(kind, cplx(b)[b_idx] if pb else cplx(b), mu, sign)
+            exact = apply_eqop(eqop, cplx(r0), y)
+            err = np.abs(cplx(r) - exact)
+            assert (err <= 8 * u_ * (np.abs(cplx(r0)) + yb) + 1e-300).all(), (kind, eqop, mu, sign, pb)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_unary_equals_identity_matrix_forms(orc, prec):
+    """With A = 1 the matrix forms reduce to the matrix-free ones bit for bit
+    (nonzero data): spproj against H eqop 1*spproj(D), sprecon against
+    D eqop sprecon(1*H), and the projection with reconstruction against
+    D eqop sprecon(1*spproj(D)).  peq / meq are also r0 +/- the eq result."""
+    n = 120
+    dt = np.float32 if prec == "F" else np.float64
+    eye = np.zeros((n, 3, 3, 2), dtype=dt)
+    eye[:, [0, 1, 2], [0, 1, 2], 0] = 1
+    for kind, mk in ((5, 1), (6, 2), (7, 3)):
+        b = lattice.random_field(USRC[kind], n, 80 + kind).astype(dt)
+        r0 = lattice.random_field(UDEST[kind], n, 90 + kind).astype(dt)
+        for ms in range(10):
+            mu, sign = ms % 5, (1 if ms < 5 else -1)
+            eqv = r0.copy()
+            orc.site(kind, 0, eqv, None, b, mu, sign)
+            for eqop in range(4):
+                got = r0.copy()
+                orc.site(kind, eqop, got, None, b, mu, sign)
+                ref = r0.copy()
+                orc.site(mk, eqop, ref, eye, b, mu, sign)
+                np.testing.assert_array_equal(got, ref, err_msg=f"kind {kind} eqop {eqop} mu {mu} sign {sign}")
+                if kind == 5 and eqop in (1, 2):
+                    np.testing.assert_array_equal(got, r0 + eqv if eqop == 1 else r0 - eqv)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_unary_eqm_keeps_macro_zero_sign(orc, prec):
+    """eqm is the macro -a.re + b.im (QLA_c_eqm_c_plus_ic), not -(a.re -
+    b.im): with a.re == b.im the reference gives +0, a negated eq gives -0."""
+    dt = np.float32 if prec == "F" else np.float64
+    d = np.zeros((1, 3, 4, 2), dtype=dt)
+    d[0, :, 0, 0] = 1.5          # d0.re
+    d[0, :, 3, 1] = 1.5          # d3.im -> mu 0, sign +: h0.re = d0.re - d3.im = 0
+    d[0, :, 1, 1] = -2.25        # d1.im
+    d[0, :, 2, 0] = 2.25         # d2.re -> h1.im = d1.im + d2.re = 0
+    for eqop in (0, 3):
+        h = np.full((1, 3, 2, 2), 7.0, dtype=dt)
+        orc.site(5, eqop, h, None, d, 0, 1)
+        assert (h[0, :, 0, 0] == 0).all() and not np.signbit(h[0, :, 0, 0]).any(), eqop
+        assert (h[0, :, 1, 1] == 0).all() and not np.signbit(h[0, :, 1, 1]).any(), eqop
