@@ -250,7 +250,8 @@ def wilson_family(lib, stream, n, combos, nbr, sets, args):
     """Other members of the Wilson-hop family on the same 32^4 sets (mu 0..3,
     sign +-1 rotated over the three input sets): the project-multiply-
     reconstruct D_vpeq_spproj_M_times_pD (FP32 and FP64), the adjoint and
-    pointer-link forms, and the multi-source V_vpeq_nM_times_npV with 4 and 8
+    pointer-link forms, the matrix-free H_veq_spproj_D, D_vpeq_sprecon_pH and
+    D_vpeq_spproj_pD, and the multi-source V_vpeq_nM_times_npV with 4 and 8
     terms.  Algorithmic bytes per site: each record read once, the
     destination read and written for peq, 8 B per pointer."""
     import torch
@@ -284,6 +285,16 @@ def wilson_family(lib, stream, n, combos, nbr, sets, args):
     f = lib.QLA_F3_D_vpeq_sprecon_pMa_times_pH
     ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), lptr[k][(mu, s)].data_ptr(), hptr[k][(mu, s)].data_ptr(), mu, s, n))
     out["D_vpeq_sprecon_pMa_times_pH_f3"] = rate(ms, 8 + 72 + 8 + 48 + 96 + 96, steps * nc)
+    # the matrix-free spin operations a split Wilson hop pairs with the above
+    f = lib.QLA_F3_H_veq_spproj_D
+    ms = run(lambda k, mu, s: f(sets[k][2].data_ptr(), sets[k][1].data_ptr(), mu, s, n))
+    out["H_veq_spproj_D_f3"] = rate(ms, 96 + 48, steps * nc)
+    f = lib.QLA_F3_D_vpeq_sprecon_pH
+    ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), hptr[k][(mu, s)].data_ptr(), mu, s, n))
+    out["D_vpeq_sprecon_pH_f3"] = rate(ms, 8 + 48 + 96 + 96, steps * nc)
+    f = lib.QLA_F3_D_vpeq_spproj_pD
+    ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), sets[k][3][(mu, s)].data_ptr(), mu, s, n))
+    out["D_vpeq_spproj_pD_f3"] = rate(ms, 8 + 96 + 96 + 96, steps * nc)
     del dsts, hsrc, hptr, lptr
 
     # multi-source: nd link fields times nd gathered vectors, accumulated
