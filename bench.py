@@ -458,7 +458,7 @@ def fields_layer(lib, stream, n, d, d2, fw, args):
 
 def batch_config3(lib, stream, args):
     """64 checkerboard sub-lattice jobs (L in {8,...,32}, seed 7) as one
-    autotuned persistent launch, as its captured hipGraph, and as 64
+    autotuned launch, as its captured hipGraph, and as 64
     sequential plain launches on the same stream."""
     import torch
     from qla_amd import batch as qb

@@ -122,10 +122,10 @@ void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n);
 /* ------------------------------------------------------------------------ */
 /* Ragged batch extension (no counterpart in the reference, where each call  */
 /* is its own site loop).  A plan holds a table of independent QLA_F3 v-form */
-/* calls; running it executes all of them in one persistent launch whose    */
-/* waves claim 64-site tiles of every job from a shared counter, so jobs of  */
-/* very different sizes share the chip.  Each job's result is bit-identical  */
-/* to its plain call.  The plan can be captured into a hipGraph.             */
+/* calls; running it executes all of them in one launch that cuts every    */
+/* job into 64-site tiles, so jobs of very different sizes share the chip.  */
+/* Each job's result is bit-identical to its plain call.  The plan can be   */
+/* captured into a hipGraph.                                                */
 /* ------------------------------------------------------------------------ */
 enum {
     QLA_MI355X_OP_V_VPEQ_M_TIMES_PV = 0,      /* QLA_F3_V_vpeq_M_times_pV         */
@@ -183,7 +183,7 @@ void QLA_MI355X_synchronize(void);
  * entry points themselves return void, like the reference. */
 int QLA_MI355X_last_error(void);
 const char *QLA_MI355X_last_error_string(void);
-/* Launch-shape knob of the persistent site kernels (spproj, sprecon,
+/* Launch-shape knob of the site kernels (spproj, sprecon,
  * matvec): tiles of 64 sites per wavefront in the grid (default 1).  Set by
  * the autotuner; results do not depend on it. */
 void QLA_MI355X_set_tiles_per_wave(int tpw);

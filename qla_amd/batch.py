@@ -2,7 +2,7 @@
 
 ``Batch(jobs)`` wraps a plan: a table of independent calls of
 V_vpeq_M_times_pV, H_veq_spproj_M_times_pD or D_vpeq_sprecon_M_times_pH run
-in one persistent launch (optionally replayed from a captured hipGraph).
+in one launch (optionally replayed from a captured hipGraph).
 
 ``checkerboard_jobs`` builds the even/odd-checkerboard workload used to
 exercise and measure it: each job is one sub-lattice L^4, its fields stored

@@ -60,13 +60,6 @@ double *scratch_for_device()
     return p;
 }
 
-// one device unsigned int after the reduction partials and the result slot
-unsigned int *counter_for_device()
-{
-    double *sc = scratch_for_device();
-    return sc ? (unsigned int *)(sc + qla_reduce_scratch_doubles() + 2) : nullptr;
-}
-
 bool is_device_pointer(const void *p)
 {
     hipPointerAttribute_t at;
