@@ -96,7 +96,9 @@ def threads() -> int:
 
 MATVEC, SPPROJ, SPRECON, WILSON = 0, 1, 2, 3
 PROJ, RECON, PROJRECON = 5, 6, 7        # matrix-free: H = spproj(b), D = sprecon(b), D = sprecon(spproj(b))
-REC_REALS = {MATVEC: 6, SPPROJ: 12, SPRECON: 24, WILSON: 24, PROJ: 12, RECON: 24, PROJRECON: 24}
+MAT_H, MAT_D, MAT_M, MAT_MA = 8, 9, 10, 11   # H, D, M eqop A*b; M eqop A*b^dagger
+REC_REALS = {MATVEC: 6, SPPROJ: 12, SPRECON: 24, WILSON: 24, PROJ: 12, RECON: 24, PROJRECON: 24,
+             MAT_H: 12, MAT_D: 24, MAT_M: 18, MAT_MA: 18}
 
 
 def _records(x: np.ndarray, idx) -> np.ndarray:

@@ -34,6 +34,8 @@ enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
  * kind 5: H eqop spproj(B) (make_code_unary.pl:183-202; pa unused, may be NULL)
  * kind 6: D eqop sprecon(B) (make_code_unary.pl:204-223)
  * kind 7: D eqop sprecon(spproj(B)) (make_code_unary.pl:243-283)
+ * kinds 8, 9, 10: H, D, M eqop A*B; kind 11: M eqop A*B^dagger
+ *   (build_qla.pl:1587-1622, expressions_tensor.pl:544-630)
  * adj selects A = a^dagger (QLA_c_peq_ca_times_c).  rindex != NULL puts the
  * result of step i at r[rindex[i]] (the per-operand destination index of the
  * QLA_INDEX_ALL forms, indirection_all.pl:66-84).  mu outside 0..4 is a

@@ -18,6 +18,7 @@ multi-source matrix times vec  V_{v,x}<eqop>_n<A>_times_n{V,pV}  (+ int nd)  mul
 spin projection                H_{v,x}<eqop>_spproj_{D,pD}                   5
 spin reconstruction            D_{v,x}<eqop>_sprecon_{H,pH}                  6
 projection with reconstruction D_{v,x}<eqop>_spproj_{D,pD}                   7
+matrix times H, D, M, M^dag    {H,D,M}_{v,x}<eqop>_<A>_times_[p]{H,D,M,Ma}  8-11
 =============================  ==========================================  =======
 
 with <eqop> in eq, peq, meq, eqm; and the mixed-precision field layer
@@ -83,7 +84,8 @@ NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144}
 FIELD_TYPES = ("V", "H", "D", "M", "P")
 # (dest, src1, src2) index flags of the per-operand index forms of a binary op
 OWN_INDEX_FORMS = [(r, a, b) for r in (False, True) for a in (False, True) for b in (False, True) if r or a or b]
-KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4, "proj": 5, "recon": 6, "projrecon": 7}
+KINDS = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "multi": 4, "proj": 5, "recon": 6, "projrecon": 7,
+         "mat_h": 8, "mat_d": 9, "mat_m": 10, "mat_ma": 11}
 # the matrix-free spin operations (one source, `a` in the QLA name)
 UNARY_KINDS = ("proj", "recon", "projrecon")
 # (dest record, second-source record, qualifier in the name)
@@ -96,6 +98,10 @@ FAMILY = {
     "proj": ("H", "D", "spproj_"),
     "recon": ("D", "H", "sprecon_"),
     "projrecon": ("D", "D", "spproj_"),
+    "mat_h": ("H", "H", ""),
+    "mat_d": ("D", "D", ""),
+    "mat_m": ("M", "M", ""),
+    "mat_ma": ("M", "M", ""),        # second factor adjoint ("Ma")
 }
 
 
@@ -145,7 +151,7 @@ class Variant:
             return f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{src}"
         n = "n" if self.multi else ""
         a = f"{n}{'p' if self.pa else ''}{'x' if self.ia else ''}M{'a' if self.adj else ''}"
-        b = f"{n}{'p' if self.pb else ''}{'x' if self.ib else ''}{self.src2}"
+        b = f"{n}{'p' if self.pb else ''}{'x' if self.ib else ''}{self.src2}{'a' if self.kind == 'mat_ma' else ''}"
         r = f"{'x' if self.ir else ''}{self.dest}"
         return f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{FAMILY[self.kind][2]}{a}_times_{b}"
 
@@ -467,6 +473,14 @@ _FAMILY_DOC = {
                "build_qla.pl:1222-1250, make_code_binary.pl:285-341"),
     "multi": ("r eqop sum_{d<nd} A_d * B_d (eqop for d = 0, then += / -=; nd <= 0 is a no-op)",
               "build_qla.pl:1693-1725, expressions_tensor.pl:983-1002, prototype.pl:683-685"),
+    "mat_h": ("colour matrix times HalfFermion, r[ic][is] eqop sum_kc A[ic][kc] b[kc][is]",
+              "build_qla.pl:1587-1622, expressions_tensor.pl:544-630"),
+    "mat_d": ("colour matrix times DiracFermion, r[ic][is] eqop sum_kc A[ic][kc] b[kc][is]",
+              "build_qla.pl:1587-1622, expressions_tensor.pl:544-630"),
+    "mat_m": ("colour matrix times colour matrix, r[ic][jc] eqop sum_kc A[ic][kc] b[kc][jc]",
+              "build_qla.pl:1587-1622, expressions_tensor.pl:544-630"),
+    "mat_ma": ("colour matrix times adjoint colour matrix, r[ic][jc] eqop sum_kc A[ic][kc] conj(b[jc][kc]) "
+               "(QLA_c_peq_c_times_ca / _ca_times_ca)", "build_qla.pl:1587-1622, expressions_tensor.pl:544-630"),
     "proj": ("spin projection of a DiracFermion, r eqop spproj(a)",
              "build_qla.pl:1087-1110, make_code_unary.pl:183-202, expressions_gamma.pl:205-276"),
     "recon": ("spin reconstruction of a HalfFermion, r eqop sprecon(a)",

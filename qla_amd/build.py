@@ -44,7 +44,7 @@ CXXFLAGS = [
 # precision and the matrix functions once per (precision, nc), so the kernel
 # instantiations spread over parallel compiles.
 MULTI_BUILD = {
-    "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in (0, 1, 2, 3, 5, 6, 7)],
+    "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in (0, 1, 2, 3, 5, 6, 7, 8, 9, 10, 11)],
     "qla_multi.hip": [{"QLA_SITE_PREC": p} for p in (0, 1)],
     "qla_matfn.hip": [{"QLA_MATFN_UNIT": u} for u in range(7)],
 }

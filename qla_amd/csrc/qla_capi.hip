@@ -164,7 +164,8 @@ void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, 
                 const char *name)
 {
     if (n <= 0) return;
-    if (kind != 0 && (mu < 0 || mu > 4)) return;     // switch without default (expressions_gamma.pl:349-381)
+    const bool spin = (kind >= 1 && kind <= 3) || (kind >= 5 && kind <= 7);
+    if (spin && (mu < 0 || mu > 4)) return;          // switch without default (expressions_gamma.pl:349-381)
     qla_site_args p;
     p.r = r;
     p.a = a;
@@ -176,7 +177,7 @@ void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, 
     p.n = n;
     p.al = ((!pa && (uintptr_t)a % 16 == 0) ? 1 : 0) | ((!pb && (uintptr_t)b % 16 == 0) ? 2 : 0) |
            (((uintptr_t)r % 16 == 0) ? 4 : 0);
-    p.ms = kind == 0 ? 0 : mu + (sign == 1 ? 0 : 5);
+    p.ms = spin ? mu + (sign == 1 ? 0 : 5) : 0;
     p.pa = pa;
     p.pb = pb;
     const int xf = (index || ir || ia || ib) ? 1 : 0;
@@ -286,6 +287,8 @@ QLA_SITE_LAUNCHER(0, 0) QLA_SITE_LAUNCHER(0, 1) QLA_SITE_LAUNCHER(0, 2) QLA_SITE
 QLA_SITE_LAUNCHER(1, 0) QLA_SITE_LAUNCHER(1, 1) QLA_SITE_LAUNCHER(1, 2) QLA_SITE_LAUNCHER(1, 3)
 QLA_SITE_LAUNCHER(0, 5) QLA_SITE_LAUNCHER(0, 6) QLA_SITE_LAUNCHER(0, 7)
 QLA_SITE_LAUNCHER(1, 5) QLA_SITE_LAUNCHER(1, 6) QLA_SITE_LAUNCHER(1, 7)
+QLA_SITE_LAUNCHER(0, 8) QLA_SITE_LAUNCHER(0, 9) QLA_SITE_LAUNCHER(0, 10) QLA_SITE_LAUNCHER(0, 11)
+QLA_SITE_LAUNCHER(1, 8) QLA_SITE_LAUNCHER(1, 9) QLA_SITE_LAUNCHER(1, 10) QLA_SITE_LAUNCHER(1, 11)
 #undef QLA_SITE_LAUNCHER
 hipError_t qla_multi_launch_p0(int, int, int, const qla_multi_args &, int, hipStream_t);
 hipError_t qla_multi_launch_p1(int, int, int, const qla_multi_args &, int, hipStream_t);
@@ -294,14 +297,17 @@ hipError_t qla_launch_site(int prec, int kind, int eqop, int adj, int xf, const 
                            hipStream_t st)
 {
     typedef hipError_t (*launcher)(int, int, int, const qla_site_args &, int, hipStream_t);
-    // kinds 0-3 with a matrix, 5-7 matrix-free (4 is the multi-source sum)
-    static const launcher tab[2][8] = {
+    // kinds 0-3 with a matrix, 5-7 matrix-free, 8-11 the colour-matrix
+    // products of H, D, M, M^dagger (4 is the multi-source sum)
+    static const launcher tab[2][12] = {
         { qla_site_launch_p0_k0, qla_site_launch_p0_k1, qla_site_launch_p0_k2, qla_site_launch_p0_k3, nullptr,
-          qla_site_launch_p0_k5, qla_site_launch_p0_k6, qla_site_launch_p0_k7 },
+          qla_site_launch_p0_k5, qla_site_launch_p0_k6, qla_site_launch_p0_k7, qla_site_launch_p0_k8,
+          qla_site_launch_p0_k9, qla_site_launch_p0_k10, qla_site_launch_p0_k11 },
         { qla_site_launch_p1_k0, qla_site_launch_p1_k1, qla_site_launch_p1_k2, qla_site_launch_p1_k3, nullptr,
-          qla_site_launch_p1_k5, qla_site_launch_p1_k6, qla_site_launch_p1_k7 },
+          qla_site_launch_p1_k5, qla_site_launch_p1_k6, qla_site_launch_p1_k7, qla_site_launch_p1_k8,
+          qla_site_launch_p1_k9, qla_site_launch_p1_k10, qla_site_launch_p1_k11 },
     };
-    if (prec < 0 || prec > 1 || kind < 0 || kind > 7 || !tab[prec][kind]) return hipErrorInvalidValue;
+    if (prec < 0 || prec > 1 || kind < 0 || kind > 11 || !tab[prec][kind]) return hipErrorInvalidValue;
     return tab[prec][kind](eqop, adj, xf, p, tpw, st);
 }
 

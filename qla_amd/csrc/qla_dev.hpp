@@ -258,6 +258,22 @@ __device__ __forceinline__ void acc_mul(cpx<T> &x, cpx<T> a, cpx<T> b)
     else      { x.re = x.re - tr; x.im = x.im - ti; }
 }
 
+// x +=/-= a*b with either factor conjugated (CA, CB), each case as its macro
+// writes it: QLA_c_{peq,meq}_c_times_c, _ca_times_c, _c_times_ca (re a.re*b.re
+// + a.im*b.im, im a.im*b.re - a.re*b.im) and _ca_times_ca (re a.re*b.re -
+// a.im*b.im, im -a.re*b.im - a.im*b.re); qla_complex.h:330-403.
+template <bool MEQ, bool CA, bool CB, class T>
+__device__ __forceinline__ void acc_mul2(cpx<T> &x, cpx<T> a, cpx<T> b)
+{
+    T tr, ti;
+    if (!CA && !CB) { tr = a.re * b.re - a.im * b.im; ti = a.re * b.im + a.im * b.re; }
+    if (CA && !CB)  { tr = a.re * b.re + a.im * b.im; ti = a.re * b.im - a.im * b.re; }
+    if (!CA && CB)  { tr = a.re * b.re + a.im * b.im; ti = a.im * b.re - a.re * b.im; }
+    if (CA && CB)   { tr = a.re * b.re - a.im * b.im; ti = -a.re * b.im - a.im * b.re; }
+    if (!MEQ) { x.re = x.re + tr; x.im = x.im + ti; }
+    else      { x.re = x.re - tr; x.im = x.im - ti; }
+}
+
 // eqop codes, as in the reference's operatortypes.pl: eq, peq, meq, eqm
 enum { EQ = 0, PEQ = 1, MEQ = 2, EQM = 3 };
 
@@ -279,6 +295,28 @@ __device__ __forceinline__ void mat_times_cols(cpx<T> *r, const cpx<T> *a, const
             for (int kc = 0; kc < 3; kc++)
                 acc_mul<SUB, ADJ>(x, ADJ ? a[kc * 3 + ic] : a[ic * 3 + kc], t[kc * NS + is]);
             r[ic * NS + is] = x;
+        }
+    }
+}
+
+// r[ic][jc] eqop sum_kc A[ic][kc] B[kc][jc] for colour matrices, A = a or
+// a^dagger (ADJ), B = b or b^dagger (ADJB: B[kc][jc] = conj(b[jc][kc])), in
+// the row-column order of expressions_tensor.pl:544-630.
+template <int EQOP, bool ADJ, bool ADJB, class T>
+__device__ __forceinline__ void mat_times_mat(cpx<T> *r, const cpx<T> *a, const cpx<T> *b)
+{
+    constexpr bool SUB = (EQOP == MEQ || EQOP == EQM);
+#pragma unroll
+    for (int ic = 0; ic < 3; ic++) {
+#pragma unroll
+        for (int jc = 0; jc < 3; jc++) {
+            cpx<T> x;
+            if (EQOP == PEQ || EQOP == MEQ) x = r[ic * 3 + jc];
+            else { x.re = T(0); x.im = T(0); }
+#pragma unroll
+            for (int kc = 0; kc < 3; kc++)
+                acc_mul2<SUB, ADJ, ADJB>(x, ADJ ? a[kc * 3 + ic] : a[ic * 3 + kc], ADJB ? b[jc * 3 + kc] : b[kc * 3 + jc]);
+            r[ic * 3 + jc] = x;
         }
     }
 }

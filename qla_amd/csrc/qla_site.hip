@@ -13,7 +13,8 @@
 //
 // This file is compiled once per (precision, kind): QLA_SITE_PREC 0 float /
 // 1 double, QLA_SITE_KIND 0..3 (K_MATVEC, K_SPPROJ, K_SPRECON, K_WILSON) or
-// 5..7 (K_PROJ, K_RECON, K_PROJRECON); each object holds the eqop x adjoint
+// 5..7 (K_PROJ, K_RECON, K_PROJRECON) or 8..11 (the colour-matrix products
+// of H, D, M and M^dagger); each object holds the eqop x adjoint
 // x form kernels of its pair (no adjoint for the matrix-free kinds) and
 // exports qla_site_launch_p<PREC>_k<KIND>.
 //

@@ -17,6 +17,10 @@
 //   K_PROJ      H  eqop spproj(B)          B: D      (make_code_unary.pl:183-202)
 //   K_RECON     D  eqop sprecon(B)         B: H      (make_code_unary.pl:204-223)
 //   K_PROJRECON D  eqop sprecon(spproj(B)) B: D      (make_code_unary.pl:243-283)
+// and the colour-matrix products of the other left-multiplied types
+// (build_qla.pl:1587-1622, expressions_tensor.pl:544-630):
+//   K_MAT_H  H eqop A * B   K_MAT_D  D eqop A * B   K_MAT_M  M eqop A * B
+//   K_MAT_MA M eqop A * B^dagger
 // with A = M or M^dagger (the `Ma` forms), A and B either fields or pointer
 // arrays (the `pM` / `pB` forms), and the v-form (site i) or the gang-indexed
 // x-form (site index[i]).  Precision, kind, eqop, adjoint and x-form are
@@ -94,8 +98,11 @@ __device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
 enum { AL_A = 1, AL_B = 2, AL_R = 4 };
 
 // kinds of site operation
-enum { K_MATVEC = 0, K_SPPROJ = 1, K_SPRECON = 2, K_WILSON = 3, K_PROJ = 5, K_RECON = 6, K_PROJRECON = 7 };
-template <int KIND> struct kind_unary { static constexpr bool v = KIND >= K_PROJ; };
+enum {
+    K_MATVEC = 0, K_SPPROJ = 1, K_SPRECON = 2, K_WILSON = 3, K_PROJ = 5, K_RECON = 6, K_PROJRECON = 7,
+    K_MAT_H = 8, K_MAT_D = 9, K_MAT_M = 10, K_MAT_MA = 11
+};
+template <int KIND> struct kind_unary { static constexpr bool v = KIND >= K_PROJ && KIND <= K_PROJRECON; };
 
 // record bytes of the second source (B) and of the destination (R)
 template <class T, int KIND> struct kind_rec;
@@ -106,6 +113,10 @@ template <class T> struct kind_rec<T, K_WILSON>  { static constexpr int B = rec<
 template <class T> struct kind_rec<T, K_PROJ>    { static constexpr int B = rec<T>::D, R = rec<T>::H; };
 template <class T> struct kind_rec<T, K_RECON>   { static constexpr int B = rec<T>::H, R = rec<T>::D; };
 template <class T> struct kind_rec<T, K_PROJRECON> { static constexpr int B = rec<T>::D, R = rec<T>::D; };
+template <class T> struct kind_rec<T, K_MAT_H>   { static constexpr int B = rec<T>::H, R = rec<T>::H; };
+template <class T> struct kind_rec<T, K_MAT_D>   { static constexpr int B = rec<T>::D, R = rec<T>::D; };
+template <class T> struct kind_rec<T, K_MAT_M>   { static constexpr int B = rec<T>::M, R = rec<T>::M; };
+template <class T> struct kind_rec<T, K_MAT_MA>  { static constexpr int B = rec<T>::M, R = rec<T>::M; };
 
 // mu/sign code: ms = mu + 5 * (sign != 1), 0..9
 template <int K> struct ms_c {
@@ -255,6 +266,15 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct si
             cpx<T> v[3];
             lds_read(v, lb);
             mat_times_cols<EQOP, ADJ, 1>(o, m, v);
+        } else if constexpr (KIND == K_MAT_H || KIND == K_MAT_D) {
+            constexpr int NS = KIND == K_MAT_H ? 2 : 4;
+            cpx<T> v[3 * NS];
+            lds_read(v, lb);
+            mat_times_cols<EQOP, ADJ, NS>(o, m, v);
+        } else if constexpr (KIND == K_MAT_M || KIND == K_MAT_MA) {
+            cpx<T> bm[9];
+            lds_read(bm, lb);
+            mat_times_mat<EQOP, ADJ, KIND == K_MAT_MA>(o, m, bm);
         } else if constexpr (KIND == K_SPPROJ) {
             cpx<T> d[12], tp[6];
             lds_read(d, lb);

@@ -36,9 +36,9 @@ def test_generated_site_abi_is_current():
     from qla_amd import abi_gen
     assert abi_gen.current()
     names = [v.name for v in abi_gen.variants()]
-    assert len(names) == len(set(names)) == 1368
+    assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 2149
+    assert len(every) == len(set(every)) == 3109
     mf = [v.name for v in abi_gen.matfuncs()]
     assert len(mf) == 321
     for s in ("QLA_F3_M_veq_expA_M", "QLA_D3_C_xeq_det_pM", "QLA_FN_M_veq_log_M", "QLA_F3_V_veq_eigenvalsH_M",
@@ -48,7 +48,8 @@ def test_generated_site_abi_is_current():
     for s in ("QLA_F3_D_vpeq_spproj_M_times_pD", "QLA_D3_V_vpeq_nM_times_npV", "QLA_F3_H_veq_spproj_Ma_times_pD",
               "QLA_F3_D_xmeq_sprecon_pMa_times_pH", "QLA_D3_V_xeqm_npMa_times_nV",
               "QLA_F3_xH_eq_spproj_xM_times_D", "QLA_D3_V_peq_M_times_xV", "QLA_F3_H_veq_spproj_D",
-              "QLA_D3_D_vpeq_sprecon_pH", "QLA_F3_D_xmeq_spproj_D", "QLA_F3_xH_eq_spproj_xD"):
+              "QLA_D3_D_vpeq_sprecon_pH", "QLA_F3_D_xmeq_spproj_D", "QLA_F3_xH_eq_spproj_xD",
+              "QLA_F3_M_veq_M_times_Ma", "QLA_D3_D_vpeq_pMa_times_pD", "QLA_F3_H_xeq_M_times_H"):
         assert s in names, s
 
 

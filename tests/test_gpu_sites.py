@@ -3,8 +3,9 @@
 Every symbol of include/qla_mi355x_sites.h (QLA_F3_* and QLA_D3_*: the four
 eqops, M and Ma, field and pointer operands, v- and x-forms, for V = M*V,
 H = M*spproj(D), D = sprecon(M*H), D = sprecon(M*spproj(D)), the
-multi-source V = sum_d M_d*V_d, and the matrix-free H = spproj(D),
-D = sprecon(H), D = sprecon(spproj(D))) is called through the C ABI and compared
+multi-source V = sum_d M_d*V_d, the matrix-free H = spproj(D),
+D = sprecon(H), D = sprecon(spproj(D)), and the products H = M*H, D = M*D,
+M = M*M, M = M*M^dagger) is called through the C ABI and compared
 with the oracle bit for bit.  Each call uses a ragged site count (the last
 tile is partial), permuted pointer targets and, for the x-forms, a parity
 subset; the spin forms run two mu/sign pairs per symbol and a full
@@ -28,7 +29,8 @@ from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
 
-KIND = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "proj": 5, "recon": 6, "projrecon": 7}
+KIND = {"matvec": 0, "spproj": 1, "sprecon": 2, "wilson": 3, "proj": 5, "recon": 6, "projrecon": 7,
+        "mat_h": 8, "mat_d": 9, "mat_m": 10, "mat_ma": 11}
 DIMS = (4, 4, 5, 6)             # 480 sites
 N = 453                          # v-form site count: 7 full tiles and a ragged one
 N_OWN = 333                      # steps of the per-operand index forms
@@ -186,7 +188,8 @@ def test_noop_cases(orc, prec, kind):
 
 
 @pytest.mark.parametrize("prec", ["F3", "D3"])
-@pytest.mark.parametrize("kind", ["matvec", "spproj", "sprecon", "wilson", "proj", "recon", "projrecon"])
+@pytest.mark.parametrize("kind", ["matvec", "spproj", "sprecon", "wilson", "proj", "recon", "projrecon",
+                                  "mat_h", "mat_d", "mat_m", "mat_ma"])
 @pytest.mark.parametrize("eqop", [0, 1])
 def test_misaligned_fields(orc, prec, kind, eqop):
     """Fields at an 8-byte (F3) offset from 16-byte alignment: the 4-byte path."""

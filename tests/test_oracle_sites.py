@@ -322,3 +322,62 @@ def test_unary_eqm_keeps_macro_zero_sign(orc, prec):
         orc.site(5, eqop, h, None, d, 0, 1)
         assert (h[0, :, 0, 0] == 0).all() and not np.signbit(h[0, :, 0, 0]).any(), eqop
         assert (h[0, :, 1, 1] == 0).all() and not np.signbit(h[0, :, 1, 1]).any(), eqop
+
+
+# ---- colour-matrix products of H, D, M (kinds 8-11) ------------------------
+MT = {8: "H", 9: "D", 10: "M", 11: "M"}
+
+
+def exact_prod(kind, A, b):
+    """(y, bound) in long double: A b for H/D/M records, A b^dagger for kind 11."""
+    B = np.conj(np.swapaxes(b, 1, 2)) if kind == 11 else b
+    return np.einsum("nik,nk...->ni...", A, B), np.einsum("nik,nk...->ni...", np.abs(A), np.abs(B))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [8, 9, 10, 11])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("adj", [False, True])
+def test_matrix_products_vs_exact(orc, prec, kind, eqop, adj):
+    n = 120
+    dt = np.float32 if prec == "F" else np.float64
+    u_ = U32 if prec == "F" else U64
+    U = lattice.random_su3(n, 11 + kind).astype(dt)
+    b = lattice.random_field(MT[kind], n, 12 + kind).astype(dt)
+    r0 = lattice.random_field(MT[kind], n, 13 + eqop).astype(dt)
+    idx = np.random.default_rng(kind * 8 + eqop).permutation(n)
+    for pb in (False, True):
+        r = r0.copy()
+        orc.site(kind, eqop, r, U, b, adj=adj, b_idx=idx if pb else None)
+        Uc = cplx(U)
+        A = np.conj(np.swapaxes(Uc, 1, 2)) if adj else Uc
+        y, yb = exact_prod(kind, A, cplx(b)[idx] if pb else cplx(b))
+        exact = apply_eqop(eqop, cplx(r0), y)
+        err = np.abs(cplx(r) - exact)
+        assert (err <= 8 * u_ * (np.abs(cplx(r0)) + yb) + 1e-300).all(), (kind, eqop, adj, pb)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_matrix_products_are_column_matvecs(orc, prec):
+    """Each destination column is the matvec of the matching source column,
+    bit for bit (the emitter's row-column loop is the same for every type);
+    for M b^dagger that column is conj(row jc of b)."""
+    n = 100
+    dt = np.float32 if prec == "F" else np.float64
+    U = lattice.random_su3(n, 5).astype(dt)
+    for kind in (8, 9, 10, 11):
+        b = lattice.random_field(MT[kind], n, 6 + kind).astype(dt)
+        r0 = lattice.random_field(MT[kind], n, 20 + kind).astype(dt)
+        ncol = b.shape[2]
+        for eqop in range(4):
+            for adj in (False, True):
+                r = r0.copy()
+                orc.site(kind, eqop, r, U, b, adj=adj)
+                for c in range(ncol):
+                    if kind == 11:
+                        col = np.ascontiguousarray(b[:, c, :, :] * np.array([1, -1], dtype=dt))
+                    else:
+                        col = np.ascontiguousarray(b[:, :, c, :])
+                    v = np.ascontiguousarray(r0[:, :, c, :])
+                    orc.site(0, eqop, v, U, col, adj=adj)
+                    np.testing.assert_array_equal(r[:, :, c, :], v, err_msg=f"kind {kind} eqop {eqop} adj {adj} col {c}")
