@@ -251,8 +251,8 @@ def wilson_family(lib, stream, n, combos, nbr, sets, args):
     sign +-1 rotated over the three input sets): the project-multiply-
     reconstruct D_vpeq_spproj_M_times_pD (FP32 and FP64), the adjoint and
     pointer-link forms, the matrix-free H_veq_spproj_D, D_vpeq_sprecon_pH and
-    D_vpeq_spproj_pD, and the multi-source V_vpeq_nM_times_npV with 4 and 8
-    terms.  Algorithmic bytes per site: each record read once, the
+    D_vpeq_spproj_pD, the products D_vpeq_M_times_pD and M_veq_M_times_Ma, and
+    the multi-source V_vpeq_nM_times_npV with 4 and 8 terms.  Algorithmic bytes per site: each record read once, the
     destination read and written for peq, 8 B per pointer."""
     import torch
     import qla_amd
@@ -295,7 +295,15 @@ def wilson_family(lib, stream, n, combos, nbr, sets, args):
     f = lib.QLA_F3_D_vpeq_spproj_pD
     ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), sets[k][3][(mu, s)].data_ptr(), mu, s, n))
     out["D_vpeq_spproj_pD_f3"] = rate(ms, 8 + 96 + 96 + 96, steps * nc)
-    del dsts, hsrc, hptr, lptr
+    # colour-matrix products of the other left-multiplied types
+    f = lib.QLA_F3_D_vpeq_M_times_pD
+    ms = run(lambda k, mu, s: f(dsts[k].data_ptr(), sets[k][0].data_ptr(), sets[k][3][(mu, s)].data_ptr(), n))
+    out["D_vpeq_M_times_pD_f3"] = rate(ms, 72 + 8 + 96 + 96 + 96, steps * nc)
+    mout = torch.empty((n, 3, 3, 2), dtype=torch.float32, device="cuda")
+    f = lib.QLA_F3_M_veq_M_times_Ma
+    ms = run(lambda k, mu, s: f(mout.data_ptr(), sets[k][0].data_ptr(), sets[(k + 1) % NSETS][0].data_ptr(), n))
+    out["M_veq_M_times_Ma_f3"] = rate(ms, 72 + 72 + 72, steps * nc)
+    del dsts, hsrc, hptr, lptr, mout
 
     # multi-source: nd link fields times nd gathered vectors, accumulated
     import ctypes
