@@ -94,6 +94,16 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 #include "qla_mi355x_matfuncs.h"
 
 /* ------------------------------------------------------------------------ */
+/* Field linear algebra: copies and increments, Hermitian conjugate,        */
+/* transpose, complex conjugate, times i, times a real or complex constant, */
+/* sums and differences, a*b +- c with a real or complex constant a; types  */
+/* V, H, D, M, P, QLA_F3 / QLA_D3, every eqop the reference generates and   */
+/* every indexing form.  Reference: build_qla.pl:356-494, 1256-1354,        */
+/* 1421-1456, 1837-1935.  The constant is read from host or device memory.  */
+/* ------------------------------------------------------------------------ */
+#include "qla_mi355x_vecops.h"
+
+/* ------------------------------------------------------------------------ */
 /* Compressed-precision extension (no counterpart in the reference, which    */
 /* stores FP32/FP64 records only).  Record formats, 4-byte aligned:          */
 /*   link 24 B: rows 0,1 of an SU(3) matrix as 12 bfloat16 (fmt 0) or IEEE    */

@@ -53,6 +53,8 @@ def load():
         for fn in ("exp", "sqrt", "invsqrt"):
             getattr(lib, f"orc_{p}_M_veq_{fn}_M").argtypes = [i32, vp, vp, i32]
         getattr(lib, f"orc_{p}_matfunc").argtypes = [i32, i32, vp, vp, vp, i32]
+        real = C.c_float if p == "F" else C.c_double
+        getattr(lib, f"orc_{p}_vecop").argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, real, real, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -295,4 +297,37 @@ def matfunc(fn: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
     r = np.empty(shape, dtype=a.dtype)
     getattr(lib, f"orc_{p}_matfunc")(MATFUNC[fn], nc, _ptr(r), _ptr(_need(a)),
                                      _ptr(_need(b)) if b is not None else None, n)
+    return r
+
+
+# field linear algebra (orc_{F,D}_vecop, qla_oracle.h)
+COPY, TIMES_I, RSCALE, CSCALE, PLUS, MINUS, R_PLUS, R_MINUS, C_PLUS, C_MINUS = range(10)
+# transposition order of the square record types (M: 3 x 3; P: 12 x 12 over (colour, spin))
+TRANSPOSE_DIM = {"M": 3, "P": 12}
+
+
+def vecop(op, eqop, r, x, y=None, s=0.0, conj=False, dim=0, index=None, x_idx=None, y_idx=None,
+          ir=None, ix=None, iy=None, n=None):
+    """One field linear-algebra call on host arrays (orc_*_vecop).
+
+    r eqop op(x, y): op COPY (conj: the conjugated source; with dim the
+    transposed one, so conj + dim is the Hermitian conjugate), TIMES_I,
+    RSCALE / CSCALE (times the real / complex constant s, conj and dim as
+    for COPY), PLUS / MINUS (x +- y), R_PLUS / R_MINUS / C_PLUS / C_MINUS
+    (s*x +- y).  x_idx / y_idx make the operand a pointer array to those
+    records; ir / ix / iy are the per-operand indices of the QLA_INDEX_ALL
+    forms; index the gang index.  Records are (n, ...) arrays with a
+    trailing (re, im) axis."""
+    lib = load()
+    ncomp = r[0].size // 2
+    px = _records(x, ix if ix is not None else x_idx)
+    py = None if y is None else _records(y, iy if iy is not None else y_idx)
+    ixx = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ixx) if ixx is not None else (len(rix) if rix is not None else r.shape[0])
+    s = complex(s)
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_vecop")(op, eqop, int(conj), int(dim), ncomp, _ptr(_need(r)), _ptr(rix), _ptr(px),
+                                   _ptr(py), s.real, s.imag, _ptr(ixx), n)
     return r

@@ -25,6 +25,7 @@
 #define SINCOSP sincosf
 #include "qla_oracle_impl.inc"
 #include "qla_oracle_matfunc.inc"
+#include "qla_oracle_vecops.inc"
 #undef REAL
 #undef P
 #undef SQRTP
@@ -44,6 +45,7 @@
 #define SINCOSP sincos
 #include "qla_oracle_impl.inc"
 #include "qla_oracle_matfunc.inc"
+#include "qla_oracle_vecops.inc"
 #undef REAL
 #undef P
 
@@ -112,6 +114,13 @@ void orc_F_matfunc(int fn, int nc, float *r, const float *a, const float *b, int
 { F_orc_matfunc(fn, nc, r, a, b, n); }
 void orc_D_matfunc(int fn, int nc, double *r, const double *a, const double *b, int n)
 { D_orc_matfunc(fn, nc, r, a, b, n); }
+
+void orc_F_vecop(int op, int eqop, int cj, int dim, int ncomp, float *r, const int *rindex,
+                 const float *const *px, const float *const *py, float sr, float si, const int *index, int n)
+{ F_orc_vecop(op, eqop, cj, dim, ncomp, r, rindex, px, py, sr, si, index, n); }
+void orc_D_vecop(int op, int eqop, int cj, int dim, int ncomp, double *r, const int *rindex,
+                 const double *const *px, const double *const *py, double sr, double si, const int *index, int n)
+{ D_orc_vecop(op, eqop, cj, dim, ncomp, r, rindex, px, py, sr, si, index, n); }
 
 void orc_set_threads(int nthreads)
 {

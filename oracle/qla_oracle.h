@@ -95,6 +95,22 @@ enum { OF_EXP = 0, OF_SQRT, OF_INVSQRT, OF_DET, OF_INVERSE, OF_INV_V, OF_INV_M, 
 void orc_F_matfunc(int fn, int nc, float *r, const float *a, const float *b, int n);
 void orc_D_matfunc(int fn, int nc, double *r, const double *a, const double *b, int n);
 
+/* field linear algebra (qla_oracle_vecops.inc): n steps of r eqop op(x, y)
+ * over records of ncomp complexes.  op 0 copy (cj: Hermitian / complex
+ * conjugate; build_qla.pl:356-494), 1 times i (:1328-1354), 2 times the real
+ * constant sr (:1256-1293), 3 times the complex constant (sr, si)
+ * (:1295-1326), 4 x + y, 5 x - y (:1421-1456; eq only), 6 sr*x + y,
+ * 7 sr*x - y (:1837-1887), 8 (sr,si)*x + y, 9 (sr,si)*x - y (:1889-1935).
+ * dim != 0 reads the source transposed as a dim x dim matrix (M: 3, P: 12).
+ * Records: destination r[rindex[i]] or r[s], sources *px[s], *py[s] with
+ * s = index[i] or i, as for the site operations. */
+enum { ORC_VOP_COPY = 0, ORC_VOP_I, ORC_VOP_RSCALE, ORC_VOP_CSCALE, ORC_VOP_PLUS, ORC_VOP_MINUS, ORC_VOP_R_PLUS,
+       ORC_VOP_R_MINUS, ORC_VOP_C_PLUS, ORC_VOP_C_MINUS };
+void orc_F_vecop(int op, int eqop, int cj, int dim, int ncomp, float *r, const int *rindex,
+                 const float *const *px, const float *const *py, float sr, float si, const int *index, int n);
+void orc_D_vecop(int op, int eqop, int cj, int dim, int ncomp, double *r, const int *rindex,
+                 const double *const *px, const double *const *py, double sr, double si, const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -74,6 +74,8 @@ FIELDS_INC_OUT = HERE / "csrc" / "qla_fields_abi.inc"
 MATFUNCS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_matfuncs.h"
 MATFUNCS_INC_OUT = HERE / "csrc" / "qla_matfuncs_abi.inc"
 GENERIC_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_generic.h"
+VECOPS_HEADER_OUT = HERE.parent / "include" / "qla_mi355x_vecops.h"
+VECOPS_INC_OUT = HERE / "csrc" / "qla_vecops_abi.inc"
 
 PRECS = ("F3", "D3")
 EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use them
@@ -399,6 +401,141 @@ def matfuncs() -> list[MatVariant]:
     return out
 
 
+# field linear algebra: family -> (op code of qla_launch_vec, constant ('', 'r' or 'c'),
+# field sources, qualifier in the name, source transform (bit 0 conjugate, bit 1 transpose),
+# record types)
+SQUARE_TYPES = ("M", "P")
+VECFAMS = {
+    "copy": (0, "", 1, "", 0, FIELD_TYPES),
+    "transpose": (0, "", 1, "transpose_", 2, SQUARE_TYPES),
+    "conj": (0, "", 1, "conj_", 1, FIELD_TYPES),
+    "i": (1, "", 1, "i_", 0, FIELD_TYPES),
+    "r_times": (2, "r", 1, "", 0, FIELD_TYPES),
+    "c_times": (3, "c", 1, "", 0, FIELD_TYPES),
+    "plus": (4, "", 2, "", 0, FIELD_TYPES),
+    "minus": (5, "", 2, "", 0, FIELD_TYPES),
+    "r_times_plus": (6, "r", 2, "", 0, FIELD_TYPES),
+    "r_times_minus": (7, "r", 2, "", 0, FIELD_TYPES),
+    "c_times_plus": (8, "c", 2, "", 0, FIELD_TYPES),
+    "c_times_minus": (9, "c", 2, "", 0, FIELD_TYPES),
+}
+# families with every eqop and an adjoint (`Ma`, `Pa`) source form
+VEC_ALL_EQOPS = ("copy", "transpose", "conj", "i", "r_times", "c_times")
+VEC_ADJ = ("copy", "r_times", "c_times")
+
+
+@dataclass(frozen=True)
+class VecVariant:
+    """A field linear-algebra call r eqop op(x[, y]) with an optional constant."""
+    prec: str       # "F3" / "D3"
+    fam: str        # key of VECFAMS
+    t: str          # record type letter
+    eqop: int
+    adj: bool = False   # copy: Hermitian conjugate; r_times / c_times: adjoint source
+    xf: bool = False
+    px: bool = False    # first field source is a pointer array
+    py: bool = False    # second field source is a pointer array
+    ir: bool = False    # per-operand index forms (QLA_INDEX_ALL)
+    ix: bool = False
+    iy: bool = False
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ix or self.iy
+
+    @property
+    def scalar(self) -> str:
+        return VECFAMS[self.fam][1]
+
+    @property
+    def nsrc(self) -> int:
+        return VECFAMS[self.fam][2]
+
+    @property
+    def xform(self) -> int:
+        return 3 if self.adj else VECFAMS[self.fam][4]
+
+    def _names(self) -> tuple[str, str]:
+        """QLA argument names of the two field sources (prototype.pl:600-640:
+        src1 a, src2 b, src3 c; a constant takes the src1 slot)."""
+        return ("b", "c") if self.scalar else ("a", "b")
+
+    @property
+    def name(self) -> str:
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + self.t
+        x = ("p" if self.px else "") + ("x" if self.ix else "") + self.t + ("a" if self.adj else "")
+        y = ("p" if self.py else "") + ("x" if self.iy else "") + self.t
+        eq = EQOPS[self.eqop]
+        head = f"QLA_{self.prec}_{r}_{form}{eq}_"
+        if self.nsrc == 1:
+            if self.scalar:
+                return f"{head}{self.scalar}_times_{x}"
+            return f"{head}{VECFAMS[self.fam][3]}{x}"
+        pm = "plus" if self.fam.endswith("plus") else "minus"
+        if self.scalar:
+            return f"{head}{self.scalar}_times_{x}_{pm}_{y}"
+        return f"{head}{x}_{pm}_{y}"
+
+    def params(self) -> list[tuple[str, str]]:
+        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        real = "D" if self.prec == "D3" else "F"
+        xn, yn = self._names()
+        out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        if self.scalar:
+            st = "Real" if self.scalar == "r" else "Complex"
+            out.append(("p", f"QLA_{real}_{st} *QLA_MI355X_RESTRICT a"))
+        out.append(("p", f"{rec} *QLA_MI355X_RESTRICT {'*' if self.px else ''}{xn}"))
+        if self.ix:
+            out.append(("p", f"int *index{xn}"))
+        if self.nsrc == 2:
+            out.append(("p", f"{rec} *QLA_MI355X_RESTRICT {'*' if self.py else ''}{yn}"))
+            if self.iy:
+                out.append(("p", f"int *index{yn}"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        xn, yn = self._names()
+        y = f"(const void *){yn}" if self.nsrc == 2 else "nullptr"
+        s = "(const void *)a" if self.scalar else "nullptr"
+        sk = {"": 0, "r": 1, "c": 2}[self.scalar]
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), (f"index{xn}", self.ix),
+                                                          (f"index{yn}", self.iy)))
+        return (f"vec_entry({prec}, {VECFAMS[self.fam][0]}, {self.eqop}, {NCOMP[self.t]}, {self.xform}, "
+                f"{int(self.px)}, {int(self.py)}, r, (const void *){xn}, {y}, {s}, {sk}, {idx}, {own}, n, "
+                f"\"{self.name}\");")
+
+
+def vecops() -> list[VecVariant]:
+    out = []
+    for prec in PRECS:
+        for fam, (_, _, nsrc, _, _, types) in VECFAMS.items():
+            eqops = range(4) if fam in VEC_ALL_EQOPS else (0,)
+            for t in types:
+                for adj in ((False, True) if fam in VEC_ADJ and t in SQUARE_TYPES else (False,)):
+                    for eqop in eqops:
+                        if nsrc == 1:
+                            out += [VecVariant(prec, fam, t, eqop, adj, xf, px) for xf in (False, True)
+                                    for px in (False, True)]
+                            out += [VecVariant(prec, fam, t, eqop, adj, ir=ir, ix=ix) for ir, ix in UNARY_OWN_FORMS]
+                        else:
+                            out += [VecVariant(prec, fam, t, eqop, adj, xf, px, py) for xf in (False, True)
+                                    for px in (False, True) for py in (False, True)]
+                            out += [VecVariant(prec, fam, t, eqop, adj, ir=ir, ix=ix, iy=iy)
+                                    for ir, ix, iy in OWN_INDEX_FORMS]
+    return out
+
+
 def reductions() -> list[RedVariant]:
     out = []
     for prec in ("F3", "D3", "DF3"):
@@ -424,8 +561,9 @@ def conversions() -> list[ConvVariant]:
 
 
 def all_symbols() -> list:
-    """Every generated symbol (site family, reductions, conversions, matrix functions)."""
-    return variants() + reductions() + conversions() + matfuncs()
+    """Every generated symbol (site family, reductions, conversions, matrix functions,
+    field linear algebra)."""
+    return variants() + reductions() + conversions() + matfuncs() + vecops()
 
 
 def variants() -> list[Variant]:
@@ -674,6 +812,71 @@ def matfuncs_inc_text() -> str:
     return "\n".join(lines)
 
 
+_VEC_DOC = {
+    "copy": ("copy or increment, r eqop a; `Ta` is the Hermitian conjugate",
+             "build_qla.pl:356-430, macros QLA_c_<eqop>_c / _ca (include/qla_complex.h:147-163)"),
+    "transpose": ("transpose, r eqop a^T (P: the (colour, spin) pairs swapped)",
+                  "build_qla.pl:432-460, lib/perl/headers.pl:68-90"),
+    "conj": ("complex conjugate, r eqop conj(a)", "build_qla.pl:462-493, QLA_c_<eqop>_ca"),
+    "i": ("times i, r eqop i a", "build_qla.pl:1328-1354, QLA_c_<eqop>_ic (include/qla_complex.h:165-172)"),
+    "r_times": ("times a real constant, r eqop a[0] b (`Ta`: the adjoint of b)",
+                "build_qla.pl:1256-1293, QLA_c_<eqop>_r_times_c / _ca (include/qla_complex.h:259-284)"),
+    "c_times": ("times a complex constant, r eqop a[0] b (`Ta`: the adjoint of b)",
+                "build_qla.pl:1295-1326, QLA_c_<eqop>_c_times_c / _ca (include/qla_complex.h:326-379)"),
+    "plus": ("sum, r = a + b", "build_qla.pl:1421-1456, QLA_c_eq_c_plus_c"),
+    "minus": ("difference, r = a - b", "build_qla.pl:1421-1456, QLA_c_eq_c_minus_c"),
+    "r_times_plus": ("r = a[0] b + c, real a", "build_qla.pl:1837-1887, QLA_c_eq_r_times_c_plus_c"),
+    "r_times_minus": ("r = a[0] b - c, real a", "build_qla.pl:1837-1887, QLA_c_eq_r_times_c_minus_c"),
+    "c_times_plus": ("r = a[0] b + c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_plus_c"),
+    "c_times_minus": ("r = a[0] b - c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_minus_c"),
+}
+
+
+def vecops_header_text() -> str:
+    lines = [
+        "/*",
+        " * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:",
+        " * copies and increments, Hermitian conjugate, transpose, complex conjugate,",
+        " * times i, times a real or complex constant, sums, differences and",
+        " * a*b +- c with a constant a, for the types V, H, D, M, P.",
+        " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
+        " * semantics are documented there).",
+        " *",
+        " * Names and argument order are QLA's (lib/perl/prototype.pl:450-688): the",
+        " * constant `a` is a pointer to one QLA_F_Real / QLA_F_Complex (D: double)",
+        " * in host or device memory; a host value is read during the call, a",
+        " * device one when the kernel runs.  A `p` operand is a pointer array;",
+        " * x-forms use every operand at index[i]; `x` operands use their own index",
+        " * arrays (QLA_INDEX_ALL forms).",
+        " */",
+        "#ifndef QLA_MI355X_VECOPS_H",
+        "#define QLA_MI355X_VECOPS_H",
+    ]
+    last = None
+    for v in vecops():
+        key = (v.prec, v.fam)
+        if key != last:
+            what, ref = _VEC_DOC[v.fam]
+            lines += ["", f"/* {v.prec}: {what}.", f" * Reference: {ref}. */"]
+            last = key
+        lines.append(_decl(v) + ";")
+    lines += ["", "#endif", ""]
+    return "\n".join(lines)
+
+
+def vecops_inc_text() -> str:
+    lines = [
+        "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
+        "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry is there.",
+    ]
+    for v in vecops():
+        lines += [_decl(v), "{", "    " + v.body(), "}"]
+    lines.append("")
+    return "\n".join(lines)
+
+
 # generic names (prototype.pl:556-575, make_define_map :423-441, selection
 # :216-260): for a library QLA_<P><C>, the fully generic QLA_<rest> when
 # QLA_Precision == P and QLA_Colors == C, the colour-generic QLA_<P>_<rest>
@@ -782,7 +985,8 @@ def generic_header_text() -> str:
 
 _OUTPUTS = ((HEADER_OUT, header_text), (INC_OUT, inc_text), (FIELDS_HEADER_OUT, fields_header_text),
             (FIELDS_INC_OUT, fields_inc_text), (MATFUNCS_HEADER_OUT, matfuncs_header_text),
-            (MATFUNCS_INC_OUT, matfuncs_inc_text), (GENERIC_HEADER_OUT, generic_header_text))
+            (MATFUNCS_INC_OUT, matfuncs_inc_text), (VECOPS_HEADER_OUT, vecops_header_text),
+            (VECOPS_INC_OUT, vecops_inc_text), (GENERIC_HEADER_OUT, generic_header_text))
 
 
 def write() -> None:

@@ -40,13 +40,14 @@ CXXFLAGS = [
 
 
 # Sources compiled more than once, each time with its own defines: the site
-# kernels once per (precision, kind), the multi-source kernels once per
-# precision and the matrix functions once per (precision, nc), so the kernel
-# instantiations spread over parallel compiles.
+# kernels once per (precision, kind), the multi-source kernels and the field
+# linear algebra once per precision and the matrix functions once per
+# (precision, nc), so the kernel instantiations spread over parallel compiles.
 MULTI_BUILD = {
     "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in (0, 1, 2, 3, 5, 6, 7, 8, 9, 10, 11)],
     "qla_multi.hip": [{"QLA_SITE_PREC": p} for p in (0, 1)],
     "qla_matfn.hip": [{"QLA_MATFN_UNIT": u} for u in range(7)],
+    "qla_vecops.hip": [{"QLA_VEC_PREC": p} for p in (0, 1)],
 }
 
 

@@ -131,6 +131,49 @@ void matfunc_entry(int prec, int fn, int nc, int pa, int pb, void *r, const void
     finish(qla_launch_matfunc(prec, fn, nc, p, ts.stream), name);
 }
 
+// One field linear-algebra call (qla_vecops_abi.inc): op and eqop codes of
+// qla_launch_vec; xform bit 0 conjugates and bit 1 transposes the source x;
+// scalar 0 none, 1 a real constant, 2 a complex constant at s, in host or
+// device memory (a host value is read here, a device one by the kernel when
+// it runs, so it may be the result of earlier work on the stream).
+void vec_entry(int prec, int op, int eqop, int ncomp, int xform, int px, int py, void *r, const void *x,
+               const void *y, const void *s, int scalar, const int *index, const int *ir, const int *ix,
+               const int *iy, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = x;
+    p.y = y;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ix;
+    p.iy = iy;
+    p.n = n;
+    p.px = px;
+    p.py = py;
+    p.conj = xform & 1;
+    p.trans = (xform >> 1) & 1;
+    if (scalar) {
+        if (!s) {
+            record(hipErrorInvalidValue, name);
+            return;
+        }
+        if (is_device_pointer(s)) {
+            p.sp = s;
+        } else if (prec == 0) {
+            p.s_re = ((const float *)s)[0];
+            p.s_im = scalar == 2 ? ((const float *)s)[1] : 0.0;
+        } else {
+            p.s_re = ((const double *)s)[0];
+            p.s_im = scalar == 2 ? ((const double *)s)[1] : 0.0;
+        }
+    }
+    p.al = ((uintptr_t)r % 16 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)y % 16 == 0) ? 1 : 0;
+    finish(qla_launch_vec(prec, op, eqop, ncomp, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -332,6 +375,13 @@ hipError_t qla_launch_multi(int prec, int eqop, int adj, int xf, const qla_multi
     return hipErrorInvalidValue;
 }
 
+hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_vec_launch_p0(op, eqop, ncomp, p, st);
+    if (prec == 1) return qla_vec_launch_p1(op, eqop, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
 extern "C" {
 
 // ----------------------------------------------------------------- control
@@ -361,6 +411,9 @@ void QLA_MI355X_set_tiles_per_wave(int tpw)
 
 // ------------------------------------------------------ colour-matrix functions
 #include "qla_matfuncs_abi.inc"
+
+// ------------------------------------------------------ field linear algebra
+#include "qla_vecops_abi.inc"
 
 // ----------------------------------------------------------------- compressed
 void QLA_MI355X_F3_M_pack12(void *r, int fmt, QLA_F3_ColorMatrix *a, int n)

@@ -82,6 +82,29 @@ hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, hipStream_t 
 hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, hipStream_t st);
 hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t st);
 
+// Field linear algebra (qla_vecops.hip): n steps of r eqop op(x, y) over
+// records of ncomp complexes (3, 6, 9, 12 or 144).  op: 0 copy, 1 times i,
+// 2 times the real constant, 3 times the complex constant (eqop 0..3);
+// 4 x + y, 5 x - y, 6 s*x + y, 7 s*x - y with a real s, 8 / 9 the same with
+// a complex s (eqop 0 only).  conj conjugates and trans transposes (records
+// of 9 or 144 complexes, as 3 x 3 or 12 x 12 matrices) the source x.  Step i
+// writes r at ir[i], index[i] (x-form) or i and reads x at ix[i], index[i] or
+// i, through the pointer array when px (likewise y).
+struct qla_vec_args {
+    void *r;
+    const void *x, *y;                  // field sources, or pointer arrays when px / py
+    const void *sp;                     // the constant in device memory (real, or re and im), or null
+    double s_re, s_im;                  // the constant's value when sp is null
+    const int *index, *ir, *ix, *iy;
+    int n, px, py;
+    int conj, trans;
+    int al;                             // r, x and y 16-byte aligned (the contiguous v-form path)
+    int wide;                           // n * ncomp >= 2^32 (set by the launcher)
+};
+hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_vec_launch_p0(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
                                hipStream_t st);

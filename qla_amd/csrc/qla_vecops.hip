@@ -1,0 +1,330 @@
+// qla_vecops.hip -- field linear algebra around the Wilson hop: copies and
+// increments, Hermitian conjugate, transpose, complex conjugate, times i,
+// times a real or complex constant, sums and differences and a*b +- c with a
+// real or complex constant a (build_qla.pl:356-494, 1256-1354, 1421-1456,
+// 1837-1935), for records of V, H, D, M and P.
+//
+// Every operation is element-wise on the complex components, so the kernels
+// stream components, not records:
+//
+//  * k_vec_flat: v-forms over fields (no index, no pointer array, no
+//    transpose).  The three arrays are one run of n*ncomp complexes; each
+//    work item moves 16 bytes per operand (two float complexes or one double
+//    complex) with non-temporal loads and stores.  The record type does not
+//    enter, so one kernel serves V, H, D, M and P.
+//  * k_vec_gather: x-forms, pointer-array sources, the per-operand index
+//    forms and the transposes.  One work item per destination complex
+//    (record i = g / NC, component g % NC); each resolves its own record
+//    addresses (the lanes of one record read the same index and pointer
+//    entries, which the memory pipeline merges), so every record is still
+//    read and written as a contiguous run.
+//
+// Both grids are capped at 2^20 blocks of 256 and stride over their work:
+// an HSA dispatch counts its work items in 32 bits, and a P field of 2^25
+// records already holds more than 2^32 complexes.
+//
+// Arithmetic.  The reference macros (include/qla_complex.h) are restated per
+// element with the conjugated-source forms (_ca) computed as the plain form
+// on the conjugated source, which is bit-identical: negating an operand is
+// exact and IEEE multiplication and addition are sign-symmetric and
+// commutative, so a*(-b) = -(a*b), x - (-y) = x + y and (-p) + q = q - p hold
+// bit for bit, signed zeros included.  The eqm form of the complex constant
+// (QLA_c_eqm_c_times_c, include/qla_complex.h:334-337) is not the negation
+// of its eq form and is kept as written.
+#include "qla_dev.hpp"
+#include "qla_launch.hpp"
+
+namespace qla {
+namespace {
+
+constexpr int VBLOCK = 256;
+
+enum { VOP_COPY = 0, VOP_I, VOP_RSCALE, VOP_CSCALE, VOP_PLUS, VOP_MINUS, VOP_R_PLUS, VOP_R_MINUS, VOP_C_PLUS,
+       VOP_C_MINUS };
+
+template <int OP> constexpr bool has_y() { return OP >= VOP_PLUS; }
+template <int OP> constexpr bool has_scalar() { return OP == VOP_RSCALE || OP == VOP_CSCALE || OP >= VOP_R_PLUS; }
+template <int OP> constexpr bool complex_scalar() { return OP == VOP_CSCALE || OP == VOP_C_PLUS || OP == VOP_C_MINUS; }
+template <int EQOP> constexpr bool reads_r() { return EQOP == 1 || EQOP == 2; }
+
+template <class T, int V> struct vtype;
+template <> struct vtype<float, 1> { typedef float v __attribute__((ext_vector_type(2))); };
+template <> struct vtype<float, 2> { typedef float v __attribute__((ext_vector_type(4))); };
+template <> struct vtype<double, 1> { typedef double v __attribute__((ext_vector_type(2))); };
+
+// the constant: from device memory (sp) or the value passed by the host
+template <class T, int OP>
+__device__ __forceinline__ void load_scalar(const qla_vec_args &p, T &sr, T &si)
+{
+    sr = T(0);
+    si = T(0);
+    if constexpr (has_scalar<OP>()) {
+        if (p.sp) {
+            const T *s = (const T *)p.sp;
+            sr = s[0];
+            if constexpr (complex_scalar<OP>()) si = s[1];
+        } else {
+            sr = (T)p.s_re;
+            si = (T)p.s_im;
+        }
+    }
+}
+
+// r eqop op(x, y) on one complex; x already conjugated for the _ca forms
+template <int OP, int EQOP, class T>
+__device__ __forceinline__ void vop(T &rr, T &ri, T xr, T xi, T yr, T yi, T sr, T si)
+{
+    T vr, vi;
+    if constexpr (OP == VOP_PLUS) {                    // QLA_c_eq_c_plus_c (qla_complex.h:287)
+        rr = xr + yr;
+        ri = xi + yi;
+        return;
+    } else if constexpr (OP == VOP_MINUS) {            // QLA_c_eq_c_minus_c (:305)
+        rr = xr - yr;
+        ri = xi - yi;
+        return;
+    } else if constexpr (OP == VOP_R_PLUS) {           // QLA_c_eq_r_times_c_plus_c (:445)
+        rr = sr * xr + yr;
+        ri = sr * xi + yi;
+        return;
+    } else if constexpr (OP == VOP_R_MINUS) {          // QLA_c_eq_r_times_c_minus_c (:449)
+        rr = sr * xr - yr;
+        ri = sr * xi - yi;
+        return;
+    } else if constexpr (OP == VOP_C_PLUS) {           // QLA_c_eq_c_times_c_plus_c (:453)
+        rr = sr * xr - si * xi + yr;
+        ri = sr * xi + si * xr + yi;
+        return;
+    } else if constexpr (OP == VOP_C_MINUS) {          // QLA_c_eq_c_times_c_minus_c (:459)
+        rr = sr * xr - si * xi - yr;
+        ri = sr * xi + si * xr - yi;
+        return;
+    } else if constexpr (OP == VOP_COPY) {             // QLA_c_<eqop>_c (:147-154)
+        vr = xr;
+        vi = xi;
+    } else if constexpr (OP == VOP_I) {                // QLA_c_<eqop>_ic (:165-172)
+        vr = -xi;
+        vi = xr;
+    } else if constexpr (OP == VOP_RSCALE) {           // QLA_c_<eqop>_c_times_r (:259-266)
+        vr = sr * xr;
+        vi = sr * xi;
+    } else {                                           // QLA_c_<eqop>_c_times_c (:326-341)
+        if constexpr (EQOP == 3) {
+            rr = -sr * xr + si * xi;
+            ri = -sr * xi - si * xr;
+            return;
+        }
+        vr = sr * xr - si * xi;
+        vi = sr * xi + si * xr;
+    }
+    if constexpr (EQOP == 0) {
+        rr = vr;
+        ri = vi;
+    } else if constexpr (EQOP == 1) {
+        rr = rr + vr;
+        ri = ri + vi;
+    } else if constexpr (EQOP == 2) {
+        rr = rr - vr;
+        ri = ri - vi;
+    } else {
+        rr = -vr;
+        ri = -vi;
+    }
+}
+
+// v-forms over fields: work item t moves complexes [t*V, t*V + V) of the run
+// of m; V = 2 (float, all three bases 16-byte aligned) or 1.  An odd float
+// run ends with one single complex in the last item.  The grid is capped
+// (a dispatch holds fewer than 2^32 work items) and strides over the items.
+template <class T, int OP, int EQOP, int V>
+__global__ void __launch_bounds__(VBLOCK) k_vec_flat(qla_vec_args p, long long m)
+{
+    typedef typename vtype<T, V>::v VT;
+    typedef typename vtype<T, 1>::v V1;
+    T sr, si;
+    load_scalar<T, OP>(p, sr, si);
+    const bool cj = p.conj != 0;
+    const long long items = (m + V - 1) / V;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long t = (long long)blockIdx.x * VBLOCK + threadIdx.x; t < items; t += stride) {
+        const long long e = t * V;
+        if (V == 1 || e + V <= m) {
+            const VT x = __builtin_nontemporal_load((const VT *)p.x + t);
+            VT y = {}, r = {};
+            if constexpr (has_y<OP>()) y = __builtin_nontemporal_load((const VT *)p.y + t);
+            if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load((const VT *)p.r + t);
+#pragma unroll
+            for (int k = 0; k < V; k++) {
+                T rr = r[2 * k], ri = r[2 * k + 1];
+                const T xr = x[2 * k], xi = cj ? -x[2 * k + 1] : x[2 * k + 1];
+                vop<OP, EQOP, T>(rr, ri, xr, xi, y[2 * k], y[2 * k + 1], sr, si);
+                r[2 * k] = rr;
+                r[2 * k + 1] = ri;
+            }
+            __builtin_nontemporal_store(r, (VT *)p.r + t);
+        } else {
+            // the last complex of an odd float run
+            const V1 x = __builtin_nontemporal_load((const V1 *)p.x + e);
+            V1 y = {}, r = {};
+            if constexpr (has_y<OP>()) y = __builtin_nontemporal_load((const V1 *)p.y + e);
+            if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load((const V1 *)p.r + e);
+            T rr = r[0], ri = r[1];
+            vop<OP, EQOP, T>(rr, ri, x[0], cj ? -x[1] : x[1], y[0], y[1], sr, si);
+            r[0] = rr;
+            r[1] = ri;
+            __builtin_nontemporal_store(r, (V1 *)p.r + e);
+        }
+    }
+}
+
+// one step's record of an operand: the pointer-array entry or the field record
+template <class T, int NC>
+__device__ __forceinline__ const cpx<T> *record_of(const void *base, int ptr, int s)
+{
+    if (ptr) return ((const cpx<T> *const *)base)[s];
+    return (const cpx<T> *)base + (size_t)s * NC;
+}
+
+// every other form: one work item per destination complex g = i*NC + c,
+// strided over a capped grid; the record number is a 32-bit quotient unless
+// the run holds 2^32 complexes or more (wide)
+template <class T, int NC, int OP, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_gather(qla_vec_args p)
+{
+    typedef typename vtype<T, 1>::v V1;
+    T sr, si;
+    load_scalar<T, OP>(p, sr, si);
+    const unsigned long long tot = (unsigned long long)p.n * NC;
+    const unsigned long long stride = (unsigned long long)gridDim.x * VBLOCK;
+    for (unsigned long long g = (unsigned long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        int i, c;
+        if (p.wide) {
+            i = (int)(g / NC);
+            c = (int)(g - (unsigned long long)i * NC);
+        } else {
+            const unsigned int g32 = (unsigned int)g;
+            i = (int)(g32 / NC);
+            c = (int)(g32 - (unsigned int)i * NC);
+        }
+        const int s = p.index ? p.index[i] : i;
+        const int sd = p.ir ? p.ir[i] : s;
+        const cpx<T> *X = record_of<T, NC>(p.x, p.px, p.ix ? p.ix[i] : s);
+        int cx = c;
+        if constexpr (NC == 9 || NC == 144) {
+            constexpr int D = NC == 9 ? 3 : 12;     // M: 3 x 3; P: 12 x 12 over (colour, spin)
+            if (p.trans) cx = (c % D) * D + c / D;
+        }
+        const V1 x = __builtin_nontemporal_load((const V1 *)(X + cx));
+        V1 y = {}, r = {};
+        if constexpr (has_y<OP>()) {
+            const cpx<T> *Y = record_of<T, NC>(p.y, p.py, p.iy ? p.iy[i] : s);
+            y = __builtin_nontemporal_load((const V1 *)(Y + c));
+        }
+        V1 *R = (V1 *)((cpx<T> *)p.r + (size_t)sd * NC + c);
+        if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load(R);
+        T rr = r[0], ri = r[1];
+        vop<OP, EQOP, T>(rr, ri, x[0], p.conj ? -x[1] : x[1], y[0], y[1], sr, si);
+        r[0] = rr;
+        r[1] = ri;
+        __builtin_nontemporal_store(r, R);
+    }
+}
+
+// blocks for `items` work items, at most VMAXBLOCKS (2^28 work items per dispatch)
+constexpr long long VMAXBLOCKS = 1 << 20;
+unsigned blocks_for(long long items)
+{
+    const long long b = (items + VBLOCK - 1) / VBLOCK;
+    return (unsigned)(b < 1 ? 1 : (b > VMAXBLOCKS ? VMAXBLOCKS : b));
+}
+
+template <class T, int OP, int EQOP>
+hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    const bool flat = !p.index && !p.ir && !p.ix && !p.iy && !p.px && !p.py && !p.trans;
+    if (flat) {
+        const long long m = (long long)p.n * ncomp;
+        if constexpr (sizeof(T) == 4) {
+            if (p.al) {
+                hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 2>), dim3(blocks_for((m + 1) / 2)),
+                                   dim3(VBLOCK), 0, st, p, m);
+                return hipGetLastError();
+            }
+        }
+        hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 1>), dim3(blocks_for(m)), dim3(VBLOCK), 0, st, p, m);
+        return hipGetLastError();
+    }
+    qla_vec_args q = p;
+    const long long tot = (long long)p.n * ncomp;
+    q.wide = tot > 0xffffffffLL ? 1 : 0;
+    const dim3 grid(blocks_for(tot));
+#define QLA_VEC_CASE(NC)                                                                          \
+    case NC:                                                                                     \
+        hipLaunchKernelGGL((k_vec_gather<T, NC, OP, EQOP>), grid, dim3(VBLOCK), 0, st, q);      \
+        return hipGetLastError();
+    switch (ncomp) {
+        QLA_VEC_CASE(3)
+        QLA_VEC_CASE(6)
+        QLA_VEC_CASE(9)
+        QLA_VEC_CASE(12)
+        QLA_VEC_CASE(144)
+    }
+#undef QLA_VEC_CASE
+    return hipErrorInvalidValue;
+}
+
+template <class T, int OP>
+hipError_t launch_eqop(int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if constexpr (OP >= VOP_PLUS) {
+        if (eqop != 0) return hipErrorInvalidValue;        // sums and a*b +- c: eq only
+        return launch_op<T, OP, 0>(ncomp, p, st);
+    } else {
+        switch (eqop) {
+        case 0: return launch_op<T, OP, 0>(ncomp, p, st);
+        case 1: return launch_op<T, OP, 1>(ncomp, p, st);
+        case 2: return launch_op<T, OP, 2>(ncomp, p, st);
+        case 3: return launch_op<T, OP, 3>(ncomp, p, st);
+        }
+        return hipErrorInvalidValue;
+    }
+}
+
+template <class T>
+hipError_t launch_t(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    if (p.trans && ncomp != 9 && ncomp != 144) return hipErrorInvalidValue;
+    switch (op) {
+    case VOP_COPY: return launch_eqop<T, VOP_COPY>(eqop, ncomp, p, st);
+    case VOP_I: return launch_eqop<T, VOP_I>(eqop, ncomp, p, st);
+    case VOP_RSCALE: return launch_eqop<T, VOP_RSCALE>(eqop, ncomp, p, st);
+    case VOP_CSCALE: return launch_eqop<T, VOP_CSCALE>(eqop, ncomp, p, st);
+    case VOP_PLUS: return launch_eqop<T, VOP_PLUS>(eqop, ncomp, p, st);
+    case VOP_MINUS: return launch_eqop<T, VOP_MINUS>(eqop, ncomp, p, st);
+    case VOP_R_PLUS: return launch_eqop<T, VOP_R_PLUS>(eqop, ncomp, p, st);
+    case VOP_R_MINUS: return launch_eqop<T, VOP_R_MINUS>(eqop, ncomp, p, st);
+    case VOP_C_PLUS: return launch_eqop<T, VOP_C_PLUS>(eqop, ncomp, p, st);
+    case VOP_C_MINUS: return launch_eqop<T, VOP_C_MINUS>(eqop, ncomp, p, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+} // namespace qla
+
+#ifndef QLA_VEC_PREC
+#define QLA_VEC_PREC 0
+#endif
+
+#if QLA_VEC_PREC == 0
+hipError_t qla_vec_launch_p0(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_t<float>(op, eqop, ncomp, p, st);
+}
+#else
+hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_t<double>(op, eqop, ncomp, p, st);
+}
+#endif

@@ -21,7 +21,7 @@ from .abi_gen import all_symbols
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libqla_mi355x.so"
 HEADERS = tuple(HERE.parent / "include" / h for h in ("qla_mi355x.h", "qla_mi355x_sites.h", "qla_mi355x_fields.h",
-                                                     "qla_mi355x_matfuncs.h"))
+                                                     "qla_mi355x_matfuncs.h", "qla_mi355x_vecops.h"))
 
 vp = C.c_void_p
 i32 = C.c_int
@@ -59,8 +59,9 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
 }
 # the generated families: the Wilson-hop building blocks (qla_mi355x_sites.h),
-# the reductions and conversions (qla_mi355x_fields.h) and the colour-matrix
-# functions (qla_mi355x_matfuncs.h)
+# the reductions and conversions (qla_mi355x_fields.h), the colour-matrix
+# functions (qla_mi355x_matfuncs.h) and the field linear algebra
+# (qla_mi355x_vecops.h)
 _SIGS.update({v.name: (None, [vp if k == "p" else i32 for k in v.ctypes_kinds()]) for v in all_symbols()})
 
 class F3Job(C.Structure):

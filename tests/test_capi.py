@@ -38,9 +38,18 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 3109
+    assert len(every) == len(set(every)) == 5857
     mf = [v.name for v in abi_gen.matfuncs()]
     assert len(mf) == 321
+    vo = [v.name for v in abi_gen.vecops()]
+    assert len(vo) == 2748
+    # the solver-side calls of QDP/QOP-style codes and the reference's test names
+    for s in ("QLA_F3_D_veq_D", "QLA_F3_D_vpeq_r_times_D", "QLA_D3_D_veq_r_times_D_plus_D",
+              "QLA_F3_D_veq_r_times_D_minus_D", "QLA_F3_D_veq_c_times_pD_plus_D", "QLA_F3_M_veq_Ma",
+              "QLA_D3_P_veqm_transpose_P", "QLA_F3_D_vpeq_conj_D", "QLA_F3_D_vmeq_i_D", "QLA_F3_M_vpeq_c_times_pMa",
+              "QLA_F3_xD_eq_D_minus_xD", "QLA_D3_V_eq_r_times_xV_plus_V", "QLA_F3_xM_peq_Ma",
+              "QLA_F3_D_xeq_pD_plus_pD", "QLA_F3_H_veq_H_plus_H"):
+        assert s in vo, s
     for s in ("QLA_F3_M_veq_expA_M", "QLA_D3_C_xeq_det_pM", "QLA_FN_M_veq_log_M", "QLA_F3_V_veq_eigenvalsH_M",
               "QLA_D3_V_veq_pM_inverse_pV", "QLA_F3_xM_eq_xM_inverse_M", "QLA_FN_M_veq_exp_M"):
         assert s in mf, s
@@ -62,7 +71,9 @@ def test_header_compiles_as_c_and_cxx(tmp_path):
                    'void g(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix **a, QLA_F_Complex *d, int *ix, int n)\n'
                    '{ QLA_F3_M_veq_expTA_pM(r, a, n); QLA_F3_C_xeq_det_M(d, r, ix, n); }\n'
                    'void h(int nc, QLA_F_Complex (*r)[4][4], QLA_F_Complex (*a)[4][4], int n)\n'
-                   '{ QLA_FN_M_veq_log_M(nc, r, a, n); }\n')
+                   '{ QLA_FN_M_veq_log_M(nc, r, a, n); }\n'
+                   'void k(QLA_D3_DiracFermion *x, QLA_D3_DiracFermion *p, QLA_D_Real *al, QLA_D_Complex *z, int n)\n'
+                   '{ QLA_D3_D_vpeq_r_times_D(x, al, p, n); QLA_D3_D_veq_c_times_D_plus_D(p + n, z, p, x, n); }\n')
     for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
         args = [cc, std, "-fsyntax-only", "-Wall", "-Werror", "-I", str(inc)]
         if cc == "g++":

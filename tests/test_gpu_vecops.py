@@ -1,0 +1,213 @@
+"""GPU parity of every field linear-algebra symbol (include/qla_mi355x_vecops.h)
+through the C ABI: copy / increment, Hermitian conjugate, transpose, complex
+conjugate, times i, times a real or complex constant, sums, differences and
+a*b +- c, for V, H, D, M, P in QLA_F3 and QLA_D3, every eqop and indexing
+form (v, x, pointer operands, per-operand indices).
+
+Bar: bit-exact against the oracle (orc_*_vecop) on the same inputs -- every
+element is one or two IEEE operations in a fixed order, so any difference is
+a bug -- including the records a call must leave untouched.  97 records
+make the float v-form run of a V field odd (291 complexes), so the last
+thread's single-complex tail is exercised.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+import qla_amd
+from qla_amd import lattice
+from qla_amd.abi_gen import NCOMP, VECFAMS, vecops
+from gpu_util import assert_bitwise, dev, host, say
+
+pytestmark = pytest.mark.gpu
+
+N = 97
+TDIM = {"M": 3, "P": 12}
+S_REAL = -0.7071067811865476
+S_CPLX = complex(0.6, -1.3)
+
+
+class Case:
+    """Fields, pointer arrays and index lists of one (precision, type)."""
+
+    def __init__(self, prec, t, n=N):
+        self.prec, self.t, self.n = prec, t, n
+        self.dt = np.float64 if prec == "D3" else np.float32
+        rng = np.random.default_rng(zlib.crc32(f"vec {prec} {t}".encode()))
+        self.x = lattice.random_field(t, n, 31, dtype=self.dt)
+        self.y = lattice.random_field(t, n, 32, dtype=self.dt)
+        self.r0 = lattice.random_field(t, n, 33, dtype=self.dt)
+        self.dx, self.dy = dev(self.x), dev(self.y)
+        self.x_idx = rng.permutation(n)
+        self.y_idx = rng.permutation(n)
+        self.px = qla_amd.pointer_array(self.dx, self.x_idx)
+        self.py = qla_amd.pointer_array(self.dy, self.y_idx)
+        self.index = np.sort(rng.choice(n, n // 2, replace=False)).astype(np.int32)
+        self.dindex = dev(self.index)
+        m = n // 3
+        self.own = {"r": rng.choice(n, m, replace=False).astype(np.int32),
+                    "x": rng.choice(n, m, replace=False).astype(np.int32),
+                    "y": rng.choice(n, m, replace=False).astype(np.int32)}
+        self.down = {k: dev(v) for k, v in self.own.items()}
+        self.sr = np.array([S_REAL], dtype=self.dt)
+        self.sc = np.array([S_CPLX.real, S_CPLX.imag], dtype=self.dt)
+
+    def scalar(self, v):
+        """(host pointer of the constant, its value as the oracle takes it)."""
+        if v.scalar == "r":
+            return self.sr.ctypes.data, float(self.sr[0])
+        if v.scalar == "c":
+            return self.sc.ctypes.data, complex(float(self.sc[0]), float(self.sc[1]))
+        return None, 0.0
+
+    def run(self, orc, v, sptr=None):
+        r = dev(self.r0)
+        args = [r]
+        if v.ir:
+            args.append(self.down["r"])
+        sp, sval = self.scalar(v)
+        if v.scalar:
+            args.append(sptr if sptr is not None else sp)
+        args.append(self.px if v.px else self.dx)
+        if v.ix:
+            args.append(self.down["x"])
+        if v.nsrc == 2:
+            args.append(self.py if v.py else self.dy)
+            if v.iy:
+                args.append(self.down["y"])
+        if v.xf:
+            args.append(self.dindex)
+        m = len(self.index) if v.xf else (len(self.own["r"]) if v.own_index else self.n)
+        say(f"{v.name} n={m}")
+        qla_amd.call(v.name, *args, m)
+        qla_amd.check()
+        # expected: the oracle on the records each step reads and writes
+        step = np.arange(m)
+        site = self.index if v.xf else step
+        rs = self.own["r"] if v.ir else site
+        xs = self.own["x"] if v.ix else site
+        ys = self.own["y"] if v.iy else site
+        gx = np.ascontiguousarray(self.x[self.x_idx[xs] if v.px else xs])
+        gy = np.ascontiguousarray(self.y[self.y_idx[ys] if v.py else ys]) if v.nsrc == 2 else None
+        sub = np.ascontiguousarray(self.r0[rs])
+        orc.vecop(VECFAMS[v.fam][0], v.eqop, sub, gx, gy, s=sval, conj=bool(v.xform & 1),
+                  dim=TDIM[v.t] if v.xform & 2 else 0)
+        exp = self.r0.copy()
+        exp[rs] = sub
+        return host(r), exp
+
+
+_cases = {}
+
+
+def case(prec, t):
+    if (prec, t) not in _cases:
+        _cases[(prec, t)] = Case(prec, t)
+    return _cases[(prec, t)]
+
+
+@pytest.mark.parametrize("v", vecops(), ids=lambda v: v.name)
+def test_vecop_symbol(orc, v):
+    got, exp = case(v.prec, v.t).run(orc, v)
+    assert_bitwise(got, exp, v.name)
+
+
+@pytest.mark.parametrize("prec", ["F3", "D3"])
+@pytest.mark.parametrize("fam", ["r_times", "c_times", "r_times_minus", "c_times_plus"])
+def test_device_constant(orc, prec, fam):
+    """The constant read from device memory gives the host-constant result."""
+    import torch
+    v = [x for x in vecops() if x.prec == prec and x.fam == fam and x.t == "D" and x.eqop == (1 if fam.endswith(
+        "times") else 0) and not (x.xf or x.px or x.py or x.own_index or x.adj)][0]
+    cs = case(prec, "D")
+    dsc = torch.from_numpy(cs.sr if v.scalar == "r" else cs.sc).to("cuda")
+    got, exp = cs.run(orc, v, sptr=dsc.data_ptr())
+    assert_bitwise(got, exp, f"{v.name} device constant")
+
+
+@pytest.mark.parametrize("prec", ["F3", "D3"])
+@pytest.mark.parametrize("t", ["V", "D", "P"])
+def test_unaligned_fields(orc, prec, t):
+    """v-form fields starting one complex past a 16-byte boundary (the float
+    run takes its one-complex-per-thread path)."""
+    import torch
+    v = [x for x in vecops() if x.prec == prec and x.fam == "c_times_plus" and x.t == t
+         and not (x.xf or x.px or x.py or x.own_index)][0]
+    cs = case(prec, t)
+    n = cs.n - 1
+    flat = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).to("cuda")   # noqa: E731
+    c2 = 2 * NCOMP[t]
+    bx, by, br = flat(cs.x), flat(cs.y), flat(cs.r0)
+    # n records of views that start one complex (2 reals) into each array
+    x, y, r = bx[2:2 + n * c2], by[2:2 + n * c2], br[2:2 + n * c2]
+    sp, sval = cs.scalar(v)
+    say(f"{v.name} unaligned n={n}")
+    qla_amd.call(v.name, r, sp, x, y, n)
+    qla_amd.check()
+    hx = cs.x.reshape(-1)[2:2 + n * c2].reshape((n,) + cs.x.shape[1:])
+    hy = cs.y.reshape(-1)[2:2 + n * c2].reshape((n,) + cs.y.shape[1:])
+    sub = np.ascontiguousarray(cs.r0.reshape(-1)[2:2 + n * c2].reshape((n,) + cs.r0.shape[1:]))
+    orc.vecop(VECFAMS[v.fam][0], 0, sub, np.ascontiguousarray(hx), np.ascontiguousarray(hy), s=sval)
+    exp = cs.r0.reshape(-1).copy()
+    exp[2:2 + n * c2] = sub.reshape(-1)
+    assert_bitwise(host(br), exp, f"{v.name} unaligned")
+
+
+def test_noop_and_errors(orc):
+    """n <= 0 writes nothing; a null constant records an error, writes nothing."""
+    import ctypes
+    cs = case("F3", "D")
+    r = dev(cs.r0)
+    say("QLA_F3_D_veq_r_times_D n=0 / null constant")
+    qla_amd.call("QLA_F3_D_veq_r_times_D", r, cs.sr.ctypes.data, cs.dx, 0)
+    qla_amd.call("QLA_F3_D_veq_D_plus_D", r, cs.dx, cs.dy, -3)
+    qla_amd.check()
+    qla_amd.call("QLA_F3_D_veq_r_times_D", r, ctypes.c_void_p(0), cs.dx, cs.n)
+    assert qla_amd.load().QLA_MI355X_last_error() != 0
+    assert_bitwise(host(r), cs.r0, "no-op calls")
+
+
+def test_full_volume_axpy(orc):
+    """The solver updates x += a p and r = a p - x on a 32^4 lattice (F3
+    DiracFermion, 1048576 records), bit-exact against the oracle."""
+    n = 32 ** 4
+    p = lattice.random_field("D", n, 41, dtype=np.float32)
+    x = lattice.random_field("D", n, 42, dtype=np.float32)
+    a = np.array([0.123456789], dtype=np.float32)
+    dp, dx = dev(p), dev(x)
+    say("QLA_F3_D_vpeq_r_times_D n=32^4")
+    qla_amd.call("QLA_F3_D_vpeq_r_times_D", dx, a.ctypes.data, dp, n)
+    qla_amd.check()
+    exp = x.copy()
+    orc.vecop(orc.RSCALE, 1, exp, p, s=float(a[0]))
+    assert_bitwise(host(dx), exp, "x += a p")
+    dr = dev(x)
+    say("QLA_F3_D_veq_r_times_D_minus_D n=32^4")
+    qla_amd.call("QLA_F3_D_veq_r_times_D_minus_D", dr, a.ctypes.data, dp, dev(x), n)
+    qla_amd.check()
+    exp = np.empty_like(x)
+    orc.vecop(orc.R_MINUS, 0, exp, p, x, s=float(a[0]))
+    assert_bitwise(host(dr), exp, "r = a p - x")
+
+
+def test_wide_transpose_of_propagators():
+    """P transposes over more than 2^32 complexes (the 64-bit step numbering
+    of the gather kernel): 29826162 DiracPropagator records, checked on the
+    device against torch's own axis swap (a transpose moves data only)."""
+    import torch
+    n = (1 << 32) // 144 + 7
+    need = 3 * n * 1152 + (1 << 30)
+    free, _ = torch.cuda.mem_get_info()
+    if free < need:
+        pytest.skip(f"needs {need >> 30} GiB of free device memory")
+    g = torch.Generator(device="cuda").manual_seed(5)
+    a = torch.randn((n, 12, 12, 2), dtype=torch.float32, device="cuda", generator=g)
+    r = torch.empty_like(a)
+    say(f"QLA_F3_P_veq_transpose_P n={n}")
+    qla_amd.call("QLA_F3_P_veq_transpose_P", r, a, n)
+    qla_amd.check()
+    ok = torch.equal(r, a.transpose(1, 2))
+    del a, r
+    torch.cuda.empty_cache()
+    assert ok

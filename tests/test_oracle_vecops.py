@@ -1,0 +1,317 @@
+"""Pins the oracle's field linear algebra (orc_*_vecop) against numpy.
+
+Each case is checked against an independent evaluation: numpy's own complex
+conjugate, axis transposes of the record shapes (the P transpose as a swap
+of the (colour, spin) axis pairs, not the 12 x 12 index formula the oracle
+uses), its IEEE single / double arithmetic on whole arrays (one rounding per
+operation, as C without contraction), exact complex products on small
+integers, long-double bounds on random data, and algebraic identities
+(i*(i*x) = -x, (x^dagger)^dagger = x, 1*x = x, 2*x = x + x, x + y - y).
+A dropped term, a sign slip, an unconjugated or untransposed source or a
+wrong eqop fails these by whole units, not ulps.  The eqm forms keep the
+reference macros' zero signs (include/qla_complex.h:149, 167, 268, 334,
+371), pinned on cases whose exact result is zero.
+"""
+import numpy as np
+import pytest
+
+from qla_amd import lattice
+
+TYPES = ("V", "H", "D", "M", "P")
+L = np.clongdouble
+
+
+def cx(a):
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def re_im(c, dtype):
+    return np.stack([c.real, c.imag], axis=-1).astype(dtype)
+
+
+def field(t, n, seed, dtype):
+    return lattice.random_field(t, n, seed, dtype=dtype)
+
+
+def transpose(t, c):
+    """Complex records (n, ...) transposed as matrices: M (n,3,3) swaps its two
+    axes; P (n,3,4,3,4) swaps (ic,is) with (jc,js)."""
+    if t == "M":
+        return np.swapaxes(c, 1, 2)
+    return np.transpose(c, (0, 3, 4, 1, 2))
+
+
+def eqop_np(eqop, r0, y):
+    """eq / peq / meq / eqm on the (re, im) arrays, one IEEE rounding each."""
+    return {0: y, 1: r0 + y, 2: r0 - y, 3: -y}[eqop]
+
+
+DT = {"F": np.float32, "D": np.float64}
+
+
+# (type, form): transposes exist for the square records M and P only
+COPY_FORMS = [(t, f) for t in TYPES for f in ("copy", "conj", "transpose", "adjoint")
+              if t in ("M", "P") or f in ("copy", "conj")]
+SCALE_FORMS = [(t, adj) for t in TYPES for adj in (False, True) if t in ("M", "P") or not adj]
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("t,form", COPY_FORMS)
+def test_copy_forms_vs_numpy(orc, prec, t, eqop, form):
+    dt = DT[prec]
+    n = 37
+    x = field(t, n, 1, dt)
+    r0 = field(t, n, 2, dt)
+    r = r0.copy()
+    conj = form in ("conj", "adjoint")
+    dim = orc.TRANSPOSE_DIM[t] if form in ("transpose", "adjoint") else 0
+    orc.vecop(orc.COPY, eqop, r, x, conj=conj, dim=dim)
+    c = cx(x)
+    if dim:
+        c = transpose(t, c)
+    if conj:
+        c = np.conj(c)
+    want = eqop_np(eqop, r0, re_im(c, dt))
+    np.testing.assert_array_equal(r, want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+def test_times_i(orc, prec, eqop):
+    dt = DT[prec]
+    x = field("D", 29, 3, dt)
+    r0 = field("D", 29, 4, dt)
+    r = r0.copy()
+    orc.vecop(orc.TIMES_I, eqop, r, x)
+    ix = np.stack([-x[..., 1], x[..., 0]], axis=-1)
+    np.testing.assert_array_equal(r, eqop_np(eqop, r0, ix))
+    # i * (i * x) = -x exactly
+    t1 = np.zeros_like(x)
+    t2 = np.zeros_like(x)
+    orc.vecop(orc.TIMES_I, 0, t1, x)
+    orc.vecop(orc.TIMES_I, 0, t2, t1)
+    np.testing.assert_array_equal(t2, -x)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("t,adj", SCALE_FORMS)
+def test_real_scale_vs_numpy(orc, prec, t, eqop, adj):
+    dt = DT[prec]
+    n = 31
+    x = field(t, n, 5, dt)
+    r0 = field(t, n, 6, dt)
+    s = dt(0.7310585786300049)
+    r = r0.copy()
+    dim = orc.TRANSPOSE_DIM[t] if adj else 0
+    orc.vecop(orc.RSCALE, eqop, r, x, s=float(s), conj=adj, dim=dim)
+    c = cx(x)
+    if adj:
+        c = np.conj(transpose(t, c))
+    src = re_im(c, dt)
+    np.testing.assert_array_equal(r, eqop_np(eqop, r0, s * src))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_real_scale_identities(orc, prec):
+    dt = DT[prec]
+    x = field("P", 9, 7, dt)
+    r = np.zeros_like(x)
+    orc.vecop(orc.RSCALE, 0, r, x, s=1.0)
+    np.testing.assert_array_equal(r, x)
+    orc.vecop(orc.RSCALE, 0, r, x, s=2.0)
+    s2 = np.zeros_like(x)
+    orc.vecop(orc.PLUS, 0, s2, x, x)
+    np.testing.assert_array_equal(r, s2)
+    orc.vecop(orc.RSCALE, 3, r, x, s=1.0)
+    m = np.zeros_like(x)
+    orc.vecop(orc.COPY, 3, m, x)
+    np.testing.assert_array_equal(r, m)
+
+
+def _int_field(t, n, seed, dt):
+    rng = np.random.default_rng(seed)
+    return rng.integers(-9, 10, size=field(t, n, 0, dt).shape).astype(dt)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("t,adj", SCALE_FORMS)
+def test_complex_scale_exact_on_integers(orc, prec, t, eqop, adj):
+    """Small integers: every product and sum is exact, so the result must
+    equal the complex128 product z*x (conj / transposed for the adjoint)."""
+    dt = DT[prec]
+    n = 23
+    x = _int_field(t, n, 8, dt)
+    r0 = _int_field(t, n, 9, dt)
+    z = complex(3, -5)
+    r = r0.copy()
+    dim = orc.TRANSPOSE_DIM[t] if adj else 0
+    orc.vecop(orc.CSCALE, eqop, r, x, s=z, conj=adj, dim=dim)
+    c = cx(x.astype(np.float64))
+    if adj:
+        c = np.conj(transpose(t, c))
+    y = z * c
+    c0 = cx(r0.astype(np.float64))
+    want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("adj", [False, True])
+def test_complex_scale_vs_long_double(orc, prec, eqop, adj):
+    dt = DT[prec]
+    u = np.finfo(dt).eps / 2
+    n = 40
+    x = field("M", n, 10, dt)
+    r0 = field("M", n, 11, dt)
+    z = complex(dt(-0.3141592653589793), dt(1.4142135623730951))
+    r = r0.copy()
+    orc.vecop(orc.CSCALE, eqop, r, x, s=z, conj=adj, dim=3 if adj else 0)
+    c = cx(x.astype(np.longdouble)).astype(L)
+    if adj:
+        c = np.conj(np.swapaxes(c, 1, 2))
+    y = L(z) * c
+    c0 = cx(r0.astype(np.longdouble)).astype(L)
+    want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
+    bound = 4 * u * (abs(z.real) + abs(z.imag)) * (np.abs(c.real) + np.abs(c.imag)) + 2 * u * np.abs(c0)
+    got = cx(r.astype(np.longdouble))
+    assert np.all(np.abs(got.real - want.real) <= bound + 1e-300)
+    assert np.all(np.abs(got.imag - want.imag) <= bound + 1e-300)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", TYPES)
+def test_sums_vs_numpy(orc, prec, t):
+    dt = DT[prec]
+    x = field(t, 17, 12, dt)
+    y = field(t, 17, 13, dt)
+    r = np.zeros_like(x)
+    orc.vecop(orc.PLUS, 0, r, x, y)
+    np.testing.assert_array_equal(r, x + y)
+    orc.vecop(orc.MINUS, 0, r, x, y)
+    np.testing.assert_array_equal(r, x - y)
+    s = dt(-1.6180339887498949)
+    orc.vecop(orc.R_PLUS, 0, r, x, y, s=float(s))
+    np.testing.assert_array_equal(r, s * x + y)
+    orc.vecop(orc.R_MINUS, 0, r, x, y, s=float(s))
+    np.testing.assert_array_equal(r, s * x - y)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", TYPES)
+@pytest.mark.parametrize("op", ["C_PLUS", "C_MINUS"])
+def test_complex_axpy(orc, prec, t, op):
+    dt = DT[prec]
+    # exact on integers
+    xi = _int_field(t, 19, 14, dt)
+    yi = _int_field(t, 19, 15, dt)
+    z = complex(-7, 2)
+    r = np.zeros_like(xi)
+    orc.vecop(getattr(orc, op), 0, r, xi, yi, s=z)
+    sgn = 1 if op == "C_PLUS" else -1
+    want = z * cx(xi.astype(np.float64)) + sgn * cx(yi.astype(np.float64))
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
+    # random data: within the rounding of (zr x - zi x) +- y
+    u = np.finfo(dt).eps / 2
+    x = field(t, 19, 16, dt)
+    y = field(t, 19, 17, dt)
+    z = complex(dt(0.25 + 1 / 3), dt(-2 / 7))
+    orc.vecop(getattr(orc, op), 0, r, x, y, s=z)
+    c = cx(x.astype(np.longdouble))
+    w = L(z) * c + sgn * cx(y.astype(np.longdouble))
+    bound = 4 * u * ((abs(z.real) + abs(z.imag)) * (np.abs(c.real) + np.abs(c.imag)) +
+                     np.abs(cx(y.astype(np.longdouble))))
+    got = cx(r.astype(np.longdouble))
+    assert np.all(np.abs(got.real - w.real) <= bound)
+    assert np.all(np.abs(got.imag - w.imag) <= bound)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_adjoint_involution_and_sum_inverse(orc, prec):
+    dt = DT[prec]
+    x = field("P", 11, 18, dt)
+    y = field("P", 11, 19, dt)
+    a = np.zeros_like(x)
+    b = np.zeros_like(x)
+    orc.vecop(orc.COPY, 0, a, x, conj=True, dim=12)
+    orc.vecop(orc.COPY, 0, b, a, conj=True, dim=12)
+    np.testing.assert_array_equal(b, x)
+    orc.vecop(orc.COPY, 0, a, x, dim=12)
+    orc.vecop(orc.COPY, 0, b, a, dim=12)
+    np.testing.assert_array_equal(b, x)
+    # integers: (x + y) - y = x exactly
+    xi = _int_field("D", 13, 20, dt)
+    yi = _int_field("D", 13, 21, dt)
+    s = np.zeros_like(xi)
+    d = np.zeros_like(xi)
+    orc.vecop(orc.PLUS, 0, s, xi, yi)
+    orc.vecop(orc.MINUS, 0, d, s, yi)
+    np.testing.assert_array_equal(d, xi)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_eqm_zero_signs_follow_the_macros(orc, prec):
+    """eqm results whose exact value is zero: the macro forms give the signs
+    -(+0) = -0 for copy / i / real scale (the negation of eq) but +0 for the
+    complex scale, whose eqm real part is -zr*xr + zi*xi, not -(zr*xr - zi*xi)."""
+    dt = DT[prec]
+    x = np.zeros((1, 3, 2), dtype=dt)
+    x[0, 0] = (1.0, 1.0)          # zr*xr == zi*xi for z = (1, 1)
+    r = np.full_like(x, 5.0)
+    orc.vecop(orc.CSCALE, 3, r, x, s=complex(1, 1))
+    assert r[0, 0, 0] == 0 and not np.signbit(r[0, 0, 0])
+    orc.vecop(orc.CSCALE, 0, r, x, s=complex(1, 1))
+    assert r[0, 0, 0] == 0 and not np.signbit(r[0, 0, 0])
+    r[:] = 5.0
+    orc.vecop(orc.COPY, 3, r, np.zeros_like(x))
+    assert np.all(np.signbit(r))
+    r[:] = 5.0
+    orc.vecop(orc.RSCALE, 3, r, np.zeros_like(x), s=2.0)
+    assert np.all(np.signbit(r))
+    # the conjugated complex scale: eqm imaginary part -zi*xr + zr*xi
+    y = np.zeros((1, 3, 2), dtype=dt)
+    y[0, 0] = (1.0, -1.0)        # z = x = (1, -1): zi*xr == zr*xi == -1
+    r[:] = 5.0
+    orc.vecop(orc.CSCALE, 3, r, y, s=complex(1, -1), conj=True)
+    assert r[0, 0, 1] == 0 and not np.signbit(r[0, 0, 1])
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_index_forms(orc, prec):
+    """x-form, pointer operands and the per-operand indices against the
+    v-form on gathered records."""
+    dt = DT[prec]
+    n = 40
+    x = field("D", n, 22, dt)
+    y = field("D", n, 23, dt)
+    rng = np.random.default_rng(24)
+    perm = rng.permutation(n)
+    sub = np.sort(rng.choice(n, size=n // 2, replace=False))
+    s = 0.375
+    # gang index: r[idx] = s*x[idx] + y[idx]
+    r = field("D", n, 25, dt)
+    want = r.copy()
+    want[sub] = dt(s) * x[sub] + y[sub]
+    orc.vecop(orc.R_PLUS, 0, r, x, y, s=s, index=sub)
+    np.testing.assert_array_equal(r, want)
+    # pointer sources: r[i] = s * x[perm[i]] - y[perm2[i]]
+    perm2 = rng.permutation(n)
+    r = np.zeros_like(x)
+    orc.vecop(orc.R_MINUS, 0, r, x, y, s=s, x_idx=perm, y_idx=perm2)
+    np.testing.assert_array_equal(r, dt(s) * x[perm] - y[perm2])
+    # per-operand: r[ir[i]] += x[ix[i]] for i < m
+    ir = rng.permutation(n)[:25]
+    ix = rng.permutation(n)[:25]
+    r = field("D", n, 26, dt)
+    want = r.copy()
+    want[ir] = want[ir] + x[ix]
+    orc.vecop(orc.COPY, 1, r, x, ir=ir, ix=ix)
+    np.testing.assert_array_equal(r, want)
+    # per-operand on the second source only: r[i] = x[i] + y[iy[i]]
+    iy = rng.permutation(n)
+    r = np.zeros_like(x)
+    orc.vecop(orc.PLUS, 0, r, x, y, iy=iy)
+    np.testing.assert_array_equal(r, x + y[iy])
