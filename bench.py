@@ -242,6 +242,8 @@ def variants(lib, stream, n, combos, nbr, sets, u64, d64, args, plain_sites_per_
     out["batch_c3"] = batch_config3(lib, stream, args)
     out["matfn_c4"] = matfn_config4(lib, stream, args)
     out["c0_16x4"] = config0(lib, stream, args)
+    out["vecops"] = vecops_layer(lib, stream, n, sets, args)
+    out["solver"] = wilson_cg(lib, stream, n, args)
     qla_amd.check()
     return out
 
@@ -329,6 +331,123 @@ def wilson_family(lib, stream, n, combos, nbr, sets, args):
     out["D_vpeq_spproj_M_times_pD_d3"] = rate(ms, 144 + 8 + 192 * 3, steps * nc)
     qla_amd.check()
     return out
+
+
+def vecops_layer(lib, stream, n, sets, args):
+    """The field linear algebra a solver runs between hops, on the 32^4 D
+    fields: x += a p, r = a p + r', r = z p - r', the copy, the even-site
+    x-form update, the FP64 update; the Hermitian conjugate of the links
+    and the P transpose on 2^20 propagators (the gather path).  Algorithmic
+    bytes per site: every record read once, the destination written (and
+    read for peq), 4 B of index for the x-form."""
+    import ctypes
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    out = {}
+    reps = 20
+    a = ctypes.c_float(0.37)
+    z = (ctypes.c_float * 2)(0.6, -1.3)
+    a64 = ctypes.c_double(0.37)
+    x = torch.zeros((n, 3, 4, 2), dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    p, q = sets[0][1], sets[1][1]
+    even = torch.from_numpy(lattice.parity_sites(DIMS, 0)).cuda()
+    x64 = x.double()
+    p64 = p.double()
+    madj = torch.empty_like(sets[0][0])
+    p_n = 1 << 20
+    prop = torch.zeros((p_n, 3, 4, 3, 4, 2), dtype=torch.float32, device="cuda").normal_()
+    prop_t = torch.empty_like(prop)
+    ref = ctypes.addressof
+    cases = {
+        "D_vpeq_r_times_D": (lambda it: lib.QLA_F3_D_vpeq_r_times_D(x.data_ptr(), ref(a), p.data_ptr(), n), 288, n),
+        "D_veq_r_times_D_plus_D": (lambda it: lib.QLA_F3_D_veq_r_times_D_plus_D(
+            y.data_ptr(), ref(a), p.data_ptr(), q.data_ptr(), n), 288, n),
+        "D_veq_c_times_D_minus_D": (lambda it: lib.QLA_F3_D_veq_c_times_D_minus_D(
+            y.data_ptr(), ref(z), p.data_ptr(), q.data_ptr(), n), 288, n),
+        "D_veq_D": (lambda it: lib.QLA_F3_D_veq_D(y.data_ptr(), p.data_ptr(), n), 192, n),
+        "D_xpeq_r_times_D_even": (lambda it: lib.QLA_F3_D_xpeq_r_times_D(
+            x.data_ptr(), ref(a), p.data_ptr(), even.data_ptr(), even.numel()), 292, even.numel()),
+        "D3_D_vpeq_r_times_D": (lambda it: lib.QLA_D3_D_vpeq_r_times_D(x64.data_ptr(), ref(a64), p64.data_ptr(), n),
+                                576, n),
+        "M_veq_Ma": (lambda it: lib.QLA_F3_M_veq_Ma(madj.data_ptr(), sets[0][0].data_ptr(), n), 144, n),
+        "P_veq_transpose_P": (lambda it: lib.QLA_F3_P_veq_transpose_P(prop_t.data_ptr(), prop.data_ptr(), p_n), 2304,
+                              p_n),
+    }
+    for name, (fn, bps, sites) in cases.items():
+        ms = _timed(stream, fn, reps, 2) / reps
+        out[name] = {"B_per_site": bps, "ms": round(ms, 4), "GBs": round(bps * sites / (ms * 1e-3) / 1e9, 1),
+                     "sites_per_s": round(sites / (ms * 1e-3), 1)}
+    qla_amd.check()
+    return out
+
+
+def wilson_cg(lib, stream, n, args):
+    """Conjugate-gradient iterations on D^dagger D for the Wilson operator
+    D = 1 - kappa sum_mu [U_mu (1 - gamma_mu) T_+mu + U_mu^dagger (1 + gamma_mu) T_-mu]
+    on 32^4 (F3 fields, DF3 reductions to host results), every field
+    operation a library call: per iteration 2 x (8 hop calls + 1 axpy), one
+    re-dot, one norm2 and three axpy forms.  Reports ms per iteration and
+    the algorithmic bytes moved per iteration over that time."""
+    import ctypes
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    fw, bw = lattice.neighbours(DIMS)
+    u = [torch.from_numpy(lattice.random_su3(n, 900 + mu).astype(np.float32)).cuda() for mu in range(4)]
+    bl = [qla_amd.pointer_array(u[mu], bw[mu]) for mu in range(4)]
+    fields = [torch.zeros((n, 3, 4, 2), dtype=torch.float32, device="cuda") for _ in range(7)]
+    x, r, p, pn, q, ap, t = fields
+    r.normal_()
+    p.copy_(r)
+    ptr = {}
+
+    def ptrs(psi):
+        k = psi.data_ptr()
+        if k not in ptr:
+            ptr[k] = ([qla_amd.pointer_array(psi, fw[mu]) for mu in range(4)],
+                      [qla_amd.pointer_array(psi, bw[mu]) for mu in range(4)])
+        return ptr[k]
+    mk = ctypes.c_float(-0.11)
+
+    def apply(dst, psi, s):
+        fwp, bwp = ptrs(psi)
+        for mu in range(4):
+            f = lib.QLA_F3_D_veq_spproj_M_times_pD if mu == 0 else lib.QLA_F3_D_vpeq_spproj_M_times_pD
+            f(t.data_ptr(), u[mu].data_ptr(), fwp[mu].data_ptr(), mu, s, n)
+            lib.QLA_F3_D_vpeq_spproj_pMa_times_pD(t.data_ptr(), bl[mu].data_ptr(), bwp[mu].data_ptr(), mu, -s, n)
+        lib.QLA_F3_D_veq_r_times_D_plus_D(dst.data_ptr(), ctypes.addressof(mk), t.data_ptr(), psi.data_ptr(), n)
+
+    v = ctypes.c_double()
+    lib.QLA_DF3_r_veq_norm2_D(ctypes.addressof(v), r.data_ptr(), n)
+    state = {"rr": v.value, "p": p, "pn": pn}
+
+    def iteration(it):
+        p, pn = state["p"], state["pn"]
+        apply(q, p, -1)
+        apply(ap, q, 1)
+        lib.QLA_DF3_r_veq_re_D_dot_D(ctypes.addressof(v), p.data_ptr(), ap.data_ptr(), n)
+        alpha = ctypes.c_float(state["rr"] / v.value)
+        lib.QLA_F3_D_vpeq_r_times_D(x.data_ptr(), ctypes.addressof(alpha), p.data_ptr(), n)
+        lib.QLA_F3_D_vmeq_r_times_D(r.data_ptr(), ctypes.addressof(alpha), ap.data_ptr(), n)
+        lib.QLA_DF3_r_veq_norm2_D(ctypes.addressof(v), r.data_ptr(), n)
+        beta = ctypes.c_float(v.value / state["rr"])
+        state["rr"] = v.value
+        lib.QLA_F3_D_veq_r_times_D_plus_D(pn.data_ptr(), ctypes.addressof(beta), p.data_ptr(), r.data_ptr(), n)
+        state["p"], state["pn"] = pn, p
+
+    iters = 20
+    ms = _timed(stream, iteration, iters, 3)
+    qla_amd.check()
+    # bytes per site of one application of D: the eq hop (72 + 8 + 96 + 96),
+    # three peq hops (72 + 8 + 3 * 96), four backward hops (8 + 72 + 8 + 3 * 96)
+    # and the final r = -kappa t + psi (3 * 96)
+    apply_b = (72 + 8 + 96 + 96) + 3 * (72 + 8 + 288) + 4 * (8 + 72 + 8 + 288) + 288
+    iter_b = 2 * apply_b + 192 + 288 + 288 + 96 + 288
+    return {"lattice": "32^4", "ms_per_iter": round(ms / iters, 4), "B_per_site_iter": iter_b,
+            "GBs": round(iter_b * n * iters / (ms * 1e-3) / 1e9, 1), "calls_per_iter": 23,
+            "rr_after": state["rr"]}
 
 
 def xforms_config1(lib, stream, n, sets, args):
