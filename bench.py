@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true",
                    help="headline only (no companion measurements), e.g. for profiler runs")
+    p.add_argument("--group", choices=("vecops", "solver"), default=None,
+                   help="run one companion group alone and print its figures (profiler runs of its kernels)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
     return p.parse_args()
@@ -92,6 +94,11 @@ def main():
     qla_amd.set_stream(stream)
     qla_amd.set_sync(False)
     f = lib.QLA_F3_H_veq_spproj_M_times_pD
+
+    if args.group:
+        res = vecops_layer(lib, stream, n, sets, args) if args.group == "vecops" else wilson_cg(lib, stream, n, args)
+        print(json.dumps({"group": args.group, "workload": "32^4 periodic", "result": res}))
+        return
 
     def step(it):
         for j, (mu, s) in enumerate(combos):
