@@ -2,7 +2,7 @@
  * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:
  * copies and increments, Hermitian conjugate, transpose, complex conjugate,
  * times i, times a real or complex constant, sums, differences and
- * a*b +- c with a constant a, for the types V, H, D, M, P.
+ * a*b +- c with a constant a, and the fills, for the types V, H, D, M, P.
  * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.
  * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device
  * semantics are documented there).
@@ -13160,5 +13160,409 @@ void QLA_D3_xP_eq_c_times_xP_minus_xP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRIC
                                       QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT c,
                                       int *indexc,
                                       int n);
+
+/* F3: zero fill, r = 0.
+ * Reference: build_qla.pl:2561-2580, QLA_c_eq_r(e, 0.). */
+void QLA_F3_V_veq_zero(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_F3_V_xeq_zero(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_F3_H_veq_zero(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_F3_H_xeq_zero(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_F3_D_veq_zero(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_F3_D_xeq_zero(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_F3_M_veq_zero(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_F3_M_xeq_zero(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_F3_P_veq_zero(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_F3_P_xeq_zero(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+
+/* F3: constant fill, r eqop a[0] (one record for every site).
+ * Reference: build_qla.pl:2584-2603, QLA_c_<eqop>_c. */
+void QLA_F3_V_veq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                    QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_V_xeq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                    QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_V_vpeq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_V_xpeq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_V_vmeq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_V_xmeq_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_V_veqm_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_V_xeqm_v(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_H_veq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                    QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_H_xeq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                    QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_H_vpeq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_H_xpeq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_H_vmeq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_H_xmeq_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_H_veqm_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_H_xeqm_h(QLA_F3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_D_veq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_D_xeq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                    QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_D_vpeq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_D_xpeq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_D_vmeq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_D_xmeq_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_D_veqm_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_D_xeqm_d(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_M_veq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_M_xeq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_M_vpeq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xpeq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_M_vmeq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xmeq_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_M_veqm_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xeqm_m(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_P_veq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                    QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_P_xeq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                    QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_P_vpeq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_P_xpeq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_P_vmeq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_P_xmeq_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_P_veqm_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_P_xeqm_p(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+
+/* F3: diagonal fill, r eqop a[0] times the unit matrix.
+ * Reference: build_qla.pl:2605-2617, expressions_tensor.pl:77-128 (QLA_c_<eqop>_r(e, 0.), then the diagonal). */
+void QLA_F3_M_veq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_F3_M_xeq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_F3_M_vpeq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xpeq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_M_vmeq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xmeq_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_F3_M_veqm_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_F3_M_xeqm_c(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_F_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+
+/* D3: zero fill, r = 0.
+ * Reference: build_qla.pl:2561-2580, QLA_c_eq_r(e, 0.). */
+void QLA_D3_V_veq_zero(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_D3_V_xeq_zero(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_D3_H_veq_zero(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_D3_H_xeq_zero(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_D3_D_veq_zero(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_D3_D_xeq_zero(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_D3_M_veq_zero(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_D3_M_xeq_zero(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+void QLA_D3_P_veq_zero(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                       int n);
+void QLA_D3_P_xeq_zero(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                       int *index,
+                       int n);
+
+/* D3: constant fill, r eqop a[0] (one record for every site).
+ * Reference: build_qla.pl:2584-2603, QLA_c_<eqop>_c. */
+void QLA_D3_V_veq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                    QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_V_xeq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                    QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_V_vpeq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_V_xpeq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_V_vmeq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_V_xmeq_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_V_veqm_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_V_xeqm_v(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorVector *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_H_veq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                    QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_H_xeq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                    QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_H_vpeq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_H_xpeq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_H_vmeq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_H_xmeq_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_H_veqm_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_H_xeqm_h(QLA_D3_HalfFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_HalfFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_D_veq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_D_xeq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                    QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_D_vpeq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_D_xpeq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_D_vmeq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_D_xmeq_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_D_veqm_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_D_xeqm_d(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_M_veq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_M_xeq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_M_vpeq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xpeq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_M_vmeq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xmeq_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_M_veqm_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xeqm_m(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_P_veq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                    QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_P_xeq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                    QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_P_vpeq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_P_xpeq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_P_vmeq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_P_xmeq_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_P_veqm_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_P_xeqm_p(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                     QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+
+/* D3: diagonal fill, r eqop a[0] times the unit matrix.
+ * Reference: build_qla.pl:2605-2617, expressions_tensor.pl:77-128 (QLA_c_<eqop>_r(e, 0.), then the diagonal). */
+void QLA_D3_M_veq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                    int n);
+void QLA_D3_M_xeq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                    QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                    int *index,
+                    int n);
+void QLA_D3_M_vpeq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xpeq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_M_vmeq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xmeq_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
+void QLA_D3_M_veqm_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int n);
+void QLA_D3_M_xeqm_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
+                     QLA_D_Complex *QLA_MI355X_RESTRICT a,
+                     int *index,
+                     int n);
 
 #endif

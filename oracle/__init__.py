@@ -55,6 +55,7 @@ def load():
         getattr(lib, f"orc_{p}_matfunc").argtypes = [i32, i32, vp, vp, vp, i32]
         real = C.c_float if p == "F" else C.c_double
         getattr(lib, f"orc_{p}_vecop").argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, real, real, vp, i32]
+        getattr(lib, f"orc_{p}_fill").argtypes = [i32, i32, i32, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -330,4 +331,22 @@ def vecop(op, eqop, r, x, y=None, s=0.0, conj=False, dim=0, index=None, x_idx=No
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_vecop")(op, eqop, int(conj), int(dim), ncomp, _ptr(_need(r)), _ptr(rix), _ptr(px),
                                    _ptr(py), s.real, s.imag, _ptr(ixx), n)
+    return r
+
+
+FILL_ZERO, FILL_REC, FILL_DIAG = 0, 1, 2
+
+
+def fill(kind, eqop, r, a=None, index=None, n=None):
+    """Fill on host arrays (orc_*_fill): FILL_ZERO r = 0; FILL_REC r eqop a
+    (a: one record, broadcast); FILL_DIAG r eqop a times the unit matrix (a:
+    one complex as a (2,) array; r colour matrices).  index: the x-form."""
+    lib = load()
+    ncomp = r[0].size // 2
+    ix = _index(index)
+    if n is None:
+        n = len(ix) if ix is not None else r.shape[0]
+    p = "F" if r.dtype == np.float32 else "D"
+    av = None if a is None else np.ascontiguousarray(a, dtype=r.dtype)
+    getattr(lib, f"orc_{p}_fill")(kind, eqop, ncomp, _ptr(_need(r)), _ptr(av), _ptr(ix), n)
     return r

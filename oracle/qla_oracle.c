@@ -122,6 +122,11 @@ void orc_D_vecop(int op, int eqop, int cj, int dim, int ncomp, double *r, const 
                  const double *const *px, const double *const *py, double sr, double si, const int *index, int n)
 { D_orc_vecop(op, eqop, cj, dim, ncomp, r, rindex, px, py, sr, si, index, n); }
 
+void orc_F_fill(int kind, int eqop, int ncomp, float *r, const float *a, const int *index, int n)
+{ F_orc_fill(kind, eqop, ncomp, r, a, index, n); }
+void orc_D_fill(int kind, int eqop, int ncomp, double *r, const double *a, const int *index, int n)
+{ D_orc_fill(kind, eqop, ncomp, r, a, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

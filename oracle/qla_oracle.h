@@ -111,6 +111,11 @@ void orc_F_vecop(int op, int eqop, int cj, int dim, int ncomp, float *r, const i
 void orc_D_vecop(int op, int eqop, int cj, int dim, int ncomp, double *r, const int *rindex,
                  const double *const *px, const double *const *py, double sr, double si, const int *index, int n);
 
+/* fills (qla_oracle_vecops.inc): kind 0 zero, 1 the record a broadcast,
+ * 2 (ncomp 9) a[0] on the diagonal; n steps writing r[index[i]] or r[i] */
+void orc_F_fill(int kind, int eqop, int ncomp, float *r, const float *a, const int *index, int n);
+void orc_D_fill(int kind, int eqop, int ncomp, double *r, const double *a, const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -536,6 +536,60 @@ def vecops() -> list[VecVariant]:
     return out
 
 
+@dataclass(frozen=True)
+class FillVariant:
+    """A fill: r = 0, r eqop a (one record a for every site), or for colour
+    matrices r eqop c on the diagonal (build_qla.pl:2561-2620); v- and x-forms
+    only (indirection.pl:163-170, @ind_fill_list)."""
+    prec: str       # "F3" / "D3"
+    kind: str       # "zero", "rec", "diag"
+    t: str
+    eqop: int
+    xf: bool
+
+    @property
+    def name(self) -> str:
+        form = "x" if self.xf else "v"
+        eq = EQOPS[self.eqop]
+        if self.kind == "zero":
+            return f"QLA_{self.prec}_{self.t}_{form}eq_zero"
+        src = self.t.lower() if self.kind == "rec" else "c"
+        return f"QLA_{self.prec}_{self.t}_{form}{eq}_{src}"
+
+    def params(self) -> list[tuple[str, str]]:
+        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
+        if self.kind == "rec":
+            out.append(("p", f"{rec} *QLA_MI355X_RESTRICT a"))
+        elif self.kind == "diag":
+            out.append(("p", f"QLA_{'D' if self.prec == 'D3' else 'F'}_Complex *QLA_MI355X_RESTRICT a"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        kind = {"zero": 0, "rec": 1, "diag": 2}[self.kind]
+        a = "nullptr" if self.kind == "zero" else "(const void *)a"
+        idx = "index" if self.xf else "nullptr"
+        return (f"fill_entry({prec}, {kind}, {self.eqop}, {NCOMP[self.t]}, r, {a}, {idx}, n, \"{self.name}\");")
+
+
+def fills() -> list[FillVariant]:
+    out = []
+    for prec in PRECS:
+        for t in FIELD_TYPES:
+            out += [FillVariant(prec, "zero", t, 0, xf) for xf in (False, True)]
+        for t in FIELD_TYPES:
+            out += [FillVariant(prec, "rec", t, e, xf) for e in range(4) for xf in (False, True)]
+        out += [FillVariant(prec, "diag", "M", e, xf) for e in range(4) for xf in (False, True)]
+    return out
+
+
 def reductions() -> list[RedVariant]:
     out = []
     for prec in ("F3", "D3", "DF3"):
@@ -562,8 +616,8 @@ def conversions() -> list[ConvVariant]:
 
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
-    field linear algebra)."""
-    return variants() + reductions() + conversions() + matfuncs() + vecops()
+    field linear algebra, fills)."""
+    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills()
 
 
 def variants() -> list[Variant]:
@@ -829,6 +883,10 @@ _VEC_DOC = {
     "r_times_minus": ("r = a[0] b - c, real a", "build_qla.pl:1837-1887, QLA_c_eq_r_times_c_minus_c"),
     "c_times_plus": ("r = a[0] b + c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_plus_c"),
     "c_times_minus": ("r = a[0] b - c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_minus_c"),
+    "zero": ("zero fill, r = 0", "build_qla.pl:2561-2580, QLA_c_eq_r(e, 0.)"),
+    "rec": ("constant fill, r eqop a[0] (one record for every site)", "build_qla.pl:2584-2603, QLA_c_<eqop>_c"),
+    "diag": ("diagonal fill, r eqop a[0] times the unit matrix",
+             "build_qla.pl:2605-2617, expressions_tensor.pl:77-128 (QLA_c_<eqop>_r(e, 0.), then the diagonal)"),
 }
 
 
@@ -838,7 +896,7 @@ def vecops_header_text() -> str:
         " * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:",
         " * copies and increments, Hermitian conjugate, transpose, complex conjugate,",
         " * times i, times a real or complex constant, sums, differences and",
-        " * a*b +- c with a constant a, for the types V, H, D, M, P.",
+        " * a*b +- c with a constant a, and the fills, for the types V, H, D, M, P.",
         " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
         " * semantics are documented there).",
@@ -854,10 +912,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops():
-        key = (v.prec, v.fam)
+    for v in vecops() + fills():
+        key = (v.prec, v.fam if isinstance(v, VecVariant) else v.kind)
         if key != last:
-            what, ref = _VEC_DOC[v.fam]
+            what, ref = _VEC_DOC[key[1]]
             lines += ["", f"/* {v.prec}: {what}.", f" * Reference: {ref}. */"]
             last = key
         lines.append(_decl(v) + ";")
@@ -869,9 +927,9 @@ def vecops_inc_text() -> str:
     lines = [
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
-        "// Included by qla_capi.hip inside extern \"C\"; vec_entry is there.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry and fill_entry are there.",
     ]
-    for v in vecops():
+    for v in vecops() + fills():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

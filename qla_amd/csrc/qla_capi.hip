@@ -174,6 +174,26 @@ void vec_entry(int prec, int op, int eqop, int ncomp, int xform, int px, int py,
     finish(qla_launch_vec(prec, op, eqop, ncomp, p, ts.stream), name);
 }
 
+// One fill (qla_vecops_abi.inc): kind 0 zero, 1 a record broadcast, 2 a
+// complex constant on the diagonal of colour matrices; the record or
+// constant a may be in host memory (copied into the launch at the call) or
+// device memory (read by the kernel).
+void fill_entry(int prec, int kind, int eqop, int ncomp, void *r, const void *a, const int *index, int n,
+                const char *name)
+{
+    if (n <= 0) return;
+    qla_fill_args p = { r, nullptr, nullptr, index, n };
+    if (kind != 0) {
+        if (!a) {
+            record(hipErrorInvalidValue, name);
+            return;
+        }
+        if (is_device_pointer(a)) p.src = a;
+        else p.host = a;
+    }
+    finish(qla_launch_fill(prec, kind, eqop, ncomp, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -379,6 +399,13 @@ hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_a
 {
     if (prec == 0) return qla_vec_launch_p0(op, eqop, ncomp, p, st);
     if (prec == 1) return qla_vec_launch_p1(op, eqop, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_fill(int prec, int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_fill_launch_p0(kind, eqop, ncomp, p, st);
+    if (prec == 1) return qla_fill_launch_p1(kind, eqop, ncomp, p, st);
     return hipErrorInvalidValue;
 }
 

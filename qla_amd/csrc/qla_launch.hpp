@@ -105,6 +105,22 @@ hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_a
 hipError_t qla_vec_launch_p0(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 
+// Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
+// ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a
+// broadcast to every step; 2 (ncomp 9) r eqop c on the diagonal and eqop 0
+// elsewhere.  The record or constant is read from src (device memory) or,
+// when src is null, copied from host into the launch at the call.
+struct qla_fill_args {
+    void *r;
+    const void *src;        // device record / constant, or null
+    const void *host;       // host record / constant when src is null
+    const int *index;
+    int n;
+};
+hipError_t qla_launch_fill(int prec, int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st);
+hipError_t qla_fill_launch_p0(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st);
+hipError_t qla_fill_launch_p1(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st);
+
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
                                hipStream_t st);

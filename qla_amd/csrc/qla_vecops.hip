@@ -31,6 +31,7 @@
 // bit for bit, signed zeros included.  The eqm form of the complex constant
 // (QLA_c_eqm_c_times_c, include/qla_complex.h:334-337) is not the negation
 // of its eq form and is kept as written.
+#include <string.h>
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -363,6 +364,150 @@ hipError_t launch_t(int op, int eqop, int ncomp, const qla_vec_args &p, hipStrea
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Fills (build_qla.pl:2561-2620).  FILL_ZERO: QLA_c_eq_r(e, 0.) on every
+// element.  FILL_REC: QLA_c_<eqop>_c(e, a.e) with one record a for every
+// step.  FILL_DIAG (colour matrices): QLA_c_<eqop>_r(e, 0.) on every element
+// -- eq (0, 0), peq re += 0, meq re -= 0, eqm (-0, 0) -- then
+// QLA_c_<eqop>_c(e, c) on the diagonal (expressions_tensor.pl:77-128).  A
+// record or constant from host memory arrives by value in the kernel
+// arguments (at most 144 complexes: 2304 bytes in double precision).
+// ---------------------------------------------------------------------------
+enum { FILL_ZERO = 0, FILL_REC = 1, FILL_DIAG = 2 };
+
+template <class T, int N> struct rec_val { cpx<T> e[N]; };
+
+// v-form zero fill: 16 bytes of zeros per work item over the run of m complexes
+template <class T>
+__global__ void __launch_bounds__(VBLOCK) k_vec_zero(cpx<T> *r, long long m)
+{
+    typedef typename vtype<T, 1>::v V1;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    if constexpr (sizeof(T) == 4) {
+        typedef typename vtype<T, 2>::v V2;
+        const long long items = (m + 1) / 2;
+        for (long long t = (long long)blockIdx.x * VBLOCK + threadIdx.x; t < items; t += stride) {
+            if (2 * t + 1 < m) __builtin_nontemporal_store(V2{}, (V2 *)r + t);
+            else __builtin_nontemporal_store(V1{}, (V1 *)r + 2 * t);
+        }
+    } else {
+        for (long long t = (long long)blockIdx.x * VBLOCK + threadIdx.x; t < m; t += stride)
+            __builtin_nontemporal_store(V1{}, (V1 *)r + t);
+    }
+}
+
+template <class T, int NC, int KIND, int EQOP, int HN>
+__global__ void __launch_bounds__(VBLOCK) k_vec_fill(qla_fill_args p, rec_val<T, HN> hv, int wide)
+{
+    typedef typename vtype<T, 1>::v V1;
+    const unsigned long long tot = (unsigned long long)p.n * NC;
+    const unsigned long long stride = (unsigned long long)gridDim.x * VBLOCK;
+    const cpx<T> *src = (const cpx<T> *)p.src;
+    for (unsigned long long g = (unsigned long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        int i, c;
+        if (wide) {
+            i = (int)(g / NC);
+            c = (int)(g - (unsigned long long)i * NC);
+        } else {
+            const unsigned int g32 = (unsigned int)g;
+            i = (int)(g32 / NC);
+            c = (int)(g32 - (unsigned int)i * NC);
+        }
+        const int s = p.index ? p.index[i] : i;
+        V1 *R = (V1 *)((cpx<T> *)p.r + (size_t)s * NC + c);
+        V1 r = {};
+        if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load(R);
+        T rr = r[0], ri = r[1];
+        if constexpr (KIND == FILL_ZERO) {
+            rr = T(0);
+            ri = T(0);
+        } else if constexpr (KIND == FILL_REC) {
+            const cpx<T> a = src ? src[c] : hv.e[c];
+            vop<VOP_COPY, EQOP, T>(rr, ri, a.re, a.im, T(0), T(0), T(0), T(0));
+        } else {
+            if constexpr (EQOP == 0) {
+                rr = T(0);
+                ri = T(0);
+            } else if constexpr (EQOP == 1) {
+                rr = rr + T(0);
+            } else if constexpr (EQOP == 2) {
+                rr = rr - T(0);
+            } else {
+                rr = -T(0);
+                ri = T(0);
+            }
+            if (c / 3 == c % 3) {
+                const cpx<T> a = src ? src[0] : hv.e[0];
+                vop<VOP_COPY, EQOP, T>(rr, ri, a.re, a.im, T(0), T(0), T(0), T(0));
+            }
+        }
+        r[0] = rr;
+        r[1] = ri;
+        __builtin_nontemporal_store(r, R);
+    }
+}
+
+template <class T, int NC, int KIND, int EQOP>
+hipError_t launch_fill_nc(const qla_fill_args &p, hipStream_t st)
+{
+    constexpr int HN = KIND == FILL_REC ? NC : 1;
+    rec_val<T, HN> hv = {};
+    if (KIND != FILL_ZERO && !p.src) memcpy(&hv, p.host, sizeof(hv));
+    const long long tot = (long long)p.n * NC;
+    const int wide = tot > 0xffffffffLL ? 1 : 0;
+    hipLaunchKernelGGL((k_vec_fill<T, NC, KIND, EQOP, HN>), dim3(blocks_for(tot)), dim3(VBLOCK), 0, st, p, hv,
+                       wide);
+    return hipGetLastError();
+}
+
+template <class T, int KIND, int EQOP>
+hipError_t launch_fill_k(int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    if constexpr (KIND == FILL_DIAG) {
+        return ncomp == 9 ? launch_fill_nc<T, 9, KIND, EQOP>(p, st) : hipErrorInvalidValue;
+    } else {
+        switch (ncomp) {
+        case 3: return launch_fill_nc<T, 3, KIND, EQOP>(p, st);
+        case 6: return launch_fill_nc<T, 6, KIND, EQOP>(p, st);
+        case 9: return launch_fill_nc<T, 9, KIND, EQOP>(p, st);
+        case 12: return launch_fill_nc<T, 12, KIND, EQOP>(p, st);
+        case 144: return launch_fill_nc<T, 144, KIND, EQOP>(p, st);
+        }
+        return hipErrorInvalidValue;
+    }
+}
+
+template <class T, int KIND>
+hipError_t launch_fill_e(int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    switch (eqop) {
+    case 0: return launch_fill_k<T, KIND, 0>(ncomp, p, st);
+    case 1: return launch_fill_k<T, KIND, 1>(ncomp, p, st);
+    case 2: return launch_fill_k<T, KIND, 2>(ncomp, p, st);
+    case 3: return launch_fill_k<T, KIND, 3>(ncomp, p, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <class T>
+hipError_t launch_fill_t(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    if (kind == FILL_ZERO) {
+        if (eqop != 0) return hipErrorInvalidValue;
+        if (!p.index) {
+            const long long m = (long long)p.n * ncomp;
+            const long long items = sizeof(T) == 4 ? (m + 1) / 2 : m;
+            hipLaunchKernelGGL(k_vec_zero<T>, dim3(blocks_for(items)), dim3(VBLOCK), 0, st, (cpx<T> *)p.r, m);
+            return hipGetLastError();
+        }
+        return launch_fill_k<T, FILL_ZERO, 0>(ncomp, p, st);
+    }
+    if (kind == FILL_REC) return launch_fill_e<T, FILL_REC>(eqop, ncomp, p, st);
+    if (kind == FILL_DIAG) return launch_fill_e<T, FILL_DIAG>(eqop, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
 } // namespace
 } // namespace qla
 
@@ -375,9 +520,17 @@ hipError_t qla_vec_launch_p0(int op, int eqop, int ncomp, const qla_vec_args &p,
 {
     return qla::launch_t<float>(op, eqop, ncomp, p, st);
 }
+hipError_t qla_fill_launch_p0(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    return qla::launch_fill_t<float>(kind, eqop, ncomp, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_t<double>(op, eqop, ncomp, p, st);
+}
+hipError_t qla_fill_launch_p1(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
+{
+    return qla::launch_fill_t<double>(kind, eqop, ncomp, p, st);
 }
 #endif

@@ -38,7 +38,8 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 5857
+    assert len(every) == len(set(every)) == 5973
+    assert len(abi_gen.fills()) == 116
     mf = [v.name for v in abi_gen.matfuncs()]
     assert len(mf) == 321
     vo = [v.name for v in abi_gen.vecops()]
@@ -50,6 +51,9 @@ def test_generated_site_abi_is_current():
               "QLA_F3_xD_eq_D_minus_xD", "QLA_D3_V_eq_r_times_xV_plus_V", "QLA_F3_xM_peq_Ma",
               "QLA_F3_D_xeq_pD_plus_pD", "QLA_F3_H_veq_H_plus_H"):
         assert s in vo, s
+    fl = [v.name for v in abi_gen.fills()]
+    for s in ("QLA_F3_D_veq_zero", "QLA_D3_V_xeq_zero", "QLA_F3_M_vpeq_c", "QLA_D3_P_veq_p", "QLA_F3_H_xmeq_h"):
+        assert s in fl, s
     for s in ("QLA_F3_M_veq_expA_M", "QLA_D3_C_xeq_det_pM", "QLA_FN_M_veq_log_M", "QLA_F3_V_veq_eigenvalsH_M",
               "QLA_D3_V_veq_pM_inverse_pV", "QLA_F3_xM_eq_xM_inverse_M", "QLA_FN_M_veq_exp_M"):
         assert s in mf, s

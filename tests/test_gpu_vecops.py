@@ -17,7 +17,7 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import NCOMP, VECFAMS, vecops
+from qla_amd.abi_gen import NCOMP, VECFAMS, fills, vecops
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -211,3 +211,41 @@ def test_wide_transpose_of_propagators():
     del a, r
     torch.cuda.empty_cache()
     assert ok
+
+
+FILL_KIND = {"zero": 0, "rec": 1, "diag": 2}
+
+
+FILL_CASES = [(v, w) for v in fills() for w in (("host",) if v.kind == "zero" else ("host", "device"))]
+
+
+@pytest.mark.parametrize("v,where", FILL_CASES, ids=lambda x: x if isinstance(x, str) else x.name)
+def test_fill_symbol(orc, v, where):
+    """Every fill with the record / constant in host or device memory, against
+    the oracle (x-forms write only the indexed records)."""
+    import torch
+    cs = case(v.prec, v.t)
+    r = dev(cs.r0)
+    if v.kind == "rec":
+        a = np.ascontiguousarray(cs.x[5])
+    elif v.kind == "diag":
+        a = np.array([0.8125, -1.75], dtype=cs.dt)
+    else:
+        a = None
+    args = [r]
+    keep = None
+    if a is not None:
+        if where == "device":
+            keep = torch.from_numpy(a).to("cuda")
+            args.append(keep.data_ptr())
+        else:
+            args.append(a.ctypes.data)
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else cs.n
+    say(f"{v.name} {where} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    exp = cs.r0.copy()
+    orc.fill(FILL_KIND[v.kind], v.eqop, exp, a, index=cs.index if v.xf else None)
+    assert_bitwise(host(r), exp, f"{v.name} {where}")

@@ -315,3 +315,66 @@ def test_index_forms(orc, prec):
     r = np.zeros_like(x)
     orc.vecop(orc.PLUS, 0, r, x, y, iy=iy)
     np.testing.assert_array_equal(r, x + y[iy])
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", TYPES)
+@pytest.mark.parametrize("eqop", range(4))
+def test_record_fill_vs_numpy(orc, prec, t, eqop):
+    """r eqop a with one record a for every site (x-form on a subset)."""
+    dt = DT[prec]
+    n = 21
+    r0 = field(t, n, 40, dt)
+    a = field(t, 1, 41, dt)[0]
+    r = r0.copy()
+    orc.fill(orc.FILL_REC, eqop, r, a)
+    np.testing.assert_array_equal(r, eqop_np(eqop, r0, np.broadcast_to(a, r0.shape)))
+    sub = np.array([3, 7, 11, 20])
+    r = r0.copy()
+    orc.fill(orc.FILL_REC, eqop, r, a, index=sub)
+    want = r0.copy()
+    want[sub] = eqop_np(eqop, r0[sub], np.broadcast_to(a, r0[sub].shape))
+    np.testing.assert_array_equal(r, want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_zero_fill(orc, prec):
+    dt = DT[prec]
+    for t in TYPES:
+        r = field(t, 9, 42, dt)
+        r[0] = -0.0
+        orc.fill(orc.FILL_ZERO, 0, r)
+        assert np.all(r == 0) and not np.any(np.signbit(r))
+        r = field(t, 9, 43, dt)
+        keep = r.copy()
+        orc.fill(orc.FILL_ZERO, 0, r, index=np.array([2, 5]))
+        keep[[2, 5]] = 0
+        np.testing.assert_array_equal(r, keep)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+def test_diagonal_fill(orc, prec, eqop):
+    """M eqop c times the unit matrix: values against numpy's c * eye(3)."""
+    dt = DT[prec]
+    r0 = field("M", 15, 44, dt)
+    c = np.array([dt(1.25), dt(-0.375)])
+    r = r0.copy()
+    orc.fill(orc.FILL_DIAG, eqop, r, c)
+    y = re_im(complex(c[0], c[1]) * np.eye(3)[None].repeat(15, 0), dt)
+    np.testing.assert_array_equal(r, eqop_np(eqop, r0, y))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_diagonal_fill_zero_signs(orc, prec):
+    """The off-diagonal elements take QLA_c_<eqop>_r(e, 0.): peq turns a -0
+    real part into +0 and leaves the imaginary part alone, meq keeps -0,
+    eqm writes (-0, +0) (include/qla_complex.h:140-145)."""
+    dt = DT[prec]
+    c = np.array([dt(2.0), dt(3.0)])
+    for eqop, want_re_neg, want_im_neg in ((1, False, True), (2, True, True), (3, True, False), (0, False, False)):
+        r = np.full((2, 3, 3, 2), -0.0, dtype=dt)
+        orc.fill(orc.FILL_DIAG, eqop, r, c)
+        off = ~np.eye(3, dtype=bool)
+        assert np.all(r[:, off, 0] == 0) and np.all(np.signbit(r[:, off, 0]) == want_re_neg), eqop
+        assert np.all(r[:, off, 1] == 0) and np.all(np.signbit(r[:, off, 1]) == want_im_neg), eqop
