@@ -2,7 +2,8 @@
  * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:
  * copies and increments, Hermitian conjugate, transpose, complex conjugate,
  * times i, times a real or complex constant, sums, differences and
- * a*b +- c with a constant a, and the fills, for the types V, H, D, M, P.
+ * a*b +- c with a constant a, the fills and the gamma-matrix multiplications,
+ * for the types V, H, D, M, P.
  * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.
  * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device
  * semantics are documented there).
@@ -13564,5 +13565,221 @@ void QLA_D3_M_xeqm_c(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
                      QLA_D_Complex *QLA_MI355X_RESTRICT a,
                      int *index,
                      int n);
+
+/* F3: left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a.
+ * Reference: build_qla.pl:1356-1384, expressions_gamma.pl:123-198, 387-470. */
+void QLA_F3_D_veq_gamma_times_D(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_D_veq_gamma_times_pD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_F3_D_xeq_gamma_times_D(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_F3_D_xeq_gamma_times_pD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracFermion *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_F3_xD_eq_gamma_times_D(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_D_eq_gamma_times_xD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_F3_xD_eq_gamma_times_xD(QLA_F3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_F3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
+void QLA_F3_P_veq_gamma_times_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_P_veq_gamma_times_pP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_F3_P_xeq_gamma_times_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_F3_P_xeq_gamma_times_pP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_F3_xP_eq_gamma_times_P(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_P_eq_gamma_times_xP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_F3_xP_eq_gamma_times_xP(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
+
+/* F3: right multiplication by Gamma_mu, r = a Gamma_mu.
+ * Reference: build_qla.pl:1386-1415, expressions_gamma.pl. */
+void QLA_F3_P_veq_P_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_P_veq_pP_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_F3_P_xeq_P_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_F3_P_xeq_pP_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_F3_xP_eq_P_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_F3_P_eq_xP_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_F3_xP_eq_xP_times_gamma(QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_F3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
+
+/* D3: left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a.
+ * Reference: build_qla.pl:1356-1384, expressions_gamma.pl:123-198, 387-470. */
+void QLA_D3_D_veq_gamma_times_D(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_D_veq_gamma_times_pD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_D3_D_xeq_gamma_times_D(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_D3_D_xeq_gamma_times_pD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracFermion *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_D3_xD_eq_gamma_times_D(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_D_eq_gamma_times_xD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_D3_xD_eq_gamma_times_xD(QLA_D3_DiracFermion *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_D3_DiracFermion *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
+void QLA_D3_P_veq_gamma_times_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_P_veq_gamma_times_pP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_D3_P_xeq_gamma_times_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_D3_P_xeq_gamma_times_pP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_D3_xP_eq_gamma_times_P(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_P_eq_gamma_times_xP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_D3_xP_eq_gamma_times_xP(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
+
+/* D3: right multiplication by Gamma_mu, r = a Gamma_mu.
+ * Reference: build_qla.pl:1386-1415, expressions_gamma.pl. */
+void QLA_D3_P_veq_P_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_P_veq_pP_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int n);
+void QLA_D3_P_xeq_P_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int *index,
+                                int n);
+void QLA_D3_P_xeq_pP_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT *a,
+                                 int mu,
+                                 int *index,
+                                 int n);
+void QLA_D3_xP_eq_P_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                int *indexr,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int mu,
+                                int n);
+void QLA_D3_P_eq_xP_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                int *indexa,
+                                int mu,
+                                int n);
+void QLA_D3_xP_eq_xP_times_gamma(QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT r,
+                                 int *indexr,
+                                 QLA_D3_DiracPropagator *QLA_MI355X_RESTRICT a,
+                                 int *indexa,
+                                 int mu,
+                                 int n);
 
 #endif

@@ -56,6 +56,7 @@ def load():
         real = C.c_float if p == "F" else C.c_double
         getattr(lib, f"orc_{p}_vecop").argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, real, real, vp, i32]
         getattr(lib, f"orc_{p}_fill").argtypes = [i32, i32, i32, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_gamma").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -349,4 +350,19 @@ def fill(kind, eqop, r, a=None, index=None, n=None):
     p = "F" if r.dtype == np.float32 else "D"
     av = None if a is None else np.ascontiguousarray(a, dtype=r.dtype)
     getattr(lib, f"orc_{p}_fill")(kind, eqop, ncomp, _ptr(_need(r)), _ptr(av), _ptr(ix), n)
+    return r
+
+
+def gamma(right, mu, r, a, index=None, a_idx=None, ir=None, ia=None, n=None):
+    """r = Gamma_mu a (right False; D or P records) or a Gamma_mu (right True;
+    P) on host arrays (orc_*_gamma), in the addressing of site()."""
+    lib = load()
+    ncomp = r[0].size // 2
+    pa = _records(a, ia if ia is not None else a_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_gamma")(int(right), ncomp, mu, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(ix), n)
     return r

@@ -127,6 +127,13 @@ void orc_F_fill(int kind, int eqop, int ncomp, float *r, const float *a, const i
 void orc_D_fill(int kind, int eqop, int ncomp, double *r, const double *a, const int *index, int n)
 { D_orc_fill(kind, eqop, ncomp, r, a, index, n); }
 
+void orc_F_gamma(int right, int ncomp, int mu, float *r, const int *rindex, const float *const *pa,
+                 const int *index, int n)
+{ F_orc_gamma(right, ncomp, mu, r, rindex, pa, index, n); }
+void orc_D_gamma(int right, int ncomp, int mu, double *r, const int *rindex, const double *const *pa,
+                 const int *index, int n)
+{ D_orc_gamma(right, ncomp, mu, r, rindex, pa, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

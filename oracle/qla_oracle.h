@@ -116,6 +116,13 @@ void orc_D_vecop(int op, int eqop, int cj, int dim, int ncomp, double *r, const 
 void orc_F_fill(int kind, int eqop, int ncomp, float *r, const float *a, const int *index, int n);
 void orc_D_fill(int kind, int eqop, int ncomp, double *r, const double *a, const int *index, int n);
 
+/* gamma-matrix multiplication (qla_oracle_vecops.inc): r = Gamma_mu a
+ * (right 0; ncomp 12 or 144) or a Gamma_mu (right 1; 144) */
+void orc_F_gamma(int right, int ncomp, int mu, float *r, const int *rindex, const float *const *pa,
+                 const int *index, int n);
+void orc_D_gamma(int right, int ncomp, int mu, double *r, const int *rindex, const double *const *pa,
+                 const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

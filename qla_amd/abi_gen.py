@@ -579,6 +579,61 @@ class FillVariant:
         return (f"fill_entry({prec}, {kind}, {self.eqop}, {NCOMP[self.t]}, r, {a}, {idx}, n, \"{self.name}\");")
 
 
+@dataclass(frozen=True)
+class GammaVariant:
+    """r = Gamma_mu a (D, P) or r = a Gamma_mu (P), eq only, every unary
+    indexing form (build_qla.pl:1356-1415)."""
+    prec: str
+    t: str
+    right: bool
+    xf: bool
+    pa: bool
+    ir: bool = False
+    ia: bool = False
+
+    @property
+    def name(self) -> str:
+        form = "" if (self.ir or self.ia) else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + self.t
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + self.t
+        if self.right:
+            return f"QLA_{self.prec}_{r}_{form}eq_{a}_times_gamma"
+        return f"QLA_{self.prec}_{r}_{form}eq_gamma_times_{a}"
+
+    def params(self) -> list[tuple[str, str]]:
+        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{rec} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out.append(("i", "int mu"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        own = ", ".join(x if f else "nullptr" for x, f in (("index" if self.xf else "nullptr", self.xf),
+                                                          ("indexr", self.ir), ("indexa", self.ia)))
+        return (f"gamma_entry({prec}, {int(self.right)}, {NCOMP[self.t]}, {int(self.pa)}, r, (const void *)a, mu, "
+                f"{own}, n, \"{self.name}\");")
+
+
+def gammas() -> list[GammaVariant]:
+    out = []
+    for prec in PRECS:
+        for t, right in (("D", False), ("P", False), ("P", True)):
+            out += [GammaVariant(prec, t, right, xf, pa) for xf in (False, True) for pa in (False, True)]
+            out += [GammaVariant(prec, t, right, False, False, ir, ia) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -616,8 +671,8 @@ def conversions() -> list[ConvVariant]:
 
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
-    field linear algebra, fills)."""
-    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills()
+    field linear algebra, fills, gamma multiplications)."""
+    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas()
 
 
 def variants() -> list[Variant]:
@@ -885,6 +940,9 @@ _VEC_DOC = {
     "c_times_minus": ("r = a[0] b - c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_minus_c"),
     "zero": ("zero fill, r = 0", "build_qla.pl:2561-2580, QLA_c_eq_r(e, 0.)"),
     "rec": ("constant fill, r eqop a[0] (one record for every site)", "build_qla.pl:2584-2603, QLA_c_<eqop>_c"),
+    "gamma_left": ("left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a",
+                   "build_qla.pl:1356-1384, expressions_gamma.pl:123-198, 387-470"),
+    "gamma_right": ("right multiplication by Gamma_mu, r = a Gamma_mu", "build_qla.pl:1386-1415, expressions_gamma.pl"),
     "diag": ("diagonal fill, r eqop a[0] times the unit matrix",
              "build_qla.pl:2605-2617, expressions_tensor.pl:77-128 (QLA_c_<eqop>_r(e, 0.), then the diagonal)"),
 }
@@ -896,7 +954,8 @@ def vecops_header_text() -> str:
         " * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:",
         " * copies and increments, Hermitian conjugate, transpose, complex conjugate,",
         " * times i, times a real or complex constant, sums, differences and",
-        " * a*b +- c with a constant a, and the fills, for the types V, H, D, M, P.",
+        " * a*b +- c with a constant a, the fills and the gamma-matrix multiplications,",
+        " * for the types V, H, D, M, P.",
         " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
         " * semantics are documented there).",
@@ -912,8 +971,11 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills():
-        key = (v.prec, v.fam if isinstance(v, VecVariant) else v.kind)
+    for v in vecops() + fills() + gammas():
+        if isinstance(v, GammaVariant):
+            key = (v.prec, "gamma_right" if v.right else "gamma_left")
+        else:
+            key = (v.prec, v.fam if isinstance(v, VecVariant) else v.kind)
         if key != last:
             what, ref = _VEC_DOC[key[1]]
             lines += ["", f"/* {v.prec}: {what}.", f" * Reference: {ref}. */"]
@@ -927,9 +989,9 @@ def vecops_inc_text() -> str:
     lines = [
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
-        "// Included by qla_capi.hip inside extern \"C\"; vec_entry and fill_entry are there.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry and gamma_entry are there.",
     ]
-    for v in vecops() + fills():
+    for v in vecops() + fills() + gammas():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

@@ -194,6 +194,29 @@ void fill_entry(int prec, int kind, int eqop, int ncomp, void *r, const void *a,
     finish(qla_launch_fill(prec, kind, eqop, ncomp, p, ts.stream), name);
 }
 
+// One gamma-matrix multiplication (qla_vecops_abi.inc): right 0 r = Gamma_mu a,
+// 1 r = a Gamma_mu.  mu outside 0..15 does nothing but print the reference's
+// message (expressions_gamma.pl:464-470, the switch default).
+void gamma_entry(int prec, int right, int ncomp, int pa, void *r, const void *a, int mu, const int *index,
+                 const int *ir, const int *ia, int n, const char *name)
+{
+    if (n <= 0) return;
+    if (mu < 0 || mu > 15) {
+        fprintf(stderr, "Gamma matrix %d not supported\n", mu);
+        return;
+    }
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.n = n;
+    p.px = pa;
+    finish(qla_launch_gamma(prec, right, mu, ncomp, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -399,6 +422,13 @@ hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_a
 {
     if (prec == 0) return qla_vec_launch_p0(op, eqop, ncomp, p, st);
     if (prec == 1) return qla_vec_launch_p1(op, eqop, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_gamma(int prec, int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_gamma_launch_p0(right, g, ncomp, p, st);
+    if (prec == 1) return qla_gamma_launch_p1(right, g, ncomp, p, st);
     return hipErrorInvalidValue;
 }
 

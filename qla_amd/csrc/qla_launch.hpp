@@ -121,6 +121,14 @@ hipError_t qla_launch_fill(int prec, int kind, int eqop, int ncomp, const qla_fi
 hipError_t qla_fill_launch_p0(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st);
 hipError_t qla_fill_launch_p1(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st);
 
+// Gamma-matrix multiplication (qla_vecops.hip): r = Gamma_g a (right == 0;
+// records of 12 or 144 complexes) or r = a Gamma_g (right == 1; 144), with
+// Gamma_g = gamma_0^b0 gamma_1^b1 gamma_2^b2 gamma_3^b3 for the bits b of
+// g = 0..15.  Addressing as qla_vec_args (x, px, ix; no y, no constant).
+hipError_t qla_launch_gamma(int prec, int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_gamma_launch_p0(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_gamma_launch_p1(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st);
+
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
                                hipStream_t st);

@@ -508,6 +508,111 @@ hipError_t launch_fill_t(int kind, int eqop, int ncomp, const qla_fill_args &p, 
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Multiplication by gamma matrices (build_qla.pl:1356-1415, eq only).  Every
+// row and column of a product of gamma matrices holds one entry i^k, so
+// Gamma_g a is a signed permutation of spin components: destination spin s
+// takes the source spin src[s] times i^ph[s], element by element with
+// QLA_c_eq_c (k = 0), QLA_c_eq_ic (1), QLA_c_eqm_c (2) and QLA_c_eqm_ic (3)
+// (expressions_gamma.pl:387-441, carith1 of headers.pl:135-170).
+// ---------------------------------------------------------------------------
+
+// gamma_mu (mu = 0..3) as rows: row r has the entry i^ph at column col.  The
+// chiral basis of the reference (expressions_gamma.pl:27-92, doc/qla.tex):
+//   g0 = [[0,0,0,i],[0,0,i,0],[0,-i,0,0],[-i,0,0,0]]
+//   g1 = [[0,0,0,-1],[0,0,1,0],[0,1,0,0],[-1,0,0,0]]
+//   g2 = [[0,0,i,0],[0,0,0,-i],[-i,0,0,0],[0,i,0,0]]
+//   g3 = [[0,0,1,0],[0,0,0,1],[1,0,0,0],[0,1,0,0]]
+struct gamma_row { int col, ph; };
+constexpr gamma_row GAMMA_ROWS[4][4] = {
+    { { 3, 1 }, { 2, 1 }, { 1, 3 }, { 0, 3 } },
+    { { 3, 2 }, { 2, 0 }, { 1, 0 }, { 0, 2 } },
+    { { 2, 1 }, { 3, 3 }, { 0, 3 }, { 1, 1 } },
+    { { 2, 0 }, { 3, 0 }, { 0, 0 }, { 1, 0 } },
+};
+
+struct gamma_tab { int src[4], ph[4]; };
+
+// Gamma_g = product over the set bits i = 0..3 of gamma_i, left to right
+// (expressions_gamma.pl:123-134); left: the source spin and phase of each
+// destination row; right: of each destination column.
+gamma_tab gamma_table(int g, int right)
+{
+    gamma_row m[4] = { { 0, 0 }, { 1, 0 }, { 2, 0 }, { 3, 0 } };
+    for (int i = 0; i < 4; i++) {
+        if (!(g & (1 << i))) continue;
+        for (int r = 0; r < 4; r++) {
+            const gamma_row b = GAMMA_ROWS[i][m[r].col];
+            m[r] = { b.col, (m[r].ph + b.ph) & 3 };
+        }
+    }
+    gamma_tab t;
+    for (int r = 0; r < 4; r++) {
+        if (!right) {
+            t.src[r] = m[r].col;
+            t.ph[r] = m[r].ph;
+        } else {
+            t.src[m[r].col] = r;
+            t.ph[m[r].col] = m[r].ph;
+        }
+    }
+    return t;
+}
+
+// one work item per destination complex; SPOS: position of the multiplied
+// spin index in the record ((colour, spin) for D; (ic, is, jc, js) for P,
+// 1 from the left, 3 from the right)
+template <class T, int NC, int SPOS>
+__global__ void __launch_bounds__(VBLOCK) k_vec_gamma(qla_vec_args p, gamma_tab tab)
+{
+    typedef typename vtype<T, 1>::v V1;
+    const unsigned long long tot = (unsigned long long)p.n * NC;
+    const unsigned long long stride = (unsigned long long)gridDim.x * VBLOCK;
+    // element stride of the spin index at SPOS
+    constexpr int SSTRIDE = NC == 12 ? 1 : (SPOS == 1 ? 12 : 1);
+    for (unsigned long long g = (unsigned long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        int i, c;
+        if (p.wide) {
+            i = (int)(g / NC);
+            c = (int)(g - (unsigned long long)i * NC);
+        } else {
+            const unsigned int g32 = (unsigned int)g;
+            i = (int)(g32 / NC);
+            c = (int)(g32 - (unsigned int)i * NC);
+        }
+        const int s = p.index ? p.index[i] : i;
+        const int sd = p.ir ? p.ir[i] : s;
+        const cpx<T> *X = record_of<T, NC>(p.x, p.px, p.ix ? p.ix[i] : s);
+        const int spin = (c / SSTRIDE) & 3;
+        const int cx = c + (tab.src[spin] - spin) * SSTRIDE;
+        const V1 x = __builtin_nontemporal_load((const V1 *)(X + cx));
+        V1 r;
+        switch (tab.ph[spin]) {
+        case 0: r[0] = x[0]; r[1] = x[1]; break;         // QLA_c_eq_c
+        case 1: r[0] = -x[1]; r[1] = x[0]; break;        // QLA_c_eq_ic
+        case 2: r[0] = -x[0]; r[1] = -x[1]; break;       // QLA_c_eqm_c
+        default: r[0] = x[1]; r[1] = -x[0]; break;       // QLA_c_eqm_ic
+        }
+        __builtin_nontemporal_store(r, (V1 *)((cpx<T> *)p.r + (size_t)sd * NC + c));
+    }
+}
+
+template <class T>
+hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const gamma_tab tab = gamma_table(g, right);
+    qla_vec_args q = p;
+    const long long tot = (long long)p.n * ncomp;
+    q.wide = tot > 0xffffffffLL ? 1 : 0;
+    const dim3 grid(blocks_for(tot));
+    if (!right && ncomp == 12) hipLaunchKernelGGL((k_vec_gamma<T, 12, 1>), grid, dim3(VBLOCK), 0, st, q, tab);
+    else if (!right && ncomp == 144) hipLaunchKernelGGL((k_vec_gamma<T, 144, 1>), grid, dim3(VBLOCK), 0, st, q, tab);
+    else if (right && ncomp == 144) hipLaunchKernelGGL((k_vec_gamma<T, 144, 3>), grid, dim3(VBLOCK), 0, st, q, tab);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 } // namespace
 } // namespace qla
 
@@ -524,6 +629,10 @@ hipError_t qla_fill_launch_p0(int kind, int eqop, int ncomp, const qla_fill_args
 {
     return qla::launch_fill_t<float>(kind, eqop, ncomp, p, st);
 }
+hipError_t qla_gamma_launch_p0(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_gamma_t<float>(right, g, ncomp, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -532,5 +641,9 @@ hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p,
 hipError_t qla_fill_launch_p1(int kind, int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
 {
     return qla::launch_fill_t<double>(kind, eqop, ncomp, p, st);
+}
+hipError_t qla_gamma_launch_p1(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_gamma_t<double>(right, g, ncomp, p, st);
 }
 #endif

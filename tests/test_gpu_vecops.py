@@ -17,7 +17,7 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import NCOMP, VECFAMS, fills, vecops
+from qla_amd.abi_gen import NCOMP, VECFAMS, fills, gammas, vecops
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -249,3 +249,46 @@ def test_fill_symbol(orc, v, where):
     exp = cs.r0.copy()
     orc.fill(FILL_KIND[v.kind], v.eqop, exp, a, index=cs.index if v.xf else None)
     assert_bitwise(host(r), exp, f"{v.name} {where}")
+
+
+GAMMA_CASES = [(v, mu) for v in gammas() for mu in (range(16) if not (v.xf or v.pa or v.ir or v.ia) else (5, 15))]
+
+
+@pytest.mark.parametrize("v,mu", GAMMA_CASES, ids=lambda x: x if isinstance(x, int) else x.name)
+def test_gamma_symbol(orc, v, mu):
+    """Every gamma multiplication symbol (all 16 mu on the v-forms) bit for bit
+    against the oracle, untouched records included."""
+    cs = case(v.prec, v.t)
+    r = dev(cs.r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(cs.px if v.pa else cs.dx)
+    if v.ia:
+        args.append(cs.down["x"])
+    args.append(mu)
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if (v.ir or v.ia) else cs.n)
+    say(f"{v.name} mu={mu} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    xs = cs.own["x"] if v.ia else site
+    gx = np.ascontiguousarray(cs.x[cs.x_idx[xs] if v.pa else xs])
+    sub = np.zeros((m,) + cs.r0.shape[1:], dtype=cs.dt)
+    orc.gamma(v.right, mu, sub, gx)
+    exp = cs.r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, f"{v.name} mu={mu}")
+
+
+def test_gamma_out_of_range_is_a_noop():
+    cs = case("F3", "D")
+    r = dev(cs.r0)
+    say("QLA_F3_D_veq_gamma_times_D mu=16 / -1")
+    qla_amd.call("QLA_F3_D_veq_gamma_times_D", r, cs.dx, 16, cs.n)
+    qla_amd.call("QLA_F3_D_veq_gamma_times_D", r, cs.dx, -1, cs.n)
+    qla_amd.check()
+    assert_bitwise(host(r), cs.r0, "mu out of range")

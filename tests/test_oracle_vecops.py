@@ -378,3 +378,75 @@ def test_diagonal_fill_zero_signs(orc, prec):
         off = ~np.eye(3, dtype=bool)
         assert np.all(r[:, off, 0] == 0) and np.all(np.signbit(r[:, off, 0]) == want_re_neg), eqop
         assert np.all(r[:, off, 1] == 0) and np.all(np.signbit(r[:, off, 1]) == want_im_neg), eqop
+
+
+def gamma_doc(mu):
+    """Gamma_mu from the documented basis: the product of G[b] over the set
+    bits b of mu, left to right."""
+    from gamma_basis import G
+    m = np.eye(4, dtype=complex)
+    for b in range(4):
+        if mu >> b & 1:
+            m = m @ G[b]
+    return m
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("mu", range(16))
+def test_gamma_left_right_vs_documented_basis(orc, prec, mu):
+    """r = Gamma_mu a for D and P and r = a Gamma_mu for P against numpy with
+    the gamma matrices of doc/qla.tex (exact: entries are +-1, +-i)."""
+    dt = DT[prec]
+    g = gamma_doc(mu)
+    d = field("D", 7, 50 + mu, dt)
+    r = np.zeros_like(d)
+    orc.gamma(False, mu, r, d)
+    np.testing.assert_array_equal(cx(r), np.einsum("st,nct->ncs", g, cx(d)))
+    pr = field("P", 5, 70 + mu, dt)
+    r = np.zeros_like(pr)
+    orc.gamma(False, mu, r, pr)
+    np.testing.assert_array_equal(cx(r), np.einsum("st,nctdu->ncsdu", g, cx(pr)))
+    r = np.zeros_like(pr)
+    orc.gamma(True, mu, r, pr)
+    np.testing.assert_array_equal(cx(r), np.einsum("ncsdt,tu->ncsdu", cx(pr), g))
+
+
+def test_gamma_identities(orc):
+    """Gamma_15 = gamma_5 = diag(1, 1, -1, -1); each gamma_mu squares to one;
+    a zero record keeps the macro signs (-1 rows give -0 through QLA_c_eqm_c)."""
+    d = field("D", 4, 90, np.float64)
+    t1 = np.zeros_like(d)
+    t2 = np.zeros_like(d)
+    for b in range(4):
+        orc.gamma(False, 1 << b, t1, d)
+        orc.gamma(False, 1 << b, t2, t1)
+        np.testing.assert_array_equal(t2, d)
+    orc.gamma(False, 15, t1, d)
+    np.testing.assert_array_equal(t1[:, :, :2], d[:, :, :2])
+    np.testing.assert_array_equal(t1[:, :, 2:], -d[:, :, 2:])
+    z = np.zeros_like(d)
+    orc.gamma(False, 15, t1, z)
+    assert not np.any(np.signbit(t1[:, :, :2])) and np.all(np.signbit(t1[:, :, 2:]))
+    t1[:] = 7.0
+    orc.gamma(False, 16, t1, d)
+    assert np.all(t1 == 7.0)
+
+
+def test_gamma_index_forms(orc):
+    d = field("D", 30, 91, np.float32)
+    rng = np.random.default_rng(92)
+    perm = rng.permutation(30)
+    r = np.zeros_like(d)
+    orc.gamma(False, 6, r, d, a_idx=perm)
+    want = np.zeros_like(d)
+    orc.gamma(False, 6, want, np.ascontiguousarray(d[perm]))
+    np.testing.assert_array_equal(r, want)
+    ir = rng.permutation(30)[:9]
+    ia = rng.permutation(30)[:9]
+    r = field("D", 30, 93, np.float32)
+    want = r.copy()
+    sub = np.zeros((9,) + d.shape[1:], dtype=d.dtype)
+    orc.gamma(False, 11, sub, np.ascontiguousarray(d[ia]))
+    want[ir] = sub
+    orc.gamma(False, 11, r, d, ir=ir, ia=ia)
+    np.testing.assert_array_equal(r, want)
