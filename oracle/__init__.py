@@ -57,6 +57,7 @@ def load():
         getattr(lib, f"orc_{p}_vecop").argtypes = [i32, i32, i32, i32, i32, vp, vp, vp, vp, real, real, vp, i32]
         getattr(lib, f"orc_{p}_fill").argtypes = [i32, i32, i32, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_gamma").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_local").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -365,4 +366,26 @@ def gamma(right, mu, r, a, index=None, a_idx=None, ir=None, ia=None, n=None):
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_gamma")(int(right), ncomp, mu, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(ix), n)
+    return r
+
+
+L_NORM2, L_DOT, L_REDOT, L_RETRACE, L_IMTRACE, L_TRACE, L_ANTIHERM = range(7)
+
+
+def local(kind, eqop, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=None, n=None):
+    """Site-local reduction or part on host arrays (orc_*_local): r is (n,)
+    reals (L_NORM2, L_REDOT, L_RETRACE, L_IMTRACE), (n, 2) complexes (L_DOT,
+    L_TRACE) or (n, 3, 3, 2) matrices (L_ANTIHERM); addressing as site()."""
+    lib = load()
+    ncomp = a[0].size // 2
+    rreals = r[0].size if r.ndim > 1 else 1
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = None if b is None else _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_local")(kind, eqop, ncomp, rreals, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
+                                   _ptr(ix), n)
     return r

@@ -134,6 +134,13 @@ void orc_D_gamma(int right, int ncomp, int mu, double *r, const int *rindex, con
                  const int *index, int n)
 { D_orc_gamma(right, ncomp, mu, r, rindex, pa, index, n); }
 
+void orc_F_local(int kind, int eqop, int ncomp, int rreals, float *r, const int *rindex, const float *const *pa,
+                 const float *const *pb, const int *index, int n)
+{ F_orc_local(kind, eqop, ncomp, rreals, r, rindex, pa, pb, index, n); }
+void orc_D_local(int kind, int eqop, int ncomp, int rreals, double *r, const int *rindex, const double *const *pa,
+                 const double *const *pb, const int *index, int n)
+{ D_orc_local(kind, eqop, ncomp, rreals, r, rindex, pa, pb, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

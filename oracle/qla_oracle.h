@@ -123,6 +123,16 @@ void orc_F_gamma(int right, int ncomp, int mu, float *r, const int *rindex, cons
 void orc_D_gamma(int right, int ncomp, int mu, double *r, const int *rindex, const double *const *pa,
                  const int *index, int n);
 
+/* site-local reductions and parts (qla_oracle_vecops.inc): kind 0 R eqop
+ * norm2(a), 1 C eqop a.b, 2 R eqop Re a.b (a, b records of ncomp
+ * complexes), 3 R = Re tr a, 4 R = Im tr a, 5 C = tr a, 6 M = the traceless
+ * anti-Hermitian part of a (a colour matrices); rreals = reals per
+ * destination record (1, 2 or 18) */
+void orc_F_local(int kind, int eqop, int ncomp, int rreals, float *r, const int *rindex, const float *const *pa,
+                 const float *const *pb, const int *index, int n);
+void orc_D_local(int kind, int eqop, int ncomp, int rreals, double *r, const int *rindex, const double *const *pa,
+                 const double *const *pb, const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -634,6 +634,102 @@ def gammas() -> list[GammaVariant]:
     return out
 
 
+# site-local reductions and parts: kind -> (code, destination, binary, qualifier, eqops, source types)
+LOCALS = {
+    "norm2": (0, "R", False, "norm2_", range(4), FIELD_TYPES),
+    "dot": (1, "C", True, "", range(4), FIELD_TYPES),
+    "redot": (2, "R", True, "re_", range(4), FIELD_TYPES),
+    "re_trace": (3, "R", False, "re_trace_", (0,), ("M",)),
+    "im_trace": (4, "R", False, "im_trace_", (0,), ("M",)),
+    "trace": (5, "C", False, "trace_", (0,), ("M",)),
+    "antiherm": (6, "M", False, "antiherm_", (0,), ("M",)),
+}
+
+
+@dataclass(frozen=True)
+class LocalVariant:
+    """A site-local reduction (R / C record per site) or the traceless
+    anti-Hermitian part of a colour matrix."""
+    prec: str
+    kind: str       # key of LOCALS
+    t: str          # source record type
+    eqop: int
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ia or self.ib
+
+    @property
+    def binary(self) -> bool:
+        return LOCALS[self.kind][2]
+
+    @property
+    def name(self) -> str:
+        _, dest, binary, qual, _, _ = LOCALS[self.kind]
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + dest
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + self.t
+        head = f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_{qual}"
+        if binary:
+            b = ("p" if self.pb else "") + ("x" if self.ib else "") + self.t
+            return f"{head}{a}_dot_{b}"
+        return f"{head}{a}"
+
+    def params(self) -> list[tuple[str, str]]:
+        dest = LOCALS[self.kind][1]
+        real = "D" if self.prec == "D3" else "F"
+        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        dt = {"R": f"QLA_{real}_Real", "C": f"QLA_{real}_Complex", "M": f"QLA_{self.prec}_ColorMatrix"}[dest]
+        out = [("p", f"{dt} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{rec} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        if self.binary:
+            out.append(("p", f"{rec} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+            if self.ib:
+                out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        b = "(const void *)b" if self.binary else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("index" if self.xf else "nullptr", self.xf),
+                                                          ("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"local_entry({prec}, {LOCALS[self.kind][0]}, {self.eqop}, {NCOMP[self.t]}, {int(self.pa)}, "
+                f"{int(self.pb)}, (void *)r, (const void *)a, {b}, {own}, n, \"{self.name}\");")
+
+
+def locals_() -> list[LocalVariant]:
+    out = []
+    for prec in PRECS:
+        for kind, (_, _, binary, _, eqops, types) in LOCALS.items():
+            for t in types:
+                for e in eqops:
+                    if binary:
+                        out += [LocalVariant(prec, kind, t, e, xf, pa, pb) for xf in (False, True)
+                                for pa in (False, True) for pb in (False, True)]
+                        out += [LocalVariant(prec, kind, t, e, ir=ir, ia=ia, ib=ib) for ir, ia, ib in OWN_INDEX_FORMS]
+                    else:
+                        out += [LocalVariant(prec, kind, t, e, xf, pa) for xf in (False, True) for pa in (False, True)]
+                        out += [LocalVariant(prec, kind, t, e, ir=ir, ia=ia) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -671,8 +767,8 @@ def conversions() -> list[ConvVariant]:
 
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
-    field linear algebra, fills, gamma multiplications)."""
-    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas()
+    field linear algebra, fills, gamma multiplications, site-local reductions)."""
+    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
 
 
 def variants() -> list[Variant]:
@@ -940,6 +1036,17 @@ _VEC_DOC = {
     "c_times_minus": ("r = a[0] b - c, complex a", "build_qla.pl:1889-1935, QLA_c_eq_c_times_c_minus_c"),
     "zero": ("zero fill, r = 0", "build_qla.pl:2561-2580, QLA_c_eq_r(e, 0.)"),
     "rec": ("constant fill, r eqop a[0] (one record for every site)", "build_qla.pl:2584-2603, QLA_c_<eqop>_c"),
+    "norm2": ("site-local squared norm, R eqop sum of |a|^2 over the record (double sum, operands promoted)",
+              "build_qla.pl:495-523, make_code_binary.pl:38-97 (make_code_binary_dot)"),
+    "dot": ("site-local inner product, C eqop sum of conj(a) b over the record (double sum)",
+            "build_qla.pl:1758-1783, make_code_binary.pl:38-97"),
+    "redot": ("real part of the site-local inner product, R eqop Re sum conj(a) b",
+              "build_qla.pl:1810-1831, make_code_binary.pl:38-97"),
+    "re_trace": ("real part of the trace, R = Re tr a", "build_qla.pl:933-981, expressions_tensor.pl:353-389"),
+    "im_trace": ("imaginary part of the trace, R = Im tr a", "build_qla.pl:933-981, expressions_tensor.pl:353-389"),
+    "trace": ("trace, C = tr a", "build_qla.pl:933-981, expressions_tensor.pl:353-389"),
+    "antiherm": ("traceless anti-Hermitian part, M = (a - a^dagger)/2 - i Im tr(a)/3",
+                 "build_qla.pl:983-1000, expressions_tensor.pl:138-232"),
     "gamma_left": ("left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a",
                    "build_qla.pl:1356-1384, expressions_gamma.pl:123-198, 387-470"),
     "gamma_right": ("right multiplication by Gamma_mu, r = a Gamma_mu", "build_qla.pl:1386-1415, expressions_gamma.pl"),
@@ -954,8 +1061,9 @@ def vecops_header_text() -> str:
         " * qla_mi355x_vecops.h -- field linear algebra of the QLA C ABI on MI355X:",
         " * copies and increments, Hermitian conjugate, transpose, complex conjugate,",
         " * times i, times a real or complex constant, sums, differences and",
-        " * a*b +- c with a constant a, the fills and the gamma-matrix multiplications,",
-        " * for the types V, H, D, M, P.",
+        " * a*b +- c with a constant a, the fills, the gamma-matrix multiplications and",
+        " * the site-local norms, inner products, traces and anti-Hermitian parts, for",
+        " * the types V, H, D, M, P.",
         " * GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         " * Included by qla_mi355x.h (types, QLA_MI355X_RESTRICT and the device",
         " * semantics are documented there).",
@@ -971,9 +1079,11 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas():
+    for v in vecops() + fills() + gammas() + locals_():
         if isinstance(v, GammaVariant):
             key = (v.prec, "gamma_right" if v.right else "gamma_left")
+        elif isinstance(v, LocalVariant):
+            key = (v.prec, v.kind)
         else:
             key = (v.prec, v.fam if isinstance(v, VecVariant) else v.kind)
         if key != last:
@@ -989,9 +1099,10 @@ def vecops_inc_text() -> str:
     lines = [
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
-        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry and gamma_entry are there.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry and local_entry",
+        "// are there.",
     ]
-    for v in vecops() + fills() + gammas():
+    for v in vecops() + fills() + gammas() + locals_():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

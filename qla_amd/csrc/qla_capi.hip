@@ -217,6 +217,27 @@ void gamma_entry(int prec, int right, int ncomp, int pa, void *r, const void *a,
     finish(qla_launch_gamma(prec, right, mu, ncomp, p, ts.stream), name);
 }
 
+// One site-local reduction or colour-matrix part (qla_vecops_abi.inc): kind
+// codes of qla_launch_local; a, b sources (b for the dots only).
+void local_entry(int prec, int kind, int eqop, int ncomp, int pa, int pb, void *r, const void *a, const void *b,
+                 const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    p.px = pa;
+    p.py = pb;
+    finish(qla_launch_local(prec, kind, eqop, ncomp, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -429,6 +450,13 @@ hipError_t qla_launch_gamma(int prec, int right, int g, int ncomp, const qla_vec
 {
     if (prec == 0) return qla_gamma_launch_p0(right, g, ncomp, p, st);
     if (prec == 1) return qla_gamma_launch_p1(right, g, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_local(int prec, int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_local_launch_p0(kind, eqop, ncomp, p, st);
+    if (prec == 1) return qla_local_launch_p1(kind, eqop, ncomp, p, st);
     return hipErrorInvalidValue;
 }
 

@@ -129,6 +129,15 @@ hipError_t qla_launch_gamma(int prec, int right, int g, int ncomp, const qla_vec
 hipError_t qla_gamma_launch_p0(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_gamma_launch_p1(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st);
 
+// Site-local reductions and parts (qla_vecops.hip), one destination record
+// per step: kind 0 R eqop norm2(x), 1 C eqop x.y, 2 R eqop Re x.y (records of
+// ncomp complexes), 3 R = Re tr x, 4 R = Im tr x, 5 C = tr x, 6 M = the
+// traceless anti-Hermitian part of x (colour matrices; kinds 3-6 eq only).
+// Addressing as qla_vec_args; r holds Real, Complex or ColorMatrix records.
+hipError_t qla_launch_local(int prec, int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_local_launch_p0(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_local_launch_p1(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
                                hipStream_t st);

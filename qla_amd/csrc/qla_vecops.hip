@@ -671,6 +671,166 @@ hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hi
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Site-local reductions and colour-matrix parts.  One work item per
+// destination record; each sums its record's components in order, as the
+// generated code does (a sequential sum fixes the rounding):
+//  * norm2 / dot / Re dot (make_code_binary.pl make_code_binary_dot): the sum
+//    runs in double with the operands promoted (F and D both promote to D,
+//    datatypes.pl:355-359), x += a.re*b.re + a.im*b.im per element (the dot
+//    adds x.im += a.re*b.im - a.im*b.re); a real destination then takes C's
+//    r op= x in double, a complex one QLA_c_<eqop>_c with x converted
+//    (expressions_scalar.pl:36-80);
+//  * traces: x = 0, x += the diagonal in order, in the field precision;
+//  * the traceless anti-Hermitian part (expressions_tensor.pl:138-232):
+//    x = Im tr / 3; D_ii = (0, Im S_ii - x); for i < j, t = (S_ij - conj S_ji)
+//    * 0.5, D_ij = t, D_ji = -conj t.
+// ---------------------------------------------------------------------------
+enum { L_NORM2 = 0, L_DOT, L_REDOT, L_RETRACE, L_IMTRACE, L_TRACE, L_ANTIHERM };
+
+template <class T, int NC, int KIND, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_local(qla_vec_args p)
+{
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long i = (long long)blockIdx.x * VBLOCK + threadIdx.x; i < p.n; i += stride) {
+        const int s = p.index ? p.index[i] : (int)i;
+        const int sd = p.ir ? p.ir[i] : s;
+        const cpx<T> *A = record_of<T, NC>(p.x, p.px, p.ix ? p.ix[i] : s);
+        if constexpr (KIND <= L_REDOT) {
+            const cpx<T> *B = KIND == L_NORM2 ? A : record_of<T, NC>(p.y, p.py, p.iy ? p.iy[i] : s);
+            double xr = 0, xi = 0;
+#pragma unroll 4
+            for (int e = 0; e < NC; e++) {
+                const cpx<T> a = A[e], b = B[e];
+                const double ar = a.re, ai = a.im, br = b.re, bi = b.im;
+                xr = xr + (ar * br + ai * bi);
+                if constexpr (KIND == L_DOT) xi = xi + (ar * bi - ai * br);
+            }
+            if constexpr (KIND != L_DOT) {
+                T *R = (T *)p.r + sd;
+                if constexpr (EQOP == 0) *R = (T)xr;
+                else if constexpr (EQOP == 1) *R = (T)((double)*R + xr);
+                else if constexpr (EQOP == 2) *R = (T)((double)*R - xr);
+                else *R = (T)(-xr);
+            } else {
+                cpx<T> *R = (cpx<T> *)p.r + sd;
+                T rr = T(0), ri = T(0);
+                if constexpr (reads_r<EQOP>()) {
+                    rr = R->re;
+                    ri = R->im;
+                }
+                vop<VOP_COPY, EQOP, T>(rr, ri, (T)xr, (T)xi, T(0), T(0), T(0), T(0));
+                R->re = rr;
+                R->im = ri;
+            }
+        } else if constexpr (KIND <= L_TRACE) {
+            T xr = T(0), xi = T(0);
+#pragma unroll
+            for (int ic = 0; ic < 3; ic++) {
+                const cpx<T> a = A[4 * ic];
+                if constexpr (KIND == L_RETRACE) xr = xr + a.re;
+                else if constexpr (KIND == L_IMTRACE) xr = xr + a.im;
+                else {
+                    xr = xr + a.re;
+                    xi = xi + a.im;
+                }
+            }
+            if constexpr (KIND == L_TRACE) {
+                cpx<T> *R = (cpx<T> *)p.r + sd;
+                R->re = xr;
+                R->im = xi;
+            } else {
+                ((T *)p.r)[sd] = xr;
+            }
+        } else {
+            cpx<T> a[9], d[9];
+#pragma unroll
+            for (int e = 0; e < 9; e++) a[e] = A[e];
+            T x = T(0);
+#pragma unroll
+            for (int ic = 0; ic < 3; ic++) x = x + a[4 * ic].im;
+            x = x / T(3);
+#pragma unroll
+            for (int ic = 0; ic < 3; ic++) {
+                d[4 * ic].re = T(0);
+                d[4 * ic].im = a[4 * ic].im - x;
+            }
+#pragma unroll
+            for (int ic = 0; ic < 2; ic++)
+#pragma unroll
+                for (int jc = ic + 1; jc < 3; jc++) {
+                    const cpx<T> u = a[3 * ic + jc], v = a[3 * jc + ic];
+                    const T tr = u.re - v.re, ti = u.im + v.im;
+                    const T hr = T(0.5) * tr, hi = T(0.5) * ti;
+                    d[3 * ic + jc].re = hr;
+                    d[3 * ic + jc].im = hi;
+                    d[3 * jc + ic].re = -hr;
+                    d[3 * jc + ic].im = hi;
+                }
+            cpx<T> *R = (cpx<T> *)p.r + (size_t)sd * 9;
+#pragma unroll
+            for (int e = 0; e < 9; e++) R[e] = d[e];
+        }
+    }
+}
+
+template <class T, int KIND, int EQOP>
+hipError_t launch_local_nc(int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    const dim3 grid(blocks_for(p.n));
+#define QLA_LOCAL_CASE(NC)                                                                      \
+    case NC:                                                                                   \
+        hipLaunchKernelGGL((k_vec_local<T, NC, KIND, EQOP>), grid, dim3(VBLOCK), 0, st, p);  \
+        return hipGetLastError();
+    if constexpr (KIND >= L_RETRACE) {
+        if (ncomp != 9) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_vec_local<T, 9, KIND, EQOP>), grid, dim3(VBLOCK), 0, st, p);
+        return hipGetLastError();
+    } else {
+        switch (ncomp) {
+            QLA_LOCAL_CASE(3)
+            QLA_LOCAL_CASE(6)
+            QLA_LOCAL_CASE(9)
+            QLA_LOCAL_CASE(12)
+            QLA_LOCAL_CASE(144)
+        }
+        return hipErrorInvalidValue;
+    }
+#undef QLA_LOCAL_CASE
+}
+
+template <class T, int KIND>
+hipError_t launch_local_e(int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if constexpr (KIND >= L_RETRACE) {
+        return eqop == 0 ? launch_local_nc<T, KIND, 0>(ncomp, p, st) : hipErrorInvalidValue;
+    } else {
+        switch (eqop) {
+        case 0: return launch_local_nc<T, KIND, 0>(ncomp, p, st);
+        case 1: return launch_local_nc<T, KIND, 1>(ncomp, p, st);
+        case 2: return launch_local_nc<T, KIND, 2>(ncomp, p, st);
+        case 3: return launch_local_nc<T, KIND, 3>(ncomp, p, st);
+        }
+        return hipErrorInvalidValue;
+    }
+}
+
+template <class T>
+hipError_t launch_local_t(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    switch (kind) {
+    case L_NORM2: return launch_local_e<T, L_NORM2>(eqop, ncomp, p, st);
+    case L_DOT: return launch_local_e<T, L_DOT>(eqop, ncomp, p, st);
+    case L_REDOT: return launch_local_e<T, L_REDOT>(eqop, ncomp, p, st);
+    case L_RETRACE: return launch_local_e<T, L_RETRACE>(eqop, ncomp, p, st);
+    case L_IMTRACE: return launch_local_e<T, L_IMTRACE>(eqop, ncomp, p, st);
+    case L_TRACE: return launch_local_e<T, L_TRACE>(eqop, ncomp, p, st);
+    case L_ANTIHERM: return launch_local_e<T, L_ANTIHERM>(eqop, ncomp, p, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 } // namespace
 } // namespace qla
 
@@ -691,6 +851,10 @@ hipError_t qla_gamma_launch_p0(int right, int g, int ncomp, const qla_vec_args &
 {
     return qla::launch_gamma_t<float>(right, g, ncomp, p, st);
 }
+hipError_t qla_local_launch_p0(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_local_t<float>(kind, eqop, ncomp, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -703,5 +867,9 @@ hipError_t qla_fill_launch_p1(int kind, int eqop, int ncomp, const qla_fill_args
 hipError_t qla_gamma_launch_p1(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_gamma_t<double>(right, g, ncomp, p, st);
+}
+hipError_t qla_local_launch_p1(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_local_t<double>(kind, eqop, ncomp, p, st);
 }
 #endif

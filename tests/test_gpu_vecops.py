@@ -17,7 +17,7 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import NCOMP, VECFAMS, fills, gammas, vecops
+from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fills, gammas, locals_, vecops
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -292,3 +292,43 @@ def test_gamma_out_of_range_is_a_noop():
     qla_amd.call("QLA_F3_D_veq_gamma_times_D", r, cs.dx, -1, cs.n)
     qla_amd.check()
     assert_bitwise(host(r), cs.r0, "mu out of range")
+
+
+LOCAL_DEST = {"R": (), "C": (2,), "M": (3, 3, 2)}
+
+
+@pytest.mark.parametrize("v", locals_(), ids=lambda v: v.name)
+def test_local_symbol(orc, v):
+    """Every site-local reduction and colour-matrix part bit for bit against
+    the oracle (the per-record double sums run in the same order)."""
+    code, dest = LOCALS[v.kind][0], LOCALS[v.kind][1]
+    cs = case(v.prec, v.t)
+    r0 = np.random.default_rng(zlib.crc32(v.name.encode())).standard_normal((cs.n,) + LOCAL_DEST[dest]).astype(cs.dt)
+    r = dev(r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(cs.px if v.pa else cs.dx)
+    if v.ia:
+        args.append(cs.down["x"])
+    if v.binary:
+        args.append(cs.py if v.pb else cs.dy)
+        if v.ib:
+            args.append(cs.down["y"])
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if v.own_index else cs.n)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    xs = cs.own["x"] if v.ia else site
+    ys = cs.own["y"] if v.ib else site
+    gx = np.ascontiguousarray(cs.x[cs.x_idx[xs] if v.pa else xs])
+    gy = np.ascontiguousarray(cs.y[cs.y_idx[ys] if v.pb else ys]) if v.binary else None
+    sub = np.ascontiguousarray(r0[rs])
+    orc.local(code, v.eqop, sub, gx, gy)
+    exp = r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)

@@ -450,3 +450,136 @@ def test_gamma_index_forms(orc):
     want[ir] = sub
     orc.gamma(False, 11, r, d, ir=ir, ia=ia)
     np.testing.assert_array_equal(r, want)
+
+
+# ---- site-local reductions and colour-matrix parts (orc_*_local) ----------
+
+def _dot_exact(a, b):
+    """per-record sum of conj(a) b in long double: (n,) complex."""
+    ca = cx(a.astype(np.longdouble)).reshape(a.shape[0], -1)
+    cb = cx(b.astype(np.longdouble)).reshape(b.shape[0], -1)
+    return np.sum(np.conj(ca) * cb, axis=1)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", TYPES)
+@pytest.mark.parametrize("eqop", range(4))
+def test_local_norm2_and_dots_vs_long_double(orc, prec, t, eqop):
+    """R eqop norm2, C eqop a.b, R eqop Re a.b per record: the double sum is
+    within (n_comp + 2) double roundings of sum |a||b| of the long-double value,
+    and the destination is that rounded to the field precision (one more
+    rounding of the result and of r0 +- x)."""
+    dt = DT[prec]
+    n = 25
+    a = field(t, n, 60, dt)
+    b = field(t, n, 61, dt)
+    ld = np.longdouble
+    ncomp = a[0].size // 2
+    sab = np.sum(np.abs(cx(a.astype(ld))) * np.abs(cx(b.astype(ld))), axis=tuple(range(1, a.ndim - 1)))
+    saa = np.sum(np.abs(cx(a.astype(ld))) ** 2, axis=tuple(range(1, a.ndim - 1)))
+    acc = {orc.L_NORM2: saa, orc.L_REDOT: sab, orc.L_DOT: sab}
+    eps_d = np.finfo(np.float64).eps
+    tol_k = {}
+    for k, s in acc.items():
+        tol_k[k] = (ncomp + 2) * eps_d * s
+    tol = lambda v, k=orc.L_REDOT: 2 * np.finfo(dt).eps * np.abs(v) + tol_k[k] + 4 * np.finfo(dt).tiny   # noqa: E731
+    r0 = field("V", n, 62, dt)[:, 0, 0].copy()             # n reals
+    c0 = np.ascontiguousarray(field("V", n, 63, dt)[:, 0])  # n complexes (n, 2)
+    for kind, want in ((orc.L_NORM2, _dot_exact(a, a).real), (orc.L_REDOT, _dot_exact(a, b).real)):
+        r = r0.copy()
+        orc.local(kind, eqop, r, a, None if kind == orc.L_NORM2 else b)
+        w = {0: want, 1: r0.astype(ld) + want, 2: r0.astype(ld) - want, 3: -want}[eqop]
+        assert np.all(np.abs(r.astype(ld) - w) <= tol(w, kind)), (kind, eqop)
+    c = c0.copy()
+    orc.local(orc.L_DOT, eqop, c, a, b)
+    want = _dot_exact(a, b)
+    c0c = c0[:, 0].astype(ld) + 1j * c0[:, 1].astype(ld)
+    w = {0: want, 1: c0c + want, 2: c0c - want, 3: -want}[eqop]
+    assert np.all(np.abs(c[:, 0].astype(ld) - w.real) <= 2 * tol(w.real, orc.L_DOT) + tol(np.abs(want), orc.L_DOT))
+    assert np.all(np.abs(c[:, 1].astype(ld) - w.imag) <= 2 * tol(w.imag, orc.L_DOT) + tol(np.abs(want), orc.L_DOT))
+
+
+def test_local_norm2_exact_on_integers(orc):
+    """Small-integer records: every step is exact, so norm2 and the dots equal
+    the integer sums; the F3 norm2 of the sum sits on the float grid."""
+    rng = np.random.default_rng(64)
+    for dt in (np.float32, np.float64):
+        a = rng.integers(-9, 10, size=(11, 3, 4, 3, 4, 2)).astype(dt)
+        b = rng.integers(-9, 10, size=a.shape).astype(dt)
+        r = np.zeros(11, dtype=dt)
+        orc.local(orc.L_NORM2, 0, r, a)
+        np.testing.assert_array_equal(r, (a.astype(np.int64) ** 2).reshape(11, -1).sum(1))
+        c = np.zeros((11, 2), dtype=dt)
+        orc.local(orc.L_DOT, 0, c, a, b)
+        ia, ib = cx(a.astype(np.int64)).reshape(11, -1), cx(b.astype(np.int64)).reshape(11, -1)
+        want = np.sum(np.conj(ia) * ib, axis=1)
+        np.testing.assert_array_equal(c[:, 0], want.real)
+        np.testing.assert_array_equal(c[:, 1], want.imag)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_traces(orc, prec):
+    dt = DT[prec]
+    m = field("M", 30, 65, dt)
+    tr = np.trace(cx(m.astype(np.float64)), axis1=1, axis2=2)
+    u = np.finfo(dt).eps
+    bound = 4 * u * np.abs(cx(m)).reshape(30, -1).sum(1)
+    r = np.zeros(30, dtype=dt)
+    orc.local(orc.L_RETRACE, 0, r, m)
+    assert np.all(np.abs(r - tr.real) <= bound)
+    orc.local(orc.L_IMTRACE, 0, r, m)
+    assert np.all(np.abs(r - tr.imag) <= bound)
+    c = np.zeros((30, 2), dtype=dt)
+    orc.local(orc.L_TRACE, 0, c, m)
+    assert np.all(np.abs(c[:, 0] - tr.real) <= bound) and np.all(np.abs(c[:, 1] - tr.imag) <= bound)
+    # the diagonal sum in order, in the field precision: exactly numpy's left-to-right float sum
+    d = cx(m)[:, [0, 1, 2], [0, 1, 2]]
+    np.testing.assert_array_equal(c[:, 0], (d.real[:, 0] + d.real[:, 1]) + d.real[:, 2])
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_antiherm_part(orc, prec):
+    """D is exactly anti-Hermitian (D_ji = -conj D_ij, real diagonal zero),
+    nearly traceless, equals (A - A^dagger)/2 - i Im tr(A)/3 within rounding,
+    and maps an anti-Hermitian traceless matrix with exactly zero trace to itself."""
+    dt = DT[prec]
+    m = field("M", 40, 66, dt)
+    d = np.zeros_like(m)
+    orc.local(orc.L_ANTIHERM, 0, d, m)
+    D = cx(d)
+    np.testing.assert_array_equal(D, -np.conj(np.swapaxes(D, 1, 2)))
+    A = cx(m.astype(np.float64))
+    want = 0.5 * (A - np.conj(np.swapaxes(A, 1, 2))) - 1j * (np.trace(A, axis1=1, axis2=2).imag / 3)[:, None, None] * np.eye(3)
+    u = np.finfo(dt).eps
+    assert np.all(np.abs(D - want) <= 8 * u * (1 + np.abs(A).max(axis=(1, 2)))[:, None, None])
+    assert np.all(np.abs(np.trace(D, axis1=1, axis2=2)) <= 8 * u * np.abs(A).max(axis=(1, 2)))
+    # anti-Hermitian input whose diagonal imaginary parts are integers summing to zero
+    q = np.zeros((3, 3), dtype=complex)
+    q[0, 1], q[0, 2], q[1, 2] = 0.5 + 0.25j, -1.0 + 2j, 0.75 - 0.5j
+    q = q - np.conj(q.T)
+    q[np.diag_indices(3)] = 1j * np.array([3.0, -1.0, -2.0])
+    qa = re_im(q[None], dt)
+    out = np.zeros_like(qa)
+    orc.local(orc.L_ANTIHERM, 0, out, qa)
+    np.testing.assert_array_equal(out, qa)
+
+
+def test_local_index_forms(orc):
+    a = field("D", 30, 67, np.float32)
+    b = field("D", 30, 68, np.float32)
+    rng = np.random.default_rng(69)
+    pa, pb = rng.permutation(30), rng.permutation(30)
+    c = np.zeros((30, 2), dtype=np.float32)
+    orc.local(orc.L_DOT, 0, c, a, b, a_idx=pa, b_idx=pb)
+    want = np.zeros((30, 2), dtype=np.float32)
+    orc.local(orc.L_DOT, 0, want, np.ascontiguousarray(a[pa]), np.ascontiguousarray(b[pb]))
+    np.testing.assert_array_equal(c, want)
+    ir = rng.permutation(30)[:8]
+    ia = rng.permutation(30)[:8]
+    r = np.full(30, 3.0, dtype=np.float32)
+    orc.local(orc.L_NORM2, 1, r, a, ir=ir, ia=ia)
+    sub = np.full(8, 3.0, dtype=np.float32)
+    orc.local(orc.L_NORM2, 1, sub, np.ascontiguousarray(a[ia]))
+    want = np.full(30, 3.0, dtype=np.float32)
+    want[ir] = sub
+    np.testing.assert_array_equal(r, want)
