@@ -58,6 +58,7 @@ def load():
         getattr(lib, f"orc_{p}_fill").argtypes = [i32, i32, i32, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_gamma").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_local").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_sum").argtypes = [i32, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -389,3 +390,19 @@ def local(kind, eqop, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None,
     getattr(lib, f"orc_{p}_local")(kind, eqop, ncomp, rreals, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
                                    _ptr(ix), n)
     return r
+
+
+def field_sum(a, index=None, a_idx=None, n=None) -> np.ndarray:
+    """Global sum of the records of a (orc_*_sum): the record's reals summed
+    in double over the steps (x-form index, pointer-array targets a_idx);
+    returns a double array of one record's shape."""
+    lib = load()
+    ncr = a[0].size
+    pa = _records(a, a_idx)
+    ix = _index(index)
+    if n is None:
+        n = len(ix) if ix is not None else a.shape[0]
+    out = np.zeros(a.shape[1:], dtype=np.float64)
+    p = "F" if a.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_sum")(ncr, _ptr(out), _ptr(pa), _ptr(ix), n)
+    return out

@@ -141,6 +141,11 @@ void orc_D_local(int kind, int eqop, int ncomp, int rreals, double *r, const int
                  const double *const *pb, const int *index, int n)
 { D_orc_local(kind, eqop, ncomp, rreals, r, rindex, pa, pb, index, n); }
 
+void orc_F_sum(int ncr, double *out, const float *const *pa, const int *index, int n)
+{ F_orc_sum(ncr, out, pa, index, n); }
+void orc_D_sum(int ncr, double *out, const double *const *pa, const int *index, int n)
+{ D_orc_sum(ncr, out, pa, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -133,6 +133,11 @@ void orc_F_local(int kind, int eqop, int ncomp, int rreals, float *r, const int 
 void orc_D_local(int kind, int eqop, int ncomp, int rreals, double *r, const int *rindex, const double *const *pa,
                  const double *const *pb, const int *index, int n);
 
+/* global sum of a field (qla_oracle_vecops.inc): the ncr real columns of
+ * the records *pa[s] summed in double over the n steps */
+void orc_F_sum(int ncr, double *out, const float *const *pa, const int *index, int n);
+void orc_D_sum(int ncr, double *out, const double *const *pa, const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -730,6 +730,65 @@ def locals_() -> list[LocalVariant]:
     return out
 
 
+# scalar-field types of the global sums (no colour): type letter -> (C type, reals per record)
+SCALAR_FIELDS = {"R": ("Real", 1), "C": ("Complex", 2)}
+
+
+@dataclass(frozen=True)
+class SumVariant:
+    """Global sum of a field, r = sum over sites of a (build_qla.pl:2449-2521):
+    F3 / D3 / DF3 for V, H, D, M, P; F / D / DF for the real and complex
+    fields R, C; v- and x-forms, field or pointer source."""
+    lib: str        # F3 D3 DF3 (coloured) or F D DF (R, C)
+    t: str
+    xf: bool
+    pa: bool
+
+    @property
+    def name(self) -> str:
+        return f"QLA_{self.lib}_{self.t.lower()}_{'x' if self.xf else 'v'}eq_sum_{'p' if self.pa else ''}{self.t}"
+
+    def _types(self) -> tuple[str, str]:
+        out_p = "D" if self.lib in ("D3", "DF3", "D", "DF") else "F"
+        in_p = "D" if self.lib in ("D3", "D") else "F"
+        if self.t in SCALAR_FIELDS:
+            ct = SCALAR_FIELDS[self.t][0]
+            return f"QLA_{out_p}_{ct}", f"QLA_{in_p}_{ct}"
+        return f"QLA_{out_p}3_{TYPE_NAMES[self.t]}", f"QLA_{in_p}3_{TYPE_NAMES[self.t]}"
+
+    def params(self) -> list[tuple[str, str]]:
+        rt, at = self._types()
+        out = [("p", f"{rt} *QLA_MI355X_RESTRICT r"),
+               ("p", f"{at} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a")]
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        in_d = 1 if self.lib in ("D3", "D") else 0
+        out_d = 1 if self.lib in ("D3", "DF3", "D", "DF") else 0
+        ncr = SCALAR_FIELDS[self.t][1] if self.t in SCALAR_FIELDS else 2 * NCOMP[self.t]
+        bytes_ = ncr * (8 if out_d else 4)
+        idx = "index" if self.xf else "nullptr"
+        return (f"sum_entry({in_d}, {out_d}, {ncr}, {int(self.pa)}, (void *)r, (const void *)a, {idx}, n, {bytes_}, "
+                f"\"{self.name}\");")
+
+
+def sums() -> list[SumVariant]:
+    out = []
+    for lib in ("F3", "D3", "DF3"):
+        for t in FIELD_TYPES:
+            out += [SumVariant(lib, t, xf, pa) for xf in (False, True) for pa in (False, True)]
+    for lib in ("F", "D", "DF"):
+        for t in SCALAR_FIELDS:
+            out += [SumVariant(lib, t, xf, pa) for xf in (False, True) for pa in (False, True)]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -767,8 +826,9 @@ def conversions() -> list[ConvVariant]:
 
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
-    field linear algebra, fills, gamma multiplications, site-local reductions)."""
-    return variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
+    field linear algebra, fills, gamma multiplications, site-local reductions, global sums)."""
+    return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
+            + sums())
 
 
 def variants() -> list[Variant]:
@@ -1047,6 +1107,8 @@ _VEC_DOC = {
     "trace": ("trace, C = tr a", "build_qla.pl:933-981, expressions_tensor.pl:353-389"),
     "antiherm": ("traceless anti-Hermitian part, M = (a - a^dagger)/2 - i Im tr(a)/3",
                  "build_qla.pl:983-1000, expressions_tensor.pl:138-232"),
+    "sum": ("global sum of a field, r = sum over sites of a (double sum per real column; DF: float fields, "
+            "double result)", "build_qla.pl:2449-2521, make_code_unary.pl:318-366"),
     "gamma_left": ("left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a",
                    "build_qla.pl:1356-1384, expressions_gamma.pl:123-198, 387-470"),
     "gamma_right": ("right multiplication by Gamma_mu, r = a Gamma_mu", "build_qla.pl:1386-1415, expressions_gamma.pl"),
@@ -1079,8 +1141,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_():
-        if isinstance(v, GammaVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums():
+        if isinstance(v, SumVariant):
+            key = (v.lib, "sum")
+        elif isinstance(v, GammaVariant):
             key = (v.prec, "gamma_right" if v.right else "gamma_left")
         elif isinstance(v, LocalVariant):
             key = (v.prec, v.kind)
@@ -1088,7 +1152,7 @@ def vecops_header_text() -> str:
             key = (v.prec, v.fam if isinstance(v, VecVariant) else v.kind)
         if key != last:
             what, ref = _VEC_DOC[key[1]]
-            lines += ["", f"/* {v.prec}: {what}.", f" * Reference: {ref}. */"]
+            lines += ["", f"/* {key[0]}: {what}.", f" * Reference: {ref}. */"]
             last = key
         lines.append(_decl(v) + ";")
     lines += ["", "#endif", ""]
@@ -1099,10 +1163,10 @@ def vecops_inc_text() -> str:
     lines = [
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
-        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry and local_entry",
-        "// are there.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry",
+        "// and sum_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_():
+    for v in vecops() + fills() + gammas() + locals_() + sums():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)
@@ -1116,6 +1180,7 @@ def vecops_inc_text() -> str:
 # names; N-colour names inject QLA_Nc as the first argument.
 _GENERIC_LIBS = {
     "F3": ("'F'", "3"), "D3": ("'D'", "3"), "FN": ("'F'", "'N'"), "DF3": (None, "3"), "FD3": (None, "3"),
+    "F": ("'F'", None), "D": ("'D'", None), "DF": (None, None),
 }
 
 
@@ -1138,6 +1203,11 @@ def generic_header_text() -> str:
             if nc:
                 return f"#define {generic}(...) {nm}(QLA_Nc, __VA_ARGS__)"
             return f"#define {generic} {nm}"
+        if col is None:
+            # colourless libraries (QLA_F, QLA_D): only the fully generic name
+            if prec is not None:
+                groups.setdefault((f"QLA_Precision == {prec}", "precision-only"), []).append(mapping(f"QLA_{rest}"))
+            continue
         if prec is not None:
             groups.setdefault((f"QLA_Precision == {prec} && QLA_Colors == {col}", "generic"), []).append(
                 mapping(f"QLA_{rest}"))

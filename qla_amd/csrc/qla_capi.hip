@@ -29,6 +29,8 @@ struct thread_state {
     hipError_t err = hipSuccess;
     // per-device reduction scratch: partials + one result slot (2 doubles)
     std::unordered_map<int, double *> scratch;
+    // per-device scratch of the global field sums: partials + one record
+    std::unordered_map<int, double *> sum_scratch;
 };
 
 thread_local thread_state ts;
@@ -57,6 +59,19 @@ double *scratch_for_device()
     hipError_t e = hipMalloc((void **)&p, bytes);
     if (e != hipSuccess) { record(e, "reduction scratch"); return nullptr; }
     ts.scratch[dev] = p;
+    return p;
+}
+
+double *sum_scratch_for_device()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = ts.sum_scratch.find(dev);
+    if (it != ts.sum_scratch.end()) return it->second;
+    double *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, sizeof(double) * (size_t)qla_sum_scratch_doubles());
+    if (e != hipSuccess) { record(e, "sum scratch"); return nullptr; }
+    ts.sum_scratch[dev] = p;
     return p;
 }
 
@@ -236,6 +251,26 @@ void local_entry(int prec, int kind, int eqop, int ncomp, int pa, int pb, void *
     p.px = pa;
     p.py = pb;
     finish(qla_launch_local(prec, kind, eqop, ncomp, p, ts.stream), name);
+}
+
+// One global sum of a field (qla_vecops_abi.inc): ncr reals per record,
+// prec_in the field precision, out_double the result precision; the result
+// record (bytes) goes to r in host or device memory; n <= 0 gives zeros.
+void sum_entry(int prec_in, int out_double, int ncr, int pa, void *r, const void *a, const int *index, int n,
+               size_t bytes, const char *name)
+{
+    double *sc = sum_scratch_for_device();
+    if (!sc) return;
+    if (n < 0) n = 0;
+    if (is_device_pointer(r)) {
+        finish(qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, r, sc, ts.stream), name);
+        return;
+    }
+    double *slot = sc + qla_sum_scratch_doubles() - 288;
+    hipError_t e = qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, slot, sc, ts.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    record(e, name);
 }
 
 // compressed-field reductions: result doubles to r (host or device)

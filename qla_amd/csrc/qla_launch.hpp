@@ -138,6 +138,15 @@ hipError_t qla_launch_local(int prec, int kind, int eqop, int ncomp, const qla_v
 hipError_t qla_local_launch_p0(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_local_launch_p1(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 
+// Global sums of fields (qla_reduce.hip): the ncr real columns of the records
+// (ncr <= 288) summed over n steps (record index[i] or i, through the pointer
+// array when pa) in double, written to the DEVICE address out as float
+// (out_double 0) or double; prec_in 0 float, 1 double fields.  scratch holds
+// qla_sum_scratch_doubles() doubles (its last 288 are free for a result).
+int qla_sum_scratch_doubles(void);
+hipError_t qla_launch_sum(int prec_in, int out_double, int ncr, const void *a, const int *index, int n, int pa,
+                          void *out, double *scratch, hipStream_t st);
+
 // Compressed-precision forms.  fmt: 0 bfloat16 links, 1 binary16 links.
 hipError_t qla_launch_spproj_c(int fmt, void *r, const void *a, const void *b, int mu, int sign, int n,
                                hipStream_t st);

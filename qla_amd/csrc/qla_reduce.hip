@@ -353,3 +353,95 @@ hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(RBLOCK), 0, st, (const double *)scratch, g, (int)dot, out, out_kind);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Global sums of fields (build_qla.pl:2449-2521, make_code_unary.pl:318-366
+// with qualifier sum): every real column of the record (ncr = reals per
+// record, at most 288) summed over the sites in double (each element
+// promoted; F and D both accumulate in D), then converted to the result
+// precision.  Stage 1: each block owns a contiguous chunk of the steps;
+// rpi = RBLOCK / ncr records are read per iteration, thread t taking column
+// t % ncr of record t / ncr (coalesced), and the rpi partial sums of a column
+// are added in order through LDS; records wider than the block (P: 288
+// reals) give each thread the columns t and t + RBLOCK.  Stage 2: one block
+// adds the per-block partials column by column in block order.
+// ---------------------------------------------------------------------------
+namespace qla {
+
+constexpr int SUM_BLOCKS = 256;
+constexpr int SUM_MAXNCR = 288;
+
+template <class T>
+__device__ __forceinline__ const T *sum_record(const void *a, const int *index, int pa, int ncr, long long i)
+{
+    const long long s = index ? index[i] : i;
+    if (pa) return ((const T *const *)a)[s];
+    return (const T *)a + s * ncr;
+}
+
+template <class T>
+__global__ void __launch_bounds__(RBLOCK)
+k_sum1(const void *a, const int *index, int pa, long long n, int ncr, double *partials)
+{
+    __shared__ double sh[RBLOCK];
+    const int t = threadIdx.x;
+    const long long per = (n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = (long long)blockIdx.x * per;
+    const long long r1 = r0 + per < n ? r0 + per : n;
+    double *out = partials + (size_t)blockIdx.x * ncr;
+    if (ncr <= RBLOCK) {
+        const int rpi = RBLOCK / ncr;
+        const int sub = t / ncr, col = t - sub * ncr;
+        double acc = 0.0;
+        if (sub < rpi)
+            for (long long r = r0 + sub; r < r1; r += rpi) acc += (double)sum_record<T>(a, index, pa, ncr, r)[col];
+        sh[t] = acc;
+        __syncthreads();
+        if (t < ncr) {
+            double s = 0.0;
+            for (int k = 0; k < rpi; k++) s += sh[k * ncr + t];
+            out[t] = s;
+        }
+    } else {
+        double acc0 = 0.0, acc1 = 0.0;
+        const int c1 = t + RBLOCK;
+        for (long long r = r0; r < r1; r++) {
+            const T *rec = sum_record<T>(a, index, pa, ncr, r);
+            acc0 += (double)rec[t];
+            if (c1 < ncr) acc1 += (double)rec[c1];
+        }
+        out[t] = acc0;
+        if (c1 < ncr) out[c1] = acc1;
+    }
+}
+
+template <class TO>
+__global__ void __launch_bounds__(RBLOCK) k_sum2(const double *partials, int nb, int ncr, TO *out)
+{
+    for (int c = threadIdx.x; c < ncr; c += RBLOCK) {
+        double s = 0.0;
+        for (int b = 0; b < nb; b++) s += partials[(size_t)b * ncr + c];
+        out[c] = (TO)s;
+    }
+}
+
+} // namespace qla
+
+int qla_sum_scratch_doubles(void) { return qla::SUM_BLOCKS * qla::SUM_MAXNCR + qla::SUM_MAXNCR; }
+
+hipError_t qla_launch_sum(int prec_in, int out_double, int ncr, const void *a, const int *index, int n, int pa,
+                          void *out, double *scratch, hipStream_t st)
+{
+    using namespace qla;
+    if (ncr < 1 || ncr > SUM_MAXNCR) return hipErrorInvalidValue;
+    const long long nn = n < 0 ? 0 : n;
+    const int rpi = ncr <= RBLOCK ? RBLOCK / ncr : 1;
+    long long want = (nn + (long long)rpi * 8 - 1) / ((long long)rpi * 8);
+    const int nb = (int)(want < 1 ? 1 : (want > SUM_BLOCKS ? SUM_BLOCKS : want));
+    if (prec_in == 0) hipLaunchKernelGGL(k_sum1<float>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
+    else hipLaunchKernelGGL(k_sum1<double>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
+    if (out_double) hipLaunchKernelGGL(k_sum2<double>, dim3(1), dim3(RBLOCK), 0, st, scratch, nb, ncr, (double *)out);
+    else hipLaunchKernelGGL(k_sum2<float>, dim3(1), dim3(RBLOCK), 0, st, scratch, nb, ncr, (float *)out);
+    return hipGetLastError();
+}
+

@@ -38,7 +38,8 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 7551
+    assert len(every) == len(set(every)) == 7635
+    assert len(abi_gen.sums()) == 84
     assert len(abi_gen.locals_()) == 1536
     assert len(abi_gen.gammas()) == 42
     assert len(abi_gen.fills()) == 116
@@ -109,6 +110,8 @@ def test_generic_names_select_the_library(tmp_path):
     assert "QLA_D3_V_veq_M_times_V(r, a, b, n)" in e and "QLA_D3_C_veq_det_M(r, a, n)" in e
     e = _expand(tmp_path, ["-DQLA_PrecisionInt=2"], "QLA_D_r_veq_norm2_V(r, a, n); QLA_F_r_veq_norm2_V(r, a, n);")
     assert "QLA_D3_r_veq_norm2_V(r, a, n)" in e and "QLA_F3_r_veq_norm2_V(r, a, n)" in e
+    e = _expand(tmp_path, ["-DQLA_Precision='D'"], "QLA_r_veq_sum_R(r, a, n); QLA_c_xeq_sum_pC(r, a, ix, n);")
+    assert "QLA_D_r_veq_sum_R(r, a, n)" in e and "QLA_D_c_xeq_sum_pC(r, a, ix, n)" in e
     e = _expand(tmp_path, ["-DQLA_Nc=5"], "QLA_M_veq_log_M(r, a, n); QLA_N_M_veq_exp_M(r, a, n); QLA_F_C_veq_det_M(r, a, n);")
     assert ("QLA_FN_M_veq_log_M(5, r, a, n)" in e and "QLA_FN_M_veq_exp_M(5, r, a, n)" in e
             and "QLA_FN_C_veq_det_M(5, r, a, n)" in e)

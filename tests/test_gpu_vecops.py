@@ -17,7 +17,7 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fills, gammas, locals_, vecops
+from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fills, gammas, locals_, sums, vecops
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -332,3 +332,57 @@ def test_local_symbol(orc, v):
     exp = r0.copy()
     exp[rs] = sub
     assert_bitwise(host(r), exp, v.name)
+
+
+SUM_N = 1000
+
+
+@pytest.mark.parametrize("v", sums(), ids=lambda v: v.name)
+def test_sum_symbol(orc, v):
+    """Every global field sum against the oracle's double sum: the summation
+    order differs (the reference's OpenMP reduction fixes none), so a double
+    result agrees within n double roundings of sum |a| per real column and a
+    float result within one float rounding more."""
+    import ctypes
+    in_d = v.lib in ("D3", "D")
+    out_d = v.lib in ("D3", "DF3", "D", "DF")
+    dt = np.float64 if in_d else np.float32
+    shape = {"R": (SUM_N,), "C": (SUM_N, 2)}.get(v.t)
+    rng = np.random.default_rng(zlib.crc32(v.name.encode()))
+    a = (rng.standard_normal(shape) if shape else lattice.random_field(v.t, SUM_N, 90, dtype=np.float64)).astype(dt)
+    da = dev(a)
+    idx = np.sort(rng.choice(SUM_N, SUM_N // 3, replace=False)).astype(np.int32)
+    tgt = rng.permutation(SUM_N)
+    src = qla_amd.pointer_array(da, tgt) if v.pa else da
+    rec_shape = a.shape[1:]
+    out = np.zeros(rec_shape, dtype=np.float64 if out_d else np.float32)
+    args = [out.ctypes.data, src]
+    if v.xf:
+        args.append(dev(idx))
+    m = len(idx) if v.xf else SUM_N
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = idx if v.xf else np.arange(m)
+    recs = a[tgt[site]] if v.pa else a[site]
+    want = orc.field_sum(np.ascontiguousarray(recs))
+    bound = m * np.finfo(np.float64).eps * np.abs(recs.astype(np.float64)).sum(axis=0)
+    if not out_d:
+        bound = bound + np.finfo(np.float32).eps * np.abs(want)
+    assert np.all(np.abs(out.astype(np.float64) - want) <= bound + 1e-300), v.name
+
+
+def test_sum_device_result_and_empty(orc):
+    import torch
+    a = lattice.random_field("D", SUM_N, 91, dtype=np.float32)
+    out = torch.zeros((3, 4, 2), dtype=torch.float64, device="cuda")
+    say("QLA_DF3_d_veq_sum_D device result")
+    qla_amd.call("QLA_DF3_d_veq_sum_D", out, dev(a), SUM_N)
+    qla_amd.check()
+    want = orc.field_sum(a)
+    got = host(out)
+    assert np.all(np.abs(got - want) <= SUM_N * np.finfo(np.float64).eps * np.abs(a.astype(np.float64)).sum(axis=0))
+    h = np.full((3, 4, 2), 7.0, dtype=np.float32)
+    qla_amd.call("QLA_F3_d_veq_sum_D", h.ctypes.data, dev(a), 0)
+    qla_amd.check()
+    assert not np.any(h)

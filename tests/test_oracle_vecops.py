@@ -583,3 +583,24 @@ def test_local_index_forms(orc):
     want = np.full(30, 3.0, dtype=np.float32)
     want[ir] = sub
     np.testing.assert_array_equal(r, want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", TYPES)
+def test_field_sum(orc, prec, t):
+    """Global sums: exact integer sums on small integers; within n double
+    roundings of the long-double sum on Gaussian data; x-form and pointer
+    targets sum the records they name."""
+    dt = DT[prec]
+    rng = np.random.default_rng(80)
+    a = rng.integers(-50, 51, size=field(t, 300, 0, dt).shape).astype(dt)
+    np.testing.assert_array_equal(orc.field_sum(a), a.astype(np.int64).sum(axis=0))
+    g = field(t, 300, 81, dt)
+    want = g.astype(np.longdouble).sum(axis=0)
+    bound = 300 * np.finfo(np.float64).eps * np.abs(g.astype(np.longdouble)).sum(axis=0)
+    assert np.all(np.abs(orc.field_sum(g) - want) <= bound)
+    idx = np.sort(rng.choice(300, 77, replace=False))
+    np.testing.assert_array_equal(orc.field_sum(a, index=idx), a[idx].astype(np.int64).sum(axis=0))
+    perm = rng.permutation(300)[:50]
+    np.testing.assert_array_equal(orc.field_sum(a, a_idx=perm, n=50), a[perm].astype(np.int64).sum(axis=0))
+    assert not np.any(orc.field_sum(a, n=0))
