@@ -156,3 +156,52 @@ def test_device_and_oracle_operators_agree(orc):
         got = host(r)
         exp = w.apply_oracle(orc, psi, dagger=dagger)
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), dagger
+
+
+def test_plaquette_two_ways(orc):
+    """sum_x sum_{mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dagger U_nu(x)^dagger
+    on 8^3 x 6 through the C ABI, twice:
+    (a) X = U_mu * U_nu(x+mu), Y = U_nu * U_mu(x+nu) (M_veq_M_times_pM with a
+        pointer array of links), then the global Re X.Y (QLA_DF3_r_veq_re_M_dot_M:
+        Re tr Y^dagger X summed);
+    (b) Z = X Y^dagger (M_veq_M_times_Ma), the site-local Re tr (R_veq_re_trace_M)
+        and the global sum of the real field (QLA_DF_r_veq_sum_R);
+    against a complex128 numpy evaluation of the same products."""
+    import torch
+    lib = qla_amd.load()
+    qla_amd.set_sync(True)
+    w = Wilson("F3")
+    n = w.n
+    U = [u.astype(np.float64) for u in w.u]
+    Uc = [u[..., 0] + 1j * u[..., 1] for u in U]
+    want = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            x = np.einsum("nij,njk->nik", Uc[mu], Uc[nu][w.fw[mu]])
+            y = np.einsum("nij,njk->nik", Uc[nu], Uc[mu][w.fw[nu]])
+            want += np.einsum("nij,nij->", x, np.conj(y)).real
+    fw_links = [[qla_amd.pointer_array(w.du[nu], w.fw[mu]) for nu in range(4)] for mu in range(4)]
+    X = torch.empty((n, 3, 3, 2), dtype=torch.float32, device="cuda")
+    Y = torch.empty_like(X)
+    Z = torch.empty_like(X)
+    tr = torch.empty(n, dtype=torch.float32, device="cuda")
+    va = ctypes.c_double()
+    vb = ctypes.c_double()
+    tot_a = tot_b = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            say(f"plaquette plane {mu}{nu}")
+            lib.QLA_F3_M_veq_M_times_pM(X.data_ptr(), w.du[mu].data_ptr(), fw_links[mu][nu].data_ptr(), n)
+            lib.QLA_F3_M_veq_M_times_pM(Y.data_ptr(), w.du[nu].data_ptr(), fw_links[nu][mu].data_ptr(), n)
+            lib.QLA_DF3_r_veq_re_M_dot_M(ctypes.addressof(va), Y.data_ptr(), X.data_ptr(), n)
+            lib.QLA_F3_M_veq_M_times_Ma(Z.data_ptr(), X.data_ptr(), Y.data_ptr(), n)
+            lib.QLA_F3_R_veq_re_trace_M(tr.data_ptr(), Z.data_ptr(), n)
+            lib.QLA_DF_r_veq_sum_R(ctypes.addressof(vb), tr.data_ptr(), n)
+            tot_a += va.value
+            tot_b += vb.value
+    qla_amd.check()
+    scale = 6 * 3 * n
+    print(f"plaquette: {want / scale:.9f} (numpy), {tot_a / scale:.9f} (dot), {tot_b / scale:.9f} (trace + sum)")
+    # float products: |error| per site-plane ~ a few float ulps of |tr| <= 3
+    assert abs(tot_a - want) <= 1e-5 * scale
+    assert abs(tot_b - want) <= 1e-5 * scale
