@@ -344,9 +344,11 @@ def vecops_layer(lib, stream, n, sets, args):
     """The field linear algebra a solver runs between hops, on the 32^4 D
     fields: x += a p, r = a p + r', r = z p - r', the copy, the even-site
     x-form update, the FP64 update; the Hermitian conjugate of the links
-    and the P transpose on 2^20 propagators (the gather path).  Algorithmic
-    bytes per site: every record read once, the destination written (and
-    read for peq), 4 B of index for the x-form."""
+    and the P transpose on 2^20 propagators; gamma_5 D; the site-local
+    norm2 and dot of D, the anti-Hermitian part of the links and the global
+    sum of a D field (DF3, result in device memory).  Algorithmic bytes per
+    site: every record read once, the destination written (and read for
+    peq), 4 B of index for the x-form."""
     import ctypes
     import torch
     import qla_amd
@@ -366,6 +368,9 @@ def vecops_layer(lib, stream, n, sets, args):
     p_n = 1 << 20
     prop = torch.zeros((p_n, 3, 4, 3, 4, 2), dtype=torch.float32, device="cuda").normal_()
     prop_t = torch.empty_like(prop)
+    rl = torch.empty(n, dtype=torch.float32, device="cuda")
+    cl = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    dsum = torch.empty((3, 4, 2), dtype=torch.float64, device="cuda")
     ref = ctypes.addressof
     cases = {
         "D_vpeq_r_times_D": (lambda it: lib.QLA_F3_D_vpeq_r_times_D(x.data_ptr(), ref(a), p.data_ptr(), n), 288, n),
@@ -381,6 +386,12 @@ def vecops_layer(lib, stream, n, sets, args):
         "M_veq_Ma": (lambda it: lib.QLA_F3_M_veq_Ma(madj.data_ptr(), sets[0][0].data_ptr(), n), 144, n),
         "P_veq_transpose_P": (lambda it: lib.QLA_F3_P_veq_transpose_P(prop_t.data_ptr(), prop.data_ptr(), p_n), 2304,
                               p_n),
+        "D_veq_gamma_times_D": (lambda it: lib.QLA_F3_D_veq_gamma_times_D(y.data_ptr(), p.data_ptr(), 15, n), 192, n),
+        "R_veq_norm2_D": (lambda it: lib.QLA_F3_R_veq_norm2_D(rl.data_ptr(), p.data_ptr(), n), 100, n),
+        "C_veq_D_dot_D": (lambda it: lib.QLA_F3_C_veq_D_dot_D(cl.data_ptr(), p.data_ptr(), q.data_ptr(), n), 200, n),
+        "M_veq_antiherm_M": (lambda it: lib.QLA_F3_M_veq_antiherm_M(madj.data_ptr(), sets[0][0].data_ptr(), n), 144,
+                             n),
+        "DF3_d_veq_sum_D": (lambda it: lib.QLA_DF3_d_veq_sum_D(dsum.data_ptr(), p.data_ptr(), n), 96, n),
     }
     for name, (fn, bps, sites) in cases.items():
         ms = _timed(stream, fn, reps, 2) / reps

@@ -364,11 +364,11 @@ hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a
 // t % ncr of record t / ncr (coalesced), and the rpi partial sums of a column
 // are added in order through LDS; records wider than the block (P: 288
 // reals) give each thread the columns t and t + RBLOCK.  Stage 2: one block
-// adds the per-block partials column by column in block order.
+// per column adds that column's per-block partials in a fixed tree.
 // ---------------------------------------------------------------------------
 namespace qla {
 
-constexpr int SUM_BLOCKS = 256;
+constexpr int SUM_BLOCKS = 2048;
 constexpr int SUM_MAXNCR = 288;
 
 template <class T>
@@ -392,9 +392,23 @@ k_sum1(const void *a, const int *index, int pa, long long n, int ncr, double *pa
     if (ncr <= RBLOCK) {
         const int rpi = RBLOCK / ncr;
         const int sub = t / ncr, col = t - sub * ncr;
+        // four independent records per iteration keep loads in flight; the
+        // partial sums are added in record order
         double acc = 0.0;
-        if (sub < rpi)
-            for (long long r = r0 + sub; r < r1; r += rpi) acc += (double)sum_record<T>(a, index, pa, ncr, r)[col];
+        if (sub < rpi) {
+            long long r = r0 + sub;
+            for (; r + 3 * rpi < r1; r += 4 * rpi) {
+                const double v0 = (double)sum_record<T>(a, index, pa, ncr, r)[col];
+                const double v1 = (double)sum_record<T>(a, index, pa, ncr, r + rpi)[col];
+                const double v2 = (double)sum_record<T>(a, index, pa, ncr, r + 2 * rpi)[col];
+                const double v3 = (double)sum_record<T>(a, index, pa, ncr, r + 3 * rpi)[col];
+                acc += v0;
+                acc += v1;
+                acc += v2;
+                acc += v3;
+            }
+            for (; r < r1; r += rpi) acc += (double)sum_record<T>(a, index, pa, ncr, r)[col];
+        }
         sh[t] = acc;
         __syncthreads();
         if (t < ncr) {
@@ -415,14 +429,16 @@ k_sum1(const void *a, const int *index, int pa, long long n, int ncr, double *pa
     }
 }
 
+// stage 2: block c adds column c's per-block partials in a fixed tree
 template <class TO>
 __global__ void __launch_bounds__(RBLOCK) k_sum2(const double *partials, int nb, int ncr, TO *out)
 {
-    for (int c = threadIdx.x; c < ncr; c += RBLOCK) {
-        double s = 0.0;
-        for (int b = 0; b < nb; b++) s += partials[(size_t)b * ncr + c];
-        out[c] = (TO)s;
-    }
+    __shared__ double sh[RBLOCK / WAVE];
+    const int c = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += RBLOCK) s += partials[(size_t)b * ncr + c];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[c] = (TO)s;
 }
 
 } // namespace qla
@@ -436,12 +452,12 @@ hipError_t qla_launch_sum(int prec_in, int out_double, int ncr, const void *a, c
     if (ncr < 1 || ncr > SUM_MAXNCR) return hipErrorInvalidValue;
     const long long nn = n < 0 ? 0 : n;
     const int rpi = ncr <= RBLOCK ? RBLOCK / ncr : 1;
-    long long want = (nn + (long long)rpi * 8 - 1) / ((long long)rpi * 8);
+    long long want = (nn + (long long)rpi * 16 - 1) / ((long long)rpi * 16);
     const int nb = (int)(want < 1 ? 1 : (want > SUM_BLOCKS ? SUM_BLOCKS : want));
     if (prec_in == 0) hipLaunchKernelGGL(k_sum1<float>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
     else hipLaunchKernelGGL(k_sum1<double>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
-    if (out_double) hipLaunchKernelGGL(k_sum2<double>, dim3(1), dim3(RBLOCK), 0, st, scratch, nb, ncr, (double *)out);
-    else hipLaunchKernelGGL(k_sum2<float>, dim3(1), dim3(RBLOCK), 0, st, scratch, nb, ncr, (float *)out);
+    if (out_double) hipLaunchKernelGGL(k_sum2<double>, dim3(ncr), dim3(RBLOCK), 0, st, scratch, nb, ncr, (double *)out);
+    else hipLaunchKernelGGL(k_sum2<float>, dim3(ncr), dim3(RBLOCK), 0, st, scratch, nb, ncr, (float *)out);
     return hipGetLastError();
 }
 

@@ -655,11 +655,47 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gamma(qla_vec_args p, gamma_tab 
     }
 }
 
+// v-form over fields: the run of m complexes, the multiplied spin index at
+// element stride SSTRIDE (1: D and a Gamma from the right; 12: P from the
+// left), so no record arithmetic is needed
+template <class T, int SSTRIDE>
+__global__ void __launch_bounds__(VBLOCK) k_vec_gamma_flat(cpx<T> *r, const cpx<T> *x, long long m, gamma_tab tab)
+{
+    typedef typename vtype<T, 1>::v V1;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < m; g += stride) {
+        const int spin = (int)((g / SSTRIDE) & 3);
+        const int src = spin == 0 ? tab.src[0] : spin == 1 ? tab.src[1] : spin == 2 ? tab.src[2] : tab.src[3];
+        const int ph = spin == 0 ? tab.ph[0] : spin == 1 ? tab.ph[1] : spin == 2 ? tab.ph[2] : tab.ph[3];
+        const V1 v = __builtin_nontemporal_load((const V1 *)(x + g + (long long)(src - spin) * SSTRIDE));
+        V1 o;
+        switch (ph) {
+        case 0: o[0] = v[0]; o[1] = v[1]; break;
+        case 1: o[0] = -v[1]; o[1] = v[0]; break;
+        case 2: o[0] = -v[0]; o[1] = -v[1]; break;
+        default: o[0] = v[1]; o[1] = -v[0]; break;
+        }
+        __builtin_nontemporal_store(o, (V1 *)(r + g));
+    }
+}
+
 template <class T>
 hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
     if (p.n <= 0) return hipSuccess;
     const gamma_tab tab = gamma_table(g, right);
+    if (!p.index && !p.ir && !p.ix && !p.px) {
+        const long long m = (long long)p.n * ncomp;
+        const dim3 grid(blocks_for(m));
+        if (!right && ncomp == 144)
+            hipLaunchKernelGGL((k_vec_gamma_flat<T, 12>), grid, dim3(VBLOCK), 0, st, (cpx<T> *)p.r,
+                               (const cpx<T> *)p.x, m, tab);
+        else if (ncomp == 12 || ncomp == 144)
+            hipLaunchKernelGGL((k_vec_gamma_flat<T, 1>), grid, dim3(VBLOCK), 0, st, (cpx<T> *)p.r,
+                               (const cpx<T> *)p.x, m, tab);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
     qla_vec_args q = p;
     const long long tot = (long long)p.n * ncomp;
     q.wide = tot > 0xffffffffLL ? 1 : 0;
