@@ -59,6 +59,8 @@ def load():
         getattr(lib, f"orc_{p}_gamma").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_local").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_sum").argtypes = [i32, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_fieldmul").argtypes = [i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_outer").argtypes = [i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -406,3 +408,37 @@ def field_sum(a, index=None, a_idx=None, n=None) -> np.ndarray:
     p = "F" if a.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_sum")(ncr, _ptr(out), _ptr(pa), _ptr(ix), n)
     return out
+
+
+def fieldmul(eqop, r, f, b, fa=False, cb=False, dim=0, index=None, f_idx=None, b_idx=None, ir=None, i_f=None,
+             ib=None, n=None):
+    """r eqop f * b with f a real field ((n,) reals) or complex field ((n, 2));
+    fa conjugates f, cb with dim the adjoint of b (orc_*_fieldmul)."""
+    lib = load()
+    ncomp = r[0].size // 2
+    fcomplex = 1 if f.ndim == 2 else 0
+    fv = f if f.ndim == 2 else f.reshape(-1, 1)
+    pf = _records(np.ascontiguousarray(fv), i_f if i_f is not None else f_idx)
+    pb = _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_fieldmul")(fcomplex, int(fa), int(cb), int(dim), eqop, ncomp, _ptr(_need(r)), _ptr(rix),
+                                      _ptr(pf), _ptr(pb), _ptr(ix), n)
+    return r
+
+
+def outer(eqop, r, a, b, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=None, n=None):
+    """M eqop a * b^dagger for colour vectors a, b (orc_*_outer)."""
+    lib = load()
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_outer")(eqop, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
+    return r

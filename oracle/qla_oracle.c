@@ -146,6 +146,19 @@ void orc_F_sum(int ncr, double *out, const float *const *pa, const int *index, i
 void orc_D_sum(int ncr, double *out, const double *const *pa, const int *index, int n)
 { D_orc_sum(ncr, out, pa, index, n); }
 
+void orc_F_fieldmul(int fcomplex, int fa, int cb, int dim, int eqop, int ncomp, float *r, const int *rindex,
+                    const float *const *pf, const float *const *pb, const int *index, int n)
+{ F_orc_fieldmul(fcomplex, fa, cb, dim, eqop, ncomp, r, rindex, pf, pb, index, n); }
+void orc_D_fieldmul(int fcomplex, int fa, int cb, int dim, int eqop, int ncomp, double *r, const int *rindex,
+                    const double *const *pf, const double *const *pb, const int *index, int n)
+{ D_orc_fieldmul(fcomplex, fa, cb, dim, eqop, ncomp, r, rindex, pf, pb, index, n); }
+void orc_F_outer(int eqop, float *r, const int *rindex, const float *const *pa, const float *const *pb,
+                 const int *index, int n)
+{ F_orc_outer(eqop, r, rindex, pa, pb, index, n); }
+void orc_D_outer(int eqop, double *r, const int *rindex, const double *const *pa, const double *const *pb,
+                 const int *index, int n)
+{ D_orc_outer(eqop, r, rindex, pa, pb, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -138,6 +138,17 @@ void orc_D_local(int kind, int eqop, int ncomp, int rreals, double *r, const int
 void orc_F_sum(int ncr, double *out, const float *const *pa, const int *index, int n);
 void orc_D_sum(int ncr, double *out, const double *const *pa, const int *index, int n);
 
+/* multiplication by a real (fcomplex 0) or complex field and the colour
+ * outer product (qla_oracle_vecops.inc) */
+void orc_F_fieldmul(int fcomplex, int fa, int cb, int dim, int eqop, int ncomp, float *r, const int *rindex,
+                    const float *const *pf, const float *const *pb, const int *index, int n);
+void orc_D_fieldmul(int fcomplex, int fa, int cb, int dim, int eqop, int ncomp, double *r, const int *rindex,
+                    const double *const *pf, const double *const *pb, const int *index, int n);
+void orc_F_outer(int eqop, float *r, const int *rindex, const float *const *pa, const float *const *pb,
+                 const int *index, int n);
+void orc_D_outer(int eqop, double *r, const int *rindex, const double *const *pa, const double *const *pb,
+                 const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

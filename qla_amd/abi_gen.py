@@ -789,6 +789,98 @@ def sums() -> list[SumVariant]:
     return out
 
 
+@dataclass(frozen=True)
+class FieldVariant:
+    """r eqop f * b with f a real (R) or complex (C, Ca) field and b a record
+    field or its adjoint (build_qla.pl:1517-1586), or the colour outer product
+    M eqop a * b^dagger of two colour vectors (:1730-1756); every binary form."""
+    prec: str
+    kind: str       # "R", "C" (factor field) or "outer"
+    t: str          # destination / second-source record type (M for outer)
+    eqop: int
+    fa: bool = False    # conjugated complex factor ("Ca")
+    adj: bool = False   # adjoint second source ("Ma", "Pa")
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ia or self.ib
+
+    @property
+    def name(self) -> str:
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        r = ("x" if self.ir else "") + self.t
+        pre_a = ("p" if self.pa else "") + ("x" if self.ia else "")
+        pre_b = ("p" if self.pb else "") + ("x" if self.ib else "")
+        head = f"QLA_{self.prec}_{r}_{form}{EQOPS[self.eqop]}_"
+        if self.kind == "outer":
+            return f"{head}{pre_a}V_times_{pre_b}Va"
+        f = self.kind + ("a" if self.fa else "")
+        return f"{head}{pre_a}{f}_times_{pre_b}{self.t}{'a' if self.adj else ''}"
+
+    def params(self) -> list[tuple[str, str]]:
+        real = "D" if self.prec == "D3" else "F"
+        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        if self.kind == "outer":
+            at = bt = f"QLA_{self.prec}_ColorVector"
+        else:
+            at = f"QLA_{real}_{'Real' if self.kind == 'R' else 'Complex'}"
+            bt = rec
+        out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{at} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out.append(("p", f"{bt} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+        if self.ib:
+            out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        if self.kind == "outer":
+            return (f"outer_entry({prec}, {self.eqop}, {int(self.pa)}, {int(self.pb)}, (void *)r, (const void *)a, "
+                    f"(const void *)b, {idx}, {own}, n, \"{self.name}\");")
+        return (f"field_entry({prec}, {int(self.kind == 'C')}, {self.eqop}, {NCOMP[self.t]}, {3 if self.adj else 0}, "
+                f"{int(self.fa)}, {int(self.pa)}, {int(self.pb)}, (void *)r, (const void *)a, (const void *)b, {idx}, "
+                f"{own}, n, \"{self.name}\");")
+
+
+def fieldmuls() -> list[FieldVariant]:
+    out = []
+
+    def forms(**kw):
+        res = [FieldVariant(xf=xf, pa=pa, pb=pb, **kw) for xf in (False, True) for pa in (False, True)
+               for pb in (False, True)]
+        res += [FieldVariant(ir=ir, ia=ia, ib=ib, **kw) for ir, ia, ib in OWN_INDEX_FORMS]
+        return res
+    for prec in PRECS:
+        for kind in ("R", "C"):
+            for fa in ((False,) if kind == "R" else (False, True)):
+                for t in FIELD_TYPES:
+                    for adj in ((False, True) if t in SQUARE_TYPES else (False,)):
+                        for e in range(4):
+                            out += forms(prec=prec, kind=kind, t=t, eqop=e, fa=fa, adj=adj)
+        for e in range(4):
+            out += forms(prec=prec, kind="outer", t="M", eqop=e)
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -826,9 +918,10 @@ def conversions() -> list[ConvVariant]:
 
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
-    field linear algebra, fills, gamma multiplications, site-local reductions, global sums)."""
+    field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
+    by real and complex fields, outer products)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums())
+            + sums() + fieldmuls())
 
 
 def variants() -> list[Variant]:
@@ -1107,6 +1200,12 @@ _VEC_DOC = {
     "trace": ("trace, C = tr a", "build_qla.pl:933-981, expressions_tensor.pl:353-389"),
     "antiherm": ("traceless anti-Hermitian part, M = (a - a^dagger)/2 - i Im tr(a)/3",
                  "build_qla.pl:983-1000, expressions_tensor.pl:138-232"),
+    "fieldR": ("times a real field, r eqop a(x) b(x) (`Ta`: the adjoint of b)",
+               "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
+    "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
+               "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "outer": ("colour matrix from the outer product, r eqop a b^dagger",
+              "build_qla.pl:1730-1756, QLA_c_<eqop>_c_times_ca"),
     "sum": ("global sum of a field, r = sum over sites of a (double sum per real column; DF: float fields, "
             "double result)", "build_qla.pl:2449-2521, make_code_unary.pl:318-366"),
     "gamma_left": ("left multiplication by Gamma_mu = g0^b0 g1^b1 g2^b2 g3^b3 (bits b of mu = 0..15), r = Gamma_mu a",
@@ -1141,8 +1240,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums():
-        if isinstance(v, SumVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls():
+        if isinstance(v, FieldVariant):
+            key = (v.prec, "outer" if v.kind == "outer" else "field" + v.kind)
+        elif isinstance(v, SumVariant):
             key = (v.lib, "sum")
         elif isinstance(v, GammaVariant):
             key = (v.prec, "gamma_right" if v.right else "gamma_left")
@@ -1163,10 +1264,10 @@ def vecops_inc_text() -> str:
     lines = [
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
-        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry",
-        "// and sum_entry are there.",
+        "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
+        "// sum_entry, field_entry and outer_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums():
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

@@ -273,6 +273,52 @@ void sum_entry(int prec_in, int out_double, int ncr, int pa, void *r, const void
     record(e, name);
 }
 
+// One multiplication by a real (fcomplex 0) or complex factor field
+// (qla_vecops_abi.inc): r eqop f * b; xform bit 0 conjugates and bit 1
+// transposes b (its adjoint), fconj conjugates f.
+void field_entry(int prec, int fcomplex, int eqop, int ncomp, int xform, int fconj, int pf, int pb, void *r,
+                 const void *f, const void *b, const int *index, const int *ir, const int *ifa, const int *ib, int n,
+                 const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = b;
+    p.px = pb;
+    p.ix = ib;
+    p.f = f;
+    p.pf = pf;
+    p.ifa = ifa;
+    p.fconj = fconj;
+    p.index = index;
+    p.ir = ir;
+    p.n = n;
+    p.conj = xform & 1;
+    p.trans = (xform >> 1) & 1;
+    finish(qla_launch_vec(prec, fcomplex ? 11 : 10, eqop, ncomp, p, ts.stream), name);
+}
+
+// One colour outer product (qla_vecops_abi.inc): M eqop a b^dagger.
+void outer_entry(int prec, int eqop, int pa, int pb, void *r, const void *a, const void *b, const int *index,
+                 const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    finish(qla_launch_outer(prec, eqop, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -492,6 +538,13 @@ hipError_t qla_launch_local(int prec, int kind, int eqop, int ncomp, const qla_v
 {
     if (prec == 0) return qla_local_launch_p0(kind, eqop, ncomp, p, st);
     if (prec == 1) return qla_local_launch_p1(kind, eqop, ncomp, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
+    if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
     return hipErrorInvalidValue;
 }
 

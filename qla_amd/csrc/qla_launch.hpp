@@ -86,7 +86,8 @@ hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t 
 // records of ncomp complexes (3, 6, 9, 12 or 144).  op: 0 copy, 1 times i,
 // 2 times the real constant, 3 times the complex constant (eqop 0..3);
 // 4 x + y, 5 x - y, 6 s*x + y, 7 s*x - y with a real s, 8 / 9 the same with
-// a complex s (eqop 0 only).  conj conjugates and trans transposes (records
+// a complex s (eqop 0 only); 10 / 11 times a real / complex factor field
+// (eqop 0..3).  conj conjugates and trans transposes (records
 // of 9 or 144 complexes, as 3 x 3 or 12 x 12 matrices) the source x.  Step i
 // writes r at ir[i], index[i] (x-form) or i and reads x at ix[i], index[i] or
 // i, through the pointer array when px (likewise y).
@@ -100,10 +101,22 @@ struct qla_vec_args {
     int conj, trans;
     int al;                             // r, x and y 16-byte aligned (the contiguous v-form path)
     int wide;                           // n * ncomp >= 2^32 (set by the launcher)
+    // ops 10 / 11: a real / complex factor field, one value per step at
+    // ifa[i], index[i] or i, through the pointer array when pf; fconj
+    // conjugates the complex factor
+    const void *f;
+    const int *ifa;
+    int pf, fconj;
 };
 hipError_t qla_launch_vec(int prec, int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_vec_launch_p0(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st);
+
+// Colour outer product (qla_vecops.hip): M eqop x y^dagger for colour-vector
+// records x, y, addressed as in qla_vec_args.
+hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_outer_launch_p0(int eqop, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_outer_launch_p1(int eqop, const qla_vec_args &p, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

@@ -41,10 +41,17 @@ namespace {
 constexpr int VBLOCK = 256;
 
 enum { VOP_COPY = 0, VOP_I, VOP_RSCALE, VOP_CSCALE, VOP_PLUS, VOP_MINUS, VOP_R_PLUS, VOP_R_MINUS, VOP_C_PLUS,
-       VOP_C_MINUS };
+       VOP_C_MINUS, VOP_RFIELD, VOP_CFIELD };
 
-template <int OP> constexpr bool has_y() { return OP >= VOP_PLUS; }
-template <int OP> constexpr bool has_scalar() { return OP == VOP_RSCALE || OP == VOP_CSCALE || OP >= VOP_R_PLUS; }
+// times a per-site real / complex factor field (build_qla.pl:1517-1586): the
+// element arithmetic of the real / complex constant with the factor of the step
+template <int OP> constexpr bool is_field() { return OP == VOP_RFIELD || OP == VOP_CFIELD; }
+template <int OP> constexpr int elem_op() { return OP == VOP_RFIELD ? VOP_RSCALE : OP == VOP_CFIELD ? VOP_CSCALE : OP; }
+template <int OP> constexpr bool has_y() { return OP >= VOP_PLUS && !is_field<OP>(); }
+template <int OP> constexpr bool has_scalar()
+{
+    return OP == VOP_RSCALE || OP == VOP_CSCALE || (OP >= VOP_R_PLUS && OP <= VOP_C_MINUS);
+}
 template <int OP> constexpr bool complex_scalar() { return OP == VOP_CSCALE || OP == VOP_C_PLUS || OP == VOP_C_MINUS; }
 template <int EQOP> constexpr bool reads_r() { return EQOP == 1 || EQOP == 2; }
 
@@ -232,6 +239,7 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gather(qla_vec_args p)
     for (unsigned long long g0 = (unsigned long long)blockIdx.x * CHUNK + threadIdx.x; g0 < tot; g0 += stride) {
         V1 x[GITEMS], y[GITEMS], r[GITEMS];
         V1 *R[GITEMS];
+        T fr[GITEMS], fi[GITEMS];
 #pragma unroll
         for (int k = 0; k < GITEMS; k++) {
             const unsigned long long g = g0 + (unsigned long long)k * VBLOCK;
@@ -239,6 +247,8 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gather(qla_vec_args p)
             y[k] = V1{};
             r[k] = V1{};
             R[k] = nullptr;
+            fr[k] = sr;
+            fi[k] = si;
             if (g >= tot) continue;
             int i, c;
             if (p.wide) {
@@ -262,6 +272,13 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gather(qla_vec_args p)
                 const cpx<T> *Y = record_of<T, NC>(p.y, p.py, p.iy ? p.iy[i] : s);
                 y[k] = __builtin_nontemporal_load((const V1 *)(Y + c));
             }
+            if constexpr (is_field<OP>()) {
+                constexpr int FW = OP == VOP_CFIELD ? 2 : 1;
+                const int fs = p.ifa ? p.ifa[i] : s;
+                const T *F = p.pf ? ((const T *const *)p.f)[fs] : (const T *)p.f + (size_t)fs * FW;
+                fr[k] = F[0];
+                if constexpr (OP == VOP_CFIELD) fi[k] = p.fconj ? -F[1] : F[1];
+            }
             R[k] = (V1 *)((cpx<T> *)p.r + (size_t)sd * NC + c);
             if constexpr (reads_r<EQOP>()) r[k] = __builtin_nontemporal_load(R[k]);
         }
@@ -269,7 +286,7 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gather(qla_vec_args p)
         for (int k = 0; k < GITEMS; k++) {
             if (!R[k]) continue;
             T rr = r[k][0], ri = r[k][1];
-            vop<OP, EQOP, T>(rr, ri, x[k][0], p.conj ? -x[k][1] : x[k][1], y[k][0], y[k][1], sr, si);
+            vop<elem_op<OP>(), EQOP, T>(rr, ri, x[k][0], p.conj ? -x[k][1] : x[k][1], y[k][0], y[k][1], fr[k], fi[k]);
             r[k][0] = rr;
             r[k][1] = ri;
             __builtin_nontemporal_store(r[k], R[k]);
@@ -337,18 +354,21 @@ template <class T, int OP, int EQOP>
 hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
 {
     const bool flat = !p.index && !p.ir && !p.ix && !p.iy && !p.px && !p.py && !p.trans;
-    if (flat) {
-        const long long m = (long long)p.n * ncomp;
-        if constexpr (sizeof(T) == 4) {
-            if (p.al) {
-                hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 2>), dim3(blocks_for(((m + 1) / 2 + FITEMS - 1) / FITEMS)),
-                                   dim3(VBLOCK), 0, st, p, m);
-                return hipGetLastError();
+    if constexpr (!is_field<OP>()) {
+        if (flat) {
+            const long long m = (long long)p.n * ncomp;
+            if constexpr (sizeof(T) == 4) {
+                if (p.al) {
+                    hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 2>),
+                                       dim3(blocks_for(((m + 1) / 2 + FITEMS - 1) / FITEMS)), dim3(VBLOCK), 0, st, p,
+                                       m);
+                    return hipGetLastError();
+                }
             }
+            hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 1>), dim3(blocks_for((m + FITEMS - 1) / FITEMS)),
+                               dim3(VBLOCK), 0, st, p, m);
+            return hipGetLastError();
         }
-        hipLaunchKernelGGL((k_vec_flat<T, OP, EQOP, 1>), dim3(blocks_for((m + FITEMS - 1) / FITEMS)), dim3(VBLOCK),
-                           0, st, p, m);
-        return hipGetLastError();
     }
     qla_vec_args q = p;
     const long long tot = (long long)p.n * ncomp;
@@ -362,6 +382,8 @@ hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
             else hipLaunchKernelGGL((k_vec_tposed<T, 144, OP, EQOP>), grid, dim3(VBLOCK), 0, st, q);
             return hipGetLastError();
         }
+    }
+    if constexpr (OP <= VOP_CSCALE || is_field<OP>()) {
         if (p.trans) {
             const dim3 grid(blocks_for((tot + TRANS_ITEMS - 1) / TRANS_ITEMS));
             if (ncomp == 9) hipLaunchKernelGGL((k_vec_gather<T, 9, OP, EQOP, TRANS_ITEMS>), grid, dim3(VBLOCK), 0, st, q);
@@ -388,7 +410,7 @@ hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
 template <class T, int OP>
 hipError_t launch_eqop(int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
-    if constexpr (OP >= VOP_PLUS) {
+    if constexpr (OP >= VOP_PLUS && !is_field<OP>()) {
         if (eqop != 0) return hipErrorInvalidValue;        // sums and a*b +- c: eq only
         return launch_op<T, OP, 0>(ncomp, p, st);
     } else {
@@ -418,8 +440,51 @@ hipError_t launch_t(int op, int eqop, int ncomp, const qla_vec_args &p, hipStrea
     case VOP_R_MINUS: return launch_eqop<T, VOP_R_MINUS>(eqop, ncomp, p, st);
     case VOP_C_PLUS: return launch_eqop<T, VOP_C_PLUS>(eqop, ncomp, p, st);
     case VOP_C_MINUS: return launch_eqop<T, VOP_C_MINUS>(eqop, ncomp, p, st);
+    case VOP_RFIELD: return launch_eqop<T, VOP_RFIELD>(eqop, ncomp, p, st);
+    case VOP_CFIELD: return launch_eqop<T, VOP_CFIELD>(eqop, ncomp, p, st);
     }
     return hipErrorInvalidValue;
+}
+
+// Colour outer product (build_qla.pl:1730-1756): destination complex (ic, jc)
+// of step i is QLA_c_<eqop>_c_times_ca(r, x[ic], y[jc]), computed as the
+// plain product with the conjugated y element (bit-identical, as above).
+template <class T, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_outer(qla_vec_args p)
+{
+    const long long tot = (long long)p.n * 9;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        const int i = (int)(g / 9), c = (int)(g - (long long)i * 9);
+        const int s = p.index ? p.index[i] : i;
+        const int sd = p.ir ? p.ir[i] : s;
+        const cpx<T> a = record_of<T, 3>(p.x, p.px, p.ix ? p.ix[i] : s)[c / 3];
+        const cpx<T> b = record_of<T, 3>(p.y, p.py, p.iy ? p.iy[i] : s)[c % 3];
+        cpx<T> *R = (cpx<T> *)p.r + (size_t)sd * 9 + c;
+        T rr = T(0), ri = T(0);
+        if constexpr (reads_r<EQOP>()) {
+            rr = R->re;
+            ri = R->im;
+        }
+        vop<VOP_CSCALE, EQOP, T>(rr, ri, b.re, -b.im, T(0), T(0), a.re, a.im);
+        R->re = rr;
+        R->im = ri;
+    }
+}
+
+template <class T>
+hipError_t launch_outer_t(int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const dim3 grid(blocks_for((long long)p.n * 9));
+    switch (eqop) {
+    case 0: hipLaunchKernelGGL((k_vec_outer<T, 0>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((k_vec_outer<T, 1>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_vec_outer<T, 2>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_vec_outer<T, 3>), grid, dim3(VBLOCK), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -891,6 +956,10 @@ hipError_t qla_local_launch_p0(int kind, int eqop, int ncomp, const qla_vec_args
 {
     return qla::launch_local_t<float>(kind, eqop, ncomp, p, st);
 }
+hipError_t qla_outer_launch_p0(int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_outer_t<float>(eqop, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -907,5 +976,9 @@ hipError_t qla_gamma_launch_p1(int right, int g, int ncomp, const qla_vec_args &
 hipError_t qla_local_launch_p1(int kind, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_local_t<double>(kind, eqop, ncomp, p, st);
+}
+hipError_t qla_outer_launch_p1(int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_outer_t<double>(eqop, p, st);
 }
 #endif

@@ -17,7 +17,7 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fills, gammas, locals_, sums, vecops
+from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fieldmuls, fills, gammas, locals_, sums, vecops
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -386,3 +386,66 @@ def test_sum_device_result_and_empty(orc):
     qla_amd.call("QLA_F3_d_veq_sum_D", h.ctypes.data, dev(a), 0)
     qla_amd.check()
     assert not np.any(h)
+
+
+_factors = {}
+
+
+def factors(prec):
+    """real and complex factor fields of N sites with device copies and pointer arrays"""
+    if prec not in _factors:
+        dt = np.float64 if prec == "D3" else np.float32
+        rng = np.random.default_rng(zlib.crc32(f"factors {prec}".encode()))
+        fr = rng.standard_normal(N).astype(dt)
+        fc = rng.standard_normal((N, 2)).astype(dt)
+        perm = rng.permutation(N)
+        dfr, dfc = dev(fr), dev(fc)
+        _factors[prec] = {"R": (fr, dfr, qla_amd.pointer_array(dfr.view(N, 1), perm)),
+                          "C": (fc, dfc, qla_amd.pointer_array(dfc, perm)), "perm": perm}
+    return _factors[prec]
+
+
+@pytest.mark.parametrize("v", fieldmuls(), ids=lambda v: v.name)
+def test_fieldmul_symbol(orc, v):
+    """Every multiplication by a real or complex field and every outer product,
+    bit for bit against the oracle (untouched records included)."""
+    cs = case(v.prec, v.t)
+    if v.kind == "outer":
+        vc = case(v.prec, "V")
+        a_h, a_d, a_p, a_perm = vc.x, vc.dx, vc.px, vc.x_idx
+        b_h, b_d, b_p, b_perm = vc.y, vc.dy, vc.py, vc.y_idx
+    else:
+        fz = factors(v.prec)
+        a_h, a_d, a_p = fz[v.kind]
+        a_perm = fz["perm"]
+        b_h, b_d, b_p, b_perm = cs.x, cs.dx, cs.px, cs.x_idx
+    r = dev(cs.r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(a_p if v.pa else a_d)
+    if v.ia:
+        args.append(cs.down["x"])
+    args.append(b_p if v.pb else b_d)
+    if v.ib:
+        args.append(cs.down["y"])
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if v.own_index else cs.n)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    as_ = cs.own["x"] if v.ia else site
+    bs = cs.own["y"] if v.ib else site
+    ga = np.ascontiguousarray(a_h[a_perm[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(b_h[b_perm[bs] if v.pb else bs])
+    sub = np.ascontiguousarray(cs.r0[rs])
+    if v.kind == "outer":
+        orc.outer(v.eqop, sub, ga, gb)
+    else:
+        orc.fieldmul(v.eqop, sub, ga, gb, fa=v.fa, cb=v.adj, dim=TDIM[v.t] if v.adj else 0)
+    exp = cs.r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)

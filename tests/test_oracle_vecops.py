@@ -604,3 +604,66 @@ def test_field_sum(orc, prec, t):
     perm = rng.permutation(300)[:50]
     np.testing.assert_array_equal(orc.field_sum(a, a_idx=perm, n=50), a[perm].astype(np.int64).sum(axis=0))
     assert not np.any(orc.field_sum(a, n=0))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("t,adj", SCALE_FORMS)
+def test_real_field_times_vs_numpy(orc, prec, eqop, t, adj):
+    """r eqop f(x) b(x) for a real field f: numpy's products, one rounding each."""
+    dt = DT[prec]
+    n = 19
+    b = field(t, n, 100, dt)
+    r0 = field(t, n, 101, dt)
+    f = np.random.default_rng(102).standard_normal(n).astype(dt)
+    r = r0.copy()
+    dim = orc.TRANSPOSE_DIM[t] if adj else 0
+    orc.fieldmul(eqop, r, f, b, cb=adj, dim=dim)
+    c = cx(b)
+    if adj:
+        c = np.conj(transpose(t, c))
+    src = re_im(c, dt)
+    fb = f.reshape((n,) + (1,) * (src.ndim - 1)) * src
+    np.testing.assert_array_equal(r, eqop_np(eqop, r0, fb))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("fa", [False, True])
+@pytest.mark.parametrize("t,adj", SCALE_FORMS)
+def test_complex_field_times_exact(orc, prec, eqop, fa, t, adj):
+    """r eqop (conj) f(x) * (adjoint) b(x) on small integers equals the
+    complex128 product."""
+    dt = DT[prec]
+    n = 17
+    b = _int_field(t, n, 103, dt)
+    r0 = _int_field(t, n, 104, dt)
+    f = np.random.default_rng(105).integers(-9, 10, size=(n, 2)).astype(dt)
+    r = r0.copy()
+    dim = orc.TRANSPOSE_DIM[t] if adj else 0
+    orc.fieldmul(eqop, r, f, b, fa=fa, cb=adj, dim=dim)
+    c = cx(b.astype(np.float64))
+    if adj:
+        c = np.conj(transpose(t, c))
+    fz = f[:, 0].astype(np.float64) + 1j * f[:, 1]
+    if fa:
+        fz = np.conj(fz)
+    y = fz.reshape((n,) + (1,) * (c.ndim - 1)) * c
+    c0 = cx(r0.astype(np.float64))
+    want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("eqop", range(4))
+def test_outer_product_exact(orc, prec, eqop):
+    dt = DT[prec]
+    a = _int_field("V", 13, 106, dt)
+    b = _int_field("V", 13, 107, dt)
+    r0 = _int_field("M", 13, 108, dt)
+    r = r0.copy()
+    orc.outer(eqop, r, a, b)
+    y = cx(a.astype(np.float64))[:, :, None] * np.conj(cx(b.astype(np.float64)))[:, None, :]
+    c0 = cx(r0.astype(np.float64))
+    want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
