@@ -61,6 +61,7 @@ def load():
         getattr(lib, f"orc_{p}_sum").argtypes = [i32, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_fieldmul").argtypes = [i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_outer").argtypes = [i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_pmul").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -441,4 +442,25 @@ def outer(eqop, r, a, b, index=None, a_idx=None, b_idx=None, ir=None, ia=None, i
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_outer")(eqop, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
+    return r
+
+
+PM_MP, PM_PM, PM_PP = 0, 1, 2
+
+
+def pmul(kind, eqop, r, a, b, ca=False, cb=False, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=None,
+         n=None):
+    """P eqop A * B (orc_*_pmul): PM_MP A colour matrices, B propagators;
+    PM_PM A propagators, B colour matrices; PM_PP both propagators; ca / cb
+    take the adjoint of A / B."""
+    lib = load()
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_pmul")(kind, eqop, int(ca), int(cb), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
+                                  _ptr(ix), n)
     return r

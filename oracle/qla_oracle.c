@@ -159,6 +159,13 @@ void orc_D_outer(int eqop, double *r, const int *rindex, const double *const *pa
                  const int *index, int n)
 { D_orc_outer(eqop, r, rindex, pa, pb, index, n); }
 
+void orc_F_pmul(int kind, int eqop, int ca, int cb, float *r, const int *rindex, const float *const *pa,
+                const float *const *pb, const int *index, int n)
+{ F_orc_pmul(kind, eqop, ca, cb, r, rindex, pa, pb, index, n); }
+void orc_D_pmul(int kind, int eqop, int ca, int cb, double *r, const int *rindex, const double *const *pa,
+                const double *const *pb, const int *index, int n)
+{ D_orc_pmul(kind, eqop, ca, cb, r, rindex, pa, pb, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

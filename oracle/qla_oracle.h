@@ -149,6 +149,13 @@ void orc_F_outer(int eqop, float *r, const int *rindex, const float *const *pa, 
 void orc_D_outer(int eqop, double *r, const int *rindex, const double *const *pa, const double *const *pb,
                  const int *index, int n);
 
+/* products with propagators (qla_oracle_vecops.inc): kind 0 P eqop M*P,
+ * 1 P eqop P*M, 2 P eqop P*P; ca / cb the adjoint of either factor */
+void orc_F_pmul(int kind, int eqop, int ca, int cb, float *r, const int *rindex, const float *const *pa,
+                const float *const *pb, const int *index, int n);
+void orc_D_pmul(int kind, int eqop, int ca, int cb, double *r, const int *rindex, const double *const *pa,
+                const double *const *pb, const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

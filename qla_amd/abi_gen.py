@@ -881,6 +881,82 @@ def fieldmuls() -> list[FieldVariant]:
     return out
 
 
+# propagator products: kind -> (first, second source record types)
+PMUL_KINDS = {0: ("M", "P"), 1: ("P", "M"), 2: ("P", "P")}
+
+
+@dataclass(frozen=True)
+class PmulVariant:
+    """P eqop a * b with a colour matrix times a propagator (build_qla.pl:
+    1587-1620), a propagator times a colour matrix or two propagators
+    (:1622-1662); either factor may be its adjoint; every binary form."""
+    prec: str
+    kind: int
+    eqop: int
+    ca: bool = False
+    cb: bool = False
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def own_index(self) -> bool:
+        return self.ir or self.ia or self.ib
+
+    @property
+    def name(self) -> str:
+        form = "" if self.own_index else ("x" if self.xf else "v")
+        ta, tb = PMUL_KINDS[self.kind]
+        pre_a = ("p" if self.pa else "") + ("x" if self.ia else "")
+        pre_b = ("p" if self.pb else "") + ("x" if self.ib else "")
+        return (f"QLA_{self.prec}_{'x' if self.ir else ''}P_{form}{EQOPS[self.eqop]}_{pre_a}{ta}{'a' if self.ca else ''}"
+                f"_times_{pre_b}{tb}{'a' if self.cb else ''}")
+
+    def params(self) -> list[tuple[str, str]]:
+        ta, tb = PMUL_KINDS[self.kind]
+        out = [("p", f"QLA_{self.prec}_DiracPropagator *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"QLA_{self.prec}_{TYPE_NAMES[ta]} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out.append(("p", f"QLA_{self.prec}_{TYPE_NAMES[tb]} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+        if self.ib:
+            out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"pmul_entry({prec}, {self.kind}, {self.eqop}, {int(self.ca)}, {int(self.cb)}, {int(self.pa)}, "
+                f"{int(self.pb)}, (void *)r, (const void *)a, (const void *)b, {idx}, {own}, n, \"{self.name}\");")
+
+
+def pmuls() -> list[PmulVariant]:
+    out = []
+    for prec in PRECS:
+        for kind in PMUL_KINDS:
+            for ca in (False, True):
+                for cb in (False, True):
+                    for e in range(4):
+                        kw = dict(prec=prec, kind=kind, eqop=e, ca=ca, cb=cb)
+                        out += [PmulVariant(xf=xf, pa=pa, pb=pb, **kw) for xf in (False, True)
+                                for pa in (False, True) for pb in (False, True)]
+                        out += [PmulVariant(ir=ir, ia=ia, ib=ib, **kw) for ir, ia, ib in OWN_INDEX_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -919,9 +995,9 @@ def conversions() -> list[ConvVariant]:
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
-    by real and complex fields, outer products)."""
+    by real and complex fields, outer products, products with propagators)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls())
+            + sums() + fieldmuls() + pmuls())
 
 
 def variants() -> list[Variant]:
@@ -1204,6 +1280,13 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "pmul0": ("propagator from a colour matrix times a propagator, r eqop a b (either factor's adjoint: "
+              "`Ma`, `Pa`; the matrix acts on colour only)",
+              "build_qla.pl:1587-1620, expressions_tensor.pl:544-630"),
+    "pmul1": ("propagator from a propagator times a colour matrix, r eqop a b (`Pa`, `Ma`: adjoints)",
+              "build_qla.pl:1622-1662, expressions_tensor.pl:544-630"),
+    "pmul2": ("propagator from a product of propagators, r eqop a b as 12 x 12 matrices over (colour, spin) "
+              "(`Pa`: adjoints)", "build_qla.pl:1622-1662, expressions_tensor.pl:544-630"),
     "outer": ("colour matrix from the outer product, r eqop a b^dagger",
               "build_qla.pl:1730-1756, QLA_c_<eqop>_c_times_ca"),
     "sum": ("global sum of a field, r = sum over sites of a (double sum per real column; DF: float fields, "
@@ -1240,8 +1323,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls():
-        if isinstance(v, FieldVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls():
+        if isinstance(v, PmulVariant):
+            key = (v.prec, "pmul" + str(v.kind))
+        elif isinstance(v, FieldVariant):
             key = (v.prec, "outer" if v.kind == "outer" else "field" + v.kind)
         elif isinstance(v, SumVariant):
             key = (v.lib, "sum")
@@ -1265,9 +1350,9 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry and outer_entry are there.",
+        "// sum_entry, field_entry, outer_entry and pmul_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls():
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

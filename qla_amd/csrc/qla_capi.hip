@@ -319,6 +319,27 @@ void outer_entry(int prec, int eqop, int pa, int pb, void *r, const void *a, con
     finish(qla_launch_outer(prec, eqop, p, ts.stream), name);
 }
 
+// products with propagators: P eqop A * B (kind 0 M*P, 1 P*M, 2 P*P), ca / cb
+// the adjoint of A / B; every binary form
+void pmul_entry(int prec, int kind, int eqop, int ca, int cb, int pa, int pb, void *r, const void *a,
+                const void *b, const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    finish(qla_launch_pmul(prec, kind, eqop, ca, cb, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -545,6 +566,13 @@ hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream
 {
     if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
     if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_pmul(int prec, int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_pmul_launch_p0(kind, eqop, ca, cb, p, st);
+    if (prec == 1) return qla_pmul_launch_p1(kind, eqop, ca, cb, p, st);
     return hipErrorInvalidValue;
 }
 

@@ -117,6 +117,11 @@ hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p,
 hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_outer_launch_p0(int eqop, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_outer_launch_p1(int eqop, const qla_vec_args &p, hipStream_t st);
+// Products with propagators (qla_vecops.hip): P eqop A * B, kind 0 M*P,
+// 1 P*M, 2 P*P; ca / cb the adjoint of A / B
+hipError_t qla_launch_pmul(int prec, int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_pmul_launch_p0(int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_pmul_launch_p1(int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

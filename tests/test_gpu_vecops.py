@@ -17,7 +17,8 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import LOCALS, NCOMP, VECFAMS, fieldmuls, fills, gammas, locals_, sums, vecops
+from qla_amd.abi_gen import (LOCALS, NCOMP, PMUL_KINDS, VECFAMS, fieldmuls, fills, gammas, locals_, pmuls, sums,
+                             vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -449,3 +450,55 @@ def test_fieldmul_symbol(orc, v):
     exp = cs.r0.copy()
     exp[rs] = sub
     assert_bitwise(host(r), exp, v.name)
+
+
+@pytest.mark.parametrize("v", pmuls(), ids=lambda v: v.name)
+def test_pmul_symbol(orc, v):
+    """Every product with a propagator (M*P, P*M, P*P, either factor's
+    adjoint), bit for bit against the oracle (untouched records included)."""
+    cs = case(v.prec, "P")
+    ta, tb = PMUL_KINDS[v.kind]
+    ca_, cb_ = case(v.prec, ta), case(v.prec, tb)
+    a_h, a_d, a_p, a_perm = ca_.x, ca_.dx, ca_.px, ca_.x_idx
+    b_h, b_d, b_p, b_perm = cb_.y, cb_.dy, cb_.py, cb_.y_idx
+    r = dev(cs.r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(a_p if v.pa else a_d)
+    if v.ia:
+        args.append(cs.down["x"])
+    args.append(b_p if v.pb else b_d)
+    if v.ib:
+        args.append(cs.down["y"])
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if v.own_index else cs.n)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    as_ = cs.own["x"] if v.ia else site
+    bs = cs.own["y"] if v.ib else site
+    ga = np.ascontiguousarray(a_h[a_perm[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(b_h[b_perm[bs] if v.pb else bs])
+    sub = np.ascontiguousarray(cs.r0[rs])
+    orc.pmul(v.kind, v.eqop, sub, ga, gb, ca=v.ca, cb=v.cb)
+    exp = cs.r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)
+
+
+def test_propagator_products_large(orc):
+    """P = P * P^dagger over 2^16 sites (4096 block steps) against the oracle, bit for bit."""
+    n = 1 << 16
+    a = lattice.random_field("P", n, 71, dtype=np.float32)
+    b = lattice.random_field("P", n, 72, dtype=np.float32)
+    r = dev(np.zeros_like(a))
+    say(f"QLA_F3_P_veq_P_times_Pa n={n}")
+    qla_amd.call("QLA_F3_P_veq_P_times_Pa", r, dev(a), dev(b), n)
+    qla_amd.check()
+    exp = np.zeros_like(a)
+    orc.pmul(2, 0, exp, a, b, cb=True)
+    assert_bitwise(host(r), exp, "QLA_F3_P_veq_P_times_Pa")

@@ -667,3 +667,58 @@ def test_outer_product_exact(orc, prec, eqop):
     c0 = cx(r0.astype(np.float64))
     want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
     np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
+
+
+def _as12(c):
+    """(n, 3, 4, 3, 4) complex propagators as (n, 12, 12) matrices over (colour, spin)"""
+    return c.reshape(c.shape[0], 12, 12)
+
+
+def _embed_m(m):
+    """colour matrix acting on colour only: M (x) 1_spin as a 12 x 12 matrix"""
+    return np.einsum("nab,st->nasbt", m, np.eye(4)).reshape(m.shape[0], 12, 12)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+@pytest.mark.parametrize("eqop", range(4))
+@pytest.mark.parametrize("ca,cb", [(False, False), (True, False), (False, True), (True, True)])
+def test_propagator_products_exact(orc, prec, kind, eqop, ca, cb):
+    """P eqop A*B for M*P, P*M and P*P, with either factor's adjoint, exactly
+    against complex128 products of the 12 x 12 matrices (colour matrices as
+    M (x) 1 over spin) on small integers."""
+    dt = DT[prec]
+    n = 5
+    ta, tb = {0: ("M", "P"), 1: ("P", "M"), 2: ("P", "P")}[kind]
+    a = _int_field(ta, n, 110 + kind, dt)
+    b = _int_field(tb, n, 120 + kind, dt)
+    r0 = _int_field("P", n, 130, dt)
+    r = r0.copy()
+    orc.pmul(kind, eqop, r, a, b, ca=ca, cb=cb)
+    A = cx(a.astype(np.float64))
+    B = cx(b.astype(np.float64))
+    A = _embed_m(A) if ta == "M" else _as12(A)
+    B = _embed_m(B) if tb == "M" else _as12(B)
+    if ca:
+        A = np.conj(np.swapaxes(A, 1, 2))
+    if cb:
+        B = np.conj(np.swapaxes(B, 1, 2))
+    y = np.einsum("nij,njk->nik", A, B)
+    c0 = _as12(cx(r0.astype(np.float64)))
+    want = {0: y, 1: c0 + y, 2: c0 - y, 3: -y}[eqop]
+    np.testing.assert_array_equal(_as12(cx(r.astype(np.float64))), want)
+
+
+def test_propagator_product_matches_colour_products(orc):
+    """M*P for each fixed (is, js) spin pair is the colour-matrix product of M
+    with that 3 x 3 colour block, bit for bit (the same kc order and macros)."""
+    m = field("M", 6, 140, np.float32)
+    pr = field("P", 6, 141, np.float32)
+    r = np.zeros_like(pr)
+    orc.pmul(0, 0, r, m, pr)
+    for is_ in range(4):
+        for js in range(4):
+            blk = np.ascontiguousarray(pr[:, :, is_, :, js])
+            want = np.zeros_like(blk)
+            orc.site(orc.MAT_M, 0, want, m, blk)
+            np.testing.assert_array_equal(r[:, :, is_, :, js], want)
