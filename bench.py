@@ -345,7 +345,9 @@ def vecops_layer(lib, stream, n, sets, args):
     fields: x += a p, r = a p + r', r = z p - r', the copy, the even-site
     x-form update, the FP64 update; the Hermitian conjugate of the links
     and the P transpose on 2^20 propagators; the products P*P and M*P on
-    2^20 propagators; gamma_5 D; the site-local
+    2^20 propagators; the spin trace of 2^20 propagators (every 128-byte
+    line of a record holds a spin-diagonal element, so the whole record is
+    counted); gamma_5 D; the site-local
     norm2 and dot of D, the anti-Hermitian part of the links and the global
     sum of a D field (DF3, result in device memory).  Algorithmic bytes per
     site: every record read once, the destination written (and read for
@@ -392,6 +394,8 @@ def vecops_layer(lib, stream, n, sets, args):
                                                                   prop2.data_ptr(), p_n), 3456, p_n),
         "P_veq_M_times_P": (lambda it: lib.QLA_F3_P_veq_M_times_P(prop_t.data_ptr(), sets[0][0].data_ptr(),
                                                                   prop.data_ptr(), p_n), 2376, p_n),
+        "M_veq_spintrace_P": (lambda it: lib.QLA_F3_M_veq_spintrace_P(madj.data_ptr(), prop.data_ptr(), p_n), 1224,
+                              p_n),
         "D_veq_gamma_times_D": (lambda it: lib.QLA_F3_D_veq_gamma_times_D(y.data_ptr(), p.data_ptr(), 15, n), 192, n),
         "R_veq_norm2_D": (lambda it: lib.QLA_F3_R_veq_norm2_D(rl.data_ptr(), p.data_ptr(), n), 100, n),
         "C_veq_D_dot_D": (lambda it: lib.QLA_F3_C_veq_D_dot_D(cl.data_ptr(), p.data_ptr(), q.data_ptr(), n), 200, n),

@@ -62,6 +62,7 @@ def load():
         getattr(lib, f"orc_{p}_fieldmul").argtypes = [i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_outer").argtypes = [i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_pmul").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_comp").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -463,4 +464,24 @@ def pmul(kind, eqop, r, a, b, ca=False, cb=False, index=None, a_idx=None, b_idx=
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_pmul")(kind, eqop, int(ca), int(cb), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb),
                                   _ptr(ix), n)
+    return r
+
+
+GET_ELEM, SET_ELEM, GET_COLORVEC, SET_COLORVEC, GET_DIRACVEC, SET_DIRACVEC, SPINTRACE = range(7)
+COMP_TYPES = {"V": 0, "H": 1, "D": 2, "M": 3, "P": 4}
+
+
+def comp(kind, t, extra, r, a, index=None, a_idx=None, ir=None, ia=None, n=None):
+    """component access and the spin trace (orc_*_comp): t the array type
+    (V, H, D, M, P), extra the reference's extra arguments in order."""
+    lib = load()
+    pa = _records(a, ia if ia is not None else a_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else a.shape[0])
+    e = np.zeros(4, dtype=np.int32)
+    e[:len(extra)] = extra
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_comp")(kind, COMP_TYPES[t], _ptr(e), _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(ix), n)
     return r

@@ -166,6 +166,13 @@ void orc_D_pmul(int kind, int eqop, int ca, int cb, double *r, const int *rindex
                 const double *const *pb, const int *index, int n)
 { D_orc_pmul(kind, eqop, ca, cb, r, rindex, pa, pb, index, n); }
 
+void orc_F_comp(int kind, int t, const int *e, float *r, const int *rindex, const float *const *pa, const int *index,
+                int n)
+{ F_orc_comp(kind, t, e, r, rindex, pa, index, n); }
+void orc_D_comp(int kind, int t, const int *e, double *r, const int *rindex, const double *const *pa,
+                const int *index, int n)
+{ D_orc_comp(kind, t, e, r, rindex, pa, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -156,6 +156,15 @@ void orc_F_pmul(int kind, int eqop, int ca, int cb, float *r, const int *rindex,
 void orc_D_pmul(int kind, int eqop, int ca, int cb, double *r, const int *rindex, const double *const *pa,
                 const double *const *pb, const int *index, int n);
 
+/* component access and the spin trace (qla_oracle_vecops.inc) */
+enum { ORC_T_V = 0, ORC_T_H = 1, ORC_T_D = 2, ORC_T_M = 3, ORC_T_P = 4 };
+enum { ORC_GET_ELEM = 0, ORC_SET_ELEM = 1, ORC_GET_COLORVEC = 2, ORC_SET_COLORVEC = 3, ORC_GET_DIRACVEC = 4,
+       ORC_SET_DIRACVEC = 5, ORC_SPINTRACE = 6 };
+void orc_F_comp(int kind, int t, const int *e, float *r, const int *rindex, const float *const *pa, const int *index,
+                int n);
+void orc_D_comp(int kind, int t, const int *e, double *r, const int *rindex, const double *const *pa,
+                const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -957,6 +957,96 @@ def pmuls() -> list[PmulVariant]:
     return out
 
 
+# component access: qualifier -> (moved part, array types, kind code of the get form)
+COMP_FAMS = {"elem": ("C", ("M", "H", "D", "V", "P"), 0), "colorvec": ("V", ("M", "H", "D", "P"), 2),
+             "diracvec": ("D", ("P",), 4), "spintrace": ("M", ("P",), 6)}
+COMP_TCODE = {"V": 0, "H": 1, "D": 2, "M": 3, "P": 4}
+# the reference's extra arguments (build_qla.pl:786-790, 841-845, 897-899)
+ELEM_ARGS = {"V": ("ic",), "H": ("ic", "is"), "D": ("ic", "is"), "M": ("ic", "jc"), "P": ("ic", "is", "jc", "js")}
+CVEC_ARGS = {"M": ("jc",), "H": ("is",), "D": ("is",), "P": ("is", "jc", "js")}
+
+
+@dataclass(frozen=True)
+class CompVariant:
+    """Component access, eq only, every unary indexing form: an element
+    (C <-> T; build_qla.pl:767-820), a colour column (V <-> M, H, D, P;
+    :822-876), a Dirac column (D <-> P; :878-932), and the spin trace
+    M = spintrace(P) (:1066-1086)."""
+    prec: str
+    qual: str
+    t: str          # the array type
+    get: bool
+    xf: bool = False
+    pa: bool = False
+    ir: bool = False
+    ia: bool = False
+
+    @property
+    def extra(self) -> tuple[str, ...]:
+        if self.qual == "elem":
+            return ELEM_ARGS[self.t]
+        if self.qual == "colorvec":
+            return CVEC_ARGS[self.t]
+        if self.qual == "diracvec":
+            return ("jc", "js")
+        return ()
+
+    @property
+    def types(self) -> tuple[str, str]:
+        part = COMP_FAMS[self.qual][0]
+        return (part, self.t) if self.get else (self.t, part)
+
+    @property
+    def name(self) -> str:
+        form = "" if (self.ir or self.ia) else ("x" if self.xf else "v")
+        d, s = self.types
+        return (f"QLA_{self.prec}_{'x' if self.ir else ''}{d}_{form}eq_{self.qual}_{'p' if self.pa else ''}"
+                f"{'x' if self.ia else ''}{s}")
+
+    def _ctype(self, x: str) -> str:
+        if x == "C":
+            return f"QLA_{'D' if self.prec == 'D3' else 'F'}_Complex"
+        return f"QLA_{self.prec}_{TYPE_NAMES[x]}"
+
+    def params(self) -> list[tuple[str, str]]:
+        d, s = self.types
+        out = [("p", f"{self._ctype(d)} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{self._ctype(s)} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out += [("i", f"int {x}") for x in self.extra]
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        kind = COMP_FAMS[self.qual][2] + (0 if self.get else 1)
+        e = list(self.extra) + ["0"] * (4 - len(self.extra))
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia)))
+        return (f"comp_entry({prec}, {kind}, {COMP_TCODE[self.t]}, {', '.join(e)}, {int(self.pa)}, (void *)r, "
+                f"(const void *)a, {idx}, {own}, n, \"{self.name}\");")
+
+
+def comps() -> list[CompVariant]:
+    out = []
+    for prec in PRECS:
+        for qual, (_, types, _) in COMP_FAMS.items():
+            for t in types:
+                for get in ((True,) if qual == "spintrace" else (True, False)):
+                    kw = dict(prec=prec, qual=qual, t=t, get=get)
+                    out += [CompVariant(xf=xf, pa=pa, **kw) for xf in (False, True) for pa in (False, True)]
+                    out += [CompVariant(ir=ir, ia=ia, **kw) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -995,9 +1085,9 @@ def conversions() -> list[ConvVariant]:
 def all_symbols() -> list:
     """Every generated symbol (site family, reductions, conversions, matrix functions,
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
-    by real and complex fields, outer products, products with propagators)."""
+    by real and complex fields, outer products, products with propagators, component access, spin trace)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls() + pmuls())
+            + sums() + fieldmuls() + pmuls() + comps())
 
 
 def variants() -> list[Variant]:
@@ -1280,6 +1370,15 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "comp_elem": ("element access, C = a(ic, ...) and the insertion r(ic, ...) = a (eq; the indices "
+                  "the type has: V ic; H, D ic, is; M ic, jc; P ic, is, jc, js; out of range: an error, "
+                  "nothing written)", "build_qla.pl:767-820, expressions_tensor.pl:297-347"),
+    "comp_colorvec": ("colour-column access, V = a(., fixed) and its insertion (M: jc; H, D: is; "
+                      "P: is, jc, js)", "build_qla.pl:822-876, expressions_tensor.pl:297-347"),
+    "comp_diracvec": ("Dirac-column access, D = P(., ., jc, js) and its insertion",
+                      "build_qla.pl:878-932, expressions_tensor.pl:297-347"),
+    "comp_spintrace": ("spin trace, M(ic, jc) = sum over is of P(ic, is, jc, is) from 0 in is order",
+                       "build_qla.pl:1066-1086, expressions_tensor.pl:394-430"),
     "pmul0": ("propagator from a colour matrix times a propagator, r eqop a b (either factor's adjoint: "
               "`Ma`, `Pa`; the matrix acts on colour only)",
               "build_qla.pl:1587-1620, expressions_tensor.pl:544-630"),
@@ -1323,8 +1422,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls():
-        if isinstance(v, PmulVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps():
+        if isinstance(v, CompVariant):
+            key = (v.prec, "comp_" + v.qual)
+        elif isinstance(v, PmulVariant):
             key = (v.prec, "pmul" + str(v.kind))
         elif isinstance(v, FieldVariant):
             key = (v.prec, "outer" if v.kind == "outer" else "field" + v.kind)
@@ -1350,9 +1451,9 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry, outer_entry and pmul_entry are there.",
+        "// sum_entry, field_entry, outer_entry, pmul_entry and comp_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls():
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

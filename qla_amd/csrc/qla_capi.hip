@@ -340,6 +340,94 @@ void pmul_entry(int prec, int kind, int eqop, int ca, int cb, int pa, int pb, vo
     finish(qla_launch_pmul(prec, kind, eqop, ca, cb, p, ts.stream), name);
 }
 
+// component access and the spin trace.  kind: 0 / 1 get / set an element,
+// 2 / 3 a colour column, 4 / 5 a Dirac column, 6 the spin trace; t the array
+// type (0 V, 1 H, 2 D, 3 M, 4 P); e0.. the reference's extra arguments in
+// order (ic, is, jc, js, those the type has; a colour column fixes jc of M,
+// is of H and D, (is, jc, js) of P; a Dirac column (jc, js)).  An index
+// outside its range records hipErrorInvalidValue and writes nothing (the
+// reference does not check them).
+void comp_entry(int prec, int kind, int t, int e0, int e1, int e2, int e3, int pa, void *r, const void *a,
+                const int *index, const int *ir, const int *ia, int n, const char *name)
+{
+    static const int rec_size[5] = { 3, 6, 12, 9, 144 };
+    // position of (ic, is, jc, js) in a record: V [ic], H [ic][2], D [ic][4], M [ic][jc], P [ic][is][jc][js]
+    auto pos = [t](int ic, int is, int jc, int js) {
+        switch (t) {
+        case 0: return ic;
+        case 1: return ic * 2 + is;
+        case 2: return ic * 4 + is;
+        case 3: return ic * 3 + jc;
+        default: return ((ic * 4 + is) * 3 + jc) * 4 + js;
+        }
+    };
+    if (t < 0 || t > 4 || kind < 0 || kind > 6) {
+        record(hipErrorInvalidValue, name);
+        return;
+    }
+    const int e[4] = { e0, e1, e2, e3 };
+    int ic = 0, is = 0, jc = 0, js = 0;
+    if (kind <= 1) {
+        int k = 0;
+        ic = e[k++];
+        if (t == 1 || t == 2 || t == 4) is = e[k++];
+        if (t == 3 || t == 4) jc = e[k++];
+        if (t == 4) js = e[k++];
+    } else if (kind <= 3) {
+        if (t == 3) jc = e[0];
+        else if (t == 1 || t == 2) is = e[0];
+        else { is = e[0]; jc = e[1]; js = e[2]; }
+    } else if (kind <= 5) {
+        jc = e[0];
+        js = e[1];
+    }
+    const int ns = t == 1 ? 2 : 4;
+    if (ic < 0 || ic >= 3 || is < 0 || is >= ns || jc < 0 || jc >= 3 || js < 0 || js >= 4) {
+        record(hipErrorInvalidValue, name);
+        return;
+    }
+    if (n <= 0) return;
+    qla_comp_map m;
+    memset(&m, 0, sizeof(m));
+    int mode = 2;
+    if (kind == 6) {
+        m.nd = 9;
+        m.nsrc = 144;
+        m.ndst = 9;
+    } else {
+        const bool get = (kind & 1) == 0;
+        mode = get ? 0 : 1;
+        if (kind <= 1) {
+            m.nd = 1;
+            m.off = pos(ic, is, jc, js);
+            m.inner = 1;
+        } else if (kind <= 3) {
+            m.nd = 3;
+            m.off = pos(0, is, jc, js);
+            m.inner = 3;
+            m.s2 = pos(1, is, jc, js) - m.off;
+        } else {
+            m.nd = 12;
+            m.off = pos(0, 0, jc, js);
+            m.inner = 4;
+            m.s1 = pos(1, 0, jc, js) - m.off;
+            m.s2 = pos(0, 1, jc, js) - m.off;
+        }
+        m.nsrc = get ? rec_size[t] : m.nd;
+        m.ndst = get ? m.nd : rec_size[t];
+    }
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.px = pa;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.n = n;
+    finish(qla_launch_comp(prec, mode, p, m, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -566,6 +654,13 @@ hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream
 {
     if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
     if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_comp(int prec, int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st)
+{
+    if (prec == 0) return qla_comp_launch_p0(mode, p, m, st);
+    if (prec == 1) return qla_comp_launch_p1(mode, p, m, st);
     return hipErrorInvalidValue;
 }
 

@@ -122,6 +122,17 @@ hipError_t qla_outer_launch_p1(int eqop, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_launch_pmul(int prec, int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_pmul_launch_p0(int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_pmul_launch_p1(int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st);
+// Component access and the spin trace (qla_vecops.hip): mode 0 extract, 1
+// insert, 2 spin trace; the source in x (px, ix), the destination in r (ir),
+// index the x-form gang index.  Element c of the moved part sits at
+// off + (c / inner) * s1 + (c % inner) * s2 of the array record; nsrc / ndst
+// are the source / destination record sizes in complexes.
+struct qla_comp_map {
+    int nd, nsrc, ndst, off, inner, s1, s2;
+};
+hipError_t qla_launch_comp(int prec, int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
+hipError_t qla_comp_launch_p0(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
+hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

@@ -3,7 +3,8 @@
 // times a real or complex constant, sums and differences and a*b +- c with a
 // real or complex constant a (build_qla.pl:356-494, 1256-1354, 1421-1456,
 // 1837-1935), for records of V, H, D, M and P, and the products with
-// propagators (build_qla.pl:1587-1662; k_vec_pmul).
+// propagators (build_qla.pl:1587-1662; k_vec_pmul), component access and the
+// spin trace (build_qla.pl:767-932, 1066-1086; k_vec_comp).
 //
 // Every operation is element-wise on the complex components, so the kernels
 // stream components, not records:
@@ -659,6 +660,61 @@ hipError_t launch_pmul_t(int kind, int eqop, int ca, int cb, const qla_vec_args 
 }
 
 // ---------------------------------------------------------------------------
+// Component access (build_qla.pl:767-932, expressions_tensor.pl:297-347) and
+// the spin trace (build_qla.pl:1066-1086, expressions_tensor.pl:394-430),
+// eq only.  The host turns the element, colour-column or Dirac-column
+// arguments into a position map (qla_comp_map): element c of the moved part
+// sits at off + (c / inner) * s1 + (c % inner) * s2 of the array record.
+// MODE 0 extracts into records of nd complexes, MODE 1 inserts (only those nd
+// positions of each destination record are written), MODE 2 is the spin
+// trace M[ic][jc] = x, x = 0 then x += P[ic][is][jc][is] for is = 0..3.  One
+// work item per moved complex.
+// ---------------------------------------------------------------------------
+template <class T, int MODE>
+__global__ void __launch_bounds__(VBLOCK) k_vec_comp(qla_vec_args p, qla_comp_map m)
+{
+    const long long tot = (long long)p.n * m.nd;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        const int i = (int)(g / m.nd), c = (int)(g - (long long)i * m.nd);
+        const int s = p.index ? p.index[i] : i;
+        const int sd = p.ir ? p.ir[i] : s;
+        const int sa = p.ix ? p.ix[i] : s;
+        const cpx<T> *A = p.px ? ((const cpx<T> *const *)p.x)[sa] : (const cpx<T> *)p.x + (size_t)sa * m.nsrc;
+        cpx<T> *R = (cpx<T> *)p.r + (size_t)sd * m.ndst;
+        if constexpr (MODE == 2) {
+            const int ic = c / 3, jc = c - ic * 3;
+            cpx<T> x = { T(0), T(0) };
+#pragma unroll
+            for (int is = 0; is < 4; is++) {
+                const cpx<T> v = A[ic * 48 + is * 12 + jc * 4 + is];
+                x.re = x.re + v.re;
+                x.im = x.im + v.im;
+            }
+            R[c] = x;
+        } else {
+            const int pos = m.off + (c / m.inner) * m.s1 + (c % m.inner) * m.s2;
+            if constexpr (MODE == 0) R[c] = A[pos];
+            else R[pos] = A[c];
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_comp_t(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const dim3 grid(blocks_for((long long)p.n * m.nd));
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((k_vec_comp<T, 0>), grid, dim3(VBLOCK), 0, st, p, m); break;
+    case 1: hipLaunchKernelGGL((k_vec_comp<T, 1>), grid, dim3(VBLOCK), 0, st, p, m); break;
+    case 2: hipLaunchKernelGGL((k_vec_comp<T, 2>), grid, dim3(VBLOCK), 0, st, p, m); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Fills (build_qla.pl:2561-2620).  FILL_ZERO: QLA_c_eq_r(e, 0.) on every
 // element.  FILL_REC: QLA_c_<eqop>_c(e, a.e) with one record a for every
 // step.  FILL_DIAG (colour matrices): QLA_c_<eqop>_r(e, 0.) on every element
@@ -1135,6 +1191,10 @@ hipError_t qla_pmul_launch_p0(int kind, int eqop, int ca, int cb, const qla_vec_
 {
     return qla::launch_pmul_t<float>(kind, eqop, ca, cb, p, st);
 }
+hipError_t qla_comp_launch_p0(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st)
+{
+    return qla::launch_comp_t<float>(mode, p, m, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -1159,5 +1219,9 @@ hipError_t qla_outer_launch_p1(int eqop, const qla_vec_args &p, hipStream_t st)
 hipError_t qla_pmul_launch_p1(int kind, int eqop, int ca, int cb, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_pmul_t<double>(kind, eqop, ca, cb, p, st);
+}
+hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st)
+{
+    return qla::launch_comp_t<double>(mode, p, m, st);
 }
 #endif

@@ -17,8 +17,8 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import (LOCALS, NCOMP, PMUL_KINDS, VECFAMS, fieldmuls, fills, gammas, locals_, pmuls, sums,
-                             vecops)
+from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, VECFAMS, comps, fieldmuls, fills, gammas, locals_,
+                             pmuls, sums, vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -502,3 +502,100 @@ def test_propagator_products_large(orc):
     exp = np.zeros_like(a)
     orc.pmul(2, 0, exp, a, b, cb=True)
     assert_bitwise(host(r), exp, "QLA_F3_P_veq_P_times_Pa")
+
+
+_cfields = {}
+
+
+def _c_field(prec, seed):
+    """a random complex field of N sites (host, device)"""
+    if (prec, seed) not in _cfields:
+        dt = np.float64 if prec == "D3" else np.float32
+        h = np.random.default_rng(seed).standard_normal((N, 2)).astype(dt)
+        _cfields[(prec, seed)] = (h, dev(h))
+    return _cfields[(prec, seed)]
+
+
+def _comp_extra(v):
+    """the symbol's element / column arguments, a seeded choice in range"""
+    rng = np.random.default_rng(zlib.crc32(v.name.encode()))
+    lim = {"ic": 3, "is": 2 if v.t == "H" else 4, "jc": 3, "js": 4}
+    return [int(rng.integers(lim[x])) for x in v.extra]
+
+
+@pytest.mark.parametrize("v", comps(), ids=lambda v: v.name)
+def test_comp_symbol(orc, v):
+    """Every element / colour-column / Dirac-column access and the spin trace,
+    bit for bit against the oracle (untouched records and elements included)."""
+    d, s = v.types
+    ix = case(v.prec, "P")     # index arrays (every case holds N records)
+    if s == "C":
+        fz = factors(v.prec)
+        a_h, a_d, a_p = fz["C"]
+        a_perm = fz["perm"]
+    else:
+        sc = case(v.prec, s)
+        a_h, a_d, a_p, a_perm = sc.x, sc.dx, sc.px, sc.x_idx
+    r0 = _c_field(v.prec, 41)[0] if d == "C" else case(v.prec, d).r0
+    extra = _comp_extra(v)
+    r = dev(r0)
+    args = [r]
+    if v.ir:
+        args.append(ix.down["r"])
+    args.append(a_p if v.pa else a_d)
+    if v.ia:
+        args.append(ix.down["x"])
+    args += extra
+    if v.xf:
+        args.append(ix.dindex)
+    m = len(ix.index) if v.xf else (len(ix.own["r"]) if (v.ir or v.ia) else N)
+    say(f"{v.name} {extra} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = ix.index if v.xf else np.arange(m)
+    rs = ix.own["r"] if v.ir else site
+    as_ = ix.own["x"] if v.ia else site
+    ga = np.ascontiguousarray(a_h[a_perm[as_] if v.pa else as_])
+    sub = np.ascontiguousarray(r0[rs])
+    kind = COMP_FAMS[v.qual][2] + (0 if v.get else 1)
+    orc.comp(kind, v.t, extra, sub, ga)
+    exp = r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)
+
+
+def test_comp_out_of_range_is_an_error():
+    """An element index outside its range records an error and writes nothing."""
+    cs = case("F3", "M")
+    c0 = _c_field("F3", 42)[0]
+    r = dev(c0)
+    say("QLA_F3_C_veq_elem_M out of range")
+    qla_amd.call("QLA_F3_C_veq_elem_M", r, cs.dx, 3, 0, N)
+    lib = qla_amd.load()
+    lib.QLA_MI355X_synchronize()
+    assert lib.QLA_MI355X_last_error() != 0
+    assert_bitwise(host(r), c0, "unchanged")
+    qla_amd.call("QLA_F3_V_veq_colorvec_H", dev(case("F3", "V").r0), case("F3", "H").dx, 2, N)
+    lib.QLA_MI355X_synchronize()
+    assert lib.QLA_MI355X_last_error() != 0
+    assert lib.QLA_MI355X_last_error() == 0
+
+
+def test_propagator_from_dirac_columns(orc):
+    """Twelve Dirac-column insertions build a propagator at 2^16 sites that
+    the spin trace and extraction read back exactly (oracle, bit for bit)."""
+    n = 1 << 16
+    p = lattice.random_field("P", n, 81, dtype=np.float32)
+    cols = [np.ascontiguousarray(p[:, :, :, jc, js]) for jc in range(3) for js in range(4)]
+    r = dev(np.zeros_like(p))
+    say(f"QLA_F3_P_veq_diracvec_D x12 n={n}")
+    for k, c in enumerate(cols):
+        qla_amd.call("QLA_F3_P_veq_diracvec_D", r, dev(c), k // 4, k % 4, n)
+    qla_amd.check()
+    assert_bitwise(host(r), p, "P rebuilt")
+    m = dev(np.zeros((n, 3, 3, 2), np.float32))
+    qla_amd.call("QLA_F3_M_veq_spintrace_P", m, r, n)
+    qla_amd.check()
+    exp = np.zeros((n, 3, 3, 2), np.float32)
+    orc.comp(orc.SPINTRACE, "P", (), exp, p)
+    assert_bitwise(host(m), exp, "spintrace")

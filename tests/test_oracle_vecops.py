@@ -722,3 +722,106 @@ def test_propagator_product_matches_colour_products(orc):
             want = np.zeros_like(blk)
             orc.site(orc.MAT_M, 0, want, m, blk)
             np.testing.assert_array_equal(r[:, :, is_, :, js], want)
+
+
+# (type, reference extra-argument count for elem / colorvec); the element is
+# indexed as the numpy record shape (ic[, is][, jc][, js]) reads it
+ELEM_ARGS = {"V": 1, "H": 2, "D": 2, "M": 2, "P": 4}
+CVEC_ARGS = {"M": 1, "H": 1, "D": 1, "P": 3}
+EXTENT = {"V": (3,), "H": (3, 2), "D": (3, 4), "M": (3, 3), "P": (3, 4, 3, 4)}
+
+
+def _cvec_slice(t, e):
+    """numpy index of the colour column of a record (every ic, the rest fixed)"""
+    if t == "M":
+        return (slice(None), slice(None), e[0])
+    if t in ("H", "D"):
+        return (slice(None), slice(None), e[0])
+    return (slice(None), slice(None), e[0], e[1], e[2])
+
+
+def _all_args(shape):
+    return [tuple(int(v) for v in ix) for ix in np.ndindex(*shape)]
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", list(ELEM_ARGS))
+def test_elem_get_set(orc, prec, t):
+    """C = elem(a, ic, ...) and its insertion, for every element of the record,
+    against numpy indexing of the record shape (an exact copy; the other
+    elements of an inserted record untouched)."""
+    dt = DT[prec]
+    n = 4
+    a = field(t, n, 150, dt)
+    for e in _all_args(EXTENT[t]):
+        c = np.zeros((n, 2), dt)
+        orc.comp(orc.GET_ELEM, t, e, c, a)
+        np.testing.assert_array_equal(c, a[(slice(None),) + e])
+        r = field(t, n, 151, dt)
+        want = r.copy()
+        src = field("V", n, 152, dt)[:, 0]
+        want[(slice(None),) + e] = src
+        orc.comp(orc.SET_ELEM, t, e, r, np.ascontiguousarray(src))
+        np.testing.assert_array_equal(r, want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", list(CVEC_ARGS))
+def test_colorvec_get_set(orc, prec, t):
+    """V = colorvec(a, ...) and its insertion for every choice of the fixed
+    indices, against numpy slicing."""
+    dt = DT[prec]
+    n = 4
+    a = field(t, n, 160, dt)
+    for e in _all_args(EXTENT[t][1:] if t != "M" else (3,)):
+        if t in ("H", "D"):
+            e = e[:1]
+        sl = _cvec_slice(t, e)
+        v = np.zeros((n, 3, 2), dt)
+        orc.comp(orc.GET_COLORVEC, t, e, v, a)
+        np.testing.assert_array_equal(v, a[sl])
+        r = field(t, n, 161, dt)
+        src = field("V", n, 162, dt)
+        want = r.copy()
+        want[sl] = src
+        orc.comp(orc.SET_COLORVEC, t, e, r, src)
+        np.testing.assert_array_equal(r, want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_diracvec_get_set(orc, prec):
+    """D = diracvec(P, jc, js) and its insertion for all 12 columns, against
+    numpy slicing; inserting all 12 columns of a propagator rebuilds it."""
+    dt = DT[prec]
+    n = 5
+    a = field("P", n, 170, dt)
+    rebuilt = np.zeros_like(a)
+    for jc in range(3):
+        for js in range(4):
+            d = np.zeros((n, 3, 4, 2), dt)
+            orc.comp(orc.GET_DIRACVEC, "P", (jc, js), d, a)
+            np.testing.assert_array_equal(d, a[:, :, :, jc, js])
+            orc.comp(orc.SET_DIRACVEC, "P", (jc, js), rebuilt, d)
+    np.testing.assert_array_equal(rebuilt, a)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_spintrace(orc, prec):
+    """M = spintrace(P): exactly the spin-diagonal sum on small integers, and
+    on random values the ordered sum ((0 + p0) + p1) + p2 + p3 in the field
+    precision; tr_colour(spintrace P) = the full 12 x 12 trace."""
+    dt = DT[prec]
+    n = 6
+    a = _int_field("P", n, 180, dt)
+    m = np.zeros((n, 3, 3, 2), dt)
+    orc.comp(orc.SPINTRACE, "P", (), m, a)
+    np.testing.assert_array_equal(cx(m.astype(np.float64)), np.einsum("nasbs->nab", cx(a.astype(np.float64))))
+    a = field("P", n, 181, dt)
+    orc.comp(orc.SPINTRACE, "P", (), m, a)
+    want = np.zeros((n, 3, 3, 2), dt)
+    for s in range(4):
+        want = (want + a[:, :, s, :, s]).astype(dt)
+    np.testing.assert_array_equal(m, want)
+    full = np.trace(cx(a.astype(np.float64)).reshape(n, 12, 12), axis1=1, axis2=2)
+    np.testing.assert_allclose(np.trace(cx(m.astype(np.float64)), axis1=1, axis2=2), full,
+                               rtol=0, atol=1e-5 if prec == "F" else 1e-13)
