@@ -82,7 +82,21 @@ EQOPS = ("eq", "peq", "meq", "eqm")          # codes 0..3, as the kernels use th
 TYPE_NAMES = {"V": "ColorVector", "H": "HalfFermion", "D": "DiracFermion", "M": "ColorMatrix",
               "P": "DiracPropagator"}
 # complex numbers per record (include/qla_types.h:29-42)
-NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144}
+NCOMP = {"V": 3, "H": 6, "D": 12, "M": 9, "P": 144, "C": 1, "R": 0}   # R: one real (the kernels' code 0)
+# the real and complex fields live in the colourless libraries QLA_F / QLA_D
+SCALAR_TYPES = {"R": "Real", "C": "Complex"}
+
+
+def lib_of(prec: str, t: str) -> str:
+    """library prefix of a record type: F3 / D3, or F / D for R and C"""
+    return prec[0] if t in SCALAR_TYPES else prec
+
+
+def rec_type(prec: str, t: str) -> str:
+    """C type of a record of type t in library prec"""
+    if t in SCALAR_TYPES:
+        return f"QLA_{prec[0]}_{SCALAR_TYPES[t]}"
+    return f"QLA_{prec}_{TYPE_NAMES[t]}"
 FIELD_TYPES = ("V", "H", "D", "M", "P")
 # (dest, src1, src2) index flags of the per-operand index forms of a binary op
 OWN_INDEX_FORMS = [(r, a, b) for r in (False, True) for a in (False, True) for b in (False, True) if r or a or b]
@@ -405,20 +419,26 @@ def matfuncs() -> list[MatVariant]:
 # field sources, qualifier in the name, source transform (bit 0 conjugate, bit 1 transpose),
 # record types)
 SQUARE_TYPES = ("M", "P")
+# the arithmetic types (datatypes.pl:64-73) without the integer fields, and
+# those a complex constant multiplies (:295-302)
+ARITH_TYPES = ("R", "C") + FIELD_TYPES
+CMULT_TYPES = ("C",) + FIELD_TYPES
 VECFAMS = {
-    "copy": (0, "", 1, "", 0, FIELD_TYPES),
+    "copy": (0, "", 1, "", 0, ARITH_TYPES),
     "transpose": (0, "", 1, "transpose_", 2, SQUARE_TYPES),
-    "conj": (0, "", 1, "conj_", 1, FIELD_TYPES),
-    "i": (1, "", 1, "i_", 0, FIELD_TYPES),
-    "r_times": (2, "r", 1, "", 0, FIELD_TYPES),
-    "c_times": (3, "c", 1, "", 0, FIELD_TYPES),
-    "plus": (4, "", 2, "", 0, FIELD_TYPES),
-    "minus": (5, "", 2, "", 0, FIELD_TYPES),
-    "r_times_plus": (6, "r", 2, "", 0, FIELD_TYPES),
-    "r_times_minus": (7, "r", 2, "", 0, FIELD_TYPES),
-    "c_times_plus": (8, "c", 2, "", 0, FIELD_TYPES),
-    "c_times_minus": (9, "c", 2, "", 0, FIELD_TYPES),
+    "conj": (0, "", 1, "conj_", 1, CMULT_TYPES),
+    "i": (1, "", 1, "i_", 0, CMULT_TYPES),
+    "r_times": (2, "r", 1, "", 0, ARITH_TYPES),
+    "c_times": (3, "c", 1, "", 0, CMULT_TYPES),
+    "plus": (4, "", 2, "", 0, ARITH_TYPES),
+    "minus": (5, "", 2, "", 0, ARITH_TYPES),
+    "r_times_plus": (6, "r", 2, "", 0, ARITH_TYPES),
+    "r_times_minus": (7, "r", 2, "", 0, ARITH_TYPES),
+    "c_times_plus": (8, "c", 2, "", 0, CMULT_TYPES),
+    "c_times_minus": (9, "c", 2, "", 0, CMULT_TYPES),
 }
+# types with an adjoint (datatypes.pl:275-279): C (its conjugate), M, P
+ADJ_TYPES = ("C",) + SQUARE_TYPES
 # families with every eqop and an adjoint (`Ma`, `Pa`) source form
 VEC_ALL_EQOPS = ("copy", "transpose", "conj", "i", "r_times", "c_times")
 VEC_ADJ = ("copy", "r_times", "c_times")
@@ -453,7 +473,9 @@ class VecVariant:
 
     @property
     def xform(self) -> int:
-        return 3 if self.adj else VECFAMS[self.fam][4]
+        if self.adj:
+            return 1 if self.t == "C" else 3
+        return VECFAMS[self.fam][4]
 
     def _names(self) -> tuple[str, str]:
         """QLA argument names of the two field sources (prototype.pl:600-640:
@@ -467,7 +489,7 @@ class VecVariant:
         x = ("p" if self.px else "") + ("x" if self.ix else "") + self.t + ("a" if self.adj else "")
         y = ("p" if self.py else "") + ("x" if self.iy else "") + self.t
         eq = EQOPS[self.eqop]
-        head = f"QLA_{self.prec}_{r}_{form}{eq}_"
+        head = f"QLA_{lib_of(self.prec, self.t)}_{r}_{form}{eq}_"
         if self.nsrc == 1:
             if self.scalar:
                 return f"{head}{self.scalar}_times_{x}"
@@ -478,7 +500,7 @@ class VecVariant:
         return f"{head}{x}_{pm}_{y}"
 
     def params(self) -> list[tuple[str, str]]:
-        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        rec = rec_type(self.prec, self.t)
         real = "D" if self.prec == "D3" else "F"
         xn, yn = self._names()
         out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
@@ -522,7 +544,7 @@ def vecops() -> list[VecVariant]:
         for fam, (_, _, nsrc, _, _, types) in VECFAMS.items():
             eqops = range(4) if fam in VEC_ALL_EQOPS else (0,)
             for t in types:
-                for adj in ((False, True) if fam in VEC_ADJ and t in SQUARE_TYPES else (False,)):
+                for adj in ((False, True) if fam in VEC_ADJ and t in ADJ_TYPES else (False,)):
                     for eqop in eqops:
                         if nsrc == 1:
                             out += [VecVariant(prec, fam, t, eqop, adj, xf, px) for xf in (False, True)
@@ -551,13 +573,14 @@ class FillVariant:
     def name(self) -> str:
         form = "x" if self.xf else "v"
         eq = EQOPS[self.eqop]
+        lib = lib_of(self.prec, self.t)
         if self.kind == "zero":
-            return f"QLA_{self.prec}_{self.t}_{form}eq_zero"
+            return f"QLA_{lib}_{self.t}_{form}eq_zero"
         src = self.t.lower() if self.kind == "rec" else "c"
-        return f"QLA_{self.prec}_{self.t}_{form}{eq}_{src}"
+        return f"QLA_{lib}_{self.t}_{form}{eq}_{src}"
 
     def params(self) -> list[tuple[str, str]]:
-        rec = f"QLA_{self.prec}_{TYPE_NAMES[self.t]}"
+        rec = rec_type(self.prec, self.t)
         out = [("p", f"{rec} *QLA_MI355X_RESTRICT r")]
         if self.kind == "rec":
             out.append(("p", f"{rec} *QLA_MI355X_RESTRICT a"))
@@ -1050,9 +1073,9 @@ def comps() -> list[CompVariant]:
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
-        for t in FIELD_TYPES:
+        for t in ARITH_TYPES:
             out += [FillVariant(prec, "zero", t, 0, xf) for xf in (False, True)]
-        for t in FIELD_TYPES:
+        for t in ARITH_TYPES:
             out += [FillVariant(prec, "rec", t, e, xf) for e in range(4) for xf in (False, True)]
         out += [FillVariant(prec, "diag", "M", e, xf) for e in range(4) for xf in (False, True)]
     return out

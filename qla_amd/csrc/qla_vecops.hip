@@ -352,9 +352,48 @@ unsigned blocks_for(long long items)
     return (unsigned)(b < 1 ? 1 : (b > VMAXBLOCKS ? VMAXBLOCKS : b));
 }
 
+// Real fields (QLA_F_Real / QLA_D_Real records, the type R of
+// build_qla.pl:356-402, 1256-1294, 1421-1456, 1837-1888): the reference's
+// real forms r = a, r += s*a, r = s*a + b ... are the real part of the
+// complex element arithmetic, which for the operations R has (copies, real
+// constants, sums) reads only real parts.  One work item per record; every
+// indexing form.  ncomp 0 selects it.
+template <class T, int OP, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_real(qla_vec_args p)
+{
+    T sr, si;
+    load_scalar<T, OP>(p, sr, si);
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        const int i = (int)g;
+        const int s = p.index ? p.index[i] : i;
+        const int xs = p.ix ? p.ix[i] : s;
+        const T x = p.px ? *((const T *const *)p.x)[xs] : ((const T *)p.x)[xs];
+        T y = T(0);
+        if constexpr (has_y<OP>()) {
+            const int ys = p.iy ? p.iy[i] : s;
+            y = p.py ? *((const T *const *)p.y)[ys] : ((const T *)p.y)[ys];
+        }
+        T *R = (T *)p.r + (p.ir ? p.ir[i] : s);
+        T rr = T(0), ri = T(0);
+        if constexpr (reads_r<EQOP>()) rr = *R;
+        vop<OP, EQOP, T>(rr, ri, x, T(0), y, T(0), sr, si);
+        *R = rr;
+    }
+}
+
 template <class T, int OP, int EQOP>
 hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
 {
+    if (ncomp == 0) {
+        if constexpr (is_field<OP>() || complex_scalar<OP>() || OP == VOP_I) {
+            return hipErrorInvalidValue;
+        } else {
+            if (p.trans || p.conj) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_vec_real<T, OP, EQOP>), dim3(blocks_for(p.n)), dim3(VBLOCK), 0, st, p);
+            return hipGetLastError();
+        }
+    }
     const bool flat = !p.index && !p.ir && !p.ix && !p.iy && !p.px && !p.py && !p.trans;
     if constexpr (!is_field<OP>()) {
         if (flat) {
@@ -399,6 +438,7 @@ hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
         hipLaunchKernelGGL((k_vec_gather<T, NC, OP, EQOP, 1>), grid, dim3(VBLOCK), 0, st, q);   \
         return hipGetLastError();
     switch (ncomp) {
+        QLA_VEC_CASE(1)
         QLA_VEC_CASE(3)
         QLA_VEC_CASE(6)
         QLA_VEC_CASE(9)
@@ -817,6 +857,7 @@ hipError_t launch_fill_k(int ncomp, const qla_fill_args &p, hipStream_t st)
         return ncomp == 9 ? launch_fill_nc<T, 9, KIND, EQOP>(p, st) : hipErrorInvalidValue;
     } else {
         switch (ncomp) {
+        case 1: return launch_fill_nc<T, 1, KIND, EQOP>(p, st);
         case 3: return launch_fill_nc<T, 3, KIND, EQOP>(p, st);
         case 6: return launch_fill_nc<T, 6, KIND, EQOP>(p, st);
         case 9: return launch_fill_nc<T, 9, KIND, EQOP>(p, st);
@@ -827,9 +868,51 @@ hipError_t launch_fill_k(int ncomp, const qla_fill_args &p, hipStream_t st)
     }
 }
 
+// fills of real fields (ncomp 0): r = 0 (QLA_r_eq_r(e, 0.)) or r eqop a with
+// the one real a from device memory or the value the host passes
+template <class T, int KIND, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_rfill(qla_fill_args p, T hv)
+{
+    const T a = KIND == FILL_ZERO ? T(0) : (p.src ? *(const T *)p.src : hv);
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        T *R = (T *)p.r + (p.index ? p.index[g] : g);
+        if constexpr (KIND == FILL_ZERO) {
+            *R = T(0);
+        } else {
+            T rr = T(0), ri = T(0);
+            if constexpr (reads_r<EQOP>()) rr = *R;
+            vop<VOP_COPY, EQOP, T>(rr, ri, a, T(0), T(0), T(0), T(0), T(0));
+            *R = rr;
+        }
+    }
+}
+
+template <class T, int KIND, int EQOP>
+hipError_t launch_rfill(const qla_fill_args &p, hipStream_t st)
+{
+    T hv = T(0);
+    if (KIND != FILL_ZERO && !p.src) memcpy(&hv, p.host, sizeof(T));
+    hipLaunchKernelGGL((k_vec_rfill<T, KIND, EQOP>), dim3(blocks_for(p.n)), dim3(VBLOCK), 0, st, p, hv);
+    return hipGetLastError();
+}
+
 template <class T, int KIND>
 hipError_t launch_fill_e(int eqop, int ncomp, const qla_fill_args &p, hipStream_t st)
 {
+    if (ncomp == 0) {
+        if constexpr (KIND == FILL_DIAG) {
+            return hipErrorInvalidValue;
+        } else {
+            switch (eqop) {
+            case 0: return launch_rfill<T, KIND, 0>(p, st);
+            case 1: return launch_rfill<T, KIND, 1>(p, st);
+            case 2: return launch_rfill<T, KIND, 2>(p, st);
+            case 3: return launch_rfill<T, KIND, 3>(p, st);
+            }
+            return hipErrorInvalidValue;
+        }
+    }
     switch (eqop) {
     case 0: return launch_fill_k<T, KIND, 0>(ncomp, p, st);
     case 1: return launch_fill_k<T, KIND, 1>(ncomp, p, st);
@@ -845,6 +928,7 @@ hipError_t launch_fill_t(int kind, int eqop, int ncomp, const qla_fill_args &p, 
     if (p.n <= 0) return hipSuccess;
     if (kind == FILL_ZERO) {
         if (eqop != 0) return hipErrorInvalidValue;
+        if (ncomp == 0) return launch_rfill<T, FILL_ZERO, 0>(p, st);
         if (!p.index) {
             const long long m = (long long)p.n * ncomp;
             const long long items = sizeof(T) == 4 ? (m + 1) / 2 : m;

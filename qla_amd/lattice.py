@@ -109,10 +109,11 @@ def random_su3(n: int, seed: int, dtype=np.float64, chunk: int = 1 << 20) -> np.
 
 
 def random_field(kind: str, n: int, seed: int, dtype=np.float64) -> np.ndarray:
-    """Complex Gaussian(0,1) field of record kind 'V', 'H', 'D', 'M' or 'P'
-    (ColorVector, HalfFermion, DiracFermion, ColorMatrix, DiracPropagator)."""
+    """Gaussian(0,1) field of record kind 'V', 'H', 'D', 'M' or 'P'
+    (ColorVector, HalfFermion, DiracFermion, ColorMatrix, DiracPropagator),
+    'C' (one complex per site) or 'R' (one real per site)."""
     shape = {"V": (n, 3, 2), "H": (n, 3, 2, 2), "D": (n, 3, 4, 2), "M": (n, 3, 3, 2),
-             "P": (n, 3, 4, 3, 4, 2)}[kind]
+             "P": (n, 3, 4, 3, 4, 2), "C": (n, 2), "R": (n,)}[kind]
     return gaussian(shape, seed, dtype=dtype)
 
 

@@ -38,18 +38,18 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 12009
+    assert len(every) == len(set(every)) == 12909
     assert len(abi_gen.fieldmuls()) == 2640
     assert len(abi_gen.pmuls()) == 1440
     assert len(abi_gen.comps()) == 294
     assert len(abi_gen.sums()) == 84
     assert len(abi_gen.locals_()) == 1536
     assert len(abi_gen.gammas()) == 42
-    assert len(abi_gen.fills()) == 116
+    assert len(abi_gen.fills()) == 156
     mf = [v.name for v in abi_gen.matfuncs()]
     assert len(mf) == 321
     vo = [v.name for v in abi_gen.vecops()]
-    assert len(vo) == 2748
+    assert len(vo) == 3608
     # the solver-side calls of QDP/QOP-style codes and the reference's test names
     for s in ("QLA_F3_D_veq_D", "QLA_F3_D_vpeq_r_times_D", "QLA_D3_D_veq_r_times_D_plus_D",
               "QLA_F3_D_veq_r_times_D_minus_D", "QLA_F3_D_veq_c_times_pD_plus_D", "QLA_F3_M_veq_Ma",

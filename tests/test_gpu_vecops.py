@@ -599,3 +599,41 @@ def test_propagator_from_dirac_columns(orc):
     exp = np.zeros((n, 3, 3, 2), np.float32)
     orc.comp(orc.SPINTRACE, "P", (), exp, p)
     assert_bitwise(host(m), exp, "spintrace")
+
+
+@pytest.mark.parametrize("name,kind", [("QLA_F3_P_vpeq_P_times_Pa", 2), ("QLA_F3_P_veq_Ma_times_P", 0),
+                                       ("QLA_F3_P_vmeq_P_times_M", 1)])
+def test_propagator_products_unaligned(orc, name, kind):
+    """Fields starting one complex past a 16-byte boundary (the record staging
+    takes its one-complex-per-thread path), bit for bit against the oracle."""
+    import torch
+    ta, tb = PMUL_KINDS[kind]
+    n = N - 1
+    ca_, cb_, cs = case("F3", ta), case("F3", tb), case("F3", "P")
+    flat = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).to("cuda")   # noqa: E731
+    sub_of = lambda a, t: a.reshape(-1)[2:2 + n * 2 * NCOMP[t]].reshape((n,) + a.shape[1:])   # noqa: E731
+    ba, bb, br = flat(ca_.x), flat(cb_.y), flat(cs.r0)
+    say(f"{name} unaligned n={n}")
+    qla_amd.call(name, br[2:], ba[2:], bb[2:], n)
+    qla_amd.check()
+    sub = np.ascontiguousarray(sub_of(cs.r0, "P"))
+    eqop = {"veq": 0, "vpeq": 1, "vmeq": 2}[name.split("_")[3]]
+    orc.pmul(kind, eqop, sub, np.ascontiguousarray(sub_of(ca_.x, ta)), np.ascontiguousarray(sub_of(cb_.y, tb)),
+             ca=name.split("_")[4].endswith("a"), cb=name.endswith("a"))
+    exp = cs.r0.reshape(-1).copy()
+    exp[2:2 + n * 288] = sub.reshape(-1)
+    assert_bitwise(host(br), exp, f"{name} unaligned")
+
+
+@pytest.mark.parametrize("prec", ["F3", "D3"])
+@pytest.mark.parametrize("t,fam", [("R", "r_times"), ("R", "r_times_minus"), ("C", "c_times_plus")])
+def test_scalar_field_device_constant(orc, prec, t, fam):
+    """R and C fields with the constant read from device memory give the
+    host-constant result."""
+    import torch
+    v = [x for x in vecops() if x.prec == prec and x.fam == fam and x.t == t and not (
+        x.xf or x.px or x.py or x.own_index or x.adj)][0]
+    cs = case(prec, t)
+    dsc = torch.from_numpy(cs.sr if v.scalar == "r" else cs.sc).to("cuda")
+    got, exp = cs.run(orc, v, sptr=dsc.data_ptr())
+    assert_bitwise(got, exp, f"{v.name} device constant")

@@ -825,3 +825,88 @@ def test_spintrace(orc, prec):
     full = np.trace(cx(a.astype(np.float64)).reshape(n, 12, 12), axis1=1, axis2=2)
     np.testing.assert_allclose(np.trace(cx(m.astype(np.float64)), axis1=1, axis2=2), full,
                                rtol=0, atol=1e-5 if prec == "F" else 1e-13)
+
+
+R_OPS = ["copy", "r_times", "plus", "minus", "r_times_plus", "r_times_minus"]
+
+
+R_TWO = ("plus", "minus", "r_times_plus", "r_times_minus")
+R_CASES = [(f, e) for f in R_OPS for e in (range(1) if f in R_TWO else range(4))]
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("fam,eqop", R_CASES)
+def test_real_field_ops(orc, prec, fam, eqop):
+    """Real fields (type R): r eqop a, r eqop s*a (every eqop), a +- b and
+    s*a +- b (eq) against numpy in the field precision (the same IEEE
+    operations one by one), signed zeros included."""
+    dt = DT[prec]
+    n = 64
+    a = field("R", n, 200, dt)
+    b = field("R", n, 201, dt)
+    a[:4] = [0.0, -0.0, 0.0, -0.0]
+    b[:4] = [-0.0, 0.0, 0.0, -0.0]
+    r0 = field("R", n, 202, dt)
+    s = dt(-0.7071067811865476)
+    op = {"copy": orc.COPY, "r_times": orc.RSCALE, "plus": orc.PLUS, "minus": orc.MINUS,
+          "r_times_plus": orc.R_PLUS, "r_times_minus": orc.R_MINUS}[fam]
+    r = r0.copy()
+    orc.vecop(op, eqop, r, a, b if fam in ("plus", "minus", "r_times_plus", "r_times_minus") else None, s=float(s))
+    v = {"copy": a, "r_times": s * a, "plus": a + b, "minus": a - b, "r_times_plus": s * a + b,
+         "r_times_minus": s * a - b}[fam]
+    want = {0: v, 1: r0 + v, 2: r0 - v, 3: -v}[eqop]
+    np.testing.assert_array_equal(r.view(np.uint32 if prec == "F" else np.uint64),
+                                  want.astype(dt).view(np.uint32 if prec == "F" else np.uint64))
+
+
+C_OPS = ["copy", "conj", "adjoint", "i", "r_times", "c_times", "plus", "minus", "r_times_plus", "r_times_minus",
+         "c_times_plus", "c_times_minus"]
+
+
+C_TWO = ("plus", "minus", "r_times_plus", "r_times_minus", "c_times_plus", "c_times_minus")
+C_CASES = [(f, e) for f in C_OPS for e in (range(1) if f in C_TWO else range(4))]
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("fam,eqop", C_CASES)
+def test_complex_field_ops(orc, prec, fam, eqop):
+    """Complex fields (type C, one complex per record) exactly against
+    complex128 on small integers, every family and eqop it has; the adjoint
+    of a complex is its conjugate."""
+    two = fam in C_TWO
+    dt = DT[prec]
+    n = 40
+    a = _int_field("C", n, 210, dt)
+    b = _int_field("C", n, 211, dt)
+    r0 = _int_field("C", n, 212, dt)
+    sr, sc = -3.0, complex(2.0, -5.0)
+    op = {"copy": orc.COPY, "conj": orc.COPY, "adjoint": orc.COPY, "i": orc.TIMES_I, "r_times": orc.RSCALE,
+          "c_times": orc.CSCALE, "plus": orc.PLUS, "minus": orc.MINUS, "r_times_plus": orc.R_PLUS,
+          "r_times_minus": orc.R_MINUS, "c_times_plus": orc.C_PLUS, "c_times_minus": orc.C_MINUS}[fam]
+    s = sc if fam.startswith("c_times") else sr
+    r = r0.copy()
+    orc.vecop(op, eqop, r, a, b if two else None, s=s, conj=fam in ("conj", "adjoint"))
+    A, B, R0 = cx(a.astype(np.float64)), cx(b.astype(np.float64)), cx(r0.astype(np.float64))
+    v = {"copy": A, "conj": np.conj(A), "adjoint": np.conj(A), "i": 1j * A, "r_times": sr * A, "c_times": sc * A,
+         "plus": A + B, "minus": A - B, "r_times_plus": sr * A + B, "r_times_minus": sr * A - B,
+         "c_times_plus": sc * A + B, "c_times_minus": sc * A - B}[fam]
+    want = {0: v, 1: R0 + v, 2: R0 - v, 3: -v}[eqop]
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), want)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", ["R", "C"])
+def test_scalar_field_fills(orc, prec, t):
+    """R and C fills: zero, and r eqop a with one constant, every eqop."""
+    dt = DT[prec]
+    n = 20
+    r0 = field(t, n, 220, dt)
+    a = np.ascontiguousarray(field(t, 1, 221, dt)[0])
+    r = r0.copy()
+    orc.fill(orc.FILL_ZERO, 0, r)
+    np.testing.assert_array_equal(r, np.zeros_like(r))
+    for eqop in range(4):
+        r = r0.copy()
+        orc.fill(orc.FILL_REC, eqop, r, a)
+        want = {0: np.broadcast_to(a, r0.shape), 1: r0 + a, 2: r0 - a, 3: np.broadcast_to(-a, r0.shape)}[eqop]
+        np.testing.assert_array_equal(r, want.astype(dt))
