@@ -173,6 +173,13 @@ void orc_D_comp(int kind, int t, const int *e, double *r, const int *rindex, con
                 const int *index, int n)
 { D_orc_comp(kind, t, e, r, rindex, pa, index, n); }
 
+void orc_F_sconv(int kind, float *r, const int *rindex, const float *const *pa, const float *const *pb,
+                 const int *index, int n)
+{ F_orc_sconv(kind, r, rindex, pa, pb, index, n); }
+void orc_D_sconv(int kind, double *r, const int *rindex, const double *const *pa, const double *const *pb,
+                 const int *index, int n)
+{ D_orc_sconv(kind, r, rindex, pa, pb, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

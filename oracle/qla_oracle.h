@@ -165,6 +165,12 @@ void orc_F_comp(int kind, int t, const int *e, float *r, const int *rindex, cons
 void orc_D_comp(int kind, int t, const int *e, double *r, const int *rindex, const double *const *pa,
                 const int *index, int n);
 
+/* real / complex conversions and scalar-field division (qla_oracle_vecops.inc) */
+void orc_F_sconv(int kind, float *r, const int *rindex, const float *const *pa, const float *const *pb,
+                 const int *index, int n);
+void orc_D_sconv(int kind, double *r, const int *rindex, const double *const *pa, const double *const *pb,
+                 const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -1070,6 +1070,82 @@ def comps() -> list[CompVariant]:
     return out
 
 
+# real / complex conversions and division: kind -> (destination, sources, name form)
+SCONV = {0: ("C", ("R",), "{a}"), 1: ("C", ("R", "R"), "{a}_plus_i_{b}"), 2: ("R", ("C",), "re_{a}"),
+         3: ("R", ("C",), "im_{a}"), 4: ("R", ("R", "R"), "{a}_divide_{b}"), 5: ("C", ("C", "C"), "{a}_divide_{b}")}
+
+
+@dataclass(frozen=True)
+class SconvVariant:
+    """Conversions between real and complex fields (build_qla.pl:623-708) and
+    the division of real and complex fields (:1457-1482), eq only, every
+    indexing form."""
+    prec: str
+    kind: int
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def binary(self) -> bool:
+        return len(SCONV[self.kind][1]) == 2
+
+    @property
+    def name(self) -> str:
+        dest, srcs, fmt = SCONV[self.kind]
+        form = "" if (self.ir or self.ia or self.ib) else ("x" if self.xf else "v")
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + srcs[0]
+        b = (("p" if self.pb else "") + ("x" if self.ib else "") + srcs[1]) if self.binary else ""
+        return f"QLA_{self.prec[0]}_{'x' if self.ir else ''}{dest}_{form}eq_{fmt.format(a=a, b=b)}"
+
+    def params(self) -> list[tuple[str, str]]:
+        dest, srcs, _ = SCONV[self.kind]
+        out = [("p", f"{rec_type(self.prec, dest)} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{rec_type(self.prec, srcs[0])} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        if self.binary:
+            out.append(("p", f"{rec_type(self.prec, srcs[1])} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+            if self.ib:
+                out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        b = "(const void *)b" if self.binary else "nullptr"
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"sconv_entry({prec}, {self.kind}, {int(self.pa)}, {int(self.pb)}, (void *)r, (const void *)a, {b}, "
+                f"{idx}, {own}, n, \"{self.name}\");")
+
+
+def sconvs() -> list[SconvVariant]:
+    out = []
+    for prec in PRECS:
+        for kind in SCONV:
+            kw = dict(prec=prec, kind=kind)
+            if len(SCONV[kind][1]) == 2:
+                out += [SconvVariant(xf=xf, pa=pa, pb=pb, **kw) for xf in (False, True) for pa in (False, True)
+                        for pb in (False, True)]
+                out += [SconvVariant(ir=ir, ia=ia, ib=ib, **kw) for ir, ia, ib in OWN_INDEX_FORMS]
+            else:
+                out += [SconvVariant(xf=xf, pa=pa, **kw) for xf in (False, True) for pa in (False, True)]
+                out += [SconvVariant(ir=ir, ia=ia, **kw) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -1110,7 +1186,7 @@ def all_symbols() -> list:
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
     by real and complex fields, outer products, products with propagators, component access, spin trace)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls() + pmuls() + comps())
+            + sums() + fieldmuls() + pmuls() + comps() + sconvs())
 
 
 def variants() -> list[Variant]:
@@ -1393,6 +1469,9 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "sconv": ("real / complex conversions C = R, C = R + iR, R = re C, R = im C and the division of "
+              "real and complex fields (eq; C / C as QLA_c_eq_c_div_c)",
+              "build_qla.pl:623-708, 1457-1482, include/qla_complex.h:252-255, 417-427"),
     "comp_elem": ("element access, C = a(ic, ...) and the insertion r(ic, ...) = a (eq; the indices "
                   "the type has: V ic; H, D ic, is; M ic, jc; P ic, is, jc, js; out of range: an error, "
                   "nothing written)", "build_qla.pl:767-820, expressions_tensor.pl:297-347"),
@@ -1445,8 +1524,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps():
-        if isinstance(v, CompVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs():
+        if isinstance(v, SconvVariant):
+            key = (v.prec[0], "sconv")
+        elif isinstance(v, CompVariant):
             key = (v.prec, "comp_" + v.qual)
         elif isinstance(v, PmulVariant):
             key = (v.prec, "pmul" + str(v.kind))
@@ -1474,9 +1555,9 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry, outer_entry, pmul_entry and comp_entry are there.",
+        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry and sconv_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps():
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

@@ -428,6 +428,27 @@ void comp_entry(int prec, int kind, int t, int e0, int e1, int e2, int e3, int p
     finish(qla_launch_comp(prec, mode, p, m, ts.stream), name);
 }
 
+// real / complex conversions and scalar-field division (kind as
+// qla_launch_sconv), eq only, every form
+void sconv_entry(int prec, int kind, int pa, int pb, void *r, const void *a, const void *b, const int *index,
+                 const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    finish(qla_launch_sconv(prec, kind, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -654,6 +675,13 @@ hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream
 {
     if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
     if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_sconv(int prec, int kind, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_sconv_launch_p0(kind, p, st);
+    if (prec == 1) return qla_sconv_launch_p1(kind, p, st);
     return hipErrorInvalidValue;
 }
 

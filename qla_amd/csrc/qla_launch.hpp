@@ -133,6 +133,11 @@ struct qla_comp_map {
 hipError_t qla_launch_comp(int prec, int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
 hipError_t qla_comp_launch_p0(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
 hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st);
+// Real / complex conversions and scalar-field division (qla_vecops.hip):
+// kind 0 C = R, 1 C = R + iR, 2 R = re C, 3 R = im C, 4 R = R/R, 5 C = C/C
+hipError_t qla_launch_sconv(int prec, int kind, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_sconv_launch_p0(int kind, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

@@ -755,6 +755,72 @@ hipError_t launch_comp_t(int mode, const qla_vec_args &p, const qla_comp_map &m,
 }
 
 // ---------------------------------------------------------------------------
+// Conversions between real and complex fields and the division of scalar
+// fields, eq only (build_qla.pl:623-708, 1457-1482): KIND 0 C = R
+// (QLA_c_eq_r_plus_ir(c, a, 0.)), 1 C = R + i R, 2 R = re C, 3 R = im C,
+// 4 R = R / R, 5 C = C / C (QLA_c_eq_c_div_c, include/qla_complex.h:417-427:
+// t = 1.0 / |b|^2 in double rounded to the field type, then
+// (a.re b.re + a.im b.im) t and (a.im b.re - a.re b.im) t).  Single-precision
+// quotients are taken in double and rounded, which is the correctly rounded
+// quotient as the reference computes it whatever the compiler's division
+// flags.  One work item per record; every indexing form.
+// ---------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T sc_real(const void *base, int ptr, int k)
+{
+    return ptr ? *((const T *const *)base)[k] : ((const T *)base)[k];
+}
+template <class T> __device__ __forceinline__ cpx<T> sc_cpx(const void *base, int ptr, int k)
+{
+    return ptr ? *((const cpx<T> *const *)base)[k] : ((const cpx<T> *)base)[k];
+}
+
+template <class T, int KIND>
+__global__ void __launch_bounds__(VBLOCK) k_vec_sconv(qla_vec_args p)
+{
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        const int i = (int)g;
+        const int s = p.index ? p.index[i] : i;
+        const int xs = p.ix ? p.ix[i] : s;
+        const int ys = p.iy ? p.iy[i] : s;
+        const int d = p.ir ? p.ir[i] : s;
+        if constexpr (KIND == 0) {
+            ((cpx<T> *)p.r)[d] = { sc_real<T>(p.x, p.px, xs), T(0) };
+        } else if constexpr (KIND == 1) {
+            ((cpx<T> *)p.r)[d] = { sc_real<T>(p.x, p.px, xs), sc_real<T>(p.y, p.py, ys) };
+        } else if constexpr (KIND == 2) {
+            ((T *)p.r)[d] = sc_cpx<T>(p.x, p.px, xs).re;
+        } else if constexpr (KIND == 3) {
+            ((T *)p.r)[d] = sc_cpx<T>(p.x, p.px, xs).im;
+        } else if constexpr (KIND == 4) {
+            ((T *)p.r)[d] = (T)((double)sc_real<T>(p.x, p.px, xs) / (double)sc_real<T>(p.y, p.py, ys));
+        } else {
+            const cpx<T> a = sc_cpx<T>(p.x, p.px, xs), b = sc_cpx<T>(p.y, p.py, ys);
+            const T n2 = b.re * b.re + b.im * b.im;
+            const T t = (T)(1.0 / (double)n2);
+            ((cpx<T> *)p.r)[d] = { (a.re * b.re + a.im * b.im) * t, (a.im * b.re - a.re * b.im) * t };
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_sconv_t(int kind, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const dim3 grid(blocks_for(p.n));
+    switch (kind) {
+    case 0: hipLaunchKernelGGL((k_vec_sconv<T, 0>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((k_vec_sconv<T, 1>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_vec_sconv<T, 2>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_vec_sconv<T, 3>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 4: hipLaunchKernelGGL((k_vec_sconv<T, 4>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 5: hipLaunchKernelGGL((k_vec_sconv<T, 5>), grid, dim3(VBLOCK), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Fills (build_qla.pl:2561-2620).  FILL_ZERO: QLA_c_eq_r(e, 0.) on every
 // element.  FILL_REC: QLA_c_<eqop>_c(e, a.e) with one record a for every
 // step.  FILL_DIAG (colour matrices): QLA_c_<eqop>_r(e, 0.) on every element
@@ -1279,6 +1345,10 @@ hipError_t qla_comp_launch_p0(int mode, const qla_vec_args &p, const qla_comp_ma
 {
     return qla::launch_comp_t<float>(mode, p, m, st);
 }
+hipError_t qla_sconv_launch_p0(int kind, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_sconv_t<float>(kind, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -1307,5 +1377,9 @@ hipError_t qla_pmul_launch_p1(int kind, int eqop, int ca, int cb, const qla_vec_
 hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_map &m, hipStream_t st)
 {
     return qla::launch_comp_t<double>(mode, p, m, st);
+}
+hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_sconv_t<double>(kind, p, st);
 }
 #endif

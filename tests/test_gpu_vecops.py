@@ -17,8 +17,8 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, VECFAMS, comps, fieldmuls, fills, gammas, locals_,
-                             pmuls, sums, vecops)
+from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, VECFAMS, comps, fieldmuls, fills, gammas,
+                             locals_, pmuls, sconvs, sums, vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -637,3 +637,41 @@ def test_scalar_field_device_constant(orc, prec, t, fam):
     dsc = torch.from_numpy(cs.sr if v.scalar == "r" else cs.sc).to("cuda")
     got, exp = cs.run(orc, v, sptr=dsc.data_ptr())
     assert_bitwise(got, exp, f"{v.name} device constant")
+
+
+@pytest.mark.parametrize("v", sconvs(), ids=lambda v: v.name)
+def test_sconv_symbol(orc, v):
+    """Every real / complex conversion and scalar-field division, bit for bit
+    against the oracle (untouched records included)."""
+    dest, srcs, _ = SCONV[v.kind]
+    cs = case(v.prec, dest)
+    sa = case(v.prec, srcs[0])
+    sb = case(v.prec, srcs[1]) if v.binary else None
+    r = dev(cs.r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(sa.px if v.pa else sa.dx)
+    if v.ia:
+        args.append(cs.down["x"])
+    if v.binary:
+        args.append(sb.py if v.pb else sb.dy)
+        if v.ib:
+            args.append(cs.down["y"])
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if (v.ir or v.ia or v.ib) else cs.n)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    as_ = cs.own["x"] if v.ia else site
+    bs = cs.own["y"] if v.ib else site
+    ga = np.ascontiguousarray(sa.x[sa.x_idx[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(sb.y[sb.y_idx[bs] if v.pb else bs]) if v.binary else None
+    sub = np.ascontiguousarray(cs.r0[rs])
+    orc.sconv(v.kind, sub, ga, gb)
+    exp = cs.r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)

@@ -910,3 +910,48 @@ def test_scalar_field_fills(orc, prec, t):
         orc.fill(orc.FILL_REC, eqop, r, a)
         want = {0: np.broadcast_to(a, r0.shape), 1: r0 + a, 2: r0 - a, 3: np.broadcast_to(-a, r0.shape)}[eqop]
         np.testing.assert_array_equal(r, want.astype(dt))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_real_complex_conversions(orc, prec):
+    """C = R, C = R + iR, R = re C, R = im C as exact copies; R / R as
+    numpy's correctly rounded quotient in the field precision."""
+    dt = DT[prec]
+    n = 50
+    ra, rb = field("R", n, 230, dt), field("R", n, 231, dt)
+    c = field("C", n, 232, dt)
+    out = np.full((n, 2), 7, dt)
+    orc.sconv(0, out, ra)
+    np.testing.assert_array_equal(out, np.stack([ra, np.zeros_like(ra)], -1))
+    orc.sconv(1, out, ra, rb)
+    np.testing.assert_array_equal(out, np.stack([ra, rb], -1))
+    rr = np.zeros(n, dt)
+    orc.sconv(2, rr, c)
+    np.testing.assert_array_equal(rr, c[:, 0])
+    orc.sconv(3, rr, c)
+    np.testing.assert_array_equal(rr, c[:, 1])
+    orc.sconv(4, rr, ra, rb)
+    np.testing.assert_array_equal(rr, (ra / rb).astype(dt))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_complex_division(orc, prec):
+    """C / C exactly against complex128 where every step is exact (integer
+    numerators, denominators with |b|^2 a power of two), and within a few
+    units in the last place of complex128 division on random values."""
+    dt = DT[prec]
+    rng = np.random.default_rng(240)
+    n = 64
+    a = rng.integers(-50, 51, size=(n, 2)).astype(dt)
+    b = np.array([[1, 0], [0, 1], [-1, 0], [0, -1], [1, 1], [1, -1], [-2, 2], [4, 0]] * 8, dt)
+    r = np.zeros((n, 2), dt)
+    orc.sconv(5, r, a, b)
+    np.testing.assert_array_equal(cx(r.astype(np.float64)), cx(a.astype(np.float64)) / cx(b.astype(np.float64)))
+    a = field("C", 4096, 241, dt)
+    b = field("C", 4096, 242, dt)
+    r = np.zeros_like(a)
+    orc.sconv(5, r, a, b)
+    want = cx(a.astype(np.float64)) / cx(b.astype(np.float64))
+    eps = np.finfo(dt).eps
+    err = np.abs(cx(r.astype(np.float64)) - want) / np.abs(want)
+    assert err.max() < 8 * eps
