@@ -40,6 +40,7 @@ extern "C" {
 /* Types (layout-identical to include/qla_complex.h:27-35, qla_types.h:29-50) */
 /* ------------------------------------------------------------------------ */
 #ifndef _QLA_TYPES_H
+typedef int QLA_Int;
 typedef float QLA_F_Real;
 typedef double QLA_D_Real;
 typedef struct { QLA_F_Real real; QLA_F_Real imag; } __attribute__((aligned(8))) QLA_F_Complex;

@@ -64,6 +64,7 @@ def load():
         getattr(lib, f"orc_{p}_pmul").argtypes = [i32, i32, i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_comp").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_sconv").argtypes = [i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_sfn").argtypes = [i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -500,4 +501,19 @@ def sconv(kind, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None, ia=No
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_sconv")(kind, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
+    return r
+
+
+def sfn(fn, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=None, n=None):
+    """elementary functions of real and complex fields (orc_*_sfn): fn as
+    there; b a real field (20..24) or an int32 field (25, ldexp)."""
+    lib = load()
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = None if b is None else _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
+    p = "F" if r.dtype == np.float32 else "D"
+    getattr(lib, f"orc_{p}_sfn")(fn, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
     return r

@@ -23,6 +23,7 @@
 #define ACOSP acosf
 #define CBRTP cbrtf
 #define SINCOSP sincosf
+#define MATHP(fn) fn##f
 #include "qla_oracle_impl.inc"
 #include "qla_oracle_matfunc.inc"
 #include "qla_oracle_vecops.inc"
@@ -33,6 +34,7 @@
 #undef ACOSP
 #undef CBRTP
 #undef SINCOSP
+#undef MATHP
 
 /* double precision instantiation */
 #define REAL double
@@ -43,6 +45,7 @@
 #define ACOSP acos
 #define CBRTP cbrt
 #define SINCOSP sincos
+#define MATHP(fn) fn
 #include "qla_oracle_impl.inc"
 #include "qla_oracle_matfunc.inc"
 #include "qla_oracle_vecops.inc"
@@ -179,6 +182,13 @@ void orc_F_sconv(int kind, float *r, const int *rindex, const float *const *pa, 
 void orc_D_sconv(int kind, double *r, const int *rindex, const double *const *pa, const double *const *pb,
                  const int *index, int n)
 { D_orc_sconv(kind, r, rindex, pa, pb, index, n); }
+
+void orc_F_sfn(int fn, float *r, const int *rindex, const float *const *pa, const void *const *pb,
+               const int *index, int n)
+{ F_orc_sfn(fn, r, rindex, pa, pb, index, n); }
+void orc_D_sfn(int fn, double *r, const int *rindex, const double *const *pa, const void *const *pb,
+               const int *index, int n)
+{ D_orc_sfn(fn, r, rindex, pa, pb, index, n); }
 
 void orc_set_threads(int nthreads)
 {

@@ -171,6 +171,12 @@ void orc_F_sconv(int kind, float *r, const int *rindex, const float *const *pa, 
 void orc_D_sconv(int kind, double *r, const int *rindex, const double *const *pa, const double *const *pb,
                  const int *index, int n);
 
+/* elementary functions of real and complex fields (qla_oracle_vecops.inc) */
+void orc_F_sfn(int fn, float *r, const int *rindex, const float *const *pa, const void *const *pb,
+               const int *index, int n);
+void orc_D_sfn(int fn, double *r, const int *rindex, const double *const *pa, const void *const *pb,
+               const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -1146,6 +1146,88 @@ def sconvs() -> list[SconvVariant]:
     return out
 
 
+# elementary functions of real and complex fields: name -> (code, destination, sources, name form)
+SFN_UNARY = ("cos", "sin", "tan", "acos", "asin", "atan", "sqrt", "fabs", "exp", "log", "sign", "cosh", "sinh",
+             "tanh", "log10", "floor", "ceil")
+SFN = {f: (k, "R", ("R",), f + "_{a}") for k, f in enumerate(SFN_UNARY)}
+SFN.update({op: (20 + k, "R", ("R", "I" if op == "ldexp" else "R"), "{a}_" + op + "_{b}")
+            for k, op in enumerate(("mod", "max", "min", "pow", "atan2", "ldexp"))})
+SFN.update({"cexpi": (30, "C", ("R",), "cexpi_{a}"), "norm": (31, "R", ("C",), "norm_{a}"),
+            "arg": (32, "R", ("C",), "arg_{a}"), "cexp": (33, "C", ("C",), "cexp_{a}"),
+            "csqrt": (34, "C", ("C",), "csqrt_{a}"), "clog": (35, "C", ("C",), "clog_{a}")})
+
+
+@dataclass(frozen=True)
+class SfnVariant:
+    """Elementary functions of real and complex fields, eq only, every
+    indexing form (build_qla.pl:194-355)."""
+    prec: str
+    fn: str
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def binary(self) -> bool:
+        return len(SFN[self.fn][2]) == 2
+
+    @property
+    def name(self) -> str:
+        _, dest, srcs, fmt = SFN[self.fn]
+        form = "" if (self.ir or self.ia or self.ib) else ("x" if self.xf else "v")
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + srcs[0]
+        b = (("p" if self.pb else "") + ("x" if self.ib else "") + srcs[1]) if self.binary else ""
+        return f"QLA_{self.prec[0]}_{'x' if self.ir else ''}{dest}_{form}eq_{fmt.format(a=a, b=b)}"
+
+    def params(self) -> list[tuple[str, str]]:
+        _, dest, srcs, _ = SFN[self.fn]
+        ct = lambda x: "QLA_Int" if x == "I" else rec_type(self.prec, x)   # noqa: E731
+        out = [("p", f"{ct(dest)} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{ct(srcs[0])} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        if self.binary:
+            out.append(("p", f"{ct(srcs[1])} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+            if self.ib:
+                out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        b = "(const void *)b" if self.binary else "nullptr"
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"sfn_entry({prec}, {SFN[self.fn][0]}, {int(self.pa)}, {int(self.pb)}, (void *)r, (const void *)a, "
+                f"{b}, {idx}, {own}, n, \"{self.name}\");")
+
+
+def sfns() -> list[SfnVariant]:
+    out = []
+    for prec in PRECS:
+        for fn, (_, _, srcs, _) in SFN.items():
+            kw = dict(prec=prec, fn=fn)
+            if len(srcs) == 2:
+                out += [SfnVariant(xf=xf, pa=pa, pb=pb, **kw) for xf in (False, True) for pa in (False, True)
+                        for pb in (False, True)]
+                out += [SfnVariant(ir=ir, ia=ia, ib=ib, **kw) for ir, ia, ib in OWN_INDEX_FORMS]
+            else:
+                out += [SfnVariant(xf=xf, pa=pa, **kw) for xf in (False, True) for pa in (False, True)]
+                out += [SfnVariant(ir=ir, ia=ia, **kw) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -1186,7 +1268,7 @@ def all_symbols() -> list:
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
     by real and complex fields, outer products, products with propagators, component access, spin trace)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls() + pmuls() + comps() + sconvs())
+            + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns())
 
 
 def variants() -> list[Variant]:
@@ -1469,6 +1551,10 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "sfn": ("elementary functions of real and complex fields (eq): R = f(R) for cos ... ceil, "
+            "R = R op R for mod max min pow atan2, R = R ldexp I, C = cexpi R, R = norm / arg C, "
+            "C = cexp / csqrt / clog C; transcendental results within a few units in the last place of the "
+            "C library's", "build_qla.pl:194-355, make_code_unary.pl:103-176, expressions_tensor.pl:1021-1075"),
     "sconv": ("real / complex conversions C = R, C = R + iR, R = re C, R = im C and the division of "
               "real and complex fields (eq; C / C as QLA_c_eq_c_div_c)",
               "build_qla.pl:623-708, 1457-1482, include/qla_complex.h:252-255, 417-427"),
@@ -1524,8 +1610,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs():
-        if isinstance(v, SconvVariant):
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns():
+        if isinstance(v, SfnVariant):
+            key = (v.prec[0], "sfn")
+        elif isinstance(v, SconvVariant):
             key = (v.prec[0], "sconv")
         elif isinstance(v, CompVariant):
             key = (v.prec, "comp_" + v.qual)
@@ -1555,9 +1643,9 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry and sconv_entry are there.",
+        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry, sconv_entry and sfn_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs():
+    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

@@ -449,6 +449,27 @@ void sconv_entry(int prec, int kind, int pa, int pb, void *r, const void *a, con
     finish(qla_launch_sconv(prec, kind, p, ts.stream), name);
 }
 
+// elementary functions of real and complex fields (fn as qla_launch_sfn),
+// eq only, every form
+void sfn_entry(int prec, int fn, int pa, int pb, void *r, const void *a, const void *b, const int *index,
+               const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    finish(qla_launch_sfn(prec, fn, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -675,6 +696,13 @@ hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream
 {
     if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
     if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_sfn(int prec, int fn, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_sfn_launch_p0(fn, p, st);
+    if (prec == 1) return qla_sfn_launch_p1(fn, p, st);
     return hipErrorInvalidValue;
 }
 

@@ -138,6 +138,12 @@ hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_ma
 hipError_t qla_launch_sconv(int prec, int kind, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sconv_launch_p0(int kind, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st);
+// Elementary functions of real and complex fields (qla_vecops.hip): fn 0..16
+// R = f(R), 20..25 R = a op b (25: b an int field), 30 C = cexpi R, 31 norm,
+// 32 arg, 33 cexp, 34 csqrt, 35 clog
+hipError_t qla_launch_sfn(int prec, int fn, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_sfn_launch_p1(int fn, const qla_vec_args &p, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

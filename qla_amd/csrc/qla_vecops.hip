@@ -821,6 +821,144 @@ hipError_t launch_sconv_t(int kind, const qla_vec_args &p, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
+// Elementary functions of real and complex fields, eq only (build_qla.pl:
+// 194-355; make_code_unary.pl:103-176, expressions_tensor.pl:1021-1075,
+// lib/cmath).  FN 0..16 R = f(R) (cos sin tan acos asin atan sqrt fabs exp
+// log sign cosh sinh tanh log10 floor ceil; the single-precision function
+// for float fields, sign as a >= 0 ? 1 : -1); 20..25 R = a op b (fmod, max
+// as a > b ? a : b, min as a > b ? b : a, pow, atan2, ldexp with an int
+// field b) taken in double and rounded, as the reference's double calls;
+// 30 C = cexpi(R); 31 R = |C|; 32 R = arg C; 33 cexp; 34 csqrt
+// (lib/cmath/QLA_csqrt.c); 35 clog (0.5 log|z|^2 in double, arg z).  The
+// transcendental functions are the device library's, within a few units in
+// the last place of the C library's; the rest are exact.
+// ---------------------------------------------------------------------------
+#define QLA_SFN_MATH(NAME)                                                               \
+    template <class T> __device__ __forceinline__ T m_##NAME(T x)                        \
+    {                                                                                    \
+        if constexpr (sizeof(T) == 4) return NAME##f(x);                                 \
+        else return NAME(x);                                                             \
+    }
+QLA_SFN_MATH(cos)
+QLA_SFN_MATH(sin)
+QLA_SFN_MATH(tan)
+QLA_SFN_MATH(acos)
+QLA_SFN_MATH(asin)
+QLA_SFN_MATH(atan)
+QLA_SFN_MATH(sqrt)
+QLA_SFN_MATH(fabs)
+QLA_SFN_MATH(exp)
+QLA_SFN_MATH(log)
+QLA_SFN_MATH(cosh)
+QLA_SFN_MATH(sinh)
+QLA_SFN_MATH(tanh)
+QLA_SFN_MATH(log10)
+QLA_SFN_MATH(floor)
+QLA_SFN_MATH(ceil)
+#undef QLA_SFN_MATH
+template <class T> __device__ __forceinline__ T m_atan2(T y, T x)
+{
+    if constexpr (sizeof(T) == 4) return atan2f(y, x);
+    else return atan2(y, x);
+}
+template <class T> __device__ __forceinline__ T m_copysign(T x, T y)
+{
+    if constexpr (sizeof(T) == 4) return copysignf(x, y);
+    else return copysign(x, y);
+}
+
+template <class T, int FN> __device__ __forceinline__ T sfn1(T a)
+{
+    if constexpr (FN == 0) return m_cos(a);
+    else if constexpr (FN == 1) return m_sin(a);
+    else if constexpr (FN == 2) return m_tan(a);
+    else if constexpr (FN == 3) return m_acos(a);
+    else if constexpr (FN == 4) return m_asin(a);
+    else if constexpr (FN == 5) return m_atan(a);
+    else if constexpr (FN == 6) return m_sqrt(a);
+    else if constexpr (FN == 7) return m_fabs(a);
+    else if constexpr (FN == 8) return m_exp(a);
+    else if constexpr (FN == 9) return m_log(a);
+    else if constexpr (FN == 10) return a >= T(0) ? T(1) : T(-1);
+    else if constexpr (FN == 11) return m_cosh(a);
+    else if constexpr (FN == 12) return m_sinh(a);
+    else if constexpr (FN == 13) return m_tanh(a);
+    else if constexpr (FN == 14) return m_log10(a);
+    else if constexpr (FN == 15) return m_floor(a);
+    else return m_ceil(a);
+}
+
+template <class T, int FN>
+__global__ void __launch_bounds__(VBLOCK) k_vec_sfn(qla_vec_args p)
+{
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        const int i = (int)g;
+        const int s = p.index ? p.index[i] : i;
+        const int xs = p.ix ? p.ix[i] : s;
+        const int ys = p.iy ? p.iy[i] : s;
+        const int d = p.ir ? p.ir[i] : s;
+        if constexpr (FN < 20) {
+            ((T *)p.r)[d] = sfn1<T, FN>(sc_real<T>(p.x, p.px, xs));
+        } else if constexpr (FN < 30) {
+            const double a = sc_real<T>(p.x, p.px, xs);
+            double v;
+            if constexpr (FN == 25) {
+                v = ldexp(a, p.py ? *((const int *const *)p.y)[ys] : ((const int *)p.y)[ys]);
+            } else {
+                const double b = sc_real<T>(p.y, p.py, ys);
+                if constexpr (FN == 20) v = fmod(a, b);
+                else if constexpr (FN == 21) v = a > b ? a : b;
+                else if constexpr (FN == 22) v = a > b ? b : a;
+                else if constexpr (FN == 23) v = pow(a, b);
+                else v = atan2(a, b);
+            }
+            ((T *)p.r)[d] = (T)v;
+        } else if constexpr (FN == 30) {
+            const T a = sc_real<T>(p.x, p.px, xs);
+            ((cpx<T> *)p.r)[d] = { m_cos(a), m_sin(a) };
+        } else {
+            const cpx<T> z = sc_cpx<T>(p.x, p.px, xs);
+            if constexpr (FN == 31) {
+                ((T *)p.r)[d] = m_sqrt(z.re * z.re + z.im * z.im);
+            } else if constexpr (FN == 32) {
+                ((T *)p.r)[d] = m_atan2(z.im, z.re);
+            } else if constexpr (FN == 33) {
+                const T mag = m_exp(z.re);
+                ((cpx<T> *)p.r)[d] = { mag * m_cos(z.im), mag * m_sin(z.im) };
+            } else if constexpr (FN == 34) {
+                const T nz = m_sqrt(z.re * z.re + z.im * z.im);
+                const T sr = m_sqrt(m_fabs((T)(0.5 * (double)(nz + z.re))));
+                const T si = m_copysign(m_sqrt(m_fabs((T)(0.5 * (double)(nz - z.re)))), z.im);
+                ((cpx<T> *)p.r)[d] = { sr, si };
+            } else {
+                const T n2 = z.re * z.re + z.im * z.im;
+                ((cpx<T> *)p.r)[d] = { (T)(0.5 * log((double)n2)), m_atan2(z.im, z.re) };
+            }
+        }
+    }
+}
+
+template <class T>
+hipError_t launch_sfn_t(int fn, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const dim3 grid(blocks_for(p.n));
+#define QLA_SFN_CASE(F)                                                                   \
+    case F: hipLaunchKernelGGL((k_vec_sfn<T, F>), grid, dim3(VBLOCK), 0, st, p); break;
+    switch (fn) {
+        QLA_SFN_CASE(0) QLA_SFN_CASE(1) QLA_SFN_CASE(2) QLA_SFN_CASE(3) QLA_SFN_CASE(4) QLA_SFN_CASE(5)
+        QLA_SFN_CASE(6) QLA_SFN_CASE(7) QLA_SFN_CASE(8) QLA_SFN_CASE(9) QLA_SFN_CASE(10) QLA_SFN_CASE(11)
+        QLA_SFN_CASE(12) QLA_SFN_CASE(13) QLA_SFN_CASE(14) QLA_SFN_CASE(15) QLA_SFN_CASE(16)
+        QLA_SFN_CASE(20) QLA_SFN_CASE(21) QLA_SFN_CASE(22) QLA_SFN_CASE(23) QLA_SFN_CASE(24) QLA_SFN_CASE(25)
+        QLA_SFN_CASE(30) QLA_SFN_CASE(31) QLA_SFN_CASE(32) QLA_SFN_CASE(33) QLA_SFN_CASE(34) QLA_SFN_CASE(35)
+    default: return hipErrorInvalidValue;
+    }
+#undef QLA_SFN_CASE
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Fills (build_qla.pl:2561-2620).  FILL_ZERO: QLA_c_eq_r(e, 0.) on every
 // element.  FILL_REC: QLA_c_<eqop>_c(e, a.e) with one record a for every
 // step.  FILL_DIAG (colour matrices): QLA_c_<eqop>_r(e, 0.) on every element
@@ -1349,6 +1487,10 @@ hipError_t qla_sconv_launch_p0(int kind, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_sconv_t<float>(kind, p, st);
 }
+hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_sfn_t<float>(fn, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -1381,5 +1523,9 @@ hipError_t qla_comp_launch_p1(int mode, const qla_vec_args &p, const qla_comp_ma
 hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_sconv_t<double>(kind, p, st);
+}
+hipError_t qla_sfn_launch_p1(int fn, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_sfn_t<double>(fn, p, st);
 }
 #endif

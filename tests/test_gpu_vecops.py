@@ -17,8 +17,8 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, VECFAMS, comps, fieldmuls, fills, gammas,
-                             locals_, pmuls, sconvs, sums, vecops)
+from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, SFN, VECFAMS, comps, fieldmuls, fills,
+                             gammas, locals_, pmuls, sconvs, sfns, sums, vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -675,3 +675,89 @@ def test_sconv_symbol(orc, v):
     exp = cs.r0.copy()
     exp[rs] = sub
     assert_bitwise(host(r), exp, v.name)
+
+
+# exact on both sides (IEEE operations, exact conversions); the rest are the
+# device library's transcendental functions against the C library's
+SFN_EXACT = ("sqrt", "fabs", "sign", "floor", "ceil", "mod", "max", "min", "ldexp", "norm", "csqrt")
+SFN_ULPS = 6
+
+_sfn_in = {}
+
+
+def _sfn_inputs(prec, fn):
+    """(a, b) host fields of N records inside the function's domain, and
+    their device copies and pointer arrays (a permutation)"""
+    key = (prec, fn)
+    if key not in _sfn_in:
+        dt = np.float64 if prec == "D3" else np.float32
+        rng = np.random.default_rng(zlib.crc32(f"sfn {prec} {fn}".encode()))
+        _, _, srcs, _ = SFN[fn]
+        if srcs[0] == "C":
+            a = rng.uniform(-4, 4, (N, 2)).astype(dt)
+        elif fn in ("acos", "asin"):
+            a = rng.uniform(-1, 1, N).astype(dt)
+        elif fn in ("sqrt", "log", "log10", "pow"):
+            a = rng.uniform(1e-2, 20, N).astype(dt)
+        else:
+            a = rng.uniform(-8, 8, N).astype(dt)
+        if len(srcs) == 2:
+            b = rng.integers(-30, 30, N).astype(np.int32) if srcs[1] == "I" else rng.uniform(-6, 6, N).astype(dt)
+        else:
+            b = None
+        perm = rng.permutation(N)
+        da = dev(a)
+        db = None if b is None else dev(b)
+        _sfn_in[key] = (a, b, da, db, qla_amd.pointer_array(da, perm),
+                        None if b is None else qla_amd.pointer_array(db, perm), perm)
+    return _sfn_in[key]
+
+
+@pytest.mark.parametrize("v", sfns(), ids=lambda v: v.name)
+def test_sfn_symbol(orc, v):
+    """Every elementary function of real and complex fields against the oracle
+    (the C library): bit for bit for the exact ones, within a few units in
+    the last place for the transcendental ones; untouched records exact."""
+    _, dest, srcs, _ = SFN[v.fn]
+    cs = case(v.prec, dest)
+    a, b, da, db, pa, pb, perm = _sfn_inputs(v.prec, v.fn)
+    r = dev(cs.r0)
+    args = [r]
+    if v.ir:
+        args.append(cs.down["r"])
+    args.append(pa if v.pa else da)
+    if v.ia:
+        args.append(cs.down["x"])
+    if v.binary:
+        args.append(pb if v.pb else db)
+        if v.ib:
+            args.append(cs.down["y"])
+    if v.xf:
+        args.append(cs.dindex)
+    m = len(cs.index) if v.xf else (len(cs.own["r"]) if (v.ir or v.ia or v.ib) else cs.n)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = cs.index if v.xf else np.arange(m)
+    rs = cs.own["r"] if v.ir else site
+    as_ = cs.own["x"] if v.ia else site
+    bs = cs.own["y"] if v.ib else site
+    ga = np.ascontiguousarray(a[perm[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(b[perm[bs] if v.pb else bs]) if v.binary else None
+    sub = np.ascontiguousarray(cs.r0[rs])
+    orc.sfn(SFN[v.fn][0], sub, ga, gb)
+    exp = cs.r0.copy()
+    exp[rs] = sub
+    got = host(r)
+    if v.fn in SFN_EXACT:
+        assert_bitwise(got, exp, v.name)
+        return
+    untouched = np.ones(cs.n, bool)
+    untouched[rs] = False
+    assert_bitwise(got[untouched], exp[untouched], v.name + " untouched")
+    it = np.int32 if cs.dt == np.float32 else np.int64
+    gi = got[rs].astype(cs.dt).view(it).astype(np.int64)
+    ei = exp[rs].astype(cs.dt).view(it).astype(np.int64)
+    gi = np.where(gi < 0, np.iinfo(it).min - gi, gi)
+    ei = np.where(ei < 0, np.iinfo(it).min - ei, ei)
+    assert np.abs(gi - ei).max() <= SFN_ULPS, (v.name, np.abs(gi - ei).max())

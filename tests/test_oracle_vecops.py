@@ -955,3 +955,95 @@ def test_complex_division(orc, prec):
     eps = np.finfo(dt).eps
     err = np.abs(cx(r.astype(np.float64)) - want) / np.abs(want)
     assert err.max() < 8 * eps
+
+
+SFN_UNARY = ["cos", "sin", "tan", "acos", "asin", "atan", "sqrt", "fabs", "exp", "log", "sign", "cosh", "sinh",
+             "tanh", "log10", "floor", "ceil"]
+_NP = {"cos": np.cos, "sin": np.sin, "tan": np.tan, "acos": np.arccos, "asin": np.arcsin, "atan": np.arctan,
+       "sqrt": np.sqrt, "fabs": np.abs, "exp": np.exp, "log": np.log, "sign": lambda x: np.where(x >= 0, 1.0, -1.0),
+       "cosh": np.cosh, "sinh": np.sinh, "tanh": np.tanh, "log10": np.log10, "floor": np.floor, "ceil": np.ceil}
+SFN_EXACT = ("sqrt", "fabs", "sign", "floor", "ceil")
+
+
+def sfn_domain(name, n, seed, dt):
+    """inputs inside the function's domain, with signed zeros where it has them"""
+    rng = np.random.default_rng(seed)
+    if name in ("acos", "asin"):
+        x = rng.uniform(-1, 1, n)
+    elif name in ("sqrt", "log", "log10"):
+        x = rng.uniform(1e-3, 50, n)
+    elif name in ("exp", "cosh", "sinh"):
+        x = rng.uniform(-20, 20, n)
+    else:
+        x = rng.uniform(-10, 10, n)
+    x = x.astype(dt)
+    if name not in ("sqrt", "log", "log10"):
+        x[:2] = [0.0, -0.0]
+    return x
+
+
+def ulps(a, b, dt):
+    """distance in units in the last place between same-typed arrays"""
+    it = np.int32 if dt == np.float32 else np.int64
+    ia, ib = a.astype(dt).view(it).astype(np.int64), b.astype(dt).view(it).astype(np.int64)
+    ia = np.where(ia < 0, np.iinfo(it).min - ia, ia)
+    ib = np.where(ib < 0, np.iinfo(it).min - ib, ib)
+    return np.abs(ia - ib)
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("k,name", list(enumerate(SFN_UNARY)))
+def test_sfn_unary(orc, prec, k, name):
+    """R = f(R) against numpy in float64 rounded to the field type: exact for
+    sqrt, fabs, sign, floor and ceil, within 2 units in the last place for
+    the transcendental functions (the C library's own accuracy)."""
+    dt = DT[prec]
+    x = sfn_domain(name, 300, 250 + k, dt)
+    r = np.zeros_like(x)
+    orc.sfn(k, r, x)
+    want = _NP[name](x.astype(np.float64)).astype(dt)
+    if name in SFN_EXACT:
+        np.testing.assert_array_equal(r, want)
+    else:
+        assert ulps(r, want, dt).max() <= 2
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_sfn_binary_and_complex(orc, prec):
+    """fmod, max, min, ldexp exactly; pow and atan2 in double rounded to the
+    field type; cexpi, norm, arg, cexp, csqrt, clog against complex128."""
+    dt = DT[prec]
+    rng = np.random.default_rng(260)
+    n = 300
+    a = rng.uniform(-10, 10, n).astype(dt)
+    b = rng.uniform(-10, 10, n).astype(dt)
+    r = np.zeros(n, dt)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    for fn, want in ((20, np.fmod(a64, b64)), (21, np.where(a > b, a, b)), (22, np.where(a > b, b, a)),
+                     (24, np.arctan2(a64, b64))):
+        orc.sfn(fn, r, a, b)
+        if fn == 24:
+            assert ulps(r, want.astype(dt), dt).max() <= 1
+        else:
+            np.testing.assert_array_equal(r, want.astype(dt))
+    pa = np.abs(a) + dt(0.5)
+    orc.sfn(23, r, pa, b)
+    assert ulps(r, np.power(pa.astype(np.float64), b64).astype(dt), dt).max() <= 1
+    e = rng.integers(-20, 20, n).astype(np.int32)
+    orc.sfn(25, r, a, e)
+    np.testing.assert_array_equal(r, np.ldexp(a64, e).astype(dt))
+    c = np.zeros((n, 2), dt)
+    orc.sfn(30, c, a)
+    z = np.exp(1j * a64)
+    assert ulps(c[:, 0], z.real.astype(dt), dt).max() <= 2 and ulps(c[:, 1], z.imag.astype(dt), dt).max() <= 2
+    zc = field("C", n, 261, dt)
+    Z = cx(zc.astype(np.float64))
+    orc.sfn(31, r, zc)
+    np.testing.assert_array_equal(r, np.sqrt((zc[:, 0] * zc[:, 0] + zc[:, 1] * zc[:, 1]).astype(dt)))
+    orc.sfn(32, r, zc)
+    assert ulps(r, np.angle(Z).astype(dt), dt).max() <= 2
+    eps = np.finfo(dt).eps
+    for fn, f in ((33, np.exp), (34, np.sqrt), (35, np.log)):
+        orc.sfn(fn, c, zc)
+        got = cx(c.astype(np.float64))
+        assert (np.abs(got - f(Z)) / np.abs(f(Z))).max() < 16 * eps
