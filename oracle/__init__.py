@@ -65,6 +65,7 @@ def load():
         getattr(lib, f"orc_{p}_comp").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_sconv").argtypes = [i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_sfn").argtypes = [i32, vp, vp, vp, vp, vp, i32]
+        getattr(lib, f"orc_{p}_mask").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -516,4 +517,23 @@ def sfn(fn, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None, ia=None, 
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
     p = "F" if r.dtype == np.float32 else "D"
     getattr(lib, f"orc_{p}_sfn")(fn, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
+    return r
+
+
+def mask(op, r, a, b, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=None, n=None, real=None):
+    """copymask (op < 0; r, a records of any floating type, b an int32 mask
+    field) or the comparison op 0..5 (== != > < >= <=) of real fields a, b
+    into the int32 field r (orc_*_mask).  real: the field's float dtype
+    (defaults to a's)."""
+    lib = load()
+    pa = _records(a, ia if ia is not None else a_idx)
+    pb = _records(b, ib if ib is not None else b_idx)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else a.shape[0] if ia is None else len(ia))
+    dt = real or a.dtype
+    p = "F" if dt == np.float32 else "D"
+    nw = 0 if op >= 0 else int(np.prod(a.shape[1:], dtype=np.int64)) if a.ndim > 1 else 1
+    getattr(lib, f"orc_{p}_mask")(op, nw, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
     return r

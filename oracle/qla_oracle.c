@@ -190,6 +190,13 @@ void orc_D_sfn(int fn, double *r, const int *rindex, const double *const *pa, co
                const int *index, int n)
 { D_orc_sfn(fn, r, rindex, pa, pb, index, n); }
 
+void orc_F_mask(int op, int nw, void *r, const int *rindex, const float *const *pa, const void *const *pb,
+                const int *index, int n)
+{ F_orc_mask(op, nw, r, rindex, pa, pb, index, n); }
+void orc_D_mask(int op, int nw, void *r, const int *rindex, const double *const *pa, const void *const *pb,
+                const int *index, int n)
+{ D_orc_mask(op, nw, r, rindex, pa, pb, index, n); }
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

@@ -177,6 +177,12 @@ void orc_F_sfn(int fn, float *r, const int *rindex, const float *const *pa, cons
 void orc_D_sfn(int fn, double *r, const int *rindex, const double *const *pa, const void *const *pb,
                const int *index, int n);
 
+/* copymask (op < 0) and the comparisons of real fields (qla_oracle_vecops.inc) */
+void orc_F_mask(int op, int nw, void *r, const int *rindex, const float *const *pa, const void *const *pb,
+                const int *index, int n);
+void orc_D_mask(int op, int nw, void *r, const int *rindex, const double *const *pa, const void *const *pb,
+                const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -1228,6 +1228,79 @@ def sfns() -> list[SfnVariant]:
     return out
 
 
+CMP_OPS = ("eq", "ne", "gt", "lt", "ge", "le")
+
+
+@dataclass(frozen=True)
+class MaskVariant:
+    """Copymask r = a where the int field b is nonzero, for every floating-point
+    type (build_qla.pl:2058-2088), and the comparisons of real fields
+    I = R op R (:1976-2018); eq only, every binary form."""
+    prec: str
+    t: str              # copied type, or "cmp"
+    op: str = ""        # comparison
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def name(self) -> str:
+        form = "" if (self.ir or self.ia or self.ib) else ("x" if self.xf else "v")
+        pre = lambda ptr, own: ("p" if ptr else "") + ("x" if own else "")   # noqa: E731
+        if self.t == "cmp":
+            return (f"QLA_{self.prec[0]}_{'x' if self.ir else ''}I_{form}eq_{pre(self.pa, self.ia)}R_{self.op}_"
+                    f"{pre(self.pb, self.ib)}R")
+        return (f"QLA_{lib_of(self.prec, self.t)}_{'x' if self.ir else ''}{self.t}_{form}eq_"
+                f"{pre(self.pa, self.ia)}{self.t}_mask_{pre(self.pb, self.ib)}I")
+
+    def params(self) -> list[tuple[str, str]]:
+        if self.t == "cmp":
+            dt, at, bt = "QLA_Int", rec_type(self.prec, "R"), rec_type(self.prec, "R")
+        else:
+            dt = at = rec_type(self.prec, self.t)
+            bt = "QLA_Int"
+        out = [("p", f"{dt} *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        out.append(("p", f"{at} *QLA_MI355X_RESTRICT {'*' if self.pa else ''}a"))
+        if self.ia:
+            out.append(("p", "int *indexa"))
+        out.append(("p", f"{bt} *QLA_MI355X_RESTRICT {'*' if self.pb else ''}b"))
+        if self.ib:
+            out.append(("p", "int *indexb"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        prec = 1 if self.prec == "D3" else 0
+        op = CMP_OPS.index(self.op) if self.t == "cmp" else -1
+        nw = 0 if self.t == "cmp" else (1 if self.t == "R" else 2 * NCOMP[self.t])
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(x if f else "nullptr" for x, f in (("indexr", self.ir), ("indexa", self.ia),
+                                                          ("indexb", self.ib)))
+        return (f"mask_entry({prec}, {op}, {nw}, {int(self.pa)}, {int(self.pb)}, (void *)r, (const void *)a, "
+                f"(const void *)b, {idx}, {own}, n, \"{self.name}\");")
+
+
+def masks() -> list[MaskVariant]:
+    out = []
+    for prec in PRECS:
+        for t, op in [(t, "") for t in ARITH_TYPES] + [("cmp", o) for o in CMP_OPS]:
+            kw = dict(prec=prec, t=t, op=op)
+            out += [MaskVariant(xf=xf, pa=pa, pb=pb, **kw) for xf in (False, True) for pa in (False, True)
+                    for pb in (False, True)]
+            out += [MaskVariant(ir=ir, ia=ia, ib=ib, **kw) for ir, ia, ib in OWN_INDEX_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -1268,7 +1341,7 @@ def all_symbols() -> list:
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
     by real and complex fields, outer products, products with propagators, component access, spin trace)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns())
+            + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns() + masks())
 
 
 def variants() -> list[Variant]:
@@ -1551,6 +1624,10 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "mask": ("copymask, r = a where the int field b is nonzero (other records untouched)",
+             "build_qla.pl:2058-2088, expressions_tensor.pl:1065-1070"),
+    "cmp": ("comparisons of real fields, the int r = (a op b) for eq ne gt lt ge le",
+            "build_qla.pl:1976-2018, expressions_tensor.pl:1009-1019"),
     "sfn": ("elementary functions of real and complex fields (eq): R = f(R) for cos ... ceil, "
             "R = R op R for mod max min pow atan2, R = R ldexp I, C = cexpi R, R = norm / arg C, "
             "C = cexp / csqrt / clog C; transcendental results within a few units in the last place of the "
@@ -1586,6 +1663,12 @@ _VEC_DOC = {
 }
 
 
+def _vec_header_variants() -> list:
+    """the families declared in qla_mi355x_vecops.h, in header order"""
+    return (vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns()
+            + masks())
+
+
 def vecops_header_text() -> str:
     lines = [
         "/*",
@@ -1610,8 +1693,10 @@ def vecops_header_text() -> str:
         "#define QLA_MI355X_VECOPS_H",
     ]
     last = None
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns():
-        if isinstance(v, SfnVariant):
+    for v in _vec_header_variants():
+        if isinstance(v, MaskVariant):
+            key = (v.prec[0] if v.t in ("cmp", "R", "C") else v.prec, "cmp" if v.t == "cmp" else "mask")
+        elif isinstance(v, SfnVariant):
             key = (v.prec[0], "sfn")
         elif isinstance(v, SconvVariant):
             key = (v.prec[0], "sconv")
@@ -1643,9 +1728,9 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry, sconv_entry and sfn_entry are there.",
+        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry, sconv_entry, sfn_entry and mask_entry are there.",
     ]
-    for v in vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns():
+    for v in _vec_header_variants():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
     lines.append("")
     return "\n".join(lines)

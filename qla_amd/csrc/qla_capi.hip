@@ -470,6 +470,27 @@ void sfn_entry(int prec, int fn, int pa, int pb, void *r, const void *a, const v
     finish(qla_launch_sfn(prec, fn, p, ts.stream), name);
 }
 
+// copymask (op < 0: nw words per record) and the comparisons of real fields
+// (op 0..5), eq only, every form
+void mask_entry(int prec, int op, int nw, int pa, int pb, void *r, const void *a, const void *b, const int *index,
+                const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    finish(op < 0 ? qla_launch_mask(prec, nw, p, ts.stream) : qla_launch_cmp(prec, op, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
@@ -696,6 +717,20 @@ hipError_t qla_launch_outer(int prec, int eqop, const qla_vec_args &p, hipStream
 {
     if (prec == 0) return qla_outer_launch_p0(eqop, p, st);
     if (prec == 1) return qla_outer_launch_p1(eqop, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_mask(int prec, int nw, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_mask_launch_p0(nw, p, st);
+    if (prec == 1) return qla_mask_launch_p1(nw, p, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t qla_launch_cmp(int prec, int op, const qla_vec_args &p, hipStream_t st)
+{
+    if (prec == 0) return qla_cmp_launch_p0(op, p, st);
+    if (prec == 1) return qla_cmp_launch_p1(op, p, st);
     return hipErrorInvalidValue;
 }
 

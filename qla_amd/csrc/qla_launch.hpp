@@ -144,6 +144,15 @@ hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_launch_sfn(int prec, int fn, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sfn_launch_p1(int fn, const qla_vec_args &p, hipStream_t st);
+// Copymask and the comparisons of real fields (qla_vecops.hip): r = a where
+// the int field y is nonzero (nw words of the field type per record); the
+// int r = (a op b) for op 0 ==, 1 !=, 2 >, 3 <, 4 >=, 5 <=
+hipError_t qla_launch_mask(int prec, int nw, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_mask_launch_p0(int nw, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_mask_launch_p1(int nw, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_launch_cmp(int prec, int op, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_cmp_launch_p0(int op, const qla_vec_args &p, hipStream_t st);
+hipError_t qla_cmp_launch_p1(int op, const qla_vec_args &p, hipStream_t st);
 
 // Fills (qla_vecops.hip), n steps writing r at index[i] or i, records of
 // ncomp complexes: kind 0 r = 0 (eqop 0 only); 1 r eqop a, one record a

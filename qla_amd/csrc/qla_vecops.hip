@@ -959,6 +959,77 @@ hipError_t launch_sfn_t(int fn, const qla_vec_args &p, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
+// Copymask (build_qla.pl:2058-2088, expressions_tensor.pl:1065-1070): the
+// record of step i is copied, r = a, where the int mask b is nonzero and left
+// as it is elsewhere; nw words of type T per record (2 * ncomp for the
+// complex types, 1 for R).  One work item per word.  Comparisons of real
+// fields (build_qla.pl:1976-2018): the int r = (a op b), 1 or 0, for op
+// ==, !=, >, <, >=, <= (OP 0..5, expressions_tensor.pl:1009-1019).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(VBLOCK) k_vec_mask(qla_vec_args p, int nw)
+{
+    const long long tot = (long long)p.n * nw;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < tot; g += stride) {
+        const int i = (int)(g / nw), w = (int)(g - (long long)i * nw);
+        const int s = p.index ? p.index[i] : i;
+        const int ys = p.iy ? p.iy[i] : s;
+        const int mask = p.py ? *((const int *const *)p.y)[ys] : ((const int *)p.y)[ys];
+        if (!mask) continue;
+        const int xs = p.ix ? p.ix[i] : s;
+        const T *A = p.px ? ((const T *const *)p.x)[xs] : (const T *)p.x + (size_t)xs * nw;
+        ((T *)p.r)[(size_t)(p.ir ? p.ir[i] : s) * nw + w] = A[w];
+    }
+}
+
+template <class T, int OP>
+__global__ void __launch_bounds__(VBLOCK) k_vec_cmp(qla_vec_args p)
+{
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        const int i = (int)g;
+        const int s = p.index ? p.index[i] : i;
+        const T a = sc_real<T>(p.x, p.px, p.ix ? p.ix[i] : s);
+        const T b = sc_real<T>(p.y, p.py, p.iy ? p.iy[i] : s);
+        int v;
+        if constexpr (OP == 0) v = a == b;
+        else if constexpr (OP == 1) v = a != b;
+        else if constexpr (OP == 2) v = a > b;
+        else if constexpr (OP == 3) v = a < b;
+        else if constexpr (OP == 4) v = a >= b;
+        else v = a <= b;
+        ((int *)p.r)[p.ir ? p.ir[i] : s] = v;
+    }
+}
+
+template <class T>
+hipError_t launch_mask_t(int nw, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    if (nw < 1 || nw > 288) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_vec_mask<T>, dim3(blocks_for((long long)p.n * nw)), dim3(VBLOCK), 0, st, p, nw);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_cmp_t(int op, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+    const dim3 grid(blocks_for(p.n));
+    switch (op) {
+    case 0: hipLaunchKernelGGL((k_vec_cmp<T, 0>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((k_vec_cmp<T, 1>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_vec_cmp<T, 2>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_vec_cmp<T, 3>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 4: hipLaunchKernelGGL((k_vec_cmp<T, 4>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 5: hipLaunchKernelGGL((k_vec_cmp<T, 5>), grid, dim3(VBLOCK), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Fills (build_qla.pl:2561-2620).  FILL_ZERO: QLA_c_eq_r(e, 0.) on every
 // element.  FILL_REC: QLA_c_<eqop>_c(e, a.e) with one record a for every
 // step.  FILL_DIAG (colour matrices): QLA_c_<eqop>_r(e, 0.) on every element
@@ -1491,6 +1562,14 @@ hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_sfn_t<float>(fn, p, st);
 }
+hipError_t qla_mask_launch_p0(int nw, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_mask_t<float>(nw, p, st);
+}
+hipError_t qla_cmp_launch_p0(int op, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_cmp_t<float>(op, p, st);
+}
 #else
 hipError_t qla_vec_launch_p1(int op, int eqop, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -1527,5 +1606,13 @@ hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st)
 hipError_t qla_sfn_launch_p1(int fn, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_sfn_t<double>(fn, p, st);
+}
+hipError_t qla_mask_launch_p1(int nw, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_mask_t<double>(nw, p, st);
+}
+hipError_t qla_cmp_launch_p1(int op, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_cmp_t<double>(op, p, st);
 }
 #endif

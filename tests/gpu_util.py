@@ -22,7 +22,7 @@ def host(t) -> np.ndarray:
 
 def bits(a: np.ndarray) -> np.ndarray:
     a = np.ascontiguousarray(a)
-    return a.view(np.uint32 if a.dtype == np.float32 else np.uint64)
+    return a.view(np.uint32 if a.dtype.itemsize == 4 else np.uint64)
 
 
 def assert_bitwise(got: np.ndarray, exp: np.ndarray, what: str = ""):

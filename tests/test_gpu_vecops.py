@@ -17,8 +17,8 @@ import pytest
 
 import qla_amd
 from qla_amd import lattice
-from qla_amd.abi_gen import (COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, SFN, VECFAMS, comps, fieldmuls, fills,
-                             gammas, locals_, pmuls, sconvs, sfns, sums, vecops)
+from qla_amd.abi_gen import (CMP_OPS, COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, SFN, VECFAMS, comps, fieldmuls, fills,
+                             gammas, locals_, masks, pmuls, sconvs, sfns, sums, vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -761,3 +761,59 @@ def test_sfn_symbol(orc, v):
     gi = np.where(gi < 0, np.iinfo(it).min - gi, gi)
     ei = np.where(ei < 0, np.iinfo(it).min - ei, ei)
     assert np.abs(gi - ei).max() <= SFN_ULPS, (v.name, np.abs(gi - ei).max())
+
+
+_mask_in = {}
+
+
+def _int_mask(prec):
+    """an int mask field of N sites (zeros and nonzeros), device copy and a pointer array"""
+    if prec not in _mask_in:
+        rng = np.random.default_rng(zlib.crc32(f"mask {prec}".encode()))
+        m = rng.integers(-2, 3, N).astype(np.int32)
+        dm = dev(m)
+        perm = rng.permutation(N)
+        _mask_in[prec] = (m, dm, qla_amd.pointer_array(dm, perm), perm)
+    return _mask_in[prec]
+
+
+@pytest.mark.parametrize("v", masks(), ids=lambda v: v.name)
+def test_mask_symbol(orc, v):
+    """Every copymask and real-field comparison, bit for bit against the oracle
+    (untouched records included)."""
+    if v.t == "cmp":
+        src = case(v.prec, "R")
+        b_h, b_d, b_p, b_perm = src.y, src.dy, src.py, src.y_idx
+        dest = np.random.default_rng(5).integers(-9, 9, N).astype(np.int32)
+    else:
+        src = case(v.prec, v.t)
+        b_h, b_d, b_p, b_perm = _int_mask(v.prec)
+        dest = src.r0
+    ix = case(v.prec, "P")
+    r = dev(dest)
+    args = [r]
+    if v.ir:
+        args.append(ix.down["r"])
+    args.append(src.px if v.pa else src.dx)
+    if v.ia:
+        args.append(ix.down["x"])
+    args.append(b_p if v.pb else b_d)
+    if v.ib:
+        args.append(ix.down["y"])
+    if v.xf:
+        args.append(ix.dindex)
+    m = len(ix.index) if v.xf else (len(ix.own["r"]) if (v.ir or v.ia or v.ib) else N)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = ix.index if v.xf else np.arange(m)
+    rs = ix.own["r"] if v.ir else site
+    as_ = ix.own["x"] if v.ia else site
+    bs = ix.own["y"] if v.ib else site
+    ga = np.ascontiguousarray(src.x[src.x_idx[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(b_h[b_perm[bs] if v.pb else bs])
+    sub = np.ascontiguousarray(dest[rs])
+    orc.mask(CMP_OPS.index(v.op) if v.t == "cmp" else -1, sub, ga, gb, real=src.dt)
+    exp = dest.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)

@@ -1047,3 +1047,33 @@ def test_sfn_binary_and_complex(orc, prec):
         orc.sfn(fn, c, zc)
         got = cx(c.astype(np.float64))
         assert (np.abs(got - f(Z)) / np.abs(f(Z))).max() < 16 * eps
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+@pytest.mark.parametrize("t", ["R", "C", "V", "D", "P"])
+def test_copymask(orc, prec, t):
+    """r = a exactly where the mask is nonzero (any nonzero int), untouched elsewhere."""
+    dt = DT[prec]
+    n = 30
+    a = field(t, n, 270, dt)
+    r0 = field(t, n, 271, dt)
+    m = np.random.default_rng(272).integers(-2, 3, n).astype(np.int32)
+    r = r0.copy()
+    orc.mask(-1, r, a, m)
+    sel = (m != 0).reshape((n,) + (1,) * (a.ndim - 1))
+    np.testing.assert_array_equal(r, np.where(sel, a, r0))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_real_comparisons(orc, prec):
+    """I = (a op b) as numpy's comparisons, with ties, signed zeros and NaN."""
+    dt = DT[prec]
+    rng = np.random.default_rng(273)
+    a = rng.integers(-3, 4, 60).astype(dt)
+    b = rng.integers(-3, 4, 60).astype(dt)
+    a[:4] = [0.0, -0.0, np.nan, 1.0]
+    b[:4] = [-0.0, 0.0, 1.0, np.nan]
+    for op, f in enumerate((np.equal, np.not_equal, np.greater, np.less, np.greater_equal, np.less_equal)):
+        r = np.full(60, 7, np.int32)
+        orc.mask(op, r, a, b)
+        np.testing.assert_array_equal(r, f(a, b).astype(np.int32))
