@@ -66,6 +66,7 @@ def load():
         getattr(lib, f"orc_{p}_sconv").argtypes = [i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_sfn").argtypes = [i32, vp, vp, vp, vp, vp, i32]
         getattr(lib, f"orc_{p}_mask").argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
+    lib.orc_int.argtypes = [i32, i32, vp, vp, vp, vp, i32, vp, i32]
     lib.orc_convert_DF.argtypes = [vp, vp, C.c_size_t]
     lib.orc_convert_FD.argtypes = [vp, vp, C.c_size_t]
     lib.orc_D_spproj.argtypes = [vp, vp, i32, i32]
@@ -500,7 +501,8 @@ def sconv(kind, r, a, b=None, index=None, a_idx=None, b_idx=None, ir=None, ia=No
     rix = _index(ir)
     if n is None:
         n = len(ix) if ix is not None else (len(rix) if rix is not None else r.shape[0])
-    p = "F" if r.dtype == np.float32 else "D"
+    fl = r.dtype if r.dtype.kind == "f" else a.dtype      # the floating operand fixes the precision
+    p = "F" if fl == np.float32 else "D"
     getattr(lib, f"orc_{p}_sconv")(kind, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
     return r
 
@@ -536,4 +538,18 @@ def mask(op, r, a, b, index=None, a_idx=None, b_idx=None, ir=None, ia=None, ib=N
     p = "F" if dt == np.float32 else "D"
     nw = 0 if op >= 0 else int(np.prod(a.shape[1:], dtype=np.int64)) if a.ndim > 1 else 1
     getattr(lib, f"orc_{p}_mask")(op, nw, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), _ptr(ix), n)
+    return r
+
+
+def int_op(op, eqop, r, a=None, b=None, sc=0, index=None, ir=None, ia=None, ib=None, n=None):
+    """integer fields (orc_int): op and eqop as there; int32 arrays."""
+    lib = load()
+    pa = None if a is None else _records(a, ia)
+    pb = None if b is None else _records(b, ib)
+    ix = _index(index)
+    rix = _index(ir)
+    if n is None:
+        n = len(ix) if ix is not None else (len(rix) if rix is not None else (a.shape[0] if a is not None
+                                                                              else r.shape[0]))
+    lib.orc_int(op, eqop, _ptr(_need(r)), _ptr(rix), _ptr(pa), _ptr(pb), int(sc), _ptr(ix), n)
     return r

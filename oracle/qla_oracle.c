@@ -197,6 +197,57 @@ void orc_D_mask(int op, int nw, void *r, const int *rindex, const double *const 
                 const int *index, int n)
 { D_orc_mask(op, nw, r, rindex, pa, pb, index, n); }
 
+/* integer fields, as the reference's generated C writes them (build_qla.pl:
+ * 356-402, 1256-1294, 1421-1516, 1837-1888, 1936-2088, 2561-2620):
+ * v = a, i*a, !a, a + b, a - b, a / b, a * b, i*a + b, i*a - b, a << b,
+ * a >> b, a % b, max, min, ==, !=, >, <, >=, <=, |, &, ^, copymask, then
+ * r = v, r += v, r -= v or r = -v; op 40 / 41 the fills r = 0 / r eqop i */
+void orc_int(int op, int eqop, int *r, const int *rindex, const int *const *px, const int *const *py, int sc,
+             const int *index, int n)
+{
+    for (int i = 0; i < n; i++) {
+        const int s = index ? index[i] : i;
+        int *R = r + (rindex ? rindex[i] : s);
+        const int a = (op < 40) ? *px[s] : 0;
+        const int b = (op >= 10 && op < 40) ? *py[s] : 0;
+        int v;
+        switch (op) {
+        case 0: v = a; break;
+        case 1: v = sc * a; break;
+        case 2: v = !a; break;
+        case 10: v = a + b; break;
+        case 11: v = a - b; break;
+        case 12: v = a / b; break;
+        case 13: v = a * b; break;
+        case 14: v = sc * a + b; break;
+        case 15: v = sc * a - b; break;
+        case 16: v = a << b; break;
+        case 17: v = a >> b; break;
+        case 18: v = a % b; break;
+        case 19: v = a > b ? a : b; break;
+        case 20: v = a > b ? b : a; break;
+        case 21: v = ( a == b ); break;
+        case 22: v = ( a != b ); break;
+        case 23: v = ( a > b ); break;
+        case 24: v = ( a < b ); break;
+        case 25: v = ( a >= b ); break;
+        case 26: v = ( a <= b ); break;
+        case 27: v = ( a | b ); break;
+        case 28: v = ( a & b ); break;
+        case 29: v = ( a ^ b ); break;
+        case 30: if (!b) continue; v = a; break;
+        case 40: v = 0; break;
+        default: v = sc; break;
+        }
+        switch (eqop) {
+        case 0: *R = v; break;
+        case 1: *R += v; break;
+        case 2: *R -= v; break;
+        default: *R = -v; break;
+        }
+    }
+}
+
 void orc_set_threads(int nthreads)
 {
 #ifdef _OPENMP

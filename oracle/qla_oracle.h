@@ -183,6 +183,10 @@ void orc_F_mask(int op, int nw, void *r, const int *rindex, const float *const *
 void orc_D_mask(int op, int nw, void *r, const int *rindex, const double *const *pa, const void *const *pb,
                 const int *index, int n);
 
+/* integer fields (qla_oracle.c) */
+void orc_int(int op, int eqop, int *r, const int *rindex, const int *const *px, const int *const *py, int sc,
+             const int *index, int n);
+
 /* thread count the OpenMP site loops use (0 = OpenMP default) */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -94,6 +94,8 @@ def lib_of(prec: str, t: str) -> str:
 
 def rec_type(prec: str, t: str) -> str:
     """C type of a record of type t in library prec"""
+    if t == "I":
+        return "QLA_Int"
     if t in SCALAR_TYPES:
         return f"QLA_{prec[0]}_{SCALAR_TYPES[t]}"
     return f"QLA_{prec}_{TYPE_NAMES[t]}"
@@ -1072,7 +1074,8 @@ def comps() -> list[CompVariant]:
 
 # real / complex conversions and division: kind -> (destination, sources, name form)
 SCONV = {0: ("C", ("R",), "{a}"), 1: ("C", ("R", "R"), "{a}_plus_i_{b}"), 2: ("R", ("C",), "re_{a}"),
-         3: ("R", ("C",), "im_{a}"), 4: ("R", ("R", "R"), "{a}_divide_{b}"), 5: ("C", ("C", "C"), "{a}_divide_{b}")}
+         3: ("R", ("C",), "im_{a}"), 4: ("R", ("R", "R"), "{a}_divide_{b}"), 5: ("C", ("C", "C"), "{a}_divide_{b}"),
+         6: ("R", ("I",), "{a}"), 7: ("I", ("R",), "trunc_{a}"), 8: ("I", ("R",), "round_{a}")}
 
 
 @dataclass(frozen=True)
@@ -1301,6 +1304,111 @@ def masks() -> list[MaskVariant]:
     return out
 
 
+# integer fields: name -> (kernel op, binary, every eqop, int constant, name form)
+INT_OPS = {"copy": (0, False, True, False, "{a}"), "i_times": (1, False, True, True, "i_times_{a}"),
+           "not": (2, False, False, False, "not_{a}"), "times": (13, True, True, False, "{a}_times_{b}"),
+           "i_times_plus": (14, True, False, True, "i_times_{a}_plus_{b}"),
+           "i_times_minus": (15, True, False, True, "i_times_{a}_minus_{b}")}
+INT_OPS.update({op: (code, True, False, False, "{a}_" + op + "_{b}") for code, op in (
+    (10, "plus"), (11, "minus"), (12, "divide"), (16, "lshift"), (17, "rshift"), (18, "mod"), (19, "max"),
+    (20, "min"), (21, "eq"), (22, "ne"), (23, "gt"), (24, "lt"), (25, "ge"), (26, "le"), (27, "or"), (28, "and"),
+    (29, "xor"), (30, "mask"))})
+
+
+@dataclass(frozen=True)
+class IntVariant:
+    """The integer fields (no precision or colour: QLA_ prefix only,
+    prototype.pl:327-353): copies, the int constant, sums, division,
+    multiplication, shifts, mod, max, min, comparisons, boolean operations,
+    copymask (build_qla.pl:356-402, 1256-1294, 1421-1516, 1837-1888,
+    1936-2088) and the fills (:2561-2620); every form."""
+    op: str             # key of INT_OPS, or "zero" / "fill"
+    eqop: int = 0
+    xf: bool = False
+    pa: bool = False
+    pb: bool = False
+    ir: bool = False
+    ia: bool = False
+    ib: bool = False
+
+    @property
+    def fill(self) -> bool:
+        return self.op in ("zero", "fill")
+
+    @property
+    def binary(self) -> bool:
+        return not self.fill and INT_OPS[self.op][1]
+
+    @property
+    def scalar(self) -> bool:
+        return self.op == "fill" or (not self.fill and INT_OPS[self.op][3])
+
+    @property
+    def name(self) -> str:
+        form = "" if (self.ir or self.ia or self.ib) else ("x" if self.xf else "v")
+        head = f"QLA_{'x' if self.ir else ''}I_{form}{EQOPS[self.eqop]}_"
+        if self.op == "zero":
+            return head + "zero"
+        if self.op == "fill":
+            return head + "i"
+        a = ("p" if self.pa else "") + ("x" if self.ia else "") + "I"
+        b = ("p" if self.pb else "") + ("x" if self.ib else "") + "I"
+        return head + INT_OPS[self.op][4].format(a=a, b=b)
+
+    def _names(self) -> tuple[str, str]:
+        return ("b", "c") if self.scalar else ("a", "b")
+
+    def params(self) -> list[tuple[str, str]]:
+        out = [("p", "QLA_Int *QLA_MI355X_RESTRICT r")]
+        if self.ir:
+            out.append(("p", "int *indexr"))
+        if self.scalar:
+            out.append(("p", "QLA_Int *QLA_MI355X_RESTRICT a"))
+        if not self.fill:
+            xn, yn = self._names()
+            out.append(("p", f"QLA_Int *QLA_MI355X_RESTRICT {'*' if self.pa else ''}{xn}"))
+            if self.ia:
+                out.append(("p", f"int *index{xn}"))
+            if self.binary:
+                out.append(("p", f"QLA_Int *QLA_MI355X_RESTRICT {'*' if self.pb else ''}{yn}"))
+                if self.ib:
+                    out.append(("p", f"int *index{yn}"))
+        if self.xf:
+            out.append(("p", "int *index"))
+        out.append(("i", "int n"))
+        return out
+
+    def ctypes_kinds(self) -> str:
+        return "".join(k for k, _ in self.params())
+
+    def body(self) -> str:
+        code = 40 if self.op == "zero" else 41 if self.op == "fill" else INT_OPS[self.op][0]
+        xn, yn = self._names()
+        s = "(const void *)a" if self.scalar else "nullptr"
+        x = "nullptr" if self.fill else f"(const void *){xn}"
+        y = f"(const void *){yn}" if self.binary else "nullptr"
+        idx = "index" if self.xf else "nullptr"
+        own = ", ".join(v if f else "nullptr" for v, f in (("indexr", self.ir), (f"index{xn}", self.ia),
+                                                          (f"index{yn}", self.ib)))
+        return (f"int_entry({code}, {self.eqop}, {int(self.pa)}, {int(self.pb)}, (void *)r, {s}, {x}, {y}, {idx}, "
+                f"{own}, n, \"{self.name}\");")
+
+
+def ints() -> list[IntVariant]:
+    out = [IntVariant("zero", 0, xf) for xf in (False, True)]
+    out += [IntVariant("fill", e, xf) for e in range(4) for xf in (False, True)]
+    for op, (_, binary, alleq, _, _) in INT_OPS.items():
+        for e in (range(4) if alleq else (0,)):
+            if binary:
+                out += [IntVariant(op, e, xf, pa, pb) for xf in (False, True) for pa in (False, True)
+                        for pb in (False, True)]
+                out += [IntVariant(op, e, ir=ir, ia=ia, ib=ib) for ir, ia, ib in OWN_INDEX_FORMS]
+            else:
+                out += [IntVariant(op, e, xf, pa) for xf in (False, True) for pa in (False, True)]
+                out += [IntVariant(op, e, ir=ir, ia=ia) for ir, ia in UNARY_OWN_FORMS]
+    return out
+
+
 def fills() -> list[FillVariant]:
     out = []
     for prec in PRECS:
@@ -1341,7 +1449,7 @@ def all_symbols() -> list:
     field linear algebra, fills, gamma multiplications, site-local reductions, global sums, multiplication
     by real and complex fields, outer products, products with propagators, component access, spin trace)."""
     return (variants() + reductions() + conversions() + matfuncs() + vecops() + fills() + gammas() + locals_()
-            + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns() + masks())
+            + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns() + masks() + ints())
 
 
 def variants() -> list[Variant]:
@@ -1624,6 +1732,9 @@ _VEC_DOC = {
                "build_qla.pl:1517-1548, QLA_c_<eqop>_r_times_{c,ca}"),
     "fieldC": ("times a complex field, r eqop a(x) b(x) (`Ca`: conj(a); `Ta`: the adjoint of b)",
                "build_qla.pl:1551-1586, QLA_c_<eqop>_{c,ca}_times_{c,ca} (include/qla_complex.h:326-403)"),
+    "int": ("integer fields (QLA_Int): copies, the int constant, sums, division, products, shifts, mod, "
+            "max, min, comparisons, or / and / xor / not, copymask and the fills; C integer semantics",
+            "build_qla.pl:356-402, 1256-1294, 1421-1516, 1837-1888, 1936-2088, 2561-2620"),
     "mask": ("copymask, r = a where the int field b is nonzero (other records untouched)",
              "build_qla.pl:2058-2088, expressions_tensor.pl:1065-1070"),
     "cmp": ("comparisons of real fields, the int r = (a op b) for eq ne gt lt ge le",
@@ -1666,7 +1777,7 @@ _VEC_DOC = {
 def _vec_header_variants() -> list:
     """the families declared in qla_mi355x_vecops.h, in header order"""
     return (vecops() + fills() + gammas() + locals_() + sums() + fieldmuls() + pmuls() + comps() + sconvs() + sfns()
-            + masks())
+            + masks() + ints())
 
 
 def vecops_header_text() -> str:
@@ -1694,7 +1805,9 @@ def vecops_header_text() -> str:
     ]
     last = None
     for v in _vec_header_variants():
-        if isinstance(v, MaskVariant):
+        if isinstance(v, IntVariant):
+            key = ("I", "int")
+        elif isinstance(v, MaskVariant):
             key = (v.prec[0] if v.t in ("cmp", "R", "C") else v.prec, "cmp" if v.t == "cmp" else "mask")
         elif isinstance(v, SfnVariant):
             key = (v.prec[0], "sfn")
@@ -1728,7 +1841,8 @@ def vecops_inc_text() -> str:
         "// qla_vecops_abi.inc -- definitions of the field linear algebra of the C ABI.",
         "// GENERATED by qla_amd/abi_gen.py from its variant table; do not edit.",
         "// Included by qla_capi.hip inside extern \"C\"; vec_entry, fill_entry, gamma_entry, local_entry,",
-        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry, sconv_entry, sfn_entry and mask_entry are there.",
+        "// sum_entry, field_entry, outer_entry, pmul_entry, comp_entry, sconv_entry, sfn_entry, mask_entry and int_entry\n"
+        "// are there.",
     ]
     for v in _vec_header_variants():
         lines += [_decl(v), "{", "    " + v.body(), "}"]
@@ -1755,7 +1869,8 @@ def _split(name: str) -> tuple[str, str]:
 
 
 def generic_header_text() -> str:
-    names = [v.name for v in all_symbols()]
+    # the integer-field functions carry no library letter: their names are already generic
+    names = [v.name for v in all_symbols() if not isinstance(v, IntVariant)]
     groups: dict[tuple[str, str], list[str]] = {}
     for nm in names:
         lib, rest = _split(nm)

@@ -491,6 +491,35 @@ void mask_entry(int prec, int op, int nw, int pa, int pb, void *r, const void *a
     finish(op < 0 ? qla_launch_mask(prec, nw, p, ts.stream) : qla_launch_cmp(prec, op, p, ts.stream), name);
 }
 
+// integer fields (op and eqop as qla_int_launch); a: the int constant of the
+// i_times forms and of the fill r eqop i, in host or device memory
+void int_entry(int op, int eqop, int pa, int pb, void *r, const void *s, const void *a, const void *b,
+               const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
+{
+    if (n <= 0) return;
+    qla_vec_args p;
+    memset(&p, 0, sizeof(p));
+    p.r = r;
+    p.x = a;
+    p.y = b;
+    p.px = pa;
+    p.py = pb;
+    p.index = index;
+    p.ir = ir;
+    p.ix = ia;
+    p.iy = ib;
+    p.n = n;
+    if (op == 1 || op == 14 || op == 15 || op == 41) {
+        if (!s) {
+            record(hipErrorInvalidValue, name);
+            return;
+        }
+        if (is_device_pointer(s)) p.sp = s;
+        else p.s_re = *(const int *)s;
+    }
+    finish(qla_int_launch(op, eqop, p, ts.stream), name);
+}
+
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {

@@ -144,6 +144,9 @@ hipError_t qla_sconv_launch_p1(int kind, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_launch_sfn(int prec, int fn, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st);
 hipError_t qla_sfn_launch_p1(int fn, const qla_vec_args &p, hipStream_t st);
+// Integer fields (qla_vecops.hip): op and eqop as k_int documents; the int
+// constant in sp (device) or s_re
+hipError_t qla_int_launch(int op, int eqop, const qla_vec_args &p, hipStream_t st);
 // Copymask and the comparisons of real fields (qla_vecops.hip): r = a where
 // the int field y is nonzero (nw words of the field type per record); the
 // int r = (a op b) for op 0 ==, 1 !=, 2 >, 3 <, 4 >=, 5 <=

@@ -758,7 +758,8 @@ hipError_t launch_comp_t(int mode, const qla_vec_args &p, const qla_comp_map &m,
 // Conversions between real and complex fields and the division of scalar
 // fields, eq only (build_qla.pl:623-708, 1457-1482): KIND 0 C = R
 // (QLA_c_eq_r_plus_ir(c, a, 0.)), 1 C = R + i R, 2 R = re C, 3 R = im C,
-// 4 R = R / R, 5 C = C / C (QLA_c_eq_c_div_c, include/qla_complex.h:417-427:
+// 4 R = R / R, 6 R = I, 7 I = trunc R ((int) a), 8 I = round R ((int) roundf / round,
+// build_qla.pl:709-766), 5 C = C / C (QLA_c_eq_c_div_c, include/qla_complex.h:417-427:
 // t = 1.0 / |b|^2 in double rounded to the field type, then
 // (a.re b.re + a.im b.im) t and (a.im b.re - a.re b.im) t).  Single-precision
 // quotients are taken in double and rounded, which is the correctly rounded
@@ -792,6 +793,14 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_sconv(qla_vec_args p)
             ((T *)p.r)[d] = sc_cpx<T>(p.x, p.px, xs).re;
         } else if constexpr (KIND == 3) {
             ((T *)p.r)[d] = sc_cpx<T>(p.x, p.px, xs).im;
+        } else if constexpr (KIND == 6) {
+            ((T *)p.r)[d] = (T)(p.px ? *((const int *const *)p.x)[xs] : ((const int *)p.x)[xs]);
+        } else if constexpr (KIND == 7) {
+            ((int *)p.r)[d] = (int)sc_real<T>(p.x, p.px, xs);
+        } else if constexpr (KIND == 8) {
+            const T a = sc_real<T>(p.x, p.px, xs);
+            if constexpr (sizeof(T) == 4) ((int *)p.r)[d] = (int)roundf(a);
+            else ((int *)p.r)[d] = (int)round(a);
         } else if constexpr (KIND == 4) {
             ((T *)p.r)[d] = (T)((double)sc_real<T>(p.x, p.px, xs) / (double)sc_real<T>(p.y, p.py, ys));
         } else {
@@ -815,6 +824,9 @@ hipError_t launch_sconv_t(int kind, const qla_vec_args &p, hipStream_t st)
     case 3: hipLaunchKernelGGL((k_vec_sconv<T, 3>), grid, dim3(VBLOCK), 0, st, p); break;
     case 4: hipLaunchKernelGGL((k_vec_sconv<T, 4>), grid, dim3(VBLOCK), 0, st, p); break;
     case 5: hipLaunchKernelGGL((k_vec_sconv<T, 5>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 6: hipLaunchKernelGGL((k_vec_sconv<T, 6>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 7: hipLaunchKernelGGL((k_vec_sconv<T, 7>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 8: hipLaunchKernelGGL((k_vec_sconv<T, 8>), grid, dim3(VBLOCK), 0, st, p); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1027,6 +1039,103 @@ hipError_t launch_cmp_t(int op, const qla_vec_args &p, hipStream_t st)
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Integer fields (QLA_Int records; the integer cases of build_qla.pl:356-402,
+// 1256-1294, 1421-1516, 1837-1888, 1936-2088, 2561-2620): r eqop v with v
+// one of (OP) 0 a, 1 i*a (the constant), 2 !a, 10 a + b, 11 a - b, 12 a / b,
+// 13 a * b, 14 i*a + b, 15 i*a - b, 16 a << b, 17 a >> b, 18 a % b, 19 max
+// (a > b ? a : b), 20 min (a > b ? b : a), 21..26 a == != > < >= <= b,
+// 27 a | b, 28 a & b, 29 a ^ b, 30 a where b is nonzero (copymask); eqop eq
+// r = v, peq r += v, meq r -= v, eqm r = -v.  C integer semantics (division
+// and % truncate toward zero).  OP 40 / 41 are the fills r = 0 and
+// r eqop i.  One work item per record.
+// ---------------------------------------------------------------------------
+template <int OP, int EQOP>
+__global__ void __launch_bounds__(VBLOCK) k_int(qla_vec_args p)
+{
+    int sc = 0;
+    if constexpr (OP == 1 || OP == 14 || OP == 15 || OP == 41) sc = p.sp ? *(const int *)p.sp : (int)p.s_re;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < p.n; g += stride) {
+        const int i = (int)g;
+        const int s = p.index ? p.index[i] : i;
+        int *R = (int *)p.r + (p.ir ? p.ir[i] : s);
+        int v = 0;
+        if constexpr (OP == 41) {
+            v = sc;
+        } else if constexpr (OP != 40) {
+            const int xs = p.ix ? p.ix[i] : s;
+            const int a = p.px ? *((const int *const *)p.x)[xs] : ((const int *)p.x)[xs];
+            int b = 0;
+            if constexpr (OP >= 10) {
+                const int ys = p.iy ? p.iy[i] : s;
+                b = p.py ? *((const int *const *)p.y)[ys] : ((const int *)p.y)[ys];
+            }
+            if constexpr (OP == 0) v = a;
+            else if constexpr (OP == 1) v = sc * a;
+            else if constexpr (OP == 2) v = !a;
+            else if constexpr (OP == 10) v = a + b;
+            else if constexpr (OP == 11) v = a - b;
+            else if constexpr (OP == 12) v = a / b;
+            else if constexpr (OP == 13) v = a * b;
+            else if constexpr (OP == 14) v = sc * a + b;
+            else if constexpr (OP == 15) v = sc * a - b;
+            else if constexpr (OP == 16) v = a << b;
+            else if constexpr (OP == 17) v = a >> b;
+            else if constexpr (OP == 18) v = a % b;
+            else if constexpr (OP == 19) v = a > b ? a : b;
+            else if constexpr (OP == 20) v = a > b ? b : a;
+            else if constexpr (OP == 21) v = a == b;
+            else if constexpr (OP == 22) v = a != b;
+            else if constexpr (OP == 23) v = a > b;
+            else if constexpr (OP == 24) v = a < b;
+            else if constexpr (OP == 25) v = a >= b;
+            else if constexpr (OP == 26) v = a <= b;
+            else if constexpr (OP == 27) v = a | b;
+            else if constexpr (OP == 28) v = a & b;
+            else if constexpr (OP == 29) v = a ^ b;
+            else {
+                if (!b) continue;
+                v = a;
+            }
+        }
+        if constexpr (EQOP == 0) *R = v;
+        else if constexpr (EQOP == 1) *R += v;
+        else if constexpr (EQOP == 2) *R -= v;
+        else *R = -v;
+    }
+}
+
+template <int OP>
+hipError_t launch_int_e(int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    const dim3 grid(blocks_for(p.n));
+    switch (eqop) {
+    case 0: hipLaunchKernelGGL((k_int<OP, 0>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 1: hipLaunchKernelGGL((k_int<OP, 1>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 2: hipLaunchKernelGGL((k_int<OP, 2>), grid, dim3(VBLOCK), 0, st, p); break;
+    case 3: hipLaunchKernelGGL((k_int<OP, 3>), grid, dim3(VBLOCK), 0, st, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_int(int op, int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    if (p.n <= 0) return hipSuccess;
+#define QLA_INT_CASE(O) \
+    case O: return launch_int_e<O>(eqop, p, st);
+    switch (op) {
+        QLA_INT_CASE(0) QLA_INT_CASE(1) QLA_INT_CASE(2) QLA_INT_CASE(10) QLA_INT_CASE(11) QLA_INT_CASE(12)
+        QLA_INT_CASE(13) QLA_INT_CASE(14) QLA_INT_CASE(15) QLA_INT_CASE(16) QLA_INT_CASE(17) QLA_INT_CASE(18)
+        QLA_INT_CASE(19) QLA_INT_CASE(20) QLA_INT_CASE(21) QLA_INT_CASE(22) QLA_INT_CASE(23) QLA_INT_CASE(24)
+        QLA_INT_CASE(25) QLA_INT_CASE(26) QLA_INT_CASE(27) QLA_INT_CASE(28) QLA_INT_CASE(29) QLA_INT_CASE(30)
+        QLA_INT_CASE(40) QLA_INT_CASE(41)
+    }
+#undef QLA_INT_CASE
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------
@@ -1557,6 +1666,10 @@ hipError_t qla_comp_launch_p0(int mode, const qla_vec_args &p, const qla_comp_ma
 hipError_t qla_sconv_launch_p0(int kind, const qla_vec_args &p, hipStream_t st)
 {
     return qla::launch_sconv_t<float>(kind, p, st);
+}
+hipError_t qla_int_launch(int op, int eqop, const qla_vec_args &p, hipStream_t st)
+{
+    return qla::launch_int(op, eqop, p, st);
 }
 hipError_t qla_sfn_launch_p0(int fn, const qla_vec_args &p, hipStream_t st)
 {

@@ -38,13 +38,14 @@ def test_generated_site_abi_is_current():
     names = [v.name for v in abi_gen.variants()]
     assert len(names) == len(set(names)) == 2328
     every = [v.name for v in abi_gen.all_symbols()]
-    assert len(every) == len(set(every)) == 13933
+    assert len(every) == len(set(every)) == 14408
     assert len(abi_gen.fieldmuls()) == 2640
     assert len(abi_gen.pmuls()) == 1440
     assert len(abi_gen.comps()) == 294
-    assert len(abi_gen.sconvs()) == 132
+    assert len(abi_gen.sconvs()) == 174
     assert len(abi_gen.sfns()) == 502
     assert len(abi_gen.masks()) == 390
+    assert len(abi_gen.ints()) == 433
     assert len(abi_gen.sums()) == 84
     assert len(abi_gen.locals_()) == 1536
     assert len(abi_gen.gammas()) == 42

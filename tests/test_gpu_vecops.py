@@ -18,7 +18,7 @@ import pytest
 import qla_amd
 from qla_amd import lattice
 from qla_amd.abi_gen import (CMP_OPS, COMP_FAMS, LOCALS, NCOMP, PMUL_KINDS, SCONV, SFN, VECFAMS, comps, fieldmuls, fills,
-                             gammas, locals_, masks, pmuls, sconvs, sfns, sums, vecops)
+                             gammas, ints, locals_, masks, pmuls, sconvs, sfns, sums, vecops)
 from gpu_util import assert_bitwise, dev, host, say
 
 pytestmark = pytest.mark.gpu
@@ -639,7 +639,7 @@ def test_scalar_field_device_constant(orc, prec, t, fam):
     assert_bitwise(got, exp, f"{v.name} device constant")
 
 
-@pytest.mark.parametrize("v", sconvs(), ids=lambda v: v.name)
+@pytest.mark.parametrize("v", [v for v in sconvs() if v.kind < 6], ids=lambda v: v.name)
 def test_sconv_symbol(orc, v):
     """Every real / complex conversion and scalar-field division, bit for bit
     against the oracle (untouched records included)."""
@@ -815,5 +815,111 @@ def test_mask_symbol(orc, v):
     sub = np.ascontiguousarray(dest[rs])
     orc.mask(CMP_OPS.index(v.op) if v.t == "cmp" else -1, sub, ga, gb, real=src.dt)
     exp = dest.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)
+
+
+class IntCase:
+    """int fields of N records (a, b, r0), device copies and pointer arrays"""
+
+    def __init__(self, op):
+        rng = np.random.default_rng(zlib.crc32(f"int {op}".encode()))
+        self.a = rng.integers(-500, 500, N).astype(np.int32)
+        self.b = rng.integers(-9, 10, N).astype(np.int32)
+        if op in ("lshift", "rshift"):
+            self.b = rng.integers(0, 20, N).astype(np.int32)
+            if op == "lshift":
+                self.a = np.abs(self.a).astype(np.int32)
+        if op in ("divide", "mod"):
+            self.b[self.b == 0] = 7
+        self.r0 = rng.integers(-100, 100, N).astype(np.int32)
+        self.perm = rng.permutation(N)
+        self.da, self.db = dev(self.a), dev(self.b)
+        self.pa = qla_amd.pointer_array(self.da, self.perm)
+        self.pb = qla_amd.pointer_array(self.db, self.perm)
+
+
+_icases = {}
+
+
+@pytest.mark.parametrize("v", ints(), ids=lambda v: v.name)
+def test_int_symbol(orc, v):
+    """Every integer-field function, exactly against the oracle (untouched
+    records included)."""
+    if v.op not in _icases:
+        _icases[v.op] = IntCase(v.op)
+    ic = _icases[v.op]
+    ix = case("F3", "P")
+    sc = np.array([-3], np.int32)
+    r = dev(ic.r0)
+    args = [r]
+    if v.ir:
+        args.append(ix.down["r"])
+    if v.scalar:
+        args.append(sc.ctypes.data)
+    if not v.fill:
+        args.append(ic.pa if v.pa else ic.da)
+        if v.ia:
+            args.append(ix.down["x"])
+        if v.binary:
+            args.append(ic.pb if v.pb else ic.db)
+            if v.ib:
+                args.append(ix.down["y"])
+    if v.xf:
+        args.append(ix.dindex)
+    m = len(ix.index) if v.xf else (len(ix.own["r"]) if (v.ir or v.ia or v.ib) else N)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = ix.index if v.xf else np.arange(m)
+    rs = ix.own["r"] if v.ir else site
+    as_ = ix.own["x"] if v.ia else site
+    bs = ix.own["y"] if v.ib else site
+    ga = None if v.fill else np.ascontiguousarray(ic.a[ic.perm[as_] if v.pa else as_])
+    gb = np.ascontiguousarray(ic.b[ic.perm[bs] if v.pb else bs]) if v.binary else None
+    sub = np.ascontiguousarray(ic.r0[rs])
+    from qla_amd.abi_gen import INT_OPS
+    code = 40 if v.op == "zero" else 41 if v.op == "fill" else INT_OPS[v.op][0]
+    orc.int_op(code, v.eqop, sub, ga, gb, sc=-3, n=m)
+    exp = ic.r0.copy()
+    exp[rs] = sub
+    assert_bitwise(host(r), exp, v.name)
+
+
+@pytest.mark.parametrize("v", [v for v in sconvs() if v.kind >= 6], ids=lambda v: v.name)
+def test_real_int_conversion_symbol(orc, v):
+    """R = I, I = trunc R, I = round R in every form, exactly against the oracle."""
+    dt = np.float64 if v.prec == "D3" else np.float32
+    rng = np.random.default_rng(zlib.crc32(v.name.encode()))
+    ix = case(v.prec, "P")
+    if v.kind == 6:
+        a = rng.integers(-10000, 10000, N).astype(np.int32)
+        r0 = rng.standard_normal(N).astype(dt)
+    else:
+        a = rng.uniform(-1000, 1000, N).astype(dt)
+        a[:4] = [2.5, -2.5, 0.5, -1.5]
+        r0 = rng.integers(-9, 9, N).astype(np.int32)
+    perm = rng.permutation(N)
+    da = dev(a)
+    r = dev(r0)
+    args = [r]
+    if v.ir:
+        args.append(ix.down["r"])
+    args.append(qla_amd.pointer_array(da, perm) if v.pa else da)
+    if v.ia:
+        args.append(ix.down["x"])
+    if v.xf:
+        args.append(ix.dindex)
+    m = len(ix.index) if v.xf else (len(ix.own["r"]) if (v.ir or v.ia) else N)
+    say(f"{v.name} n={m}")
+    qla_amd.call(v.name, *args, m)
+    qla_amd.check()
+    site = ix.index if v.xf else np.arange(m)
+    rs = ix.own["r"] if v.ir else site
+    as_ = ix.own["x"] if v.ia else site
+    ga = np.ascontiguousarray(a[perm[as_] if v.pa else as_])
+    sub = np.ascontiguousarray(r0[rs])
+    orc.sconv(v.kind, sub, ga)
+    exp = r0.copy()
     exp[rs] = sub
     assert_bitwise(host(r), exp, v.name)

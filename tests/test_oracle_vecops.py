@@ -1077,3 +1077,84 @@ def test_real_comparisons(orc, prec):
         r = np.full(60, 7, np.int32)
         orc.mask(op, r, a, b)
         np.testing.assert_array_equal(r, f(a, b).astype(np.int32))
+
+
+def _c_div(a, b):
+    """C integer division and remainder (truncation toward zero)"""
+    q = np.abs(a) // np.abs(b) * np.sign(a) * np.sign(b)
+    return q.astype(np.int32), (a - q * b).astype(np.int32)
+
+
+INT_REF = {
+    0: lambda a, b, s: a, 1: lambda a, b, s: s * a, 2: lambda a, b, s: (a == 0).astype(np.int32),
+    10: lambda a, b, s: a + b, 11: lambda a, b, s: a - b, 12: lambda a, b, s: _c_div(a, b)[0],
+    13: lambda a, b, s: a * b, 14: lambda a, b, s: s * a + b, 15: lambda a, b, s: s * a - b,
+    16: lambda a, b, s: np.left_shift(np.abs(a), np.abs(b) % 8) if False else None,
+    18: lambda a, b, s: _c_div(a, b)[1], 19: lambda a, b, s: np.maximum(a, b), 20: lambda a, b, s: np.minimum(a, b),
+    21: lambda a, b, s: a == b, 22: lambda a, b, s: a != b, 23: lambda a, b, s: a > b, 24: lambda a, b, s: a < b,
+    25: lambda a, b, s: a >= b, 26: lambda a, b, s: a <= b, 27: lambda a, b, s: a | b, 28: lambda a, b, s: a & b,
+    29: lambda a, b, s: a ^ b}
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 10, 11, 12, 13, 14, 15, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29])
+@pytest.mark.parametrize("eqop", range(4))
+def test_integer_fields(orc, op, eqop):
+    """Integer-field operations against numpy's int32 arithmetic (C division
+    and remainder truncating toward zero), every eqop."""
+    rng = np.random.default_rng(280 + op)
+    n = 80
+    a = rng.integers(-50, 51, n).astype(np.int32)
+    b = rng.integers(-9, 10, n).astype(np.int32)
+    b[b == 0] = 3
+    a[:3] = [0, 0, -7]
+    r0 = rng.integers(-20, 21, n).astype(np.int32)
+    r = r0.copy()
+    orc.int_op(op, eqop, r, a, b if op >= 10 else None, sc=-3)
+    v = np.asarray(INT_REF[op](a, b, -3)).astype(np.int32)
+    want = {0: v, 1: r0 + v, 2: r0 - v, 3: -v}[eqop]
+    np.testing.assert_array_equal(r, want)
+
+
+def test_integer_shifts_mask_and_fills(orc):
+    """<< and >> on non-negative counts below 31 (arithmetic right shift of
+    negatives), copymask with any nonzero mask, and the fills."""
+    rng = np.random.default_rng(290)
+    n = 60
+    a = rng.integers(-1000, 1000, n).astype(np.int32)
+    b = rng.integers(0, 20, n).astype(np.int32)
+    r = np.zeros(n, np.int32)
+    orc.int_op(16, 0, r, np.abs(a).astype(np.int32), b)
+    np.testing.assert_array_equal(r, np.left_shift(np.abs(a).astype(np.int32), b))
+    orc.int_op(17, 0, r, a, b)
+    np.testing.assert_array_equal(r, np.right_shift(a, b))
+    m = rng.integers(-2, 3, n).astype(np.int32)
+    r0 = rng.integers(-5, 5, n).astype(np.int32)
+    r = r0.copy()
+    orc.int_op(30, 0, r, a, m)
+    np.testing.assert_array_equal(r, np.where(m != 0, a, r0))
+    orc.int_op(40, 0, r, n=n)
+    np.testing.assert_array_equal(r, 0)
+    for eqop, want in enumerate((lambda x: np.full_like(x, 9), lambda x: x + 9, lambda x: x - 9,
+                                 lambda x: np.full_like(x, -9))):
+        r = r0.copy()
+        orc.int_op(41, eqop, r, sc=9, n=n)
+        np.testing.assert_array_equal(r, want(r0))
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_real_integer_conversions(orc, prec):
+    """R = I exactly; I = trunc R and I = round R as numpy's trunc and
+    round-half-away-from-zero."""
+    dt = DT[prec]
+    rng = np.random.default_rng(295)
+    i = rng.integers(-1000, 1000, 50).astype(np.int32)
+    r = np.zeros(50, dt)
+    orc.sconv(6, r, i)
+    np.testing.assert_array_equal(r, i.astype(dt))
+    x = (rng.uniform(-100, 100, 50)).astype(dt)
+    x[:4] = [2.5, -2.5, 0.5, -1.5]
+    out = np.zeros(50, np.int32)
+    orc.sconv(7, out, x)
+    np.testing.assert_array_equal(out, np.trunc(x).astype(np.int32))
+    orc.sconv(8, out, x)
+    np.testing.assert_array_equal(out, (np.sign(x) * np.floor(np.abs(x) + dt(0.5))).astype(np.int32))
