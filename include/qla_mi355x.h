@@ -194,6 +194,19 @@ void QLA_MI355X_synchronize(void);
  * entry points themselves return void, like the reference. */
 int QLA_MI355X_last_error(void);
 const char *QLA_MI355X_last_error_string(void);
+/* Device scratch of the global reductions and sums: one small buffer (32 KiB
+ * of partials, or 4.5 MiB for the field sums) per (host thread, device,
+ * stream) that calls them, allocated and zeroed on that triple's first
+ * reduction (which therefore must not be inside a stream capture: the call
+ * then records hipErrorStreamCaptureUnsupported and does nothing; later
+ * reductions on that stream can be captured when their result pointer is
+ * device memory -- a host result waits on the host and is refused under
+ * capture the same way).  Reductions on different streams never share a buffer.  A
+ * thread's buffers are released when it exits; QLA_MI355X_finalize releases
+ * every buffer of every thread (no library call may be in flight), and
+ * QLA_MI355X_scratch_buffers counts the live ones. */
+void QLA_MI355X_finalize(void);
+int QLA_MI355X_scratch_buffers(void);
 /* Launch-shape knob of the site kernels (spproj, sprecon,
  * matvec): tiles of 64 sites per wavefront in the grid (default 1).  Set by
  * the autotuner; results do not depend on it. */

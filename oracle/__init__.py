@@ -48,6 +48,9 @@ def load():
         f.argtypes, f.restype = [vp, i32, i32], C.c_double
         f = getattr(lib, f"orc_{p}_norm2_ld")
         f.argtypes, f.restype = [vp, i32, i32], C.c_longdouble
+        f = getattr(lib, f"orc_{p}_norm2_terms_ld")
+        f.argtypes, f.restype = [vp, i32, i32], C.c_longdouble
+        getattr(lib, f"orc_{p}_dot_terms_ld").argtypes = [vp, vp, vp, vp, vp, i32, i32]
         getattr(lib, f"orc_{p}_dot").argtypes = [vp, vp, vp, vp, i32, i32]
         getattr(lib, f"orc_{p}_dot_ld").argtypes = [vp, vp, vp, vp, i32, i32]
         for fn in ("exp", "sqrt", "invsqrt"):
@@ -257,6 +260,25 @@ def dot_ld(a: np.ndarray, b: np.ndarray):
     getattr(lib, f"orc_{p}_dot_ld")(C.byref(re), C.byref(im), _ptr(_need(a)), _ptr(_need(b)),
                                      a[0].size // 2, a.shape[0])
     return re.value, im.value
+
+
+def norm2_terms_ld(a: np.ndarray):
+    """The reference's component norms (formed in the field's precision)
+    summed in long double: numpy longdouble."""
+    lib = load()
+    p = "F" if a.dtype == np.float32 else "D"
+    return np.longdouble(getattr(lib, f"orc_{p}_norm2_terms_ld")(_ptr(_need(a)), a[0].size // 2, a.shape[0]))
+
+
+def dot_terms_ld(a: np.ndarray, b: np.ndarray):
+    """The reference's double dot terms summed in long double:
+    (re, im, sum of |terms|) as numpy longdouble."""
+    lib = load()
+    p = "F" if a.dtype == np.float32 else "D"
+    re, im, sa = C.c_longdouble(), C.c_longdouble(), C.c_longdouble()
+    getattr(lib, f"orc_{p}_dot_terms_ld")(C.byref(re), C.byref(im), C.byref(sa), _ptr(_need(a)), _ptr(_need(b)),
+                                           a[0].size // 2, a.shape[0])
+    return np.longdouble(re.value), np.longdouble(im.value), np.longdouble(sa.value)
 
 
 def spproj1(d: np.ndarray, mu: int, sign: int) -> np.ndarray:

@@ -68,6 +68,14 @@ void orc_D3_multi(int eqop, int adj, double *r, const int *rindex, const double 
 double orc_F_norm2(const float *a, int ncomp, int n) { return F_orc_norm2(a, ncomp, n); }
 double orc_D_norm2(const double *a, int ncomp, int n) { return D_orc_norm2(a, ncomp, n); }
 long double orc_F_norm2_ld(const float *a, int ncomp, int n) { return F_orc_norm2_ld(a, ncomp, n); }
+long double orc_F_norm2_terms_ld(const float *a, int ncomp, int n) { return F_orc_norm2_terms_ld(a, ncomp, n); }
+long double orc_D_norm2_terms_ld(const double *a, int ncomp, int n) { return D_orc_norm2_terms_ld(a, ncomp, n); }
+void orc_F_dot_terms_ld(long double *re, long double *im, long double *sa, const float *a, const float *b, int ncomp,
+                        int n)
+{ F_orc_dot_terms_ld(re, im, sa, a, b, ncomp, n); }
+void orc_D_dot_terms_ld(long double *re, long double *im, long double *sa, const double *a, const double *b,
+                        int ncomp, int n)
+{ D_orc_dot_terms_ld(re, im, sa, a, b, ncomp, n); }
 long double orc_D_norm2_ld(const double *a, int ncomp, int n) { return D_orc_norm2_ld(a, ncomp, n); }
 void orc_F_dot(double *re, double *im, const float *a, const float *b, int ncomp, int n)
 { F_orc_dot(re, im, a, b, ncomp, n); }

@@ -63,6 +63,14 @@ void orc_D_dot(double *re, double *im, const double *a, const double *b, int nco
  * test/src/protocol_idx.m4:128-183) */
 long double orc_F_norm2_ld(const float *a, int ncomp, int n);
 long double orc_D_norm2_ld(const double *a, int ncomp, int n);
+/* the reference's rounded terms (component norms / double dot terms) summed
+ * in long double; *sa = sum |term| */
+long double orc_F_norm2_terms_ld(const float *a, int ncomp, int n);
+long double orc_D_norm2_terms_ld(const double *a, int ncomp, int n);
+void orc_F_dot_terms_ld(long double *re, long double *im, long double *sa, const float *a, const float *b, int ncomp,
+                        int n);
+void orc_D_dot_terms_ld(long double *re, long double *im, long double *sa, const double *a, const double *b,
+                        int ncomp, int n);
 void orc_F_dot_ld(long double *re, long double *im, const float *a, const float *b, int ncomp, int n);
 void orc_D_dot_ld(long double *re, long double *im, const double *a, const double *b, int ncomp, int n);
 

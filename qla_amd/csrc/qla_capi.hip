@@ -14,7 +14,8 @@
 #include <atomic>
 #include <map>
 #include <mutex>
-#include <unordered_map>
+#include <thread>
+#include <tuple>
 #include <vector>
 #include "qla_launch.hpp"
 #include "../../include/qla_mi355x.h"
@@ -27,10 +28,6 @@ struct thread_state {
     hipStream_t stream = nullptr;
     int sync = 1;
     hipError_t err = hipSuccess;
-    // per-device reduction scratch: partials + one result slot (2 doubles)
-    std::unordered_map<int, double *> scratch;
-    // per-device scratch of the global field sums: partials + one record
-    std::unordered_map<int, double *> sum_scratch;
 };
 
 thread_local thread_state ts;
@@ -48,31 +45,108 @@ void finish(hipError_t e, const char *where)
     if (e == hipSuccess && ts.sync) record(hipStreamSynchronize(ts.stream), where);
 }
 
-double *scratch_for_device()
+// the calling thread's stream is being captured into a graph
+bool capturing()
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(ts.stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+// ----------------------------------------------------------- reduction scratch
+// Device scratch of the global reductions and sums: one buffer of each kind
+// per (host thread, device, stream), so reductions that run concurrently on
+// different streams, or that two threads issue on one stream (each copies
+// its own result slot back), never share partials or a result slot.  A
+// buffer is allocated (and zeroed: the single-launch reductions' ticket
+// counter starts at zero) on its first use, which must not fall inside a
+// stream capture; thread exit releases the thread's buffers and
+// QLA_MI355X_finalize releases all of them.
+enum { SCRATCH_REDUCE = 0, SCRATCH_SUM = 1 };
+
+struct scratch_key {
+    std::thread::id tid;
+    int dev;
+    hipStream_t stream;
+    int kind;
+    bool operator<(const scratch_key &o) const
+    {
+        return std::tie(tid, dev, stream, kind) < std::tie(o.tid, o.dev, o.stream, o.kind);
+    }
+};
+
+std::mutex g_scratch_mutex;
+std::map<scratch_key, double *> g_scratch;
+const std::thread::id g_main_thread = std::this_thread::get_id();
+
+// frees the calling thread's buffers when the thread ends (the main thread's
+// live until the process ends or QLA_MI355X_finalize)
+struct scratch_owner {
+    bool used = false;
+    ~scratch_owner()
+    {
+        if (!used || std::this_thread::get_id() == g_main_thread) return;
+        std::lock_guard<std::mutex> lk(g_scratch_mutex);
+        for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+            if (it->first.tid == std::this_thread::get_id()) {
+                int cur = 0;
+                if (hipGetDevice(&cur) == hipSuccess && cur != it->first.dev) (void)hipSetDevice(it->first.dev);
+                (void)hipFree(it->second);
+                if (cur != it->first.dev) (void)hipSetDevice(cur);
+                it = g_scratch.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+};
+thread_local scratch_owner t_scratch_owner;
+
+size_t scratch_doubles(int kind)
+{
+    // reductions: partials and ticket counter, then a 4-double result slot
+    return kind == SCRATCH_REDUCE ? (size_t)qla_reduce_scratch_doubles() + 4 : (size_t)qla_sum_scratch_doubles();
+}
+
+double *scratch_for(int kind, const char *name)
 {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    auto it = ts.scratch.find(dev);
-    if (it != ts.scratch.end()) return it->second;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { record(e, name); return nullptr; }
+    const scratch_key key = { std::this_thread::get_id(), dev, ts.stream, kind };
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mutex);
+        auto it = g_scratch.find(key);
+        if (it != g_scratch.end()) return it->second;
+    }
+    if (capturing()) {
+        fprintf(stderr, "qla_mi355x: %s: the first reduction on a stream allocates its scratch and cannot be "
+                        "captured; run one reduction on this stream before capturing\n", name);
+        record(hipErrorStreamCaptureUnsupported, name);
+        return nullptr;
+    }
     double *p = nullptr;
-    size_t bytes = sizeof(double) * (qla_reduce_scratch_doubles() + 4);
-    hipError_t e = hipMalloc((void **)&p, bytes);
-    if (e != hipSuccess) { record(e, "reduction scratch"); return nullptr; }
-    ts.scratch[dev] = p;
+    const size_t bytes = sizeof(double) * scratch_doubles(kind);
+    e = hipMalloc((void **)&p, bytes);
+    if (e == hipSuccess) e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) {
+        if (p) (void)hipFree(p);
+        record(e, name);
+        return nullptr;
+    }
+    t_scratch_owner.used = true;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    g_scratch[key] = p;
     return p;
 }
 
-double *sum_scratch_for_device()
+// a result delivered to host memory waits on the host, which a stream
+// capture cannot contain
+bool refuse_host_result_in_capture(const char *name)
 {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    auto it = ts.sum_scratch.find(dev);
-    if (it != ts.sum_scratch.end()) return it->second;
-    double *p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, sizeof(double) * (size_t)qla_sum_scratch_doubles());
-    if (e != hipSuccess) { record(e, "sum scratch"); return nullptr; }
-    ts.sum_scratch[dev] = p;
-    return p;
+    if (!capturing()) return false;
+    fprintf(stderr, "qla_mi355x: %s: a host result pointer cannot be captured; pass device memory\n", name);
+    record(hipErrorStreamCaptureUnsupported, name);
+    return true;
 }
 
 bool is_device_pointer(const void *p)
@@ -88,12 +162,13 @@ bool is_device_pointer(const void *p)
 template <class L>
 void reduce_to(L launch, void *r, size_t bytes, const char *name)
 {
-    double *sc = scratch_for_device();
+    double *sc = scratch_for(SCRATCH_REDUCE, name);
     if (!sc) return;
     if (is_device_pointer(r)) {
         finish(launch(r, sc), name);
         return;
     }
+    if (refuse_host_result_in_capture(name)) return;
     double *slot = sc + qla_reduce_scratch_doubles();
     hipError_t e = launch((void *)slot, sc);
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
@@ -259,13 +334,14 @@ void local_entry(int prec, int kind, int eqop, int ncomp, int pa, int pb, void *
 void sum_entry(int prec_in, int out_double, int ncr, int pa, void *r, const void *a, const int *index, int n,
                size_t bytes, const char *name)
 {
-    double *sc = sum_scratch_for_device();
+    double *sc = scratch_for(SCRATCH_SUM, name);
     if (!sc) return;
     if (n < 0) n = 0;
     if (is_device_pointer(r)) {
         finish(qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, r, sc, ts.stream), name);
         return;
     }
+    if (refuse_host_result_in_capture(name)) return;
     double *slot = sc + qla_sum_scratch_doubles() - 288;
     hipError_t e = qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, slot, sc, ts.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
@@ -523,13 +599,14 @@ void int_entry(int op, int eqop, int pa, int pb, void *r, const void *s, const v
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
-    double *sc = scratch_for_device();
+    double *sc = scratch_for(SCRATCH_REDUCE, name);
     if (!sc) return;
     if (n < 0) n = 0;
     if (is_device_pointer(r)) {
         finish(qla_launch_reduce_dh(a, b, n, (double *)r, sc, ts.stream), name);
         return;
     }
+    if (refuse_host_result_in_capture(name)) return;
     double *slot = sc + qla_reduce_scratch_doubles();
     hipError_t e = qla_launch_reduce_dh(a, b, n, slot, sc, ts.stream);
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
@@ -573,12 +650,20 @@ void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, 
     finish(qla_launch_site(prec, kind, eqop, adj, xf, p, g_tiles_per_wave.load(), ts.stream), name);
 }
 
-// nd term addresses from host or device memory
+// nd term addresses from host or device memory.  A device-resident list is
+// read on the calling thread's stream (after the work already queued there,
+// which may have written it); that copy waits on the host, so it cannot be
+// part of a stream capture.
 hipError_t term_list(const void **dst, const void *src, int nd)
 {
-    if (is_device_pointer(src)) return hipMemcpy(dst, src, sizeof(void *) * (size_t)nd, hipMemcpyDeviceToHost);
-    memcpy(dst, src, sizeof(void *) * (size_t)nd);
-    return hipSuccess;
+    if (!is_device_pointer(src)) {
+        memcpy(dst, src, sizeof(void *) * (size_t)nd);
+        return hipSuccess;
+    }
+    if (capturing()) return hipErrorStreamCaptureUnsupported;
+    hipError_t e = hipMemcpyAsync(dst, src, sizeof(void *) * (size_t)nd, hipMemcpyDeviceToHost, ts.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    return e;
 }
 
 // One multi-source call: terms in launches of at most QLA_MULTI_MAXND, the
@@ -812,6 +897,23 @@ int QLA_MI355X_last_error(void)
     return e;
 }
 const char *QLA_MI355X_last_error_string(void) { return hipGetErrorString(ts.err); }
+void QLA_MI355X_finalize(void)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    int cur = 0;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    for (auto &kv : g_scratch) {
+        (void)hipSetDevice(kv.first.dev);
+        record(hipFree(kv.second), "QLA_MI355X_finalize");
+    }
+    if (have_cur) (void)hipSetDevice(cur);
+    g_scratch.clear();
+}
+int QLA_MI355X_scratch_buffers(void)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    return (int)g_scratch.size();
+}
 const char *QLA_MI355X_version(void) { return QLA_MI355X_VERSION_STRING; }
 void QLA_MI355X_set_tiles_per_wave(int tpw)
 {

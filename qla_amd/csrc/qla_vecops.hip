@@ -1151,7 +1151,8 @@ enum { FILL_ZERO = 0, FILL_REC = 1, FILL_DIAG = 2 };
 
 template <class T, int N> struct rec_val { cpx<T> e[N]; };
 
-// v-form zero fill: 16 bytes of zeros per work item over the run of m complexes
+// v-form zero fill: 16 bytes of zeros per work item over the run of m
+// complexes (r 16-byte aligned)
 template <class T>
 __global__ void __launch_bounds__(VBLOCK) k_vec_zero(cpx<T> *r, long long m)
 {
@@ -1313,7 +1314,9 @@ hipError_t launch_fill_t(int kind, int eqop, int ncomp, const qla_fill_args &p, 
     if (kind == FILL_ZERO) {
         if (eqop != 0) return hipErrorInvalidValue;
         if (ncomp == 0) return launch_rfill<T, FILL_ZERO, 0>(p, st);
-        if (!p.index) {
+        // the 16-byte stores of k_vec_zero need a 16-byte aligned float
+        // field; an 8-byte aligned one takes the per-complex fill
+        if (!p.index && (sizeof(T) == 8 || (uintptr_t)p.r % 16 == 0)) {
             const long long m = (long long)p.n * ncomp;
             const long long items = sizeof(T) == 4 ? (m + 1) / 2 : m;
             hipLaunchKernelGGL(k_vec_zero<T>, dim3(blocks_for(items)), dim3(VBLOCK), 0, st, (cpx<T> *)p.r, m);

@@ -57,6 +57,8 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_last_error_string": (C.c_char_p, []),
     "QLA_MI355X_version": (C.c_char_p, []),
     "QLA_MI355X_set_tiles_per_wave": (None, [i32]),
+    "QLA_MI355X_finalize": (None, []),
+    "QLA_MI355X_scratch_buffers": (i32, []),
 }
 # the generated families: the Wilson-hop building blocks (qla_mi355x_sites.h),
 # the reductions and conversions (qla_mi355x_fields.h), the colour-matrix

@@ -134,12 +134,13 @@ def test_compressed_reductions(orc):
     out = (C.c_double * 2)()
     qla_amd.call("QLA_MI355X_DF3_r_veq_norm2_Dh", C.cast(out, C.c_void_p).value, da, n)
     qla_amd.check()
-    ref = orc.norm2(xa)
-    assert abs(out[0] - ref) <= 1e-12 * ref
+    # the decoded records' terms summed in long double; 1e-14 of the scale
+    # (sum |terms|), the DF3 bar of the plain reductions
+    ref = orc.norm2_terms_ld(xa)
+    assert abs(np.longdouble(out[0]) - ref) <= 1e-14 * ref
     qla_amd.call("QLA_MI355X_DF3_c_veq_Dh_dot_Dh", C.cast(out, C.c_void_p).value, da, db, n)
     qla_amd.check()
-    ref = orc.dot(xa, xb)
-    scale = np.sqrt(orc.norm2(xa) * orc.norm2(xb))
-    assert abs(out[0] - ref.real) <= 1e-12 * scale and abs(out[1] - ref.imag) <= 1e-12 * scale
+    re, im, scale = orc.dot_terms_ld(xa, xb)
+    assert abs(np.longdouble(out[0]) - re) <= 1e-14 * scale and abs(np.longdouble(out[1]) - im) <= 1e-14 * scale
     # the compressed norm is within the format's precision of the FP64 one
     assert abs(orc.norm2(xa) - orc.norm2(a.astype(np.float64))) <= 2e-3 * orc.norm2(a.astype(np.float64))

@@ -155,6 +155,27 @@ def test_unaligned_fields(orc, prec, t):
     assert_bitwise(host(br), exp, f"{v.name} unaligned")
 
 
+@pytest.mark.parametrize("prec", ["F3", "D3"])
+@pytest.mark.parametrize("t", ["V", "D", "M"])
+def test_unaligned_zero_fill(orc, prec, t):
+    """The v-form zero fill of a field starting one complex past a 16-byte
+    boundary writes exactly its n records (8-byte stores for float fields)
+    and leaves the complex before and after them alone."""
+    cs = case(prec, t)
+    n = cs.n - 1
+    c2 = 2 * NCOMP[t]
+    import torch
+    flat = torch.from_numpy(np.ascontiguousarray(cs.r0).reshape(-1)).to("cuda")
+    view = flat[2:2 + n * c2]
+    name = f"QLA_{prec}_{t}_veq_zero"
+    say(f"{name} unaligned n={n}")
+    qla_amd.call(name, view, n)
+    qla_amd.check()
+    exp = cs.r0.reshape(-1).copy()
+    exp[2:2 + n * c2] = 0
+    assert_bitwise(host(flat), exp, f"{name} unaligned")
+
+
 def test_noop_and_errors(orc):
     """n <= 0 writes nothing; a null constant records an error, writes nothing."""
     import ctypes

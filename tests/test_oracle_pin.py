@@ -181,3 +181,36 @@ def test_reductions_vs_long_double(orc, prec):
         ac, bc = cplx(a).reshape(-1), cplx(b).reshape(-1)
         ex = np.sum(np.conj(ac) * bc)
         assert abs(got - complex(ex)) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_term_sums_vs_exact(orc, prec):
+    """The long-double sums of the reference's rounded terms (the GPU
+    reductions' bar) against exact sums (math.fsum) of the same terms formed
+    independently in numpy: the component norm re*re + im*im rounded to the
+    field precision (two numpy operations of that precision), the dot terms
+    ar*br + ai*bi and ar*bi - ai*br in float64.  A dropped term, a missing
+    rounding step or a wrong sign fails this far above the long-double
+    error (~1e-19 relative)."""
+    import math
+    n = 5000
+    dt = np.float32 if prec == "F" else np.float64
+    a = lattice.random_field("D", n, 15).astype(dt)
+    b = lattice.random_field("D", n, 16).astype(dt)
+    ar, ai = a[..., 0].reshape(-1), a[..., 1].reshape(-1)
+    br, bi = b[..., 0].reshape(-1), b[..., 1].reshape(-1)
+    terms = ar * ar + ai * ai                     # in the field precision
+    ex = math.fsum(terms.astype(np.float64))
+    got = orc.norm2_terms_ld(a)
+    assert abs(float(got) - ex) <= 1e-15 * ex
+    # the rounding to the field precision is part of the semantics
+    if prec == "F":
+        exact_sq = math.fsum((ar.astype(np.float64) ** 2 + ai.astype(np.float64) ** 2))
+        assert abs(float(got) - exact_sq) > 1e-12 * ex
+    a64 = [x.astype(np.float64) for x in (ar, ai, br, bi)]
+    tr = a64[0] * a64[2] + a64[1] * a64[3]
+    ti = a64[0] * a64[3] - a64[1] * a64[2]
+    re, im, sa = orc.dot_terms_ld(a, b)
+    assert abs(float(re) - math.fsum(tr)) <= 1e-15 * float(sa)
+    assert abs(float(im) - math.fsum(ti)) <= 1e-15 * float(sa)
+    assert abs(float(sa) - math.fsum(np.abs(tr)) - math.fsum(np.abs(ti))) <= 1e-15 * float(sa)
