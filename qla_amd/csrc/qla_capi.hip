@@ -57,8 +57,7 @@ bool capturing()
 // per (host thread, device, stream), so reductions that run concurrently on
 // different streams, or that two threads issue on one stream (each copies
 // its own result slot back), never share partials or a result slot.  A
-// buffer is allocated (and zeroed: the single-launch reductions' ticket
-// counter starts at zero) on its first use, which must not fall inside a
+// buffer is allocated on its first use, which must not fall inside a
 // stream capture; thread exit releases the thread's buffers and
 // QLA_MI355X_finalize releases all of them.
 enum { SCRATCH_REDUCE = 0, SCRATCH_SUM = 1 };
@@ -103,7 +102,7 @@ thread_local scratch_owner t_scratch_owner;
 
 size_t scratch_doubles(int kind)
 {
-    // reductions: partials and ticket counter, then a 4-double result slot
+    // reductions: partials, then a 4-double result slot
     return kind == SCRATCH_REDUCE ? (size_t)qla_reduce_scratch_doubles() + 4 : (size_t)qla_sum_scratch_doubles();
 }
 
@@ -127,7 +126,6 @@ double *scratch_for(int kind, const char *name)
     double *p = nullptr;
     const size_t bytes = sizeof(double) * scratch_doubles(kind);
     e = hipMalloc((void **)&p, bytes);
-    if (e == hipSuccess) e = hipMemset(p, 0, bytes);
     if (e != hipSuccess) {
         if (p) (void)hipFree(p);
         record(e, name);
@@ -157,13 +155,12 @@ bool is_device_pointer(const void *p)
     return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
 }
 
-// run a reduction and deliver `bytes` of result to r (host or device);
+// run a reduction with scratch sc (partials, then a 4-double
+// result slot) and deliver `bytes` of result to r (host or device);
 // launch(out, scratch) enqueues the kernels that write the result to out
 template <class L>
-void reduce_to(L launch, void *r, size_t bytes, const char *name)
+void reduce_with(double *sc, L launch, void *r, size_t bytes, const char *name)
 {
-    double *sc = scratch_for(SCRATCH_REDUCE, name);
-    if (!sc) return;
     if (is_device_pointer(r)) {
         finish(launch(r, sc), name);
         return;
@@ -174,6 +171,13 @@ void reduce_to(L launch, void *r, size_t bytes, const char *name)
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
     record(e, name);
+}
+
+template <class L>
+void reduce_to(L launch, void *r, size_t bytes, const char *name)
+{
+    double *sc = scratch_for(SCRATCH_REDUCE, name);
+    if (sc) reduce_with(sc, launch, r, bytes, name);
 }
 
 // One QLA global reduction (qla_fields_abi.inc): kind 0 norm2, 1 dot (the

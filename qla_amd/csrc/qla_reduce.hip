@@ -14,10 +14,11 @@
 // per-block partials in a fixed tree and writes the result.  Contiguous
 // chunks keep each block's reads within few DRAM pages (6.7 TB/s measured
 // for a 510 MB norm2, against 5.4 TB/s with a grid-stride walk).
-// Small contiguous inputs (at most FUSE_MAX_BLOCKS blocks) run both stages in
-// one launch: each block publishes its partial and takes a ticket, and the
-// block that draws the last ticket adds the partials with stage 2's code and
-// order (so either form gives the same bits) and re-arms the ticket counter.
+// A single-launch form (the last block to finish, found with an atomic
+// ticket, adds the partials) was measured slower than the two launches at
+// every size: the device-scope fence each block needs before its ticket
+// writes back its XCD's L2 (3.9 us for two launches against 4.3-6.3 us at
+// 1K-65K sites, as graph replays).
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -26,7 +27,6 @@ namespace qla {
 constexpr int RBLOCK = 256;
 constexpr int RMAXGRID = 2048;      // partial slots in the scratch (gathered forms)
 constexpr int RCHUNKS = 1024;       // blocks of the contiguous stage 1
-constexpr int FUSE_MAX_BLOCKS = 128; // contiguous inputs of at most this many blocks: one launch
 
 template <class T> struct vec2_of;
 template <> struct vec2_of<float> { typedef float2 type; };
@@ -79,21 +79,11 @@ __device__ __forceinline__ void acc_dot(double &sr, double &si, T ar, T ai, T br
     si += xr * yi - xi * yr;
 }
 
-// single-launch form: where the result goes and the ticket counter
-struct fuse_args {
-    void *out;
-    int out_kind;
-    unsigned *ticket;
-};
-
-__device__ __forceinline__ void combine_if_last(const double *part, bool dot, void *out, int out_kind,
-                                                unsigned *ticket, double *sh);
-
-// stage 1, one complex per load (any record alignment); FUSED: also stage 2
-template <class T, bool DOT, bool FUSED>
+// stage 1, one complex per load (any record alignment)
+template <class T, bool DOT>
 __global__ void __launch_bounds__(RBLOCK)
 k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long m,
-          double *__restrict__ part, fuse_args f)
+          double *__restrict__ part)
 {
     __shared__ double sh[RBLOCK / WAVE];
     double sr = 0, si = 0;
@@ -123,7 +113,6 @@ k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long 
         double ti = block_sum(si, sh);
         if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = ti;
     }
-    if (FUSED) combine_if_last(part, DOT, f.out, f.out_kind, f.ticket, sh);
 }
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -136,10 +125,10 @@ __device__ __forceinline__ float4 ld_stream(const float4 *p)
 }
 
 // stage 1, two float complexes per 16-byte load (a, b 16-byte aligned, m even)
-template <bool DOT, bool FUSED>
+template <bool DOT>
 __global__ void __launch_bounds__(RBLOCK)
 k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long long m2,
-             double *__restrict__ part, fuse_args f)
+             double *__restrict__ part)
 {
     __shared__ double sh[RBLOCK / WAVE];
     double sr = 0, si = 0;
@@ -180,7 +169,6 @@ k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long lo
         double ti = block_sum(si, sh);
         if (threadIdx.x == 0) part[gridDim.x + blockIdx.x] = ti;
     }
-    if (FUSED) combine_if_last(part, DOT, f.out, f.out_kind, f.ticket, sh);
 }
 
 // stage 1 over gathered records: the record of site i is *pa[s] (pointer
@@ -273,32 +261,13 @@ k_reduce2(const double *__restrict__ part, int g, int dot, void *out, int out_ki
     combine_partials(part, g, dot != 0, out, out_kind, sh);
 }
 
-// Single-launch tail: the block's partial(s) are in part[] (written by
-// thread 0); thread 0 makes them visible device-wide and takes a ticket, and
-// the block that draws ticket gridDim.x - 1 combines every block's partials
-// and sets the counter back to zero for the next call.
-__device__ __forceinline__ void combine_if_last(const double *part, bool dot, void *out, int out_kind,
-                                                unsigned *ticket, double *sh)
-{
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1u;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    combine_partials(part, (int)gridDim.x, dot, out, out_kind, sh);
-    if (threadIdx.x == 0) atomicExch(ticket, 0u);
-}
-
 } // namespace qla
 
 using namespace qla;
 
-// scratch layout: 2 * RMAXGRID partial slots, then the ticket counter of the
-// single-launch form (one 8-byte slot, zero between calls) and one spare slot
-int qla_reduce_scratch_doubles(void) { return 2 * RMAXGRID + 2; }
+// scratch layout: 2 * RMAXGRID partial slots (real parts, then imaginary)
+static_assert(QLA_REDUCE_SCRATCH_DOUBLES == 2 * RMAXGRID, "scratch layout");
+int qla_reduce_scratch_doubles(void) { return QLA_REDUCE_SCRATCH_DOUBLES; }
 
 static int grid_for(long long work)
 {
@@ -308,33 +277,20 @@ static int grid_for(long long work)
     return (int)g;
 }
 
-// one stage-1 kernel over contiguous data, fused with stage 2 or not
-template <class T, bool DOT>
-static void launch_contig(const void *a, const void *b, long long m, int g, double *scratch, const fuse_args &f,
-                          hipStream_t st)
+static bool use_f4(int prec, bool dot, const void *a, const void *b, long long m)
 {
-    if (f.ticket)
-        hipLaunchKernelGGL((k_reduce1<T, DOT, true>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<T> *)a,
-                           (const cpx<T> *)b, m, scratch, f);
-    else
-        hipLaunchKernelGGL((k_reduce1<T, DOT, false>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<T> *)a,
-                           (const cpx<T> *)b, m, scratch, f);
+    return prec == 0 && ((uintptr_t)a % 16 == 0) && (!dot || (uintptr_t)b % 16 == 0) && (m % 2 == 0);
 }
-template <bool DOT>
-static void launch_contig_f4(const void *a, const void *b, long long m2, int g, double *scratch, const fuse_args &f,
-                             hipStream_t st)
+
+int qla_reduce2_partials(int prec, int kind, const void *a, const void *b, long long m)
 {
-    if (f.ticket)
-        hipLaunchKernelGGL((k_reduce1_f4<DOT, true>), dim3(g), dim3(RBLOCK), 0, st, (const float4 *)a,
-                           (const float4 *)b, m2, scratch, f);
-    else
-        hipLaunchKernelGGL((k_reduce1_f4<DOT, false>), dim3(g), dim3(RBLOCK), 0, st, (const float4 *)a,
-                           (const float4 *)b, m2, scratch, f);
+    if (m <= 0) return 0;
+    const bool dot = (kind == 1);
+    const int g = grid_for(use_f4(prec, dot, a, b, m) ? m / 2 : m);
+    return dot ? 2 * g : g;
 }
 
 // prec 0/1; kind 0 norm2 / 1 dot; m complex elements; out device address.
-// A stage-1 grid of at most FUSE_MAX_BLOCKS blocks runs both stages in one
-// launch (same partials, same combine order, same bits).
 hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, long long m, void *out,
                               int out_kind, double *scratch, hipStream_t st)
 {
@@ -344,22 +300,32 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
                            out, out_kind);
         return hipGetLastError();
     }
-    const bool f4 = prec == 0 && ((uintptr_t)a % 16 == 0) && (!dot || (uintptr_t)b % 16 == 0) && (m % 2 == 0);
+    const bool f4 = use_f4(prec, dot, a, b, m);
     const int g = grid_for(f4 ? m / 2 : m);
-    fuse_args f = { out, out_kind, nullptr };
-    if (g <= FUSE_MAX_BLOCKS) f.ticket = (unsigned *)(scratch + 2 * RMAXGRID);
     if (f4) {
-        if (dot) launch_contig_f4<true>(a, b, m / 2, g, scratch, f, st);
-        else launch_contig_f4<false>(a, b, m / 2, g, scratch, f, st);
+        if (dot)
+            hipLaunchKernelGGL(k_reduce1_f4<true>, dim3(g), dim3(RBLOCK), 0, st, (const float4 *)a,
+                               (const float4 *)b, m / 2, scratch);
+        else
+            hipLaunchKernelGGL(k_reduce1_f4<false>, dim3(g), dim3(RBLOCK), 0, st, (const float4 *)a,
+                               (const float4 *)b, m / 2, scratch);
     } else if (prec == 0) {
-        if (dot) launch_contig<float, true>(a, b, m, g, scratch, f, st);
-        else launch_contig<float, false>(a, b, m, g, scratch, f, st);
+        if (dot)
+            hipLaunchKernelGGL((k_reduce1<float, true>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<float> *)a,
+                               (const cpx<float> *)b, m, scratch);
+        else
+            hipLaunchKernelGGL((k_reduce1<float, false>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<float> *)a,
+                               (const cpx<float> *)b, m, scratch);
     } else {
-        if (dot) launch_contig<double, true>(a, b, m, g, scratch, f, st);
-        else launch_contig<double, false>(a, b, m, g, scratch, f, st);
+        if (dot)
+            hipLaunchKernelGGL((k_reduce1<double, true>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<double> *)a,
+                               (const cpx<double> *)b, m, scratch);
+        else
+            hipLaunchKernelGGL((k_reduce1<double, false>), dim3(g), dim3(RBLOCK), 0, st, (const cpx<double> *)a,
+                               (const cpx<double> *)b, m, scratch);
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || f.ticket) return e;
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(RBLOCK), 0, st, (const double *)scratch, g, (int)dot, out,
                        out_kind);
     return hipGetLastError();

@@ -148,8 +148,8 @@ def _norm2_dev(lib, out, a, n):
 
 def test_two_streams_async(orc):
     """One host thread, asynchronous calls, DF3 norm2 and dot alternating
-    between two streams with device results, at a size that takes the
-    two-kernel path and one that takes the single-launch path: each stream's
+    between two streams with device results, at a size with a full stage-1
+    grid and one with a few blocks: each stream's
     reductions have their own scratch, so every result equals the one a
     synchronous call gives, bit for bit (same kernels, same order)."""
     import torch
@@ -238,7 +238,7 @@ def test_reduction_capture(orc):
     would allocate) and allocates nothing; once the stream has run one
     reduction, reductions with device results capture into a hipGraph whose
     replays -- with eager work between them that changes the input -- each
-    give the eager call's bits (single-launch and two-kernel sizes).  A host
+    give the eager call's bits (a few-block and a full-grid size).  A host
     result is refused under capture."""
     import torch
     lib = qla_amd.load()
