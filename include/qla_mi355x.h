@@ -131,6 +131,23 @@ void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n);
 void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n);
 
 /* ------------------------------------------------------------------------ */
+/* Allocation-free reduction (no counterpart in the reference).  For callers */
+/* that cannot allocate device memory: code under HIP graph capture, tight   */
+/* serving loops, harnesses that forbid hipMalloc.  Same result, bit for     */
+/* bit, as QLA_DF3_r_veq_norm2_D (same kernels), but its scratch is a       */
+/* static 32 KiB device array (one copy per GPU, held until the process      */
+/* ends) instead of the per-stream buffer the other reductions allocate on  */
+/* first use, and it calls no hipMalloc / hipFree.  Size limit: none below  */
+/* INT_MAX sites (the first stage runs at most 1024 blocks whatever n is,   */
+/* and the array holds their partials; a launch needing more would be       */
+/* refused with hipErrorInvalidValue).  r may be host or device memory (a   */
+/* host result waits on the host, so only a device r can be captured).     */
+/* Every call on one GPU shares that GPU's array: calls that overlap on one */
+/* GPU (two streams, or two threads) race; order them.                      */
+/* ------------------------------------------------------------------------ */
+void QLA_MI355X_DF3_r_veq_norm2_D_noalloc(QLA_D_Real *r, QLA_F3_DiracFermion *a, int n);
+
+/* ------------------------------------------------------------------------ */
 /* Ragged batch extension (no counterpart in the reference, where each call  */
 /* is its own site loop).  A plan holds a table of independent QLA_F3 v-form */
 /* calls; running it executes all of them in one launch that cuts every    */

@@ -180,6 +180,37 @@ void reduce_to(L launch, void *r, size_t bytes, const char *name)
     if (sc) reduce_with(sc, launch, r, bytes, name);
 }
 
+// ------------------------------------------------ allocation-free reduction
+// Scratch of QLA_MI355X_DF3_r_veq_norm2_D_noalloc: the partials and the
+// result slot, laid out as the allocated reduction scratch, in one static
+// device array (32 KiB; every GPU the library runs on holds its own copy
+// until the process ends).  Stage 1 writes every partial slot stage 2
+// reads, so nothing in it needs clearing between calls.  Host code never names the array in a
+// launch or copy: it takes each device's address once with
+// hipGetSymbolAddress and passes that.
+constexpr int NOALLOC_DOUBLES = QLA_REDUCE_SCRATCH_DOUBLES + 4;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+std::atomic<double *> g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// this device's address of `scratch`, looked up on the first call on the device
+double *noalloc_scratch(const char *name)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) { record(e, name); return nullptr; }
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { record(hipErrorInvalidDevice, name); return nullptr; }
+    double *p = g_noalloc_ptr[dev].load(std::memory_order_acquire);
+    if (p) return p;
+    e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        record(e != hipSuccess ? e : hipErrorInvalidSymbol, name);
+        return nullptr;
+    }
+    g_noalloc_ptr[dev].store(p, std::memory_order_release);
+    return p;
+}
+
 // One QLA global reduction (qla_fields_abi.inc): kind 0 norm2, 1 dot (the
 // real-part forms take out_kind 4 / 5); ncomp complexes per record.  The
 // v-form over fields streams the arrays as one run of n*ncomp complexes;
@@ -959,6 +990,24 @@ void QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh(void *r, void *a, int fmt, void
 {
     finish(qla_launch_sprecon_c(fmt, r, a, (const void *)b, mu, sign, n, ts.stream),
            "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh");
+}
+void QLA_MI355X_DF3_r_veq_norm2_D_noalloc(QLA_D_Real *r, QLA_F3_DiracFermion *a, int n)
+{
+    static const char name[] = "QLA_MI355X_DF3_r_veq_norm2_D_noalloc";
+    double *sc = noalloc_scratch(name);
+    if (!sc) return;
+    const long long m = n > 0 ? 12LL * n : 0;
+    // the first stage's grid is capped (1024 blocks) whatever n is, so its
+    // partials always fit; checked all the same, before anything launches
+    if (qla_reduce2_partials(0, 0, a, nullptr, m) > QLA_REDUCE_SCRATCH_DOUBLES) {
+        fprintf(stderr, "qla_mi355x: %s: n = %d needs more partial slots than the static scratch holds (%d)\n", name,
+                n, QLA_REDUCE_SCRATCH_DOUBLES);
+        record(hipErrorInvalidValue, name);
+        return;
+    }
+    reduce_with(sc, [&](void *out, double *s) {
+        return qla_launch_reduce2(0, 0, a, nullptr, m, out, 0, s, ts.stream);
+    }, r, sizeof(double), name);
 }
 void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n)
 {
