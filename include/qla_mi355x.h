@@ -66,8 +66,9 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 /* then x +=/-= a[ic][kc]*b[kc], kc = 0,1,2, include/qla_complex.h:330-341),*/
 /* build_qla.pl:1162-1250 (spin forms), :1693-1725 (multi-source).  The SSE */
 /* variants lib/optimized/sse/src/QLA_F3_V_vpeq_M_times_pV.c,               */
-/* ..._D_vpeq_sprecon_M_times_pH.c, ..._D_vpeq_spproj_M_times_pD.c          */
-/* re-associate; this library keeps the generic C order for every symbol.   */
+/* ..._D_vpeq_sprecon_M_times_pH.c, ..._D_vpeq_spproj_M_times_pD.c (and    */
+/* V_veq_M_times_pV, V_veq_Ma_times_V) re-associate; this library keeps the */
+/* generic C order unless QLA_MI355X_set_arith_order selects theirs.        */
 /* ------------------------------------------------------------------------ */
 #include "qla_mi355x_sites.h"
 
@@ -201,6 +202,21 @@ void QLA_MI355X_batch_destroy(QLA_MI355X_batch *p);
  * selects the default stream. */
 void QLA_MI355X_set_stream(void *stream);
 void *QLA_MI355X_get_stream(void);
+/* Per host thread: the association order of the floating-point sums.
+ * QLA_MI355X_ORDER_GENERIC (default): every symbol rounds exactly like the
+ * reference's generated C (a default build), bit for bit.
+ * QLA_MI355X_ORDER_SSE3: the five QLA_F3 symbols that a reference built
+ * --enable-sse3 replaces with re-associated SSE code -- V_vpeq_M_times_pV,
+ * V_veq_M_times_pV, V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and
+ * D_vpeq_spproj_M_times_pD (lib/optimized/sse/src/) -- round exactly like that
+ * build instead, including its choice of order by the alignment of r, a and
+ * b[0] and, for the V symbols, by the position of a site in its 1- and
+ * 2-site loops; every other symbol, and the batches, keep the generic order.
+ * Both orders are within 3 ulp (of the sum of the term magnitudes) of the
+ * exact result. */
+enum { QLA_MI355X_ORDER_GENERIC = 0, QLA_MI355X_ORDER_SSE3 = 1 };
+void QLA_MI355X_set_arith_order(int order);
+int QLA_MI355X_get_arith_order(void);
 /* Per host thread: 1 (default) = each call returns with results complete,
  * 0 = calls are stream-ordered and return after the launch. */
 void QLA_MI355X_set_sync(int on);

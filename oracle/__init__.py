@@ -43,6 +43,7 @@ def load():
     for pre in ("F3", "D3"):
         getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32, i32, vp, i32]
         getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, vp, i32, vp, i32]
+    lib.orc_F3_sse_site.argtypes = [i32, i32, i32, vp, vp, vp, i32, i32, i32, i32, i32, i32]
     for p in ("F", "D"):
         f = getattr(lib, f"orc_{p}_norm2")
         f.argtypes, f.restype = [vp, i32, i32], C.c_double
@@ -198,6 +199,30 @@ def D_eqop_sprecon_M_times_pH(eqop, r, a, h, h_idx, mu, sign, index=None, adj=Fa
 def D_eqop_spproj_M_times_pD(eqop, r, a, d, d_idx, mu, sign, index=None, adj=False):
     n = None if index is not None else len(d_idx)
     return site(WILSON, eqop, r, a, d, mu, sign, index=index, adj=adj, b_idx=d_idx, n=n)
+
+
+# (kind, eqop, adj) of the symbols whose --enable-sse3 build rounds in
+# another order (qla_oracle_sse.inc)
+SSE_SYMBOLS = {"V_vpeq_M_times_pV": (MATVEC, PEQ, 0), "V_veq_M_times_pV": (MATVEC, EQ, 0),
+               "V_veq_Ma_times_V": (MATVEC, EQ, 1), "D_vpeq_sprecon_M_times_pH": (SPRECON, PEQ, 0),
+               "D_vpeq_spproj_M_times_pD": (WILSON, PEQ, 0)}
+
+
+def sse_site(symbol, r, a, b, b_idx=None, mu=0, sign=1, r_al=-1, a_al=-1, b0_al=-1):
+    """QLA_F3_<symbol> (a key of SSE_SYMBOLS) in the operation order of a QLA
+    built --enable-sse3 (qla_oracle_sse.inc), in place on r.  b is the field
+    whose records b_idx[i] the pointer array names (a field operand: b_idx
+    None).  r_al / a_al / b0_al are the byte offsets mod 16 of r, a and b[0]
+    in the call being modelled (the reference picks its order from them); -1
+    takes these host arrays' own."""
+    kind, eqop, adj = SSE_SYMBOLS[symbol]
+    assert r.dtype == np.float32
+    lib = load()
+    pa = _records(a, None)
+    pb = _records(b, b_idx)
+    n = r.shape[0] if b_idx is None else len(b_idx)
+    lib.orc_F3_sse_site(kind, eqop, adj, _ptr(_need(r)), _ptr(pa), _ptr(pb), mu, sign, n, r_al, a_al, b0_al)
+    return r
 
 
 def records(a: np.ndarray, index=None, a_idx=None, own=None) -> np.ndarray:

@@ -27,6 +27,7 @@
 #include "qla_oracle_impl.inc"
 #include "qla_oracle_matfunc.inc"
 #include "qla_oracle_vecops.inc"
+#include "qla_oracle_sse.inc"
 #undef REAL
 #undef P
 #undef SQRTP

@@ -40,6 +40,13 @@ enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
  * result of step i at r[rindex[i]] (the per-operand destination index of the
  * QLA_INDEX_ALL forms, indirection_all.pl:66-84).  mu outside 0..4 is a
  * no-op for kinds 1-3. */
+/* the operation order of a QLA built --enable-sse3 for V_vpeq_M_times_pV
+ * (kind 0, PEQ), V_veq_M_times_pV (0, EQ), V_veq_Ma_times_V (0, EQ, adj),
+ * D_vpeq_sprecon_M_times_pH (2, PEQ) and D_vpeq_spproj_M_times_pD (3, PEQ);
+ * r_al / a_al / b0_al: byte offsets mod 16 of r, a, b[0] in the modelled call
+ * (-1: these arrays' own) -- qla_oracle_sse.inc */
+void orc_F3_sse_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb, int mu,
+                     int sign, int n, int r_al, int a_al, int b0_al);
 void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const float *const *pa,
                  const float *const *pb, int mu, int sign, const int *index, int n);
 void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,

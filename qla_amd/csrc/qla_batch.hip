@@ -38,6 +38,7 @@ __device__ __forceinline__ site_args job_args(const batch_job &j)
     p.ir = p.ia = p.ib = nullptr;
     p.n = j.n;
     p.al = j.al;
+    p.order = 0;
     p.ms = j.ms;
     p.pa = 0;
     p.pb = 1;

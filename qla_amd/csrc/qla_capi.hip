@@ -27,6 +27,7 @@ namespace {
 struct thread_state {
     hipStream_t stream = nullptr;
     int sync = 1;
+    int order = QLA_MI355X_ORDER_GENERIC;
     hipError_t err = hipSuccess;
 };
 
@@ -677,8 +678,17 @@ void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, 
     p.ib = ib;
     p.n = n;
     p.al = ((!pa && (uintptr_t)a % 16 == 0) ? 1 : 0) | ((!pb && (uintptr_t)b % 16 == 0) ? 2 : 0) |
-           (((uintptr_t)r % 16 == 0) ? 4 : 0);
+           (((uintptr_t)r % 16 == 0) ? 4 : 0) | (((uintptr_t)r % 8 == 0) ? 8 : 0) |
+           ((!pa && (uintptr_t)a % 8 == 0) ? 16 : 0);
     p.ms = spin ? mu + (sign == 1 ? 0 : 5) : 0;
+    // the SSE3-build order exists for the five F3 v-form symbols that build
+    // overrides with a different association (QLA_F3_V_vpeq_M_times_pV,
+    // V_veq_M_times_pV, V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH,
+    // D_vpeq_spproj_M_times_pD); every other symbol keeps the generic order
+    const bool v_sym = kind == 0 && !pa && ((!adj && pb && (eqop == 0 || eqop == 1)) || (adj && !pb && eqop == 0));
+    const bool spin_sym = (kind == 2 || kind == 3) && !adj && !pa && pb && eqop == 1;
+    p.order = (ts.order == QLA_MI355X_ORDER_SSE3 && prec == 0 && !index && !ir && !ia && !ib && (v_sym || spin_sym))
+                  ? 1 : 0;
     p.pa = pa;
     p.pb = pb;
     const int xf = (index || ir || ia || ib) ? 1 : 0;
@@ -924,6 +934,15 @@ extern "C" {
 void QLA_MI355X_set_stream(void *stream) { ts.stream = (hipStream_t)stream; }
 void *QLA_MI355X_get_stream(void) { return (void *)ts.stream; }
 void QLA_MI355X_set_sync(int on) { ts.sync = on ? 1 : 0; }
+void QLA_MI355X_set_arith_order(int order)
+{
+    if (order != QLA_MI355X_ORDER_GENERIC && order != QLA_MI355X_ORDER_SSE3) {
+        record(hipErrorInvalidValue, "QLA_MI355X_set_arith_order");
+        return;
+    }
+    ts.order = order;
+}
+int QLA_MI355X_get_arith_order(void) { return ts.order; }
 void QLA_MI355X_synchronize(void) { record(hipStreamSynchronize(ts.stream), "synchronize"); }
 int QLA_MI355X_last_error(void)
 {

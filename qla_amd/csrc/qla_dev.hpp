@@ -322,6 +322,128 @@ __device__ __forceinline__ void mat_times_mat(cpx<T> *r, const cpx<T> *a, const 
 }
 
 // ---------------------------------------------------------------------------
+// The association orders of the reference built --enable-sse3 (its
+// lib/optimized/sse versions of V_vpeq_M_times_pV, V_veq_M_times_pV,
+// V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and D_vpeq_spproj_M_times_pD).
+// SSE float lanes round like scalar float operations and x + (-y) == x - y,
+// so each is a fixed sequence of float operations; ar / ai are the real and
+// imaginary parts of the matrix elements of one row (or, for the adjoint,
+// one column), x_k the source column with stride XS.
+// ---------------------------------------------------------------------------
+// vector order of the V bodies (QLA_F3_V_vpeq_M_times_pV_{1,2}_sse.c):
+// re = ((r.re + ar0 x0r) + ar1 x1r) + ar2 x2r - ((ai0 x0i + ai1 x1i) + ai2 x2i),
+// im = ((r.im + ai0 x0r) + ai1 x1r) + ai2 x2r + ((ar0 x0i + ar1 x1i) + ar2 x2i);
+// without r (V_veq) the first product starts each sum (sse_half)
+template <class T>
+__device__ __forceinline__ cpx<T> sse_pair(cpx<T> r, const cpx<T> *a, const cpx<T> *x)
+{
+    T R = r.re + a[0].re * x[0].re, I = r.im + a[0].im * x[0].re;
+    T Qr = a[0].re * x[0].im, Qi = a[0].im * x[0].im;
+    R = R + a[1].re * x[1].re;
+    I = I + a[1].im * x[1].re;
+    Qr = Qr + a[1].re * x[1].im;
+    Qi = Qi + a[1].im * x[1].im;
+    R = R + a[2].re * x[2].re;
+    I = I + a[2].im * x[2].re;
+    Qr = Qr + a[2].re * x[2].im;
+    Qi = Qi + a[2].im * x[2].im;
+    return { R - Qi, I + Qr };
+}
+// the multiply of the sprecon / spproj bodies (and V_veq's vector order):
+// t.re = ((ar0 x0r + ar1 x1r) + ar2 x2r) - ((ai0 x0i + ai1 x1i) + ai2 x2i),
+// t.im = ((ar0 x0i + ar1 x1i) + ar2 x2i) + ((ai0 x0r + ai1 x1r) + ai2 x2r)
+template <int XS, class T>
+__device__ __forceinline__ cpx<T> sse_half(const cpx<T> *a, const cpx<T> *x)
+{
+    T R = a[0].re * x[0].re, I = a[0].re * x[0].im;
+    T Qr = a[0].im * x[0].re, Qi = a[0].im * x[0].im;
+    R = R + a[1].re * x[XS].re;
+    I = I + a[1].re * x[XS].im;
+    Qr = Qr + a[1].im * x[XS].re;
+    Qi = Qi + a[1].im * x[XS].im;
+    R = R + a[2].re * x[2 * XS].re;
+    I = I + a[2].re * x[2 * XS].im;
+    Qr = Qr + a[2].im * x[2 * XS].re;
+    Qi = Qi + a[2].im * x[2 * XS].im;
+    return { R - Qi, I + Qr };
+}
+// the unaligned scalar fallback of the sprecon / spproj files:
+// t.re = (((((0 + ar0 x0r) - ai0 x0i) + ar1 x1r) - ai1 x1i) + ar2 x2r) - ai2 x2i,
+// t.im = (((((0 + ai0 x0r) + ar0 x0i) + ai1 x1r) + ar1 x1i) + ai2 x2r) + ar2 x2i
+template <int XS, class T>
+__device__ __forceinline__ cpx<T> sse_scalar(const cpx<T> *a, const cpx<T> *x)
+{
+    T tr = T(0), ti = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const T xr = x[k * XS].re, xi = x[k * XS].im;
+        tr = tr + a[k].re * xr;
+        ti = ti + a[k].im * xr;
+        tr = tr - a[k].im * xi;
+        ti = ti + a[k].re * xi;
+    }
+    return { tr, ti };
+}
+// row 2 of the 1-site V bodies, left to right, added to r (vpeq) or stored:
+// ((((ar0 x0r - ai0 x0i) + ar1 x1r) - ai1 x1i) + ar2 x2r) - ai2 x2i and
+// ((((ar0 x0i + ai0 x0r) + ar1 x1i) + ai1 x1r) + ar2 x2i) + ai2 x2r
+template <bool HAVE_R, class T>
+__device__ __forceinline__ cpx<T> sse_row2(cpx<T> r, const cpx<T> *a, const cpx<T> *x)
+{
+    T er = a[0].re * x[0].re;
+    er = er - a[0].im * x[0].im;
+    er = er + a[1].re * x[1].re;
+    er = er - a[1].im * x[1].im;
+    er = er + a[2].re * x[2].re;
+    er = er - a[2].im * x[2].im;
+    T ei = a[0].re * x[0].im;
+    ei = ei + a[0].im * x[0].re;
+    ei = ei + a[1].re * x[1].im;
+    ei = ei + a[1].im * x[1].re;
+    ei = ei + a[2].re * x[2].im;
+    ei = ei + a[2].im * x[2].re;
+    if (HAVE_R) return { r.re + er, r.im + ei };
+    return { er, ei };
+}
+// V_veq_Ma_times_V, a_k = a[k][ic] (column stride 3):
+// vector order  re = ((ai0 x0i + ai1 x1i) + ai2 x2i) + ((ar0 x0r + ar1 x1r) + ar2 x2r),
+//               im = ((ar0 x0i + ar1 x1i) + ar2 x2i) - ((ai0 x0r + ai1 x1r) + ai2 x2r);
+// row 2         re = ((((ar0 x0r + ai0 x0i) + ar1 x1r) + ai1 x1i) + ar2 x2r) + ai2 x2i,
+//               im = ((((ar0 x0i - ai0 x0r) + ar1 x1i) - ai1 x1r) + ar2 x2i) - ai2 x2r
+template <class T>
+__device__ __forceinline__ cpx<T> sse_adj_pair(const cpx<T> *a, const cpx<T> *x)
+{
+    T Rr = a[0].re * x[0].re, Ri = a[0].im * x[0].re;
+    T Ir = a[0].re * x[0].im, Ii = a[0].im * x[0].im;
+    Rr = Rr + a[3].re * x[1].re;
+    Ri = Ri + a[3].im * x[1].re;
+    Ir = Ir + a[3].re * x[1].im;
+    Ii = Ii + a[3].im * x[1].im;
+    Rr = Rr + a[6].re * x[2].re;
+    Ri = Ri + a[6].im * x[2].re;
+    Ir = Ir + a[6].re * x[2].im;
+    Ii = Ii + a[6].im * x[2].im;
+    return { Ii + Rr, Ir - Ri };
+}
+template <class T>
+__device__ __forceinline__ cpx<T> sse_adj_row2(const cpx<T> *a, const cpx<T> *x)
+{
+    T er = a[0].re * x[0].re;
+    er = er + a[0].im * x[0].im;
+    er = er + a[3].re * x[1].re;
+    er = er + a[3].im * x[1].im;
+    er = er + a[6].re * x[2].re;
+    er = er + a[6].im * x[2].im;
+    T ei = a[0].re * x[0].im;
+    ei = ei - a[0].im * x[0].re;
+    ei = ei + a[3].re * x[1].im;
+    ei = ei - a[3].im * x[1].re;
+    ei = ei + a[6].re * x[2].im;
+    ei = ei - a[6].im * x[2].re;
+    return { er, ei };
+}
+
+// ---------------------------------------------------------------------------
 // Spin projection (expressions_gamma.pl:244-276) for one colour row d[0..3].
 // ---------------------------------------------------------------------------
 template <class T> __device__ __forceinline__ cpx<T> c_plus(cpx<T> a, cpx<T> b)  { return { a.re + b.re, a.im + b.im }; }

@@ -17,7 +17,11 @@ struct qla_site_args {
     const int *ia;      // record of site i is r[ir[i]], a[ia[i]], b[ib[i]]; an
     const int *ib;      // operand without one uses the gang index or i
     int n;              // sites (v-form) / index entries (x-form)
-    int al;             // 16-byte alignment of the contiguous fields: 1 a, 2 b, 4 r
+    int al;             // 16-byte alignment of the contiguous fields: 1 a, 2 b, 4 r;
+                        // 8-byte alignment: 8 r, 16 a
+    int order;          // 1: the SSE3-build order (F3 V_vpeq_M_times_pV,
+                        // V_veq_M_times_pV, V_veq_Ma_times_V, the vpeq sprecon
+                        // and spproj-multiply-reconstruct), else 0 (generic)
     int ms;             // mu + 5 * (sign != 1), 0..9
     int pa, pb;         // a / b are pointer arrays
 };

@@ -105,10 +105,10 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
     }
 }
 
-template <class T, int KIND, int EQOP, bool ADJ, bool XF, int WPB>
+template <class T, int KIND, int EQOP, bool ADJ, bool XF, int WPB, bool SSEO = false>
 __global__ void __launch_bounds__(WPB * WAVE) k_site(site_args p)
 {
-    run_tiles<site_op<T, KIND, EQOP, ADJ, XF>, T, KIND, XF, WPB>(p);
+    run_tiles<site_op<T, KIND, EQOP, ADJ, XF, -1, SSEO>, T, KIND, XF, WPB>(p);
 }
 
 // waves per block: two for FP32, one for FP64 (the LDS tile of a wave is
@@ -125,8 +125,14 @@ static hipError_t launch_one(const site_args &p, int tpw, hipStream_t st)
     const long long ntiles = ((long long)p.n + WAVE - 1) / WAVE;
     const long long per_block = (long long)W * (tpw < 1 ? 1 : tpw);
     const long long g = (ntiles + per_block - 1) / per_block;
-    hipLaunchKernelGGL((k_site<T, SITE_KIND, EQOP, ADJ, XF, W>), dim3((unsigned)(g < 1 ? 1 : g)), dim3(W * WAVE), 0,
-                       st, p);
+    const dim3 grid((unsigned)(g < 1 ? 1 : g)), block(W * WAVE);
+    if constexpr (sse_order<T, SITE_KIND, EQOP, ADJ, XF>::v) {
+        if (p.order) {
+            hipLaunchKernelGGL((k_site<T, SITE_KIND, EQOP, ADJ, XF, W, true>), grid, block, 0, st, p);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((k_site<T, SITE_KIND, EQOP, ADJ, XF, W>), grid, block, 0, st, p);
     return hipGetLastError();
 }
 

@@ -27,6 +27,7 @@
 // compile-time; mu/sign and the pointer flags are wave-uniform run-time
 // values (a scalar branch per tile).
 #pragma once
+#include <type_traits>
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -94,8 +95,9 @@ __device__ __forceinline__ void lds_write(char *p, const cpx<T> (&v)[N])
     for (int i = 0; i < N; i++) q[i] = v[i];
 }
 
-// alignment flags of the contiguous operands (bit set: 16-byte aligned)
-enum { AL_A = 1, AL_B = 2, AL_R = 4 };
+// alignment flags of the contiguous operands (bit set: 16-byte aligned;
+// AL_R8 / AL_A8: r / a 8-byte aligned, which the SSE-order dispatch reads)
+enum { AL_A = 1, AL_B = 2, AL_R = 4, AL_R8 = 8, AL_A8 = 16 };
 
 // kinds of site operation
 enum {
@@ -192,11 +194,70 @@ __device__ __forceinline__ lane_addr addresses_of(const site_args &p, const lane
 }
 
 // ---------------------------------------------------------------------------
+// The SSE3-build order (SSEO): which order site i of a v-form call gets.
+// The V symbols split the call as the reference's dispatch does
+// (QLA_F3_V_vpeq_M_times_pV.c:59-103): nb1 = 1 when r and a are 8-byte but r
+// not 16-byte aligned; sites [nb1, nb1 + n1), n1 = n - nb1 rounded down to
+// even, run its 2-site body (vector order in every row), the others its
+// 1-site body (vector order in rows 0-1, the row-2 expression in row 2).
+// sprecon / spproj take the SSE multiply when r is 16-, a 8- and b[0]
+// 16-byte aligned (QLA_F3_D_vpeq_sprecon_M_times_pH.c:108), else the scalar
+// fallback's.
+// ---------------------------------------------------------------------------
+template <class T, int KIND, int EQOP, bool ADJ, bool XF> struct sse_order {
+    static constexpr bool v = std::is_same<T, float>::value && !XF &&
+                              ((KIND == K_MATVEC && !ADJ && (EQOP == PEQ || EQOP == EQ)) ||
+                               (KIND == K_MATVEC && ADJ && EQOP == EQ) ||
+                               ((KIND == K_SPRECON || KIND == K_WILSON) && !ADJ && EQOP == PEQ));
+};
+
+__device__ __forceinline__ bool sse_two_site(const site_args &p, int i)
+{
+    const int nb1 = ((p.al & AL_R8) && (p.al & AL_A8) && !(p.al & AL_R)) ? 1 : 0;
+    const int n1 = (p.n - nb1) & ~1;
+    return i >= nb1 && i < nb1 + n1;
+}
+
+// r[ic] (eqop) A x for the three V symbols in the SSE build's order
+template <int EQOP, bool ADJ, class T>
+__device__ __forceinline__ void sse_matvec(cpx<T> *r, const cpx<T> *a, const cpx<T> *x, bool two)
+{
+    if constexpr (ADJ) {
+        r[0] = sse_adj_pair(a + 0, x);
+        r[1] = sse_adj_pair(a + 1, x);
+        r[2] = two ? sse_adj_pair(a + 2, x) : sse_adj_row2(a + 2, x);
+    } else if constexpr (EQOP == PEQ) {
+        r[0] = sse_pair(r[0], a, x);
+        r[1] = sse_pair(r[1], a + 3, x);
+        r[2] = two ? sse_pair(r[2], a + 6, x) : sse_row2<true>(r[2], a + 6, x);
+    } else {
+        r[0] = sse_half<1>(a, x);
+        r[1] = sse_half<1>(a + 3, x);
+        r[2] = two ? sse_half<1>(a + 6, x) : sse_row2<false>(r[2], a + 6, x);
+    }
+}
+
+// t[ic][s] = sum_k a[ic][k] x[k][s] (2 spins) in the SSE multiply or the
+// scalar fallback order
+template <class T>
+__device__ __forceinline__ void sse_half_cols(cpx<T> *t, const cpx<T> *a, const cpx<T> *x, bool aligned)
+{
+#pragma unroll
+    for (int ic = 0; ic < 3; ic++) {
+#pragma unroll
+        for (int is = 0; is < 2; is++)
+            t[2 * ic + is] = aligned ? sse_half<2>(a + 3 * ic, x + is) : sse_scalar<2>(a + 3 * ic, x + is);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The tile operation.  LDS per wave: A tile (none for the matrix-free
 // kinds), B tile, and (peq/meq) the destination tile; eq/eqm write their
-// result over the A/B region when it fits there, else after it.
+// result over the A/B region when it fits there, else after it.  SSEO
+// selects the SSE3-build order for the symbols sse_order covers.
 // ---------------------------------------------------------------------------
-template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct site_op {
+template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1, bool SSEO = false> struct site_op {
+    static_assert(!SSEO || sse_order<T, KIND, EQOP, ADJ, XF>::v, "no SSE-build order for this form");
     typedef rec<T> R;
     typedef kind_rec<T, KIND> K;
     static constexpr bool UNARY = kind_unary<KIND>::v;
@@ -265,7 +326,8 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct si
         } else if constexpr (KIND == K_MATVEC) {
             cpx<T> v[3];
             lds_read(v, lb);
-            mat_times_cols<EQOP, ADJ, 1>(o, m, v);
+            if constexpr (SSEO) sse_matvec<EQOP, ADJ>(o, m, v, sse_two_site(p, base + lane));
+            else mat_times_cols<EQOP, ADJ, 1>(o, m, v);
         } else if constexpr (KIND == K_MAT_H || KIND == K_MAT_D) {
             constexpr int NS = KIND == K_MAT_H ? 2 : 4;
             cpx<T> v[3 * NS];
@@ -286,10 +348,16 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct si
             mat_times_cols<EQOP, ADJ, 2>(o, m, tp);
         } else {
             cpx<T> tv[6];
+            // SSE order: the reference's alignment test, with b[0] the first
+            // pointer-array entry (a wave-uniform load)
+            bool sse_al = false;
+            if constexpr (SSEO)
+                sse_al = (p.al & AL_R) && (p.al & AL_A8) && ((*(const uint64_t *)p.b) & 15) == 0;
             if constexpr (KIND == K_SPRECON) {
                 cpx<T> h[6];
                 lds_read(h, lb);
-                mat_times_cols<EQ, ADJ, 2>(tv, m, h);
+                if constexpr (SSEO) sse_half_cols(tv, m, h, sse_al);
+                else mat_times_cols<EQ, ADJ, 2>(tv, m, h);
             } else {
                 cpx<T> d[12], tp[6];
                 lds_read(d, lb);
@@ -298,7 +366,8 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1> struct si
                     for (int ic = 0; ic < 3; ic++)
                         spproj_row<decltype(c)::MU, decltype(c)::SIGN>(tp[2 * ic], tp[2 * ic + 1], d + 4 * ic);
                 });
-                mat_times_cols<EQ, ADJ, 2>(tv, m, tp);
+                if constexpr (SSEO) sse_half_cols(tv, m, tp, sse_al);
+                else mat_times_cols<EQ, ADJ, 2>(tv, m, tp);
             }
             with_ms<MS>(p.ms, [&](auto c) {
                 typedef recon_tab<decltype(c)::MU, decltype(c)::SIGN> TAB;

@@ -54,6 +54,8 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_set_stream": (None, [vp]),
     "QLA_MI355X_get_stream": (vp, []),
     "QLA_MI355X_set_sync": (None, [i32]),
+    "QLA_MI355X_set_arith_order": (None, [i32]),
+    "QLA_MI355X_get_arith_order": (i32, []),
     "QLA_MI355X_synchronize": (None, []),
     "QLA_MI355X_last_error": (i32, []),
     "QLA_MI355X_last_error_string": (C.c_char_p, []),
