@@ -11,13 +11,16 @@
 // the general kernels; the Hermitian eigenvalues exist for nc = 3 only (the
 // template's QR iteration for larger nc is not provided).
 //
-// One thread owns one matrix; the grid has one thread per matrix, so the
-// hardware dispatcher balances the data-dependent work of exp, sqrt,
+// One thread owns one matrix (for nc = 5..8 exp, sqrt and invsqrt, whose
+// per-thread temporaries would not fit in registers, a group of 8 lanes owns
+// one, a row per lane: qla_matfn_rows.hpp); the grid covers every matrix, so
+// the hardware dispatcher balances the data-dependent work of exp, sqrt,
 // invsqrt and log (squarings, Newton steps) by starting a new workgroup
 // wherever one retires.  (A global work counter claimed per wave costs one
 // same-address atomic per 64 matrices, which serialises at the L2 and capped
 // every function at about 4.4e9 matrices/s.)
 #include "qla_matfn.hpp"
+#include "qla_matfn_rows.hpp"
 #include "qla_launch.hpp"
 
 #ifndef QLA_MATFN_UNIT
@@ -200,9 +203,49 @@ __global__ void __launch_bounds__(MF_BLOCK) k_matfunc(qla_matfunc_args p)
     LOOP(k, S::OUT) dst[k] = out[k];
 }
 
+// nc = 5..8 exp / sqrt / invsqrt: one matrix per group of RG lanes, one row
+// per lane (qla_matfn_rows.hpp); every indexing form
+template <int NC, int FN> struct by_rows {
+    static constexpr bool v = NC >= 5 && (FN == MF_EXP || FN == MF_SQRT || FN == MF_INVSQRT);
+};
+
+template <class T, int NC, int FN>
+__global__ void __launch_bounds__(MF_BLOCK) k_matfunc_rows(qla_matfunc_args p)
+{
+    constexpr int GPB = MF_BLOCK / RG;
+    __shared__ rgrp<T, NC> sh[GPB];
+    const int g = threadIdx.x / RG;
+    const long long i = (long long)blockIdx.x * GPB + g;
+    if (i >= p.n) return;           // whole groups: RG divides the block
+    const int row = (int)(threadIdx.x % RG);
+    const rctx<T, NC> c = { &sh[g], row, row < NC };
+    const long long site = p.index ? (long long)p.index[i] : i;
+    const long long rs = p.ir ? (long long)p.ir[i] : site;
+    const long long as = p.ia ? (long long)p.ia[i] : site;
+    rrow<T, NC> a, r;
+    if (c.act) {
+        const cpx<T> *src = operand<T>(p.a, p.pa, as, NC * NC) + row * NC;
+        LOOP(j, NC) a.e[j] = src[j];
+    } else {
+        LOOP(j, NC) a.e[j] = { T(0), T(0) };
+    }
+    if constexpr (FN == MF_EXP) r_exp(r, a, c);
+    else if constexpr (FN == MF_SQRT) r_sqrt(r, a, c);
+    else r_invsqrt(r, a, c);
+    if (c.act) {
+        cpx<T> *dst = (cpx<T> *)p.r + (size_t)rs * (NC * NC) + row * NC;
+        LOOP(j, NC) dst[j] = r.e[j];
+    }
+}
+
 template <class T, int NC, int FN>
 static hipError_t launch_t(const qla_matfunc_args &p, hipStream_t st)
 {
+    if constexpr (by_rows<NC, FN>::v) {
+        const long long gblocks = ((long long)p.n + MF_BLOCK / RG - 1) / (MF_BLOCK / RG);
+        hipLaunchKernelGGL((k_matfunc_rows<T, NC, FN>), dim3((unsigned)gblocks), dim3(MF_BLOCK), 0, st, p);
+        return hipGetLastError();
+    }
     const long long blocks = ((long long)p.n + MF_BLOCK - 1) / MF_BLOCK;
     const bool contiguous = !p.index && !p.ir && !p.ia && !p.ib && !p.pa && !p.pb;
     if constexpr (stage_of<T, NC, FN>::OK) {
