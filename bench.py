@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true",
                    help="headline only (no companion measurements), e.g. for profiler runs")
-    p.add_argument("--group", choices=("vecops", "solver"), default=None,
+    p.add_argument("--group", choices=("vecops", "solver", "matfn"), default=None,
                    help="run one companion group alone and print its figures (profiler runs of its kernels)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -96,8 +96,13 @@ def main():
     f = lib.QLA_F3_H_veq_spproj_M_times_pD
 
     if args.group:
-        res = vecops_layer(lib, stream, n, sets, args) if args.group == "vecops" else wilson_cg(lib, stream, n, args)
-        print(json.dumps({"group": args.group, "workload": "32^4 periodic", "result": res}))
+        if args.group == "vecops":
+            res, wl = vecops_layer(lib, stream, n, sets, args), "32^4 periodic"
+        elif args.group == "solver":
+            res, wl = wilson_cg(lib, stream, n, args), "32^4 periodic"
+        else:
+            res, wl = matfn_config4(lib, stream, args), "2,097,152 Nc=3 / 262,144 Nc=4,8 matrices"
+        print(json.dumps({"group": args.group, "workload": wl, "result": res}))
         return
 
     def step(it):
@@ -639,36 +644,68 @@ def batch_config3(lib, stream, args):
             "speedup_graph_vs_sequential": round(ms_seq / ms_graph, 3)}
 
 
+# FP32 flops per matrix of the reference's own model (examples/qla_bench.c:35-48,
+# one Newton step assumed): exp, sqrt, invsqrt at nc = 3 and the general formulas
+def _ref_flops(fn: str, nc: int) -> float:
+    if nc == 3:
+        return {"exp": 699.0, "sqrt": 708.0, "invsqrt": 1018.0}[fn]
+    if fn == "exp":
+        return ((44 * nc + 29.5) * nc + 2.5) * nc + 3
+    if fn == "sqrt":
+        return ((40 * nc + 23) * nc + 5) * nc + 3
+    return ((52 * nc + 20.5) * nc + 7.5) * nc + 4
+
+
+FP32_PEAK_TFLOPS = 157.3
+
+
 def matfn_config4(lib, stream, args):
     """M_veq_{exp,sqrt,invsqrt}_M on 2,097,152 Nc=3 matrices (F3) and on
     262,144 Nc=4 / Nc=8 matrices (FN), then the other F3 colour-matrix
     functions (expA, expTA, sqrtPH, invsqrtPH, log, det, inverse,
     eigenvalsH, M_inverse_V) on the 2,097,152 Nc=3 matrices: matrices per
-    second and algorithmic GB/s (one record read, one written)."""
+    second, algorithmic GB/s (one record read, one written) and its fraction
+    of the 8 TB/s HBM peak, and for exp / sqrt / invsqrt the FP32 rate by the
+    reference's own flop model (examples/qla_bench.c) and its fraction of the
+    157.3 TFLOP/s FP32 peak.  Two input / output sets alternate between calls
+    (Nc = 3: 2 x 302 MB, Nc = 8: 2 x 268 MB, more than the 256 MB Infinity
+    Cache; the Nc = 4 sets, 2 x 67 MB, stay cache-resident)."""
     import torch
     from qla_amd import lattice
     out = {}
     n3 = 2097152
-    inputs = {"exp": lattice.random_antihermitian(n3, 3, 41), "sqrt": lattice.random_hpd(n3, 3, 42)}
-    inputs["invsqrt"] = inputs["sqrt"]
-    for fn in ("exp", "sqrt", "invsqrt"):
-        a = torch.from_numpy(inputs[fn].astype(np.float32)).cuda()
-        r = torch.empty_like(a)
-        f = getattr(lib, f"QLA_F3_M_veq_{fn}_M")
-        ms = _timed(stream, lambda it: f(r.data_ptr(), a.data_ptr(), n3), 10, 2) / 10
-        out[f"F3_{fn}"] = {"n": n3, "ms": round(ms, 3), "matrices_per_s": round(n3 / (ms * 1e-3), 1),
-                           "GBs": round(144 * n3 / (ms * 1e-3) / 1e9, 1)}
-    del inputs
-    for nc in (4, 8):
-        nn = 262144
-        ins = {"exp": lattice.random_antihermitian(nn, nc, 43), "sqrt": lattice.random_hpd(nn, nc, 44)}
-        ins["invsqrt"] = ins["sqrt"]
-        for fn in ("exp", "sqrt", "invsqrt"):
-            a = torch.from_numpy(ins[fn].astype(np.float32)).cuda()
-            r = torch.empty_like(a)
+    gens = {"exp": (lattice.random_antihermitian, 41), "sqrt": (lattice.random_hpd, 42),
+            "invsqrt": (lattice.random_hpd, 42)}
+
+    def measure(fn, nc, nn, reps):
+        gen, seed = gens[fn]
+        sets = []
+        for k in range(2):
+            a = torch.from_numpy(gen(nn, nc, seed + 100 * k + (0 if nc == 3 else 2)).astype(np.float32)).cuda()
+            sets.append((a, torch.empty_like(a)))
+        if nc == 3:
+            f = getattr(lib, f"QLA_F3_M_veq_{fn}_M")
+            call = lambda it: f(sets[it % 2][1].data_ptr(), sets[it % 2][0].data_ptr(), nn)      # noqa: E731
+        else:
             f = getattr(lib, f"QLA_FN_M_veq_{fn}_M")
-            ms = _timed(stream, lambda it: f(nc, r.data_ptr(), a.data_ptr(), nn), 6, 2) / 6
-            out[f"FN{nc}_{fn}"] = {"n": nn, "ms": round(ms, 3), "matrices_per_s": round(nn / (ms * 1e-3), 1)}
+            call = lambda it: f(nc, sets[it % 2][1].data_ptr(), sets[it % 2][0].data_ptr(), nn)  # noqa: E731
+        ms = _timed(stream, call, reps, 2) / reps
+        bpm = 2 * nc * nc * 8
+        gbs = bpm * nn / (ms * 1e-3) / 1e9
+        tfl = _ref_flops(fn, nc) * nn / (ms * 1e-3) / 1e12
+        del sets
+        return {"n": nn, "ms": round(ms, 4), "matrices_per_s": round(nn / (ms * 1e-3), 1), "GBs": round(gbs, 1),
+                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "ref_model_flops_per_matrix": _ref_flops(fn, nc),
+                "ref_model_TFLOPs": round(tfl, 2), "valu_frac": round(tfl / FP32_PEAK_TFLOPS, 4)}
+
+    for fn in ("exp", "sqrt", "invsqrt"):
+        out[f"F3_{fn}"] = measure(fn, 3, n3, 10)
+    for nc in (4, 8):
+        for fn in ("exp", "sqrt", "invsqrt"):
+            out[f"FN{nc}_{fn}"] = measure(fn, nc, 262144, 6)
+    if args.group == "matfn":
+        torch.cuda.synchronize()
+        return out
     # the other colour-matrix functions on the same 2,097,152 Nc=3 matrices
     rng = np.random.default_rng(45)
     q = lattice.random_antihermitian(n3, 3, 46)
@@ -694,7 +731,7 @@ def matfn_config4(lib, stream, args):
             nbytes = 72 + rbytes
         ms = _timed(stream, call, 10, 2) / 10
         out[f"F3_{fn}"] = {"n": n3, "ms": round(ms, 3), "matrices_per_s": round(n3 / (ms * 1e-3), 1),
-                           "GBs": round(nbytes * n3 / (ms * 1e-3) / 1e9, 1)}
+                           "GBs": round(nbytes * n3 / (ms * 1e-3) / 1e9, 1), "cache": "one set (working set <= 256 MB)"}
         del a, r
     torch.cuda.synchronize()
     return out
