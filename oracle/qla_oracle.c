@@ -14,6 +14,11 @@
 #endif
 #include "qla_oracle.h"
 
+/* the small per-site helpers are inlined wherever they are called, so a
+ * loop that passes constant mu / sign / eqop gets them folded (the CPU
+ * baseline's loops, below) */
+#define ORC_HOT static inline __attribute__((always_inline))
+
 /* single precision instantiation */
 #define REAL float
 #define P(x) F_##x
@@ -56,6 +61,12 @@
 void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const float *const *pa,
                  const float *const *pb, int mu, int sign, const int *index, int n)
 { F_orc_3_site(kind, eqop, adj, r, rindex, pa, pb, mu, sign, index, n); }
+void orc_F3_site_vf(int kind, int eqop, int adj, float *r, const float *af, const float *const *pb, int mu, int sign,
+                    const int *index, int n)
+{ F_orc_3_site_vf(kind, eqop, adj, r, af, pb, mu, sign, index, n); }
+void orc_D3_site_vf(int kind, int eqop, int adj, double *r, const double *af, const double *const *pb, int mu,
+                    int sign, const int *index, int n)
+{ D_orc_3_site_vf(kind, eqop, adj, r, af, pb, mu, sign, index, n); }
 void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,
                  const double *const *pb, int mu, int sign, const int *index, int n)
 { D_orc_3_site(kind, eqop, adj, r, rindex, pa, pb, mu, sign, index, n); }
@@ -272,4 +283,67 @@ int orc_get_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* ---- benchmark loops of the CPU baseline (oracle/cpu_bench.py) ------------
+ * QLA_F3_H_veq_spproj_M_times_pD in the v-form (a a field, b a QLA pointer
+ * array) and QLA_F3_D_vpeq_sprecon_M_times_pH in the SSE build's order, each
+ * with mu and sign constant inside its site loop, as the generated code has
+ * them (one loop per case of the switch outside it, expressions_gamma.pl:
+ * 349-381), so the spin tables fold away.  Same helpers, same order, same
+ * results as orc_F3_site_vf / orc_F3_sse_site_vf (tests/test_oracle_sse.py). */
+ORC_HOT void bench_spproj_site(float *r, const float *af, const float *b, int mu, int sign)
+{
+    F_cpx t[6];
+    for (int ic = 0; ic < 3; ic++) F_proj_row(&t[2 * ic], &t[2 * ic + 1], (const F_cpx *)b + 4 * ic, mu, sign);
+    F_mat_times_cols(ORC_EQ, 0, (F_cpx *)r, (const F_cpx *)af, t, 2);
+}
+
+ORC_HOT void bench_sprecon_sse_site(float *r, const float *af, const float *b, int mu, int sign, int aligned)
+{
+    F_recon_t tab;
+    F_recon_table(&tab, mu, sign);
+    const F_cpx *A = (const F_cpx *)af, *B = (const F_cpx *)b;
+    F_cpx *R = (F_cpx *)r, tv[6];
+    for (int ic = 0; ic < 3; ic++)
+        for (int s = 0; s < 2; s++)
+            tv[2 * ic + s] = aligned ? F_sse_half(A + 3 * ic, B + s, 2) : F_sse_scalar(A + 3 * ic, B + s, 2);
+    for (int ic = 0; ic < 3; ic++)
+        for (int sp = 0; sp < 4; sp++) F_recon_apply(ORC_PEQ, tab.op[sp], &R[4 * ic + sp], tv[2 * ic + tab.src[sp]]);
+}
+
+#define BENCH_CASES(X) X(0, 1) X(1, 1) X(2, 1) X(3, 1) X(4, 1) X(0, -1) X(1, -1) X(2, -1) X(3, -1) X(4, -1)
+
+void orc_F3_bench_spproj(float *r, const float *af, const float *const *pb, int mu, int sign, int n)
+{
+    if (mu < 0 || mu > 4 || n <= 0) return;
+    switch (mu + (sign == 1 ? 0 : 5)) {
+#define X(MU, SG)                                                                                              \
+    case MU + (SG == 1 ? 0 : 5):                                                                               \
+        _Pragma("omp parallel for schedule(static)") for (int i = 0; i < n; i++)                              \
+            bench_spproj_site(r + 12 * (size_t)i, af + 18 * (size_t)i, pb[i], MU, SG);                         \
+        break;
+        BENCH_CASES(X)
+#undef X
+    }
+}
+
+void orc_F3_bench_sprecon_sse(float *r, const float *af, const float *const *pb, int mu, int sign, int n)
+{
+    if (mu < 0 || mu > 4 || n <= 0) return;
+    const int aligned = (size_t)r % 16 == 0 && (size_t)af % 8 == 0 && (size_t)pb[0] % 16 == 0;
+    switch (mu + (sign == 1 ? 0 : 5)) {
+#define X(MU, SG)                                                                                              \
+    case MU + (SG == 1 ? 0 : 5):                                                                               \
+        if (aligned) {                                                                                         \
+            _Pragma("omp parallel for schedule(static)") for (int i = 0; i < n; i++)                          \
+                bench_sprecon_sse_site(r + 24 * (size_t)i, af + 18 * (size_t)i, pb[i], MU, SG, 1);             \
+        } else {                                                                                               \
+            _Pragma("omp parallel for schedule(static)") for (int i = 0; i < n; i++)                          \
+                bench_sprecon_sse_site(r + 24 * (size_t)i, af + 18 * (size_t)i, pb[i], MU, SG, 0);             \
+        }                                                                                                      \
+        break;
+        BENCH_CASES(X)
+#undef X
+    }
 }

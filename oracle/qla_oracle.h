@@ -47,6 +47,17 @@ enum { ORC_EQ = 0, ORC_PEQ = 1, ORC_MEQ = 2, ORC_EQM = 3 };
  * (-1: these arrays' own) -- qla_oracle_sse.inc */
 void orc_F3_sse_site(int kind, int eqop, int adj, float *r, const float *const *pa, const float *const *pb, int mu,
                      int sign, int n, int r_al, int a_al, int b0_al);
+/* orc_F3_site / orc_D3_site with the colour matrices a contiguous field af */
+void orc_F3_site_vf(int kind, int eqop, int adj, float *r, const float *af, const float *const *pb, int mu, int sign,
+                    const int *index, int n);
+void orc_D3_site_vf(int kind, int eqop, int adj, double *r, const double *af, const double *const *pb, int mu,
+                    int sign, const int *index, int n);
+void orc_F3_sse_site_vf(int kind, int eqop, int adj, float *r, const float *af, const float *const *pb, int mu,
+                        int sign, int n, int r_al, int a_al, int b0_al);
+/* the CPU baseline's loops (mu / sign constant per loop): v-form
+ * H = spproj(M pD) and D += sprecon(M pH) in the SSE build's order */
+void orc_F3_bench_spproj(float *r, const float *af, const float *const *pb, int mu, int sign, int n);
+void orc_F3_bench_sprecon_sse(float *r, const float *af, const float *const *pb, int mu, int sign, int n);
 void orc_F3_site(int kind, int eqop, int adj, float *r, const int *rindex, const float *const *pa,
                  const float *const *pb, int mu, int sign, const int *index, int n);
 void orc_D3_site(int kind, int eqop, int adj, double *r, const int *rindex, const double *const *pa,

@@ -232,3 +232,37 @@ def test_within_4ulp_of_exact(orc, sym):
                     assert err.max() <= 4.0, (sym, mu, sign, r_al, float(err.max()))
     assert differ > 0, "the SSE and generic orders never differed"
     print(f"{sym}: worst {worst:.2f} ulp of the scale")
+
+
+def test_bench_loops_equal_oracle(orc):
+    """The CPU baseline's loops (mu / sign constant per loop, a as a field)
+    give the generic oracle's spproj and the SSE-order sprecon bit for bit,
+    every mu / sign, aligned and unaligned destination."""
+    import ctypes as C
+    lib = orc.load()
+    vp, i32 = C.c_void_p, C.c_int
+    for f in (lib.orc_F3_bench_spproj, lib.orc_F3_bench_sprecon_sse):
+        f.argtypes = [vp, vp, vp, i32, i32, i32]
+    n = 1000
+    rng = np.random.default_rng(12)
+    u = lattice.random_su3(n, 3).astype(np.float32)
+    d = lattice.random_field("D", n, 4).astype(np.float32)
+    h = lattice.random_field("H", n, 5).astype(np.float32)
+    r0 = lattice.random_field("D", n + 1, 6).astype(np.float32)
+    idx = rng.permutation(n)
+    pd, ph = orc.pointer_array(d, idx), orc.pointer_array(h, idx)     # kept alive across the calls
+    for mu in range(5):
+        for sign in (1, -1):
+            got = np.zeros((n, 3, 2, 2), np.float32)
+            lib.orc_F3_bench_spproj(got.ctypes.data, u.ctypes.data, pd.ctypes.data, mu, sign, n)
+            exp = np.zeros_like(got)
+            orc.H_eqop_spproj_M_times_pD(orc.EQ, exp, u, d, idx, mu, sign)
+            np.testing.assert_array_equal(got, exp)
+            for off in (0, 1):       # a D record view 0 or 8 bytes past the array start
+                flat = r0.reshape(-1).copy()
+                view = flat[2 * off:2 * off + n * 24].reshape(n, 3, 4, 2)
+                exp = np.ascontiguousarray(view).copy()
+                orc.sse_site("D_vpeq_sprecon_M_times_pH", exp, u, h, idx, mu, sign,
+                             r_al=view.ctypes.data % 16, a_al=u.ctypes.data % 16, b0_al=h[idx[0]].ctypes.data % 16)
+                lib.orc_F3_bench_sprecon_sse(view.ctypes.data, u.ctypes.data, ph.ctypes.data, mu, sign, n)
+                np.testing.assert_array_equal(view, exp)
