@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true",
                    help="headline only (no companion measurements), e.g. for profiler runs")
-    p.add_argument("--group", choices=("vecops", "solver", "matfn"), default=None,
+    p.add_argument("--group", choices=("vecops", "solver", "matfn", "xform"), default=None,
                    help="run one companion group alone and print its figures (profiler runs of its kernels)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -96,7 +96,14 @@ def main():
     f = lib.QLA_F3_H_veq_spproj_M_times_pD
 
     if args.group:
-        if args.group == "vecops":
+        if args.copy_mib:       # the counter calibration copies, as in the headline run
+            src = torch.empty(args.copy_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
+            dst = torch.empty_like(src)
+            _timed(stream, lambda it: dst.copy_(src), 5, 1)
+            del src, dst
+        if args.group == "xform":
+            res, wl = xforms_config1(lib, stream, n, sets, args), "32^4 periodic, even sites"
+        elif args.group == "vecops":
             res, wl = vecops_layer(lib, stream, n, sets, args), "32^4 periodic"
         elif args.group == "solver":
             res, wl = wilson_cg(lib, stream, n, args), "32^4 periodic"
