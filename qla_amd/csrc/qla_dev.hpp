@@ -208,6 +208,19 @@ __device__ __forceinline__ void store_contig(char *g, const char *lds, int nsite
     }
 }
 
+// Store of a piece of a record whose cache line the call may write only in
+// part: the destination of an x-form (every other record of a parity
+// subset) or of an index form.  A write-back store, not a non-temporal one:
+// the L2 then merges the pieces of a line before the line goes to HBM,
+// where non-temporal stores leave as separate partial-line writes.  On the
+// even sites of 32^4 (three input sets rotating between calls) the x-forms
+// H_xeq_spproj_M_times_pD and D_xpeq_sprecon_M_times_pH took 40.0 and
+// 56.7 us with write-back scatters against 46.4 and 68.6 us with
+// non-temporal ones.  Contiguous destinations, written in whole lines, keep
+// the non-temporal stores, and so do all loads: write-back loads made the
+// x-form slower (52.7 us) and the v-forms 13% slower.
+template <class C> __device__ __forceinline__ void scatter_store(C *p, C v) { *p = v; }
+
 template <int BPS, int CH>
 __device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds, int nsites, int lane)
 {
@@ -219,7 +232,7 @@ __device__ __forceinline__ void store_scatter(uint64_t my_addr, const char *lds,
         const int s = c / NPS;
         const int p = c - s * NPS;
         const uint64_t a = shfl64(my_addr, s);
-        if (s < nsites) stream_store((C *)(a + (uint64_t)p * CH), *(const C *)(lds + c * LS));
+        if (s < nsites) scatter_store((C *)(a + (uint64_t)p * CH), *(const C *)(lds + c * LS));
     }
 }
 
