@@ -111,15 +111,33 @@ __device__ __forceinline__ void stage_gather(char *lds, uint64_t my_addr, int ns
 {
     static_assert(BPS % CH == 0, "chunk must divide the record");
     constexpr int NPS = BPS / CH, LS = lds_stride<CH>::v;
-    // wide chunks: fully unrolled; 4-byte chunks (the misaligned fallback):
-    // a rolled loop, to keep the fast path's register budget
-#pragma unroll(CH == 4 ? 1 : NPS)
-    for (int k = 0; k < NPS; k++) {
-        const int c = k * WAVE + lane;
-        const int s = c / NPS;
-        const int p = c - s * NPS;
-        const uint64_t a = shfl64(my_addr, s);
-        if (s < nsites) dma_load<CH>((const void *)(a + (uint64_t)p * CH), lds + k * WAVE * LS);
+    if constexpr (CH == 4) {
+        // 4-byte chunks (the misaligned fallback): a rolled loop, to keep the
+        // fast path's register budget
+#pragma unroll 1
+        for (int k = 0; k < NPS; k++) {
+            const int c = k * WAVE + lane;
+            const int s = c / NPS;
+            const int p = c - s * NPS;
+            const uint64_t a = shfl64(my_addr, s);
+            if (s < nsites) dma_load<CH>((const void *)(a + (uint64_t)p * CH), lds + k * WAVE * LS);
+        }
+    } else {
+        // wide chunks: every chunk address is fetched from its record's lane
+        // before the first load is issued, so the shuffles overlap instead of
+        // each load waiting for its own
+        uint64_t a[NPS];
+#pragma unroll
+        for (int k = 0; k < NPS; k++) {
+            const int c = k * WAVE + lane;
+            const int s = c / NPS;
+            a[k] = shfl64(my_addr, s) + (uint64_t)(c - s * NPS) * CH;     // s < WAVE: c < NPS * WAVE
+        }
+#pragma unroll
+        for (int k = 0; k < NPS; k++) {
+            const int s = (k * WAVE + lane) / NPS;
+            if (s < nsites) dma_load<CH>((const void *)a[k], lds + k * WAVE * LS);
+        }
     }
 }
 

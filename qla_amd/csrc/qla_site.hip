@@ -71,7 +71,9 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
         nsite = sites_of<XF>(p, i1, nact);
     }
     // the loop body then never waits on a prologue load: its only memory
-    // wait is the one in finish
+    // wait is the one in finish.  (Issuing the first tile's contiguous
+    // records before this wait measured 0.5% faster for spproj but 1.2%
+    // slower for sprecon and 2.2% for the D3 spproj, so it is not done.)
     vm_wait();
     for (;;) {
         const int base = tile * WAVE, ns = min(WAVE, p.n - base);
@@ -111,11 +113,14 @@ __global__ void __launch_bounds__(WPB * WAVE) k_site(site_args p)
     run_tiles<site_op<T, KIND, EQOP, ADJ, XF, -1, SSEO>, T, KIND, XF, WPB>(p);
 }
 
-// waves per block: two for FP32, one for FP64 (the LDS tile of a wave is
-// twice as large); the grid covers the tiles with about `tpw` tiles per wave
-template <class T> struct site_shape;
-template <> struct site_shape<float> { static constexpr int WPB = 2; };
-template <> struct site_shape<double> { static constexpr int WPB = 1; };
+// one wave per block: a block's LDS is released when its last wave ends, so
+// single-wave blocks return each tile's LDS as soon as that tile is done.
+// Measured on one device against two-wave blocks: on 32^4 F3, spproj
+// 37.31 against 37.67 us and sprecon 53.24 against 53.38 us; on 48^4 the
+// other way round (195.4 against 194.1 us, 276.4 against 271.7 us).  One
+// wave per block serves the 32^4 calls.  The grid covers the tiles with
+// about `tpw` tiles per wave.
+template <class T> struct site_shape { static constexpr int WPB = 1; };
 
 template <int EQOP, bool ADJ, bool XF>
 static hipError_t launch_one(const site_args &p, int tpw, hipStream_t st)
