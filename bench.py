@@ -167,7 +167,7 @@ def main():
                    "copy_gbs": None if copy_gbs is None else round(copy_gbs, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "kernel": "qla::k_site<float, 1, 0, false, false, 1> (spproj, eq, v-form)", "avg_launch_ms": round(avg_ms, 5),
+                     "kernel": "qla::k_site<float, 1, 0, false, false, 1, false> (spproj, eq, v-form)", "avg_launch_ms": round(avg_ms, 5),
                      "alg_bytes_per_launch": BYTES_PER_SITE * n},
         "cpu_baseline": cpu,
         "extra": extra,
