@@ -61,7 +61,21 @@ bool capturing()
 // buffer is allocated on its first use, which must not fall inside a
 // stream capture; thread exit releases the thread's buffers and
 // QLA_MI355X_finalize releases all of them.
+//
+// Each buffer comes with a result slot in pinned (page-locked, coherent)
+// host memory.  A reduction whose result pointer is in host memory has its
+// last kernel write the result there, waits for the stream and copies it
+// out on the host: no copy command is queued, and the caller's pageable
+// memory is never the target of a device transfer (a 16^4 F3 norm2 into
+// pageable host memory took 24.4 us that way against 17.0 us into pinned
+// memory and 15.0 us into device memory).
 enum { SCRATCH_REDUCE = 0, SCRATCH_SUM = 1 };
+constexpr int HOST_SLOT_DOUBLES = 288;     // the largest result: a propagator sum, 144 complexes
+
+struct scratch_buf {
+    double *dev = nullptr;      // partials (and the device result slot)
+    double *host = nullptr;     // pinned result slot, HOST_SLOT_DOUBLES
+};
 
 struct scratch_key {
     std::thread::id tid;
@@ -75,7 +89,7 @@ struct scratch_key {
 };
 
 std::mutex g_scratch_mutex;
-std::map<scratch_key, double *> g_scratch;
+std::map<scratch_key, scratch_buf> g_scratch;
 const std::thread::id g_main_thread = std::this_thread::get_id();
 
 // frees the calling thread's buffers when the thread ends (the main thread's
@@ -90,7 +104,8 @@ struct scratch_owner {
             if (it->first.tid == std::this_thread::get_id()) {
                 int cur = 0;
                 if (hipGetDevice(&cur) == hipSuccess && cur != it->first.dev) (void)hipSetDevice(it->first.dev);
-                (void)hipFree(it->second);
+                (void)hipFree(it->second.dev);
+                (void)hipHostFree(it->second.host);
                 if (cur != it->first.dev) (void)hipSetDevice(cur);
                 it = g_scratch.erase(it);
             } else {
@@ -107,11 +122,13 @@ size_t scratch_doubles(int kind)
     return kind == SCRATCH_REDUCE ? (size_t)qla_reduce_scratch_doubles() + 4 : (size_t)qla_sum_scratch_doubles();
 }
 
-double *scratch_for(int kind, const char *name)
+// the calling thread's buffer of this kind for its device and stream
+// (.dev == nullptr after an error, which is recorded)
+scratch_buf scratch_for(int kind, const char *name)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) { record(e, name); return nullptr; }
+    if (e != hipSuccess) { record(e, name); return {}; }
     const scratch_key key = { std::this_thread::get_id(), dev, ts.stream, kind };
     {
         std::lock_guard<std::mutex> lk(g_scratch_mutex);
@@ -122,20 +139,22 @@ double *scratch_for(int kind, const char *name)
         fprintf(stderr, "qla_mi355x: %s: the first reduction on a stream allocates its scratch and cannot be "
                         "captured; run one reduction on this stream before capturing\n", name);
         record(hipErrorStreamCaptureUnsupported, name);
-        return nullptr;
+        return {};
     }
-    double *p = nullptr;
+    scratch_buf b;
     const size_t bytes = sizeof(double) * scratch_doubles(kind);
-    e = hipMalloc((void **)&p, bytes);
+    e = hipMalloc((void **)&b.dev, bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b.host, sizeof(double) * HOST_SLOT_DOUBLES, hipHostMallocCoherent);
     if (e != hipSuccess) {
-        if (p) (void)hipFree(p);
+        if (b.dev) (void)hipFree(b.dev);
+        if (b.host) (void)hipHostFree(b.host);
         record(e, name);
-        return nullptr;
+        return {};
     }
     t_scratch_owner.used = true;
     std::lock_guard<std::mutex> lk(g_scratch_mutex);
-    g_scratch[key] = p;
-    return p;
+    g_scratch[key] = b;
+    return b;
 }
 
 // a result delivered to host memory waits on the host, which a stream
@@ -156,17 +175,26 @@ bool is_device_pointer(const void *p)
     return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
 }
 
-// run a reduction with scratch sc (partials, then a 4-double
+// run a reduction with device scratch sc (partials, then a 4-double
 // result slot) and deliver `bytes` of result to r (host or device);
-// launch(out, scratch) enqueues the kernels that write the result to out
+// launch(out, scratch) enqueues the kernels that write the result to out.
+// A host result goes through the pinned slot `pinned` when there is one,
+// else through sc's device result slot and a copy.
 template <class L>
-void reduce_with(double *sc, L launch, void *r, size_t bytes, const char *name)
+void reduce_with(double *sc, double *pinned, L launch, void *r, size_t bytes, const char *name)
 {
     if (is_device_pointer(r)) {
         finish(launch(r, sc), name);
         return;
     }
     if (refuse_host_result_in_capture(name)) return;
+    if (pinned) {
+        hipError_t e = launch((void *)pinned, sc);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+        if (e == hipSuccess) memcpy(r, pinned, bytes);
+        record(e, name);
+        return;
+    }
     double *slot = sc + qla_reduce_scratch_doubles();
     hipError_t e = launch((void *)slot, sc);
     if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
@@ -177,8 +205,8 @@ void reduce_with(double *sc, L launch, void *r, size_t bytes, const char *name)
 template <class L>
 void reduce_to(L launch, void *r, size_t bytes, const char *name)
 {
-    double *sc = scratch_for(SCRATCH_REDUCE, name);
-    if (sc) reduce_with(sc, launch, r, bytes, name);
+    const scratch_buf sc = scratch_for(SCRATCH_REDUCE, name);
+    if (sc.dev) reduce_with(sc.dev, sc.host, launch, r, bytes, name);
 }
 
 // ------------------------------------------------ allocation-free reduction
@@ -370,18 +398,18 @@ void local_entry(int prec, int kind, int eqop, int ncomp, int pa, int pb, void *
 void sum_entry(int prec_in, int out_double, int ncr, int pa, void *r, const void *a, const int *index, int n,
                size_t bytes, const char *name)
 {
-    double *sc = scratch_for(SCRATCH_SUM, name);
-    if (!sc) return;
+    const scratch_buf sc = scratch_for(SCRATCH_SUM, name);
+    if (!sc.dev) return;
     if (n < 0) n = 0;
     if (is_device_pointer(r)) {
-        finish(qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, r, sc, ts.stream), name);
+        finish(qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, r, sc.dev, ts.stream), name);
         return;
     }
     if (refuse_host_result_in_capture(name)) return;
-    double *slot = sc + qla_sum_scratch_doubles() - 288;
-    hipError_t e = qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, slot, sc, ts.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
+    // the last kernel writes the result into the pinned slot (bytes <= 288 doubles)
+    hipError_t e = qla_launch_sum(prec_in, out_double, ncr, a, index, n, pa, sc.host, sc.dev, ts.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    if (e == hipSuccess) memcpy(r, sc.host, bytes);
     record(e, name);
 }
 
@@ -635,18 +663,17 @@ void int_entry(int op, int eqop, int pa, int pb, void *r, const void *s, const v
 // compressed-field reductions: result doubles to r (host or device)
 void reduce_dh(const void *a, const void *b, int n, void *r, size_t bytes, const char *name)
 {
-    double *sc = scratch_for(SCRATCH_REDUCE, name);
-    if (!sc) return;
+    const scratch_buf sc = scratch_for(SCRATCH_REDUCE, name);
+    if (!sc.dev) return;
     if (n < 0) n = 0;
     if (is_device_pointer(r)) {
-        finish(qla_launch_reduce_dh(a, b, n, (double *)r, sc, ts.stream), name);
+        finish(qla_launch_reduce_dh(a, b, n, (double *)r, sc.dev, ts.stream), name);
         return;
     }
     if (refuse_host_result_in_capture(name)) return;
-    double *slot = sc + qla_reduce_scratch_doubles();
-    hipError_t e = qla_launch_reduce_dh(a, b, n, slot, sc, ts.stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(r, slot, bytes, hipMemcpyDeviceToHost, ts.stream);
+    hipError_t e = qla_launch_reduce_dh(a, b, n, sc.host, sc.dev, ts.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ts.stream);
+    if (e == hipSuccess) memcpy(r, sc.host, bytes);
     record(e, name);
 }
 
@@ -958,7 +985,8 @@ void QLA_MI355X_finalize(void)
     const bool have_cur = hipGetDevice(&cur) == hipSuccess;
     for (auto &kv : g_scratch) {
         (void)hipSetDevice(kv.first.dev);
-        record(hipFree(kv.second), "QLA_MI355X_finalize");
+        record(hipFree(kv.second.dev), "QLA_MI355X_finalize");
+        record(hipHostFree(kv.second.host), "QLA_MI355X_finalize");
     }
     if (have_cur) (void)hipSetDevice(cur);
     g_scratch.clear();
@@ -1024,7 +1052,7 @@ void QLA_MI355X_DF3_r_veq_norm2_D_noalloc(QLA_D_Real *r, QLA_F3_DiracFermion *a,
         record(hipErrorInvalidValue, name);
         return;
     }
-    reduce_with(sc, [&](void *out, double *s) {
+    reduce_with(sc, nullptr, [&](void *out, double *s) {
         return qla_launch_reduce2(0, 0, a, nullptr, m, out, 0, s, ts.stream);
     }, r, sizeof(double), name);
 }

@@ -125,6 +125,44 @@ def test_reduce_empty():
     assert run_host_result("QLA_F3_c_veq_D_dot_D", C.c_float, a, a, 0) == [0.0, 0.0]
 
 
+def test_result_memory_kinds(orc):
+    """The result pointer may be pageable host memory, page-locked host
+    memory the caller allocated, or device memory; a host result comes back
+    through the library's own page-locked slot.  All three give the same
+    bits, for the reductions and for a global sum (up to 288 doubles), on
+    consecutive calls that reuse the slot."""
+    import torch
+    n = 3000
+    a = lattice.random_field("D", n, 90).astype(np.float32)     # an F3 field
+    da = dev(a)
+    pinned = torch.zeros(288, dtype=torch.float64, pin_memory=True)
+    for rep in range(2):
+        say(f"norm2 / dot / sum result kinds, call {rep}")
+        ref = run_host_result("QLA_DF3_r_veq_norm2_D", C.c_double, da, n)[0]
+        qla_amd.call("QLA_DF3_r_veq_norm2_D", pinned.data_ptr(), da, n)
+        qla_amd.check()
+        assert pinned[0].item() == ref
+        assert run_device_result("QLA_DF3_r_veq_norm2_D", torch.float64, da, n)[0] == ref
+        dref = run_host_result("QLA_DF3_c_veq_D_dot_D", C.c_double, da, da, n)
+        qla_amd.call("QLA_DF3_c_veq_D_dot_D", pinned.data_ptr(), da, da, n)
+        qla_amd.check()
+        assert pinned[:2].tolist() == dref
+        out = (C.c_double * 24)()
+        qla_amd.call("QLA_DF3_d_veq_sum_D", C.cast(out, C.c_void_p).value, da, n)
+        qla_amd.check()
+        qla_amd.call("QLA_DF3_d_veq_sum_D", pinned.data_ptr(), da, n)
+        qla_amd.check()
+        assert pinned[:24].tolist() == list(out)
+        dsum = torch.zeros(24, dtype=torch.float64, device="cuda")
+        qla_amd.call("QLA_DF3_d_veq_sum_D", dsum, da, n)
+        qla_amd.check()
+        assert host(dsum).tolist() == list(out)
+        exp = a.astype(np.float64).reshape(n, 24).sum(axis=0)
+        assert np.allclose(list(out), exp, rtol=0, atol=1e-12 * np.abs(a).sum())
+        da.mul_(0.5)
+        a = a * np.float32(0.5)
+
+
 def test_misaligned_field(orc):
     """A float field starting 8 bytes past a 16-byte boundary (legal: float
     complex is 8-byte aligned) takes the per-complex path and must agree."""
