@@ -11,6 +11,7 @@
 // logical positions).
 #pragma once
 #include "qla_dev.hpp"
+#include "qla_hostmath.hpp"
 #include <float.h>
 
 namespace qla {
@@ -233,22 +234,27 @@ template <class T, int NC> __device__ void f_invsqrt(mat<T, NC> &r, const mat<T,
 }
 
 // ---------------------------------------------------------------------------
-// Scalar helpers of the closed forms.  The transcendental functions are the
-// device library's (ocml) versions of the C99 functions the templates call
+// Scalar helpers of the closed forms: the C99 functions the templates call
 // (sqrtP, fabsP, acosP, cbrtP and the sincos of QLA_*_cexpi,
-// lib/cmath/QLA_cexpi.c:12-31); sqrt is IEEE-rounded, the others are
-// faithful to an ulp or two, so results that pass through them agree with a
-// CPU evaluation to rounding rather than bit for bit.
+// lib/cmath/QLA_cexpi.c:12-31).  sqrt is IEEE-rounded on both sides.  In
+// single precision acos, cbrt and sincos are the host C library's own
+// algorithms restated (qla_hostmath.hpp), so the F3 closed forms return the
+// reference's bits; sincos of |x| >= 120 and the double-precision functions
+// are the device library's (ocml), faithful to an ulp or two, so D3 results
+// that pass through them agree with a CPU evaluation to rounding.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float m_fabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double m_fabs(double x) { return fabs(x); }
-__device__ __forceinline__ float m_acos(float x) { return acosf(x); }
+__device__ __forceinline__ float m_acos(float x) { return hm_acosf(x); }
 __device__ __forceinline__ double m_acos(double x) { return acos(x); }
-__device__ __forceinline__ float m_cbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ float m_cbrt(float x) { return hm_cbrtf(x); }
 __device__ __forceinline__ double m_cbrt(double x) { return cbrt(x); }
-__device__ __forceinline__ void m_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void m_sincos(float x, float *s, float *c)
+{
+    if (!hm_sincosf(x, s, c)) sincosf(x, s, c);
+}
 __device__ __forceinline__ void m_sincos(double x, double *s, double *c) { sincos(x, s, c); }
 // QLA_*_cexpi(t) = (cos t, sin t); the argument is rounded to T first
 template <class T> __device__ __forceinline__ cpx<T> cexpi(T t)

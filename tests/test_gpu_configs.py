@@ -274,3 +274,41 @@ def test_c4_fn_full(orc, fn, nc):
     e = rel_err(got, orc.matfn(fn, a.astype(np.float64)))
     print(f"c4 FN{nc} {fn}: rel err vs FP64 {e:.3e}")
     assert e <= C4_TOL[fn]
+
+
+C4_CLOSED = {
+    # fn: (input, symbol suffix)
+    "expA": ("antiherm_phase", "M_veq_expA_M"),
+    "expTA": ("antiherm", "M_veq_expTA_M"),
+    "sqrtPH": ("hpd", "M_veq_sqrtPH_M"),
+    "invsqrtPH": ("hpd", "M_veq_invsqrtPH_M"),
+    "eigenvalsH": ("herm", "V_veq_eigenvalsH_M"),
+}
+
+
+@pytest.mark.parametrize("fn", sorted(C4_CLOSED))
+def test_c4_closed_forms_full(orc, fn):
+    """The Nc = 3 closed forms on 2,097,152 F3 matrices, bit-exact against
+    the oracle: exp of i*H (spectra in [-4, 4]; expA with a phase on the
+    diagonal), the square roots of B^dagger B + eps, the eigenvalues of H.
+    Their acosf / cbrtf / sincosf are the host library's algorithms on the
+    device, so every branch and every rounding is the reference's."""
+    n = 2097152
+    kind, sym = C4_CLOSED[fn]
+    if kind == "hpd":
+        a64 = lattice.random_hpd(n, 3, 51)
+    else:
+        a64 = lattice.random_antihermitian(n, 3, 52)
+        if kind == "antiherm_phase":
+            ph = np.random.default_rng(53).uniform(-2.0, 2.0, n)
+            a64[:, [0, 1, 2], [0, 1, 2], 1] += ph[:, None]
+        elif kind == "herm":
+            # H = -i * (i H): (re, im) of i*H -> (im, -re)
+            a64 = np.stack([a64[..., 1], -a64[..., 0]], axis=-1)
+    a = np.ascontiguousarray(a64.astype(np.float32))
+    out_shape = (n, 3, 2) if fn == "eigenvalsH" else a.shape
+    r = dev(np.zeros(out_shape, np.float32))
+    say(f"c4 QLA_F3_{sym} n={n}")
+    qla_amd.call(f"QLA_F3_{sym}", r, dev(a), n)
+    qla_amd.check()
+    assert_bitwise(host(r), orc.matfunc(fn, a), f"c4 F3 {fn}")
