@@ -205,15 +205,18 @@ void *QLA_MI355X_get_stream(void);
 /* Per host thread: the association order of the floating-point sums.
  * QLA_MI355X_ORDER_GENERIC (default): every symbol rounds exactly like the
  * reference's generated C (a default build), bit for bit.
- * QLA_MI355X_ORDER_SSE3: the five QLA_F3 symbols that a reference built
- * --enable-sse3 replaces with re-associated SSE code -- V_vpeq_M_times_pV,
- * V_veq_M_times_pV, V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and
- * D_vpeq_spproj_M_times_pD (lib/optimized/sse/src/) -- round exactly like that
- * build instead, including its choice of order by the alignment of r, a and
- * b[0] and, for the V symbols, by the position of a site in its 1- and
- * 2-site loops; every other symbol, and the batches, keep the generic order.
- * Both orders are within 3 ulp (of the sum of the term magnitudes) of the
- * exact result. */
+ * QLA_MI355X_ORDER_SSE3: the symbols that a reference built --enable-sse3
+ * replaces with re-associated SSE code round exactly like that build
+ * instead: the five QLA_F3 ones -- V_vpeq_M_times_pV, V_veq_M_times_pV,
+ * V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and D_vpeq_spproj_M_times_pD
+ * (lib/optimized/sse/src/) -- including that build's choice of order by the
+ * alignment of r, a and b[0] and, for the V symbols, by the position of a
+ * site in its 1- and 2-site loops; and the four QLA_D3 ones that
+ * --enable-sse3 takes from its SSE2 versions -- V_veq_M_times_V,
+ * V_veq_M_times_pV, V_vpeq_M_times_V and V_vpeq_M_times_pV
+ * (lib/optimized/sse2/src/).  Every other symbol, and the batches, keep the
+ * generic order.  Both orders are within 3 ulp (of the sum of the term
+ * magnitudes) of the exact result. */
 enum { QLA_MI355X_ORDER_GENERIC = 0, QLA_MI355X_ORDER_SSE3 = 1 };
 void QLA_MI355X_set_arith_order(int order);
 int QLA_MI355X_get_arith_order(void);

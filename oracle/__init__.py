@@ -44,6 +44,7 @@ def load():
         getattr(lib, f"orc_{pre}_site").argtypes = [i32, i32, i32, vp, vp, vp, vp, i32, i32, vp, i32]
         getattr(lib, f"orc_{pre}_multi").argtypes = [i32, i32, vp, vp, vp, vp, i32, vp, i32]
     lib.orc_F3_sse_site.argtypes = [i32, i32, i32, vp, vp, vp, i32, i32, i32, i32, i32, i32]
+    lib.orc_D3_sse2_site.argtypes = [i32, vp, vp, vp, i32]
     for p in ("F", "D"):
         f = getattr(lib, f"orc_{p}_norm2")
         f.argtypes, f.restype = [vp, i32, i32], C.c_double
@@ -222,6 +223,27 @@ def sse_site(symbol, r, a, b, b_idx=None, mu=0, sign=1, r_al=-1, a_al=-1, b0_al=
     pb = _records(b, b_idx)
     n = r.shape[0] if b_idx is None else len(b_idx)
     lib.orc_F3_sse_site(kind, eqop, adj, _ptr(_need(r)), _ptr(pa), _ptr(pb), mu, sign, n, r_al, a_al, b0_al)
+    return r
+
+
+# the D3 symbols an --enable-sse3 build takes from its SSE2 versions, with
+# their eqop (qla_oracle_sse.inc)
+SSE2_D3_SYMBOLS = {"V_veq_M_times_V": EQ, "V_veq_M_times_pV": EQ, "V_vpeq_M_times_V": PEQ,
+                   "V_vpeq_M_times_pV": PEQ}
+
+
+def sse2_d3_site(symbol, r, a, b, b_idx=None):
+    """QLA_D3_<symbol> (a key of SSE2_D3_SYMBOLS) in the operation order of
+    the SSE2 version an --enable-sse3 build links, in place on r.  b is the
+    field whose records b_idx[i] the pointer array names (field operand:
+    b_idx None)."""
+    eqop = SSE2_D3_SYMBOLS[symbol]
+    assert r.dtype == np.float64
+    lib = load()
+    pa = _records(a, None)
+    pb = _records(b, b_idx)
+    n = r.shape[0] if b_idx is None else len(b_idx)
+    lib.orc_D3_sse2_site(eqop, _ptr(_need(r)), _ptr(pa), _ptr(pb), n)
     return r
 
 

@@ -52,6 +52,9 @@ void orc_F3_site_vf(int kind, int eqop, int adj, float *r, const float *af, cons
                     const int *index, int n);
 void orc_D3_site_vf(int kind, int eqop, int adj, double *r, const double *af, const double *const *pb, int mu,
                     int sign, const int *index, int n);
+/* QLA_D3_V_{veq,vpeq}_M_times_{V,pV} in the order of the SSE2 versions a
+ * QLA built --enable-sse3 links (eqop ORC_EQ or ORC_PEQ) -- qla_oracle_sse.inc */
+void orc_D3_sse2_site(int eqop, double *r, const double *const *pa, const double *const *pb, int n);
 void orc_F3_sse_site_vf(int kind, int eqop, int adj, float *r, const float *af, const float *const *pb, int mu,
                         int sign, int n, int r_al, int a_al, int b0_al);
 /* the CPU baseline's loops (mu / sign constant per loop): v-form

@@ -711,10 +711,14 @@ void site_entry(int prec, int kind, int eqop, int adj, int pa, int pb, void *r, 
     // the SSE3-build order exists for the five F3 v-form symbols that build
     // overrides with a different association (QLA_F3_V_vpeq_M_times_pV,
     // V_veq_M_times_pV, V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH,
-    // D_vpeq_spproj_M_times_pD); every other symbol keeps the generic order
+    // D_vpeq_spproj_M_times_pD)
     const bool v_sym = kind == 0 && !pa && ((!adj && pb && (eqop == 0 || eqop == 1)) || (adj && !pb && eqop == 0));
     const bool spin_sym = (kind == 2 || kind == 3) && !adj && !pa && pb && eqop == 1;
-    p.order = (ts.order == QLA_MI355X_ORDER_SSE3 && prec == 0 && !index && !ir && !ia && !ib && (v_sym || spin_sym))
+    // and the four D3 matvecs it takes from its SSE2 versions (QLA_D3_V_veq_M_times_V,
+    // V_veq_M_times_pV, V_vpeq_M_times_V, V_vpeq_M_times_pV)
+    const bool d3_sym = kind == 0 && !adj && !pa && (eqop == 0 || eqop == 1);
+    p.order = (ts.order == QLA_MI355X_ORDER_SSE3 && !index && !ir && !ia && !ib &&
+               ((prec == 0 && (v_sym || spin_sym)) || (prec == 1 && d3_sym)))
                   ? 1 : 0;
     p.pa = pa;
     p.pb = pb;

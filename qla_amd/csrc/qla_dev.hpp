@@ -477,6 +477,43 @@ __device__ __forceinline__ cpx<T> sse_adj_row2(const cpx<T> *a, const cpx<T> *x)
 // ---------------------------------------------------------------------------
 // Spin projection (expressions_gamma.pl:244-276) for one colour row d[0..3].
 // ---------------------------------------------------------------------------
+// QLA_D3_V_{eq,peq}_M_times_V of the SSE2 versions an --enable-sse3 build
+// links (lib/optimized/sse2/include/nas/QLA_D3_V_peq_M_times_V.s): per row
+// the (re, im) pair starts from Re a[ic][k] * x_k for the first column of
+// the row's "re" order, adds the other two, then adds Im a[ic][k] * i x_k
+// = (-Im a x.im, Im a x.re) in the row's "im" order, and (peq) r last.
+//   re order: rows 0, 1, 2: k = 0 1 2 | 0 2 1 | 0 1 2
+//   im order: rows 0, 1, 2: k = 0 1 2 | 1 0 2 | 2 0 1
+template <int EQOP>
+__device__ __forceinline__ void sse2_d3_matvec(cpx<double> *r, const cpx<double> *a, const cpx<double> *x)
+{
+    constexpr int RE[3][3] = { { 0, 1, 2 }, { 0, 2, 1 }, { 0, 1, 2 } };
+    constexpr int IM[3][3] = { { 0, 1, 2 }, { 1, 0, 2 }, { 2, 0, 1 } };
+#pragma unroll
+    for (int ic = 0; ic < 3; ic++) {
+        const cpx<double> *row = a + 3 * ic;
+        double tr = row[RE[ic][0]].re * x[RE[ic][0]].re, ti = row[RE[ic][0]].re * x[RE[ic][0]].im;
+#pragma unroll
+        for (int j = 1; j < 3; j++) {
+            const int k = RE[ic][j];
+            tr = tr + row[k].re * x[k].re;
+            ti = ti + row[k].re * x[k].im;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int k = IM[ic][j];
+            tr = tr + row[k].im * (-x[k].im);
+            ti = ti + row[k].im * x[k].re;
+        }
+        if (EQOP == PEQ) {
+            tr = tr + r[ic].re;
+            ti = ti + r[ic].im;
+        }
+        r[ic].re = tr;
+        r[ic].im = ti;
+    }
+}
+
 template <class T> __device__ __forceinline__ cpx<T> c_plus(cpx<T> a, cpx<T> b)  { return { a.re + b.re, a.im + b.im }; }
 template <class T> __device__ __forceinline__ cpx<T> c_minus(cpx<T> a, cpx<T> b) { return { a.re - b.re, a.im - b.im }; }
 // QLA_c_eq_c_plus_ic: (a.re - b.im, a.im + b.re)

@@ -205,10 +205,13 @@ __device__ __forceinline__ lane_addr addresses_of(const site_args &p, const lane
 // fallback's.
 // ---------------------------------------------------------------------------
 template <class T, int KIND, int EQOP, bool ADJ, bool XF> struct sse_order {
-    static constexpr bool v = std::is_same<T, float>::value && !XF &&
-                              ((KIND == K_MATVEC && !ADJ && (EQOP == PEQ || EQOP == EQ)) ||
-                               (KIND == K_MATVEC && ADJ && EQOP == EQ) ||
-                               ((KIND == K_SPRECON || KIND == K_WILSON) && !ADJ && EQOP == PEQ));
+    static constexpr bool v = !XF && ((std::is_same<T, float>::value &&
+                                       ((KIND == K_MATVEC && !ADJ && (EQOP == PEQ || EQOP == EQ)) ||
+                                        (KIND == K_MATVEC && ADJ && EQOP == EQ) ||
+                                        ((KIND == K_SPRECON || KIND == K_WILSON) && !ADJ && EQOP == PEQ))) ||
+                                      // the D3 matvecs of the SSE2 versions (field or pointer source)
+                                      (std::is_same<T, double>::value && KIND == K_MATVEC && !ADJ &&
+                                       (EQOP == PEQ || EQOP == EQ)));
 };
 
 __device__ __forceinline__ bool sse_two_site(const site_args &p, int i)
@@ -326,7 +329,8 @@ template <class T, int KIND, int EQOP, bool ADJ, bool XF, int MS = -1, bool SSEO
         } else if constexpr (KIND == K_MATVEC) {
             cpx<T> v[3];
             lds_read(v, lb);
-            if constexpr (SSEO) sse_matvec<EQOP, ADJ>(o, m, v, sse_two_site(p, base + lane));
+            if constexpr (SSEO && std::is_same<T, double>::value) sse2_d3_matvec<EQOP>(o, m, v);
+            else if constexpr (SSEO) sse_matvec<EQOP, ADJ>(o, m, v, sse_two_site(p, base + lane));
             else mat_times_cols<EQOP, ADJ, 1>(o, m, v);
         } else if constexpr (KIND == K_MAT_H || KIND == K_MAT_D) {
             constexpr int NS = KIND == K_MAT_H ? 2 : 4;

@@ -10,6 +10,11 @@ V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and D_vpeq_spproj_M_times_pD.
   b[0] 8 bytes off (scalar fallback, 1-site V head).  The oracle is told the
   device call's address offsets, since the reference picks its order from
   them.
+* The four QLA_D3 matvecs that --enable-sse3 takes from its SSE2 versions
+  (lib/optimized/sse2/src: V_veq_M_times_V, V_veq_M_times_pV,
+  V_vpeq_M_times_V, V_vpeq_M_times_pV) likewise in SSE3 mode, bit-exact
+  against the oracle's restatement on the 32^4 lattice, field and pointer
+  sources; their default order within 4 ulp (double) of it.
 * Default order: bit-exact with the generic oracle (the other tests), and
   against the SSE build within the north star's 4 ulp, measured on the
   cancellation-aware scale |r| + sum_k |a_k| |x_k| of each component.  (Each
@@ -144,3 +149,57 @@ def test_generic_order_close_to_sse3_build(sym):
         worst = max(worst, float(d.max()))
         assert d.max() <= 4.0, (sym, mu, sign, float(d.max()))
     print(f"{sym}: generic vs SSE3 build, worst {worst:.2f} ulp of the scale")
+
+
+D3_SYMS = list(oracle.SSE2_D3_SYMBOLS)
+_d3 = {}
+
+
+def d3_data():
+    if not _d3:
+        n = lattice.volume(DIMS)
+        fw, _ = lattice.neighbours(DIMS)
+        _d3.update(n=n, idx=fw[0], u=lattice.random_su3(n, 11), v=lattice.random_field("V", n, 12),
+                   r=lattice.random_field("V", n, 13))
+        _d3["du"], _d3["dv"] = dev(_d3["u"]), dev(_d3["v"])
+    return _d3
+
+
+def run_d3(sym, order):
+    D = d3_data()
+    n = D["n"]
+    r = dev(D["r"])
+    pointer = sym.endswith("pV")
+    b_arg = qla_amd.pointer_array(D["dv"], D["idx"]) if pointer else D["dv"]
+    lib = qla_amd.load()
+    lib.QLA_MI355X_set_arith_order(order)
+    try:
+        say(f"QLA_D3_{sym} 32^4 order={order}")
+        qla_amd.call("QLA_D3_" + sym, r, D["du"], b_arg, n)
+        qla_amd.check()
+    finally:
+        lib.QLA_MI355X_set_arith_order(ORDER_GENERIC)
+    exp = oracle.sse2_d3_site(sym, D["r"].copy(), D["u"], D["v"], D["idx"] if pointer else None)
+    return host(r), exp
+
+
+@pytest.mark.parametrize("sym", D3_SYMS)
+def test_d3_sse3_order_bit_exact(sym):
+    got, exp = run_d3(sym, ORDER_SSE3)
+    assert_bitwise(got, exp, f"QLA_D3_{sym} SSE3 order")
+
+
+@pytest.mark.parametrize("sym", D3_SYMS)
+def test_d3_generic_order_close_to_sse3_build(sym):
+    D = d3_data()
+    got, exp = run_d3(sym, ORDER_GENERIC)
+    src = D["v"][D["idx"]] if sym.endswith("pV") else D["v"]
+    am = np.hypot(D["u"][..., 0], D["u"][..., 1])
+    xm = np.hypot(src[..., 0], src[..., 1])
+    scale = np.einsum("nik,nk->ni", am, xm)[..., None]
+    if oracle.SSE2_D3_SYMBOLS[sym] == oracle.PEQ:
+        scale = scale + np.abs(D["r"])
+    d = np.abs(got - exp) / np.spacing(scale)
+    assert d.max() <= 4.0, (sym, float(d.max()))
+    assert np.count_nonzero(got != exp) > 0
+    print(f"QLA_D3_{sym}: generic vs SSE3 build, worst {float(d.max()):.2f} ulp of the scale")

@@ -266,3 +266,113 @@ def test_bench_loops_equal_oracle(orc):
                              r_al=view.ctypes.data % 16, a_al=u.ctypes.data % 16, b0_al=h[idx[0]].ctypes.data % 16)
                 lib.orc_F3_bench_sprecon_sse(view.ctypes.data, u.ctypes.data, ph.ctypes.data, mu, sign, n)
                 np.testing.assert_array_equal(view, exp)
+
+
+# ---------------------------------------------------------------------------
+# The D3 matvecs an --enable-sse3 build takes from its SSE2 versions
+# (qla_oracle_sse.inc, orc_D3_sse2_site).
+# ---------------------------------------------------------------------------
+P53 = float(2 ** 53)
+D3_SYMS = ["V_veq_M_times_V", "V_veq_M_times_pV", "V_vpeq_M_times_V", "V_vpeq_M_times_pV"]
+
+
+@pytest.mark.parametrize("sym", D3_SYMS)
+def test_d3_small_integers_equal_generic(orc, sym):
+    """Small integers keep every product and partial sum exact in double, so
+    the SSE2 order must equal the generic oracle bit for bit."""
+    rng = np.random.default_rng(21)
+    n = 301
+    a = rng.integers(-8, 9, size=(n, 3, 3, 2)).astype(np.float64)
+    v = rng.integers(-8, 9, size=(n, 3, 2)).astype(np.float64)
+    r0 = rng.integers(-8, 9, size=(n, 3, 2)).astype(np.float64)
+    eqop = orc.SSE2_D3_SYMBOLS[sym]
+    idx = rng.permutation(n) if sym.endswith("pV") else None
+    exp = r0.copy()
+    orc.V_eqop_M_times_V(eqop, exp, a, v, b_idx=idx, n=n)
+    got = orc.sse2_d3_site(sym, r0.copy(), a, v, idx)
+    np.testing.assert_array_equal(got, exp)
+
+
+def _d3_order_sites():
+    """Sites whose results tell the orders apart (double: 2^53 + 1 -> 2^53).
+    Each entry: (a, b, r0, row, lane, SSE2 value, generic value); b = 1 + i
+    or 1, hand-evaluated:
+      A  row 0, Re a = (2^53, 1, 1), Im a = (2^53, 0, 0), b = 1+i, re lane:
+         SSE2 ((2^53 + 1) + 1) - 2^53 - 0 - 0 = 0;   generic sum_k (Re - Im) = 2
+      B  row 1, Re a = (2^53, 1, -2^53), b = 1+i, re lane: SSE2 columns 0, 2, 1:
+         (2^53 - 2^53) + 1 = 1;   generic (2^53 + 1) - 2^53 = 0
+      C  row 1, Re a = (2^53, 0, 0), Im a = (1, -2^53, 0), b = 1+i, im lane:
+         SSE2 Re a * Im b first: 2^53, then Im columns 1, 0, 2:
+         (2^53 - 2^53) + 1 + 0 = 1;   generic (2^53 + 1) + (0 - 2^53) = 0
+      D  row 2, Re a = (2^53, 0, 0), Im a = (1, 0, -2^53), b = 1+i, im lane:
+         SSE2 2^53, then Im columns 2, 0, 1: (2^53 - 2^53) + 1 + 0 = 1;  generic 0
+      E  vpeq, row 0, Re a = (2^53, -2^53, 0), b = 1, r = 1, re lane:
+         SSE2 (2^53 - 2^53 + 0) + r = 1 (r added last);  generic r first: 0
+    """
+    sites = []
+    one_i = np.array([[1.0, 1.0]] * 3)
+    one = np.array([[1.0, 0.0]] * 3)
+    a = np.zeros((3, 3, 2))
+    a[0, :, 0] = (P53, 1.0, 1.0)
+    a[0, 0, 1] = P53
+    sites.append((a, one_i, np.zeros((3, 2)), 0, 0, 0.0, 2.0, "EQ"))
+    a = np.zeros((3, 3, 2))
+    a[1, :, 0] = (P53, 1.0, -P53)
+    sites.append((a, one_i, np.zeros((3, 2)), 1, 0, 1.0, 0.0, "EQ"))
+    a = np.zeros((3, 3, 2))
+    a[1, :, 0] = (P53, 0.0, 0.0)
+    a[1, :, 1] = (1.0, -P53, 0.0)
+    sites.append((a, one_i, np.zeros((3, 2)), 1, 1, 1.0, 0.0, "EQ"))
+    a = np.zeros((3, 3, 2))
+    a[2, :, 0] = (P53, 0.0, 0.0)
+    a[2, :, 1] = (1.0, 0.0, -P53)
+    sites.append((a, one_i, np.zeros((3, 2)), 2, 1, 1.0, 0.0, "EQ"))
+    a = np.zeros((3, 3, 2))
+    a[0, :, 0] = (P53, -P53, 0.0)
+    r0 = np.zeros((3, 2))
+    r0[0, 0] = 1.0
+    sites.append((a, one, r0, 0, 0, 1.0, 0.0, "PEQ"))
+    return sites
+
+
+def test_d3_known_answers(orc):
+    for k, (a, b, r0, row, lane, sse2, gen, eq) in enumerate(_d3_order_sites()):
+        sym = "V_veq_M_times_V" if eq == "EQ" else "V_vpeq_M_times_V"
+        eqop = orc.EQ if eq == "EQ" else orc.PEQ
+        got = orc.sse2_d3_site(sym, r0[None].copy(), a[None], b[None])
+        assert got[0, row, lane] == sse2, (k, got[0, row, lane], sse2)
+        exp = r0[None].copy()
+        orc.V_eqop_M_times_V(eqop, exp, a[None], b[None])
+        assert exp[0, row, lane] == gen, (k, exp[0, row, lane], gen)
+
+
+@pytest.mark.parametrize("sym", D3_SYMS)
+def test_d3_within_4ulp_of_exact(orc, sym):
+    """Every component within 4 ulp (double) of the long-double result on
+    the scale |r| + sum_k |a_k||x_k|, as is the generic order; the two
+    orders do differ on some components."""
+    n = 4096
+    rng = np.random.default_rng(33)
+    u = lattice.random_su3(n, 5)
+    v = lattice.random_field("V", n, 6)
+    r0 = lattice.random_field("V", n, 7)
+    eqop = orc.SSE2_D3_SYMBOLS[sym]
+    idx = rng.permutation(n) if sym.endswith("pV") else None
+    got = orc.sse2_d3_site(sym, r0.copy(), u, v, idx)
+    gen = r0.copy()
+    orc.V_eqop_M_times_V(eqop, gen, u, v, b_idx=idx, n=n)
+    src = v if idx is None else v[idx]
+    A = u[..., 0].astype(np.longdouble) + 1j * u[..., 1].astype(np.longdouble)
+    X = src[..., 0].astype(np.longdouble) + 1j * src[..., 1].astype(np.longdouble)
+    prod = np.einsum("nij,nj->ni", A, X)
+    mag = np.einsum("nij,nj->ni", np.abs(A), np.abs(X))
+    if eqop == orc.PEQ:
+        prod = prod + (r0[..., 0] + 1j * r0[..., 1])
+        mag = mag + np.abs(r0[..., 0] + 1j * r0[..., 1])
+    ex = np.stack([prod.real, prod.imag], axis=-1)
+    scale = np.stack([mag, mag], axis=-1).astype(np.float64)
+    ulp = np.spacing(scale)
+    for res in (got, gen):
+        err = (np.abs(res.astype(np.longdouble) - ex) / ulp).astype(np.float64)
+        assert err.max() <= 4.0, (sym, float(err.max()))
+    assert np.count_nonzero(got != gen) > 0
