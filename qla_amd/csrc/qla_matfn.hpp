@@ -175,11 +175,13 @@ template <class T, int NC> __device__ void f_exp(mat<T, NC> &r, const mat<T, NC>
     m_peq_r_times_m(pb, T(2), qb);
     if (s == 1) { r = pb; return; }
     // binary powering: va = prod of the squares vb selected by the bits of s
+    // (M_eq_exp_M.c:354-365; the square formed after the top bit is never
+    // used, so it is skipped)
     mat<T, NC> va, tmp;
     m_eq_d(va, T(1));
     for (; s > 0; s >>= 1) {
         if (s & 1) { m_eq_m_times_m(tmp, va, pb); va = tmp; }
-        m_eq_m_times_m(tmp, pb, pb); pb = tmp;
+        if (s > 1) { m_eq_m_times_m(tmp, pb, pb); pb = tmp; }
     }
     r = va;
 }
