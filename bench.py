@@ -752,7 +752,11 @@ def matfn_config4(lib, stream, args):
     reference's own flop model (examples/qla_bench.c) and its fraction of the
     157.3 TFLOP/s FP32 peak.  Two input / output sets alternate between calls
     (Nc = 3: 2 x 302 MB, Nc = 8: 2 x 268 MB, more than the 256 MB Infinity
-    Cache; the Nc = 4 sets, 2 x 67 MB, stay cache-resident)."""
+    Cache; the Nc = 4 sets, 2 x 67 MB, stay cache-resident).  Every Nc = 3
+    result carries its error against the QLA_D3 symbol on the same inputs,
+    and the Cayley-Hamilton forms with an FP64 coefficient step
+    (QLA_MI355X_F3_M_veq_{expA,sqrtPH,invsqrtPH}_M_ch) run on the same inputs
+    beside exp, sqrt and invsqrt."""
     import torch
     from qla_amd import lattice
     out = {}
@@ -760,14 +764,20 @@ def matfn_config4(lib, stream, args):
     gens = {"exp": (lattice.random_antihermitian, 41), "sqrt": (lattice.random_hpd, 42),
             "invsqrt": (lattice.random_hpd, 42)}
 
-    def measure(fn, nc, nn, reps):
+    def errors(x, ref):
+        """max and mean over the matrices of ||x - ref||_F / ||ref||_F"""
+        d = torch.linalg.vector_norm((x.double() - ref).reshape(ref.shape[0], -1), dim=1)
+        e = d / torch.linalg.vector_norm(ref.reshape(ref.shape[0], -1), dim=1)
+        return {"max": float(e.max()), "mean": float(e.mean())}
+
+    def measure(fn, nc, nn, reps, sym=None):
         gen, seed = gens[fn]
         sets = []
         for k in range(2):
             a = torch.from_numpy(gen(nn, nc, seed + 100 * k + (0 if nc == 3 else 2)).astype(np.float32)).cuda()
             sets.append((a, torch.empty_like(a)))
         if nc == 3:
-            f = getattr(lib, f"QLA_F3_M_veq_{fn}_M")
+            f = getattr(lib, sym or f"QLA_F3_M_veq_{fn}_M")
             call = lambda it: f(sets[it % 2][1].data_ptr(), sets[it % 2][0].data_ptr(), nn)      # noqa: E731
         else:
             f = getattr(lib, f"QLA_FN_M_veq_{fn}_M")
@@ -775,14 +785,33 @@ def matfn_config4(lib, stream, args):
         ms = _timed(stream, call, reps, 2) / reps
         bpm = 2 * nc * nc * 8
         gbs = bpm * nn / (ms * 1e-3) / 1e9
-        tfl = _ref_flops(fn, nc) * nn / (ms * 1e-3) / 1e12
+        res = {"n": nn, "ms": round(ms, 4), "matrices_per_s": round(nn / (ms * 1e-3), 1), "GBs": round(gbs, 1),
+               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if sym is None:
+            tfl = _ref_flops(fn, nc) * nn / (ms * 1e-3) / 1e12
+            res.update({"ref_model_flops_per_matrix": _ref_flops(fn, nc), "ref_model_TFLOPs": round(tfl, 2),
+                        "valu_frac": round(tfl / FP32_PEAK_TFLOPS, 4)})
+        if nc == 3:
+            # the FP32 result of set 1 against the QLA_D3 symbol on the same
+            # inputs held in double
+            torch.cuda.synchronize()
+            a64 = sets[1][0].double()
+            r64 = torch.empty_like(a64)
+            getattr(lib, f"QLA_D3_M_veq_{fn}_M")(r64.data_ptr(), a64.data_ptr(), nn)
+            torch.cuda.synchronize()
+            res["rel_err_vs_fp64"] = errors(sets[1][1], r64)
+            del a64, r64
         del sets
-        return {"n": nn, "ms": round(ms, 4), "matrices_per_s": round(nn / (ms * 1e-3), 1), "GBs": round(gbs, 1),
-                "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "ref_model_flops_per_matrix": _ref_flops(fn, nc),
-                "ref_model_TFLOPs": round(tfl, 2), "valu_frac": round(tfl / FP32_PEAK_TFLOPS, 4)}
+        return res
 
     for fn in ("exp", "sqrt", "invsqrt"):
         out[f"F3_{fn}"] = measure(fn, 3, n3, 10)
+    # the Cayley-Hamilton forms with an FP64 coefficient step on the same
+    # inputs (exp of the traceless anti-Hermitian i H, square roots of B+B + eps)
+    for fn, ch in (("exp", "expA"), ("sqrt", "sqrtPH"), ("invsqrt", "invsqrtPH")):
+        r = measure(fn, 3, n3, 10, sym=f"QLA_MI355X_F3_M_veq_{ch}_M_ch")
+        r["speedup_vs_F3_" + fn] = round(out[f"F3_{fn}"]["ms"] / r["ms"], 3)
+        out[f"F3_{ch}_ch"] = r
     for nc in (4, 8):
         for fn in ("exp", "sqrt", "invsqrt"):
             out[f"FN{nc}_{fn}"] = measure(fn, nc, 262144, 6)

@@ -132,6 +132,30 @@ void QLA_MI355X_DF3_r_veq_norm2_Dh(QLA_D_Real *r, void *a, int n);
 void QLA_MI355X_DF3_c_veq_Dh_dot_Dh(QLA_D_Complex *r, void *a, void *b, int n);
 
 /* ------------------------------------------------------------------------ */
+/* Cayley-Hamilton matrix functions with an FP64 coefficient step (no        */
+/* counterpart in the reference; QLA_F3_M_veq_{expA,sqrtPH,invsqrtPH}_M are  */
+/* the closest, and evaluate their eigenvalues in single precision).  F3    */
+/* records in and out: f(X) = f0 + f1 X + f2 X^2, with X's invariants, its  */
+/* eigenvalues, the coefficients, X^2 and the sum in double and each result */
+/* component rounded to float once.                                          */
+/*   expA:      exp of the anti-Hermitian part of a (a itself when a is      */
+/*              anti-Hermitian, as for QLA's expA); exp(0) = 1.               */
+/*   sqrtPH:    square root of the Hermitian part of a, which must be        */
+/*              positive semi-definite (eigenvalues below 0 count as 0).      */
+/*   invsqrtPH: its inverse; a zero eigenvalue gives +inf on the diagonal.   */
+/* Results are not bit-identical to the QLA symbols: they agree with the     */
+/* exact function to 1e-7 relative to the result's norm (one float rounding */
+/* per component).  The x-forms act on the records index[0..n) as QLA's     */
+/* x-forms do.                                                               */
+/* ------------------------------------------------------------------------ */
+void QLA_MI355X_F3_M_veq_expA_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n);
+void QLA_MI355X_F3_M_xeq_expA_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n);
+void QLA_MI355X_F3_M_veq_sqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n);
+void QLA_MI355X_F3_M_xeq_sqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n);
+void QLA_MI355X_F3_M_veq_invsqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n);
+void QLA_MI355X_F3_M_xeq_invsqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n);
+
+/* ------------------------------------------------------------------------ */
 /* Allocation-free reduction (no counterpart in the reference).  For callers */
 /* that cannot allocate device memory: code under HIP graph capture, tight   */
 /* serving loops, harnesses that forbid hipMalloc.  Same result, bit for     */

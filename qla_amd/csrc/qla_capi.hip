@@ -1042,6 +1042,26 @@ void QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh(void *r, void *a, int fmt, void
     finish(qla_launch_sprecon_c(fmt, r, a, (const void *)b, mu, sign, n, ts.stream),
            "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh");
 }
+// --------------------------------------- Cayley-Hamilton forms, FP64 coefficients
+// function codes 13..15 of qla_launch_matfunc (F3, nc = 3 only)
+void QLA_MI355X_F3_M_veq_expA_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n)
+{ matfunc_entry(0, 13, 3, 0, 0, r, a, nullptr, nullptr, nullptr, nullptr, nullptr, n, "QLA_MI355X_F3_M_veq_expA_M_ch"); }
+void QLA_MI355X_F3_M_xeq_expA_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n)
+{ matfunc_entry(0, 13, 3, 0, 0, r, a, nullptr, index, nullptr, nullptr, nullptr, n, "QLA_MI355X_F3_M_xeq_expA_M_ch"); }
+void QLA_MI355X_F3_M_veq_sqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n)
+{ matfunc_entry(0, 14, 3, 0, 0, r, a, nullptr, nullptr, nullptr, nullptr, nullptr, n, "QLA_MI355X_F3_M_veq_sqrtPH_M_ch"); }
+void QLA_MI355X_F3_M_xeq_sqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n)
+{ matfunc_entry(0, 14, 3, 0, 0, r, a, nullptr, index, nullptr, nullptr, nullptr, n, "QLA_MI355X_F3_M_xeq_sqrtPH_M_ch"); }
+void QLA_MI355X_F3_M_veq_invsqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int n)
+{
+    matfunc_entry(0, 15, 3, 0, 0, r, a, nullptr, nullptr, nullptr, nullptr, nullptr, n,
+                  "QLA_MI355X_F3_M_veq_invsqrtPH_M_ch");
+}
+void QLA_MI355X_F3_M_xeq_invsqrtPH_M_ch(QLA_F3_ColorMatrix *r, QLA_F3_ColorMatrix *a, int *index, int n)
+{
+    matfunc_entry(0, 15, 3, 0, 0, r, a, nullptr, index, nullptr, nullptr, nullptr, n,
+                  "QLA_MI355X_F3_M_xeq_invsqrtPH_M_ch");
+}
 void QLA_MI355X_DF3_r_veq_norm2_D_noalloc(QLA_D_Real *r, QLA_F3_DiracFermion *a, int n)
 {
     static const char name[] = "QLA_MI355X_DF3_r_veq_norm2_D_noalloc";

@@ -19,7 +19,9 @@
 // wherever one retires.  (A global work counter claimed per wave costs one
 // same-address atomic per 64 matrices, which serialises at the L2 and capped
 // every function at about 4.4e9 matrices/s.)
+#include <type_traits>
 #include "qla_matfn.hpp"
+#include "qla_matfn_ch.hpp"
 #include "qla_matfn_rows.hpp"
 #include "qla_launch.hpp"
 
@@ -32,7 +34,10 @@ namespace qla {
 // function codes (qla_launch.hpp)
 enum {
     MF_EXP = 0, MF_SQRT, MF_INVSQRT, MF_DET, MF_INVERSE, MF_INV_V, MF_INV_M, MF_EXPA, MF_EXPTA, MF_SQRTPH,
-    MF_INVSQRTPH, MF_LOG, MF_EIGH
+    MF_INVSQRTPH, MF_LOG, MF_EIGH,
+    // Cayley-Hamilton forms with the coefficient step in double (F3 only;
+    // no QLA counterpart, qla_matfn_ch.hpp)
+    MF_EXPA_CH, MF_SQRTPH_CH, MF_INVSQRTPH_CH
 };
 
 // complexes per result record and per second-operand record
@@ -64,6 +69,9 @@ __device__ __forceinline__ void evaluate(cpx<T> *out, const mat<T, NC> &a, const
         else if constexpr (FN == MF_EXPTA) f_expA3<T, true>(o, a);
         else if constexpr (FN == MF_SQRTPH) f_sqrtPH3<T, false>(o, a);
         else if constexpr (FN == MF_INVSQRTPH) f_sqrtPH3<T, true>(o, a);
+        else if constexpr (FN == MF_EXPA_CH) f_expA_ch(o, a);
+        else if constexpr (FN == MF_SQRTPH_CH) f_sqrtPH_ch<false>(o, a);
+        else if constexpr (FN == MF_INVSQRTPH_CH) f_sqrtPH_ch<true>(o, a);
         else f_log(o, a);
         LOOP(k, NC * NC) out[k] = o.e[k];
     }
@@ -134,7 +142,8 @@ template <class T, int NC, int FN> struct stage_of {
     static constexpr int RB = S::B * (int)sizeof(cpx<T>);
     static constexpr int RO = S::OUT * (int)sizeof(cpx<T>);
     static constexpr bool UNIFORM = FN == MF_DET || FN == MF_INVERSE || FN == MF_EXPA || FN == MF_EXPTA ||
-                                    FN == MF_SQRTPH || FN == MF_INVSQRTPH || FN == MF_EIGH;
+                                    FN == MF_SQRTPH || FN == MF_INVSQRTPH || FN == MF_EIGH || FN == MF_EXPA_CH ||
+                                    FN == MF_SQRTPH_CH || FN == MF_INVSQRTPH_CH;
     static constexpr bool OK = RA <= MF_STAGE_MAX && UNIFORM;
 };
 
@@ -279,8 +288,17 @@ static hipError_t launch_nc(int fn, const qla_matfunc_args &p, hipStream_t st)
         case MF_SQRTPH: return launch_t<T, 3, MF_SQRTPH>(p, st);
         case MF_INVSQRTPH: return launch_t<T, 3, MF_INVSQRTPH>(p, st);
         case MF_EIGH: return launch_t<T, 3, MF_EIGH>(p, st);
-        default: return hipErrorInvalidValue;
+        default: break;
         }
+        if constexpr (std::is_same<T, float>::value) {
+            switch (fn) {
+            case MF_EXPA_CH: return launch_t<float, 3, MF_EXPA_CH>(p, st);
+            case MF_SQRTPH_CH: return launch_t<float, 3, MF_SQRTPH_CH>(p, st);
+            case MF_INVSQRTPH_CH: return launch_t<float, 3, MF_INVSQRTPH_CH>(p, st);
+            default: break;
+            }
+        }
+        return hipErrorInvalidValue;
     } else {
         // the templates' general branches (M_eq_expA_M.c:357-420 and
         // M_eq_sqrtPH_M.c:270-310 are M_eq_exp_M / M_eq_sqrt_M's)

@@ -39,6 +39,13 @@ _SIGS: dict[str, tuple[type, list]] = {
     "QLA_MI355X_F3_Dh_vpeq_sprecon_M12_times_pHh": (None, [vp, vp, i32, vp, i32, i32, i32]),
     "QLA_MI355X_DF3_r_veq_norm2_Dh": (None, [vp, vp, i32]),
     "QLA_MI355X_DF3_c_veq_Dh_dot_Dh": (None, [vp, vp, vp, i32]),
+    # Cayley-Hamilton matrix functions with FP64 coefficients
+    "QLA_MI355X_F3_M_veq_expA_M_ch": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_M_xeq_expA_M_ch": (None, [vp, vp, vp, i32]),
+    "QLA_MI355X_F3_M_veq_sqrtPH_M_ch": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_M_xeq_sqrtPH_M_ch": (None, [vp, vp, vp, i32]),
+    "QLA_MI355X_F3_M_veq_invsqrtPH_M_ch": (None, [vp, vp, i32]),
+    "QLA_MI355X_F3_M_xeq_invsqrtPH_M_ch": (None, [vp, vp, vp, i32]),
     # allocation-free reduction
     "QLA_MI355X_DF3_r_veq_norm2_D_noalloc": (None, [vp, vp, i32]),
     # ragged batches
