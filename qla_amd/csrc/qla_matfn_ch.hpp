@@ -66,6 +66,20 @@ __device__ __forceinline__ double herm_det3(double d0, double d1, double d2, zd 
     return d0 * d1 * d2 + 2.0 * acb - d0 * nc - d1 * nb - d2 * na;
 }
 
+// cos(theta / 3) for t = cos(theta) in [0, 1]: the root in [cos(pi/6), 1] of
+// 4 x^3 - 3 x = t, where the derivative 12 x^2 - 3 >= 6, from a
+// single-precision seed and two Newton steps in double (the seed's error
+// 1e-7 falls below double rounding in two quadratic steps)
+__device__ __forceinline__ double cos_third(double t)
+{
+    double x = (double)cosf(acosf((float)t) * (1.0f / 3.0f));
+    LOOP(k, 2) {
+        const double x2 = x * x;
+        x = x - fma(fma(4.0, x2, -3.0), x, -t) / fma(12.0, x2, -3.0);
+    }
+    return x;
+}
+
 // f0, f1, f2 of exp(iQ) = f0 + f1 Q + f2 Q^2, Q traceless Hermitian with
 // c0 = det Q and c1 = tr Q^2 / 2 >= 0
 __device__ __forceinline__ void ch_exp_coeffs(zd f[3], double c0, double c1)
@@ -80,8 +94,12 @@ __device__ __forceinline__ void ch_exp_coeffs(zd f[3], double c0, double c1)
     const double sc13 = sqrt(c13);
     const double c0max = 2.0 * c13 * sc13;
     const double t = fmin(fabs(c0) / c0max, 1.0);
-    double sth, cth;
-    sincos(acos(t) / 3.0, &sth, &cth);
+    // cos and sin of theta / 3; 1 - cth is exact (Sterbenz), and an error of
+    // sth near theta = 0, where the two eigenvalues -u +- w nearly meet,
+    // moves them symmetrically about their mean and changes exp only at
+    // second order
+    const double cth = cos_third(t);
+    const double sth = sqrt(fmax(0.0, (1.0 - cth) * (1.0 + cth)));
     const double u = sc13 * cth, w = sqrt(c1) * sth;
     const double u2 = u * u, w2 = w * w;
     double sw, cw, su, cu;
@@ -164,14 +182,51 @@ template <bool INV> __device__ __forceinline__ void ch_sqrt_coeffs(double f[3], 
     }
 }
 
-// e = c0 1 + c1 x + c2 x^2, evaluated in double and rounded to float
+// x^2 of a Hermitian x (real diagonal): the diagonal sum_k |x_ik|^2 and the
+// upper triangle; the lower triangle is its conjugate
+__device__ __forceinline__ void herm_square(zd x2[9], const zd x[9])
+{
+    LOOP(i, 3) {
+        double d = 0.0;
+        LOOP(k, 3) d = fma(x[i * 3 + k].re, x[i * 3 + k].re, fma(x[i * 3 + k].im, x[i * 3 + k].im, d));
+        x2[i * 4] = { d, 0.0 };
+    }
+    LOOP(i, 3) LOOP(j, 3) {
+        if (j > i) {
+            zd t = zmul(x[i * 3], x[j]);
+            LOOP(k, 2) t = zmadd(x[i * 3 + k + 1], x[(k + 1) * 3 + j], t);
+            x2[i * 3 + j] = t;
+            x2[j * 3 + i] = { t.re, -t.im };
+        }
+    }
+}
+
+// e = c0 1 + c1 x + c2 x^2 for a Hermitian x, evaluated in double and rounded
+// to float
 __device__ __forceinline__ void ch_combine(mat<float, 3> &e, const zd x[9], zd c0, zd c1, zd c2)
 {
+    zd x2[9];
+    herm_square(x2, x);
+    LOOP(k, 9) {
+        const zd r = zmadd(c2, x2[k], zmadd(c1, x[k], k % 4 == 0 ? c0 : zd{ 0.0, 0.0 }));
+        e.e[k] = { (float)r.re, (float)r.im };
+    }
+}
+
+// the same with real coefficients: the result is Hermitian, its diagonal real
+__device__ __forceinline__ void ch_combine_real(mat<float, 3> &e, const zd x[9], double c0, double c1, double c2)
+{
+    zd x2[9];
+    herm_square(x2, x);
     LOOP(i, 3) LOOP(j, 3) {
-        zd x2 = zmul(x[i * 3], x[j]);
-        LOOP(k, 2) x2 = zmadd(x[i * 3 + k + 1], x[(k + 1) * 3 + j], x2);
-        const zd r = zmadd(c2, x2, zmadd(c1, x[i * 3 + j], i == j ? c0 : zd{ 0.0, 0.0 }));
-        e.e[i * 3 + j] = { (float)r.re, (float)r.im };
+        if (j == i) {
+            e.e[i * 4] = { (float)fma(c2, x2[i * 4].re, fma(c1, x[i * 4].re, c0)), 0.0f };
+        } else if (j > i) {
+            const int k = i * 3 + j;
+            const zd r = { fma(c2, x2[k].re, c1 * x[k].re), fma(c2, x2[k].im, c1 * x[k].im) };
+            e.e[k] = { (float)r.re, (float)r.im };
+            e.e[j * 3 + i] = { (float)r.re, (float)-r.im };
+        }
     }
 }
 
@@ -210,7 +265,7 @@ template <bool INV> __device__ void f_sqrtPH_ch(mat<float, 3> &e, const mat<floa
     double lam[3], f[3];
     herm_eig3(lam, x[0].re, x[4].re, x[8].re, x[1], x[2], x[5]);
     ch_sqrt_coeffs<INV>(f, lam);
-    ch_combine(e, x, { f[0], 0.0 }, { f[1], 0.0 }, { f[2], 0.0 });
+    ch_combine_real(e, x, f[0], f[1], f[2]);
 }
 
 } // namespace qla

@@ -119,6 +119,19 @@ def test_special_inputs():
     assert np.abs(v[2] - rot(1 / np.sqrt([2.0, 3.0, 3.0]))).max() < 1e-5
 
 
+def test_nan_propagates():
+    """A NaN anywhere in the input gives NaN results, not a wrong finite matrix."""
+    for fn in SYM:
+        a = inputs(fn, 64, 3)
+        a[5, 1, 2, 0] = np.nan
+        say(f"ch {fn} NaN input")
+        got = run(fn, a)
+        assert np.isnan(got[5]).any(), fn
+        clean = np.ones(64, dtype=bool)
+        clean[5] = False
+        assert np.isfinite(got[clean]).all(), fn
+
+
 def test_n_zero_is_noop():
     a = inputs("expA", 8, 1)
     r0 = np.full_like(a, 3.0)
