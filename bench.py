@@ -189,7 +189,8 @@ TARGET_SPEEDUP, TARGET_REL_ERR = 1.6, 1e-3
 def precision_forms(lib, stream, dims, sets, hsets, u64, d64, h64, steps):
     """Plain FP32 spproj and sprecon and the compressed forms with binary16
     and bfloat16 links on the `dims` lattice: rates over the 8 (mu, sign)
-    combinations, the input sets rotating between calls; each compressed
+    combinations, the input sets rotating between calls, each the median of
+    7 timed blocks of `steps` steps; each compressed
     form's sites/s over the plain form's; and every result's relative error
     ||x - x64|| / ||x64|| against the FP64 result of the same inputs (the
     QLA_D3 kernels, bit-identical to the FP64 oracle): spproj mu 0, sign +1
@@ -210,11 +211,12 @@ def precision_forms(lib, stream, dims, sets, hsets, u64, d64, h64, steps):
         return {"B_per_site": bps, "ms_per_call": round(ms / c, 5), "GBs": round(bps * n * c / (ms * 1e-3) / 1e9, 1),
                 "sites_per_s": round(n * c / (ms * 1e-3), 1)}
 
-    def rotate(call):
+    def rotate(call, reps=7):
+        """median over `reps` timed blocks of `steps` steps (8 calls each)"""
         def step(it):
             for j, c in enumerate(combos):
                 call((it * nc + j) % K, c)
-        return _timed(stream, step, steps, 2)
+        return float(np.median([_timed(stream, step, steps, 2 if r == 0 else 0) for r in range(reps)]))
 
     dptr = [{c: qla_amd.pointer_array(sets[k][1], nbr[c]) for c in combos} for k in range(K)]
     hptr = [{c: qla_amd.pointer_array(hsets[k], nbr[c]) for c in combos} for k in range(K)]
