@@ -17,6 +17,7 @@
 // the gathered Dh/Hh records are moved as 12-byte chunks for the binary16
 // part plus one 4-byte chunk for the scale; outputs are written back
 // contiguously through LDS.
+#include <atomic>
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -386,12 +387,12 @@ k_reduce_dh2(const double *__restrict__ part, int g, int dot, double *__restrict
 // one wave per block (a block's LDS is released when its last wave ends):
 // 32^4 binary16-link spproj 21.96 us against 22.31 us with four-wave blocks
 constexpr int CWPB = 1;
-static int g_cmp_tiles_per_wave = 1;
+static std::atomic<int> g_cmp_tiles_per_wave{ 1 };     // set from any host thread
 
 static dim3 cgrid(int n)
 {
     const long long ntiles = ((long long)n + WAVE - 1) / WAVE;
-    const long long per_block = (long long)CWPB * g_cmp_tiles_per_wave;
+    const long long per_block = (long long)CWPB * g_cmp_tiles_per_wave.load(std::memory_order_relaxed);
     return dim3((unsigned)((ntiles + per_block - 1) / per_block));
 }
 
@@ -442,7 +443,7 @@ QLA_CMP_DISPATCH(sprecon_c)
 
 using namespace qla;
 
-void qla_set_cmp_tiles_per_wave(int tpw) { g_cmp_tiles_per_wave = tpw < 1 ? 1 : tpw; }
+void qla_set_cmp_tiles_per_wave(int tpw) { g_cmp_tiles_per_wave.store(tpw < 1 ? 1 : tpw, std::memory_order_relaxed); }
 
 static inline int cmp_align(const void *a, const void *r)
 {
