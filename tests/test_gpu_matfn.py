@@ -72,3 +72,36 @@ def test_matfn_bad_nc_is_error():
     qla_amd.call("QLA_FN_M_veq_exp_M", 2, r, dev(a), 4)
     assert qla_amd.load().QLA_MI355X_last_error() != 0
     np.testing.assert_array_equal(host(r), a)
+
+
+def general(n, nc, seed, fn, dt):
+    """P D P^-1 with a Gaussian complex P and Gaussian complex eigenvalues
+    (Re <= 0 for the roots, imaginary for exp), the construction of the
+    reference's own tests (protocol_tensor_sng.m4:481-596): general,
+    non-normal matrices."""
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    d = rng.standard_normal((n, nc)) + 1j * rng.standard_normal((n, nc))
+    d = 1j * d.real if fn == "exp" else np.where(d.real > 0, -d, d)
+    m = p @ (d[:, :, None] * np.linalg.inv(p))
+    return np.stack([m.real, m.imag], axis=-1).astype(dt)
+
+
+@pytest.mark.parametrize("fn", ["exp", "sqrt", "invsqrt"])
+@pytest.mark.parametrize("lib", ["F3", "D3", "FN4", "FN8"])
+def test_matfn_general_matrices(orc, fn, lib):
+    """Non-Hermitian inputs: bit-exact against the oracle (the kernels take
+    no shortcut that assumes Hermitian or anti-Hermitian structure)."""
+    n = 1031
+    nc = 3 if lib in ("F3", "D3") else int(lib[2:])
+    dt = np.float64 if lib == "D3" else np.float32
+    a = general(n, nc, 500 + nc, fn, dt)
+    if lib.startswith("FN"):
+        name = f"QLA_FN_M_veq_{fn}_M"
+        say(f"{name} nc={nc} general n={n}")
+        got = run(name, nc, a, True)
+    else:
+        name = f"QLA_{lib}_M_veq_{fn}_M"
+        say(f"{name} general n={n}")
+        got = run(name, 3, a, False)
+    assert_bitwise(got, orc.matfn(fn, a), f"{name} nc={nc} general")

@@ -79,3 +79,35 @@ def test_sqrt_zero(orc, prec):
     assert not orc.matfn("sqrt", z).any()
     y = c(orc.matfn("invsqrt", z))[0]
     assert np.isinf(np.diag(y).real).all() and not y[~np.eye(3, dtype=bool)].any()
+
+
+def similar(n, nc, seed, kind):
+    """M = P D P^-1 with a Gaussian complex P and a diagonal D drawn as in the
+    reference's own tests (protocol_tensor_sng.m4:481-596): Gaussian complex
+    eigenvalues, with Re <= 0 for sqrt / invsqrt (an eigenvalue of positive
+    real part is negated there) and purely imaginary ones for exp; returns
+    (M, P, d) in complex128."""
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    d = rng.standard_normal((n, nc)) + 1j * rng.standard_normal((n, nc))
+    if kind == "exp":
+        d = 1j * d.real
+    else:
+        d = np.where(d.real > 0, -d, d)
+    m = p @ (d[:, :, None] * np.linalg.inv(p))
+    return m, p, d
+
+
+@pytest.mark.parametrize("fn", ["exp", "sqrt", "invsqrt"])
+@pytest.mark.parametrize("nc", [3, 4, 8])
+def test_similarity_transform(orc, fn, nc):
+    """f(P D P^-1) = P f(D) P^-1 on general (non-normal) matrices, the
+    reference's own construction; principal branches (numpy's complex sqrt),
+    in double, within 1e-9 times cond(P)^2."""
+    m, p, d = similar(40, nc, 300 + nc, fn)
+    f = {"exp": np.exp, "sqrt": np.sqrt, "invsqrt": lambda z: 1 / np.sqrt(z)}[fn]
+    ref = p @ (f(d)[:, :, None] * np.linalg.inv(p))
+    got = c(orc.matfn(fn, pack(m)))
+    cond = np.linalg.cond(p)
+    err = np.abs(got - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2))
+    assert (err <= 1e-9 * cond ** 2).all(), (fn, nc, (err / cond ** 2).max())
