@@ -65,8 +65,8 @@ def _timed(stream, fn, steps, warmup):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not a measured path)", file=sys.stderr)
+        print(f"bench.py: this benchmark measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is designed, not a measured path)", file=sys.stderr)
         sys.exit(2)
     import torch
     import qla_amd

@@ -1,5 +1,5 @@
 """Numpy model of the Cayley-Hamilton matrix functions with an FP64
-coefficient step (an addition of this build with no reference counterpart).
+coefficient step (an addition of this library with no reference counterpart).
 
 TEST INFRASTRUCTURE ONLY: the checker of the QLA_MI355X_F3_M_*_ch kernels, also
 used by the benchmark for their error figures.

@@ -9,8 +9,8 @@ is checked at full size in test_gpu_site.py and test_gpu_sse_order.py.)
   bit-exact, F3 and D3; DF3 norm2 / dot within 1e-14 of the scale; both
   compressed link formats (binary16, bfloat16) bit-exact against the format
   model on the first 65,536 sites and, over the whole lattice, within their
-  accuracy bar of the FP64 result: binary16 links 1e-3 (the north star's
-  bar); bfloat16 links 5e-3 -- bfloat16 keeps 8 significand bits, a unit
+  accuracy bar of the FP64 result: binary16 links 1e-3 (the compressed
+  form's target); bfloat16 links 5e-3 -- bfloat16 keeps 8 significand bits, a unit
   roundoff of 2^-8 = 3.9e-3 on every stored link real, so no 12-real
   bfloat16 link form can reach 1e-3 (measured 1.8e-3).
 * c3: the 64 jobs of draw_jobs(64, seed=7), L in {8..32}, in one batch

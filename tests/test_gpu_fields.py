@@ -6,7 +6,7 @@ pointer operands): the reference's summation order depends on its OpenMP
 thread count, so no order is privileged; the terms are fixed (component
 norms in the field precision, dot terms in double), and the oracle sums
 exactly those terms in long double.  The GPU's result must agree with that
-sum to 1e-14 of the sum's scale (sum |terms|): the north star's QLA_D3 bar; a
+sum to 1e-14 of the sum's scale (sum |terms|), the QLA_D3 accuracy bar; a
 float result (F3) may add one float ulp of rounding.  Conversions (DF3 float -> double, FD3 double -> float) are
 bit-exact.  Sizes give ragged last tiles; x-forms use a parity subset,
 pointer operands permuted targets; host and device result pointers and

@@ -6,7 +6,7 @@ the generated code forms it (component norms in the field precision, dot
 terms in double).  The oracle sums exactly those terms in long double
 (norm2_terms_ld / dot_terms_ld), and the bars are:
   * D3 / DF3 (double results): within 1e-14 x scale, scale = sum |terms|
-    (for norm2 the sum itself) -- the north star's QLA_D3 bar; a double sum
+    (for norm2 the sum itself) -- the QLA_D3 accuracy bar; a double sum
     of n terms in any order is within ~log2(n) * 2^-53 * scale of it;
   * F3 (float results): within one float ulp of the long-double sum (the
     double sum rounded to float can land on either side of a rounding

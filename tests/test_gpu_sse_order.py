@@ -16,7 +16,7 @@ V_veq_Ma_times_V, D_vpeq_sprecon_M_times_pH and D_vpeq_spproj_M_times_pD.
   against the oracle's restatement on the 32^4 lattice, field and pointer
   sources; their default order within 4 ulp (double) of it.
 * Default order: bit-exact with the generic oracle (the other tests), and
-  against the SSE build within the north star's 4 ulp, measured on the
+  against the SSE build within 4 ulp (the F3 parity bar), measured on the
   cancellation-aware scale |r| + sum_k |a_k| |x_k| of each component.  (Each
   order is within 3 ulp of the exact result, tests/test_oracle_sse.py, so two
   orders could in principle differ by 6; on these inputs the largest
