@@ -11,9 +11,9 @@
 // the general kernels; the Hermitian eigenvalues exist for nc = 3 only (the
 // template's QR iteration for larger nc is not provided).
 //
-// One thread owns one matrix (for nc = 5..8 exp, sqrt and invsqrt, whose
-// per-thread temporaries would not fit in registers, a group of 8 lanes owns
-// one, a row per lane: qla_matfn_rows.hpp); the grid covers every matrix, so
+// One thread owns one matrix (for exp, sqrt, invsqrt and log at the larger
+// nc, whose per-thread temporaries would not fit in registers, a group of nc
+// lanes owns one, a row per lane: qla_matfn_rows.hpp); the grid covers every matrix, so
 // the hardware dispatcher balances the data-dependent work of exp, sqrt,
 // invsqrt and log (squarings, Newton steps) by starting a new workgroup
 // wherever one retires.  (A global work counter claimed per wave costs one
@@ -212,59 +212,71 @@ __global__ void __launch_bounds__(MF_BLOCK) k_matfunc(qla_matfunc_args p)
     LOOP(k, S::OUT) dst[k] = out[k];
 }
 
-// nc = 5..8 exp / sqrt / invsqrt: one matrix per group of RG lanes, one row
-// per lane (qla_matfn_rows.hpp); every indexing form
+// exp / sqrt / invsqrt at nc = 6..8 and log at nc = 7, 8: one matrix per
+// group of nc lanes, one row per lane (qla_matfn_rows.hpp); every indexing
+// form.  Measured on 262,144 matrices against one thread per matrix, the row
+// form runs exp 1.30 / 1.41 / 1.90x, sqrt 1.01 / 1.10 / 1.31x, invsqrt
+// 1.08 / 1.17 / 1.32x and log 0.92 / 1.08 / 1.20x at nc = 6 / 7 / 8; at
+// nc = 5, where one thread's temporaries still fit in its registers, it runs
+// 0.74-1.01x.
 template <int NC, int FN> struct by_rows {
-    static constexpr bool v = NC >= 5 && (FN == MF_EXP || FN == MF_SQRT || FN == MF_INVSQRT);
+    static constexpr bool v = (NC >= 6 && (FN == MF_EXP || FN == MF_SQRT || FN == MF_INVSQRT)) ||
+                              (NC >= 7 && FN == MF_LOG);
+};
+
+// groups per wavefront and per block of the row kernel
+template <int NC> struct rows_shape {
+    static constexpr int GPW = WAVE / NC;
+    static constexpr int GPB = GPW * (MF_BLOCK / WAVE);
 };
 
 template <class T, int NC, int FN>
 __global__ void __launch_bounds__(MF_BLOCK) k_matfunc_rows(qla_matfunc_args p)
 {
-    constexpr int GPB = MF_BLOCK / RG;
-    __shared__ rgrp<T, NC> sh[GPB];
-    const int g = threadIdx.x / RG;
-    const long long i = (long long)blockIdx.x * GPB + g;
-    if (i >= p.n) return;           // whole groups: RG divides the block
-    const int row = (int)(threadIdx.x % RG);
-    const rctx<T, NC> c = { &sh[g], row, row < NC };
+    constexpr int GPW = rows_shape<NC>::GPW;
+    __shared__ rgrp<T, NC> sh[rows_shape<NC>::GPB];
+    const int lane = (int)(threadIdx.x % WAVE);
+    const int gw = lane / NC;
+    if (gw >= GPW) return;          // the wave's last 64 mod nc lanes
+    const int g = (int)(threadIdx.x / WAVE) * GPW + gw;
+    const long long i = (long long)blockIdx.x * rows_shape<NC>::GPB + g;
+    if (i >= p.n) return;           // whole groups
+    const int row = lane - gw * NC;
+    const rctx<T, NC> c = { &sh[g], row };
     const long long site = p.index ? (long long)p.index[i] : i;
     const long long rs = p.ir ? (long long)p.ir[i] : site;
     const long long as = p.ia ? (long long)p.ia[i] : site;
     rrow<T, NC> a, r;
-    if (c.act) {
-        const cpx<T> *src = operand<T>(p.a, p.pa, as, NC * NC) + row * NC;
-        LOOP(j, NC) a.e[j] = src[j];
-    } else {
-        LOOP(j, NC) a.e[j] = { T(0), T(0) };
-    }
+    const cpx<T> *src = operand<T>(p.a, p.pa, as, NC * NC) + row * NC;
+    LOOP(j, NC) a.e[j] = src[j];
     if constexpr (FN == MF_EXP) r_exp(r, a, c);
     else if constexpr (FN == MF_SQRT) r_sqrt(r, a, c);
-    else r_invsqrt(r, a, c);
-    if (c.act) {
-        cpx<T> *dst = (cpx<T> *)p.r + (size_t)rs * (NC * NC) + row * NC;
-        LOOP(j, NC) dst[j] = r.e[j];
-    }
+    else if constexpr (FN == MF_INVSQRT) r_invsqrt(r, a, c);
+    else r_log(r, a, c);
+    cpx<T> *dst = (cpx<T> *)p.r + (size_t)rs * (NC * NC) + row * NC;
+    LOOP(j, NC) dst[j] = r.e[j];
 }
 
 template <class T, int NC, int FN>
 static hipError_t launch_t(const qla_matfunc_args &p, hipStream_t st)
 {
     if constexpr (by_rows<NC, FN>::v) {
-        const long long gblocks = ((long long)p.n + MF_BLOCK / RG - 1) / (MF_BLOCK / RG);
+        constexpr int GPB = rows_shape<NC>::GPB;
+        const long long gblocks = ((long long)p.n + GPB - 1) / GPB;
         hipLaunchKernelGGL((k_matfunc_rows<T, NC, FN>), dim3((unsigned)gblocks), dim3(MF_BLOCK), 0, st, p);
         return hipGetLastError();
-    }
-    const long long blocks = ((long long)p.n + MF_BLOCK - 1) / MF_BLOCK;
-    const bool contiguous = !p.index && !p.ir && !p.ia && !p.ib && !p.pa && !p.pb;
-    if constexpr (stage_of<T, NC, FN>::OK) {
-        if (contiguous) {
-            hipLaunchKernelGGL((k_matfunc_stage<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
-            return hipGetLastError();
+    } else {
+        const long long blocks = ((long long)p.n + MF_BLOCK - 1) / MF_BLOCK;
+        const bool contiguous = !p.index && !p.ir && !p.ia && !p.ib && !p.pa && !p.pb;
+        if constexpr (stage_of<T, NC, FN>::OK) {
+            if (contiguous) {
+                hipLaunchKernelGGL((k_matfunc_stage<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
+                return hipGetLastError();
+            }
         }
+        hipLaunchKernelGGL((k_matfunc<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
+        return hipGetLastError();
     }
-    hipLaunchKernelGGL((k_matfunc<T, NC, FN>), dim3((unsigned)blocks), dim3(MF_BLOCK), 0, st, p);
-    return hipGetLastError();
 }
 
 template <class T, int NC>

@@ -1,11 +1,14 @@
-// qla_matfn_rows.hpp -- exp, sqrt and invsqrt of nc x nc complex matrices
-// (nc = 5..8) with one matrix per group of RG = 8 lanes, one row per lane.
+// qla_matfn_rows.hpp -- exp, sqrt, invsqrt and log of nc x nc complex matrices
+// (nc = 5..8; the launcher picks this form where it measured faster) with one
+// matrix per group of nc lanes, one row per lane.
 //
 // The per-thread form (qla_matfn.hpp) holds a whole matrix in each thread:
 // at nc = 8 the Pade / Newton temporaries are six 64-complex matrices, more
 // than the 512 VGPRs a lane can address, so it spills.  Here lane r of a
 // group holds row r of every matrix (8 complexes at nc = 8) and the group
-// shares what a row needs of the other rows through its own LDS area:
+// shares what a row needs of the other rows through its own LDS area.  A
+// wavefront holds 64 / nc whole groups (12, 10, 9, 8 at nc = 5..8); its
+// last 64 mod nc lanes hold none.  A group's lanes:
 //   * a product x = a b: each lane publishes its row of b, then forms
 //     x[r][j] = sum_k a[r][k] b[k][j] with x = 0 and k ascending, the
 //     reference's order (M_eq_M_times_M, expressions_tensor.pl:544-630);
@@ -31,8 +34,6 @@
 
 namespace qla {
 
-constexpr int RG = 8;   // lanes per matrix; lanes r >= nc of a group hold no row
-
 template <class T, int NC> struct rrow { cpx<T> e[NC]; };
 
 // one group's LDS: a published matrix (product operand / row reorder), the
@@ -47,7 +48,6 @@ template <class T, int NC> struct rgrp {
 template <class T, int NC> struct rctx {
     rgrp<T, NC> *s;
     int r;          // this lane's row
-    bool act;       // r < NC
 };
 
 template <class T, int NC> __device__ __forceinline__ void r_eq_d(rrow<T, NC> &x, T d, int r)
@@ -77,7 +77,7 @@ template <class T, int NC>
 __device__ __forceinline__ void r_times(rrow<T, NC> &x, const rrow<T, NC> &a, const rrow<T, NC> &b,
                                         const rctx<T, NC> &c)
 {
-    if (c.act) LOOP(j, NC) c.s->m[c.r * NC + j] = b.e[j];
+    LOOP(j, NC) c.s->m[c.r * NC + j] = b.e[j];
     wave_sync();
     cpx<T> acc[NC];
     LOOP(j, NC) acc[j] = { T(0), T(0) };
@@ -94,7 +94,7 @@ __device__ __forceinline__ void r_times(rrow<T, NC> &x, const rrow<T, NC> &a, co
 // row-major order (maxev of the templates)
 template <class T, int NC> __device__ __forceinline__ double r_maxev2(const rrow<T, NC> &a, const rctx<T, NC> &c)
 {
-    if (c.act) LOOP(j, NC) c.s->nrm[c.r * NC + j] = norm2c(a.e[j]);
+    LOOP(j, NC) c.s->nrm[c.r * NC + j] = norm2c(a.e[j]);
     wave_sync();
     double f = 0;
     LOOP(k, NC * NC) f += (double)c.s->nrm[k];
@@ -112,7 +112,7 @@ __device__ __forceinline__ void r_inverse(rrow<T, NC> &x, const rrow<T, NC> &a, 
     else d = b;
     int lr = ctx.r;     // the logical row this lane holds
     LOOP(k, NC) {
-        if (ctx.act) ctx.s->key[lr] = norm2c(c.e[k]);
+        ctx.s->key[lr] = norm2c(c.e[k]);
         wave_sync();
         T rmax = ctx.s->key[k];
         int imax = k;
@@ -125,7 +125,7 @@ __device__ __forceinline__ void r_inverse(rrow<T, NC> &x, const rrow<T, NC> &a, 
         // rows k and imax trade places
         lr = lr == imax ? k : (lr == k ? imax : lr);
         rmax = T(1) / rmax;
-        if (ctx.act && lr == k) {
+        if (lr == k) {
             cpx<T> s;
             s.re = rmax * c.e[k].re;        // QLA_c_eq_r_times_ca
             s.im = -(rmax * c.e[k].im);
@@ -133,7 +133,7 @@ __device__ __forceinline__ void r_inverse(rrow<T, NC> &x, const rrow<T, NC> &a, 
             LOOP(j, NC) { ctx.s->pc[j] = c.e[j]; ctx.s->pd[j] = d.e[j]; }
         }
         wave_sync();
-        if (ctx.act && lr != k) {
+        if (lr != k) {
             const cpx<T> t = cmul(ctx.s->pc[k], c.e[k]);
             LOOP(j, NC) if (j > k) acc_mul<true, false>(c.e[j], t, ctx.s->pc[j]);
             LOOP(j, NC) acc_mul<true, false>(d.e[j], t, ctx.s->pd[j]);
@@ -143,9 +143,9 @@ __device__ __forceinline__ void r_inverse(rrow<T, NC> &x, const rrow<T, NC> &a, 
     // x[lr][j] = c[lr][lr] d[lr][j], then every lane takes its own row back
     cpx<T> cii = c.e[0];
     LOOP(j, NC) if (j == lr) cii = c.e[j];
-    if (ctx.act) LOOP(j, NC) ctx.s->m[lr * NC + j] = cmul(cii, d.e[j]);
+    LOOP(j, NC) ctx.s->m[lr * NC + j] = cmul(cii, d.e[j]);
     wave_sync();
-    if (ctx.act) LOOP(j, NC) x.e[j] = ctx.s->m[ctx.r * NC + j];
+    LOOP(j, NC) x.e[j] = ctx.s->m[ctx.r * NC + j];
     wave_sync();
 }
 
@@ -239,6 +239,47 @@ template <class T, int NC> __device__ __forceinline__ void r_invsqrt(rrow<T, NC>
         enorm = r_maxev2(e, c);
     } while (nit < 20 && enorm > estop);
     r_eq_r_times(r, (T)(1 / sqrt(ds)), x);
+}
+
+// code-templates/M_eq_log_M.c:49-70, 158-200 (f_log): LOG_NROOTS square
+// roots of the scaled matrix by the row form of the Newton iteration, the
+// Pade form of log(1 + X) with one Gauss-Jordan inverse, then the scale
+template <class T, int NC> __device__ __forceinline__ void r_log(rrow<T, NC> &r, const rrow<T, NC> &a,
+                                                                 const rctx<T, NC> &c)
+{
+    double ds = sqrt(r_maxev2(a, c));
+    if (ds == 0) { r_eq_d(r, (T)(-1. / 0.), c.r); return; }
+    rrow<T, NC> as, as2, as3, as4, pa, qa;
+    r_eq_r_times(as, (T)(1 / ds), a);
+    for (int i = 0; i < LOG_NROOTS; i++) {
+        as2 = as;
+        r_sqrt(as, as2, c);
+    }
+    LOOP(j, NC) if (j == c.r) as.e[j].re = as.e[j].re - 1;
+    r_eq_d(pa, (T)LOG_P[0], c.r);
+    r_eq_d(qa, (T)LOG_Q[0], c.r);
+    r_peq_r_times(pa, (T)LOG_P[1], as);
+    r_peq_r_times(qa, (T)LOG_Q[1], as);
+    r_times(as2, as, as, c);
+    r_peq_r_times(pa, (T)LOG_P[2], as2);
+    r_peq_r_times(qa, (T)LOG_Q[2], as2);
+    r_times(as3, as, as2, c);
+    r_peq_r_times(pa, (T)LOG_P[3], as3);
+    r_peq_r_times(qa, (T)LOG_Q[3], as3);
+    r_times(as4, as2, as2, c);
+    r_peq_r_times(pa, (T)LOG_P[4], as4);
+    r_peq_r_times(qa, (T)LOG_Q[4], as4);
+    r_times(as4, as, as4, c);               // as5 of the template
+    r_peq_r_times(pa, (T)LOG_P[5], as4);
+    r_peq(qa, as4);
+    r_inverse<T, NC, true>(as4, qa, qa, c);
+    r_times(qa, pa, as4, c);
+    r_times(pa, as3, qa, c);
+    r_peq_r_times(pa, (T)-0.5, as2);
+    r_peq(pa, as);
+    r_eq_r_times(r, (T)(1 << LOG_NROOTS), pa);
+    ds = log(ds);
+    LOOP(j, NC) if (j == c.r) r.e[j].re = (T)((double)r.e[j].re + ds);
 }
 
 } // namespace qla

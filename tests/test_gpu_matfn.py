@@ -41,7 +41,7 @@ def test_matfn_nc3(orc, fn, prec):
 
 
 @pytest.mark.parametrize("fn", ["exp", "sqrt", "invsqrt"])
-@pytest.mark.parametrize("nc", [3, 4, 5, 8])
+@pytest.mark.parametrize("nc", [3, 4, 5, 6, 7, 8])
 def test_matfn_fn(orc, fn, nc):
     n = 777
     gen = lattice.random_antihermitian if fn == "exp" else lattice.random_hpd

@@ -178,7 +178,10 @@ def case(orc, lib, fn, nc):
 
 
 def fn_nc(v):
-    return 3 if v.lib != "FN" or v.fn == "eigenvalsH" else 5
+    # nc = 7 runs exp, sqrt, invsqrt, log (and expA, expTA, sqrtPH,
+    # invsqrtPH, which take their branches) on the row kernel, one matrix per
+    # lane group, and det and the inverses on one thread per matrix
+    return 3 if v.lib != "FN" or v.fn == "eigenvalsH" else 7
 
 
 @pytest.mark.parametrize("v", matfuncs(), ids=lambda v: v.name)
@@ -192,7 +195,7 @@ def test_matfunc_symbol(orc, v):
     check(v.fn, nc, got[rs], exp[rs], ga, v.name)
 
 
-@pytest.mark.parametrize("nc", [3, 4, 6, 7, 8])
+@pytest.mark.parametrize("nc", [3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("fn", [f for f in MATFUNCS if f != "eigenvalsH"])
 def test_fn_colours(orc, fn, nc):
     """Every QLA_FN v-form at each supported colour count."""
@@ -235,6 +238,19 @@ def test_log_special(orc):
         qla_amd.call(f"QLA_{lib}_M_veq_log_M", r, dev(z), 3)
         qla_amd.check()
         assert_bitwise(host(r), orc.matfunc("log", z), f"QLA_{lib}_M_veq_log_M zero")
+    # nc = 6 runs one thread per matrix, nc = 7 and 8 one matrix per lane
+    # group; a zero matrix between nonzero ones leaves its neighbours untouched
+    for nc in (6, 7, 8):
+        a = lattice.random_hpd(16, nc, 3, eps_lo=1e-2).astype(np.float32)
+        a[[0, 9]] = 0
+        r = dev(np.ones_like(a))
+        say(f"QLA_FN_M_veq_log_M zero nc={nc}")
+        qla_amd.call("QLA_FN_M_veq_log_M", nc, r, dev(a), len(a))
+        qla_amd.check()
+        got, exp = host(r), orc.matfunc("log", a)
+        assert_bitwise(got[[0, 9]], exp[[0, 9]], f"QLA_FN_M_veq_log_M zero nc={nc}")
+        rest = np.setdiff1d(np.arange(len(a)), [0, 9])
+        check("log", nc, got[rest], exp[rest], a[rest], f"QLA_FN_M_veq_log_M nc={nc}")
 
 
 def test_errors_and_empty():
