@@ -119,10 +119,12 @@ k_spproj_c(char *__restrict__ r, const char *__restrict__ a, const char *const *
     char *ll = smem + (threadIdx.x / WAVE) * BYTES;
     char *ld = ll + LB, *ls = ld + DB, *lo = ls + SB;
     const int lane = lane_id();
-    const int ntiles = (n + WAVE - 1) / WAVE, stride = gridDim.x * WPB;
+    const int ntiles = tiles_of(n), stride = gridDim.x * WPB;
     int tile = blockIdx.x * WPB + threadIdx.x / WAVE;
     if (tile >= ntiles) return;
-    uint64_t pd = (tile * WAVE + lane < n) ? (uint64_t)b[tile * WAVE + lane] : 0;
+    bool act;
+    int i = lane_site(tile, lane, n, act);
+    uint64_t pd = act ? (uint64_t)b[i] : 0;
     auto stage_links = [&](int base, int ns) {
         if (al & 1) stage_bytes<16, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);
         else stage_bytes<4, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);
@@ -140,7 +142,11 @@ k_spproj_c(char *__restrict__ r, const char *__restrict__ a, const char *const *
         stage_gather<4, 4>(ls, pd + 48, ns, lane);
         // next tile's pointers travel with this tile's records
         const int next = tile + stride;
-        const uint64_t npd = (next < ntiles && next * WAVE + lane < n) ? (uint64_t)b[next * WAVE + lane] : 0;
+        uint64_t npd = 0;
+        if (next < ntiles) {
+            i = lane_site(next, lane, n, act);
+            if (act) npd = (uint64_t)b[i];
+        }
         cmp_dma_wait();
         uint32_t lw[6], dw[12];
 #pragma unroll
@@ -183,10 +189,12 @@ k_sprecon_c(char *__restrict__ r, const char *__restrict__ a, const char *const 
     char *ll = smem + (threadIdx.x / WAVE) * BYTES;
     char *lh = ll + LB, *ls = lh + HB, *lr = ls + SB;
     const int lane = lane_id();
-    const int ntiles = (n + WAVE - 1) / WAVE, stride = gridDim.x * WPB;
+    const int ntiles = tiles_of(n), stride = gridDim.x * WPB;
     int tile = blockIdx.x * WPB + threadIdx.x / WAVE;
     if (tile >= ntiles) return;
-    uint64_t ph = (tile * WAVE + lane < n) ? (uint64_t)b[tile * WAVE + lane] : 0;
+    bool act;
+    int i = lane_site(tile, lane, n, act);
+    uint64_t ph = act ? (uint64_t)b[i] : 0;
     auto stage_contig_ops = [&](int base, int ns) {
         if (al & 1) stage_bytes<16, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);
         else stage_bytes<4, WAVE * LINKB>(ll, a + (size_t)base * LINKB, ns * LINKB, lane);
@@ -205,7 +213,11 @@ k_sprecon_c(char *__restrict__ r, const char *__restrict__ a, const char *const 
         stage_gather<24, 12>(lh, ph, ns, lane);
         stage_gather<4, 4>(ls, ph + 24, ns, lane);
         const int next = tile + stride;
-        const uint64_t nph = (next < ntiles && next * WAVE + lane < n) ? (uint64_t)b[next * WAVE + lane] : 0;
+        uint64_t nph = 0;
+        if (next < ntiles) {
+            i = lane_site(next, lane, n, act);
+            if (act) nph = (uint64_t)b[i];
+        }
         cmp_dma_wait();
         uint32_t lw[6], hw[6], dw[12];
 #pragma unroll
@@ -472,7 +484,7 @@ hipError_t qla_launch_sprecon_c(int fmt, void *r, const void *a, const void *b, 
 hipError_t qla_launch_convert(int kind, int fmt, void *r, const void *a, int n, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    const dim3 g((n + 255) / 256), b(256);
+    const dim3 g((unsigned)(((long long)n + 255) / 256)), b(256);
     if ((kind == 0 || kind == 1) && fmt != FMT_BF16 && fmt != FMT_F16) return hipErrorInvalidValue;
     switch (kind) {
     case 0:

@@ -39,6 +39,18 @@ __device__ __forceinline__ void wave_sync()
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// Tiles of WAVE sites over n >= 0 sites, formed so that nothing overflows
+// int for any n up to INT_MAX: a tile below tiles_of(n) starts at
+// tile * WAVE <= n - 1, and lane_site forms a lane's site only when it is
+// below n (the tile's first site otherwise).
+__device__ __forceinline__ int tiles_of(int n) { return n / WAVE + (n % WAVE != 0 ? 1 : 0); }
+__device__ __forceinline__ int lane_site(int tile, int lane, int n, bool &act)
+{
+    const int base = tile * WAVE;
+    act = lane < n - base;
+    return act ? base + lane : base;
+}
+
 // Wait until every vector-memory operation of this wave has completed
 // (s_waitcnt vmcnt(0): simm16 0x0F70 on gfx9).  Issued through the builtin so
 // the compiler's wait-count pass sees it: loads issued before it count as

@@ -37,12 +37,12 @@ __global__ void __launch_bounds__(WPB * WAVE) k_multi(multi_args p)
     char *lds = smem + (threadIdx.x / WAVE) * OP::BYTES;
     const int lane = lane_id();
     const int stride = gridDim.x * WPB;
-    const int ntiles = (p.n + WAVE - 1) / WAVE;
+    const int ntiles = tiles_of(p.n);
     int tile = blockIdx.x * WPB + (int)(threadIdx.x / WAVE);
     if (tile >= ntiles) return;
 
-    const int i0 = tile * WAVE + lane;
-    bool act = i0 < p.n;
+    bool act;
+    const int i0 = lane_site(tile, lane, p.n, act);
     lane_sites site = OP::sites(p, i0, act);
     uint64_t ga, gb;
     OP::term_addresses(p, 0, site, act, ga, gb);
@@ -62,8 +62,7 @@ __global__ void __launch_bounds__(WPB * WAVE) k_multi(multi_args p)
                 else gather_tile<R::V>(lds + OP::OFF_R, gr, ns, lane);
             }
             if (d == 0 && more) {
-                const int i1 = next * WAVE + lane;
-                nact = i1 < p.n;
+                const int i1 = lane_site(next, lane, p.n, nact);
                 nsite = OP::sites(p, i1, nact);
             }
             // gather addresses of the next phase: term d + 1, or term 0 of the next tile

@@ -42,12 +42,13 @@ k_precision_gather(TO *__restrict__ r, const void *__restrict__ a, const int *__
 {
     const int lane = threadIdx.x % WAVE;
     const int nwaves = gridDim.x * (PBLOCK / WAVE);
-    const int ntiles = (n + WAVE - 1) / WAVE;
+    const int ntiles = tiles_of(n);
     for (int tile = blockIdx.x * (PBLOCK / WAVE) + (int)(threadIdx.x / WAVE); tile < ntiles; tile += nwaves) {
-        const int i = tile * WAVE + lane;
+        bool act;
+        const int i = lane_site(tile, lane, n, act);
         const int ns = min(WAVE, n - tile * WAVE);
         uint64_t ga = 0, gr = 0;
-        if (i < n) {
+        if (act) {
             const int g = index ? index[i] : i;
             const int sa = ia ? ia[i] : g;
             const int sr = ir ? ir[i] : g;

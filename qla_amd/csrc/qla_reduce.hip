@@ -187,12 +187,13 @@ k_reduce1_gather(const void *__restrict__ a, const void *__restrict__ b, const i
     double sr = 0, si = 0;
     const int lane = threadIdx.x % WAVE;
     const int nwaves = gridDim.x * (RBLOCK / WAVE);
-    const int ntiles = (n + WAVE - 1) / WAVE;
+    const int ntiles = tiles_of(n);
     for (int tile = blockIdx.x * (RBLOCK / WAVE) + (int)(threadIdx.x / WAVE); tile < ntiles; tile += nwaves) {
-        const int i = tile * WAVE + lane;
+        bool act;
+        const int i = lane_site(tile, lane, n, act);
         const int ns = min(WAVE, n - tile * WAVE);
         uint64_t ga = 0, gb = 0;
-        if (i < n) {
+        if (act) {
             const int g = index ? index[i] : i;
             const int sa = ia ? ia[i] : g;
             ga = pa ? (uint64_t)((const void *const *)a)[sa] : (uint64_t)((const cpx<T> *)a + (size_t)sa * NC);

@@ -55,19 +55,18 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
     char *lds = smem + (threadIdx.x / WAVE) * OP::BYTES;
     const int lane = lane_id();
     const int stride = gridDim.x * WPB;
-    const int ntiles = (p.n + WAVE - 1) / WAVE;
+    const int ntiles = tiles_of(p.n);
     int tile = blockIdx.x * WPB + (int)(threadIdx.x / WAVE);
     if (tile >= ntiles) return;
 
-    const int i0 = tile * WAVE + lane;
-    const bool act0 = i0 < p.n;
+    bool act0;
+    const int i0 = lane_site(tile, lane, p.n, act0);
     lane_addr cur = addresses_of<T, KIND, XF>(p, sites_of<XF>(p, i0, act0), act0);
     int next = tile + stride;
     lane_sites nsite = { 0, 0, 0 };     // XF forms: this lane's records in the next tile
     bool nact = false;
     if (XF && next < ntiles) {
-        const int i1 = next * WAVE + lane;
-        nact = i1 < p.n;
+        const int i1 = lane_site(next, lane, p.n, nact);
         nsite = sites_of<XF>(p, i1, nact);
     }
     // the loop body then never waits on a prologue load: its only memory
@@ -83,15 +82,13 @@ __device__ __forceinline__ void run_tiles(const site_args &p)
         bool nnact = false;
         if (next < ntiles) {
             if (!XF) {
-                const int i1 = next * WAVE + lane;
-                nact = i1 < p.n;
+                const int i1 = lane_site(next, lane, p.n, nact);
                 nsite = sites_of<XF>(p, i1, nact);
             }
             nxt = addresses_of<T, KIND, XF>(p, nsite, nact);
             const int nn = next + stride;
             if (XF && nn < ntiles) {
-                const int i2 = nn * WAVE + lane;
-                nnact = i2 < p.n;
+                const int i2 = lane_site(nn, lane, p.n, nnact);
                 nnsite = sites_of<XF>(p, i2, nnact);
             }
         }

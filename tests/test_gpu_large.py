@@ -1,4 +1,5 @@
-"""Maximum-size checks: fields whose byte offsets pass 4 GiB, through the C ABI.
+"""Maximum-size checks through the C ABI: fields whose byte offsets pass 4 GiB,
+and site counts up to INT_MAX, the largest an int argument holds.
 
 The site kernels, the reductions and the matrix functions address records
 with 64-bit offsets; these tests run them on 48,000,037 sites (a D field of
@@ -7,19 +8,25 @@ records that includes the last 1,024 against the oracle bit for bit (the
 sampled records' inputs are gathered back to the host and evaluated as a
 small call), and the global norm against a double sum formed by torch.
 Inputs are drawn on the device; neighbour maps jump by about n/2 so the
-gathers reach far addresses.  Each test is skipped when the device has too
-little free memory.
+gathers reach far addresses.  The INT_MAX tests run 2,147,483,647 sites
+(33,554,432 tiles of 64, the last holding 63) with every pointer naming one
+record, so each result holds one value everywhere; outputs start as NaN or
+0xFF bytes, so a site left unwritten fails.  Each test is skipped when the
+device has too little free memory.
 """
 import numpy as np
 import pytest
 
 import qla_amd
-from gpu_util import assert_bitwise, say
+from oracle import compressed as cm
+from qla_amd import lattice
+from gpu_util import assert_bitwise, dev, say
 
 pytestmark = pytest.mark.gpu
 
 N_SITES = 48_000_037
 N_MATS = 50_000_017
+N_INT_MAX = 2**31 - 1
 
 
 def need(nbytes):
@@ -110,4 +117,82 @@ def test_matrix_exp_past_4GiB(orc):
     exp = orc.matfn("exp", a[st].cpu().numpy())
     assert_bitwise(r[st].cpu().numpy(), exp, "exp past 4 GiB")
     del a, r
+    torch.cuda.empty_cache()
+
+
+def _differs(r, rec):
+    """whether any record of r (n, ...) differs from rec (1, ...), bytewise,
+    compared in slices of 2^28 records"""
+    import torch
+    flat, ref = r.view(r.shape[0], -1).view(torch.uint8), rec.reshape(1, -1).view(torch.uint8)
+    step = 1 << 28
+    return any(bool((flat[i:i + step] != ref).any()) for i in range(0, r.shape[0], step))
+
+
+def test_matvec_at_int_max(orc):
+    """V = M V with M and V through pointer arrays (QLA_F3_V_veq_pM_times_pV)
+    on INT_MAX sites: the site kernel's tile loop and its last, partial tile."""
+    import torch
+    n = N_INT_MAX
+    need(n * (8 + 8 + 24) + (1 << 28) * 24)
+    m = lattice.random_su3(1, 21).astype(np.float32)
+    v = np.random.default_rng(22).standard_normal((1, 3, 2)).astype(np.float32)
+    dm, dv = dev(m), dev(v)
+    pm = torch.full((n,), dm.data_ptr(), dtype=torch.int64, device="cuda")
+    pv = torch.full((n,), dv.data_ptr(), dtype=torch.int64, device="cuda")
+    r = torch.full((n, 3, 2), float("nan"), dtype=torch.float32, device="cuda")
+    say(f"QLA_F3_V_veq_pM_times_pV n={n}")
+    qla_amd.call("QLA_F3_V_veq_pM_times_pV", r, pm, pv, n)
+    qla_amd.check()
+    exp = orc.V_eqop_M_times_V(orc.EQ, np.zeros((1, 3, 2), np.float32), m, v)
+    assert not _differs(r, torch.from_numpy(exp).cuda())
+    del pm, pv, r
+    torch.cuda.empty_cache()
+
+
+def test_norm2_at_int_max():
+    """norm2 through a pointer array (QLA_F3_r_veq_norm2_pV, the gathered
+    reduction) on INT_MAX sites: n times the record's norm, the float norms
+    added in double, rounded to float."""
+    import torch
+    n = N_INT_MAX
+    need(n * 8)
+    v = np.random.default_rng(23).standard_normal((1, 3, 2)).astype(np.float32)
+    dv = dev(v)
+    pv = torch.full((n,), dv.data_ptr(), dtype=torch.int64, device="cuda")
+    r = torch.zeros(1, dtype=torch.float32, device="cuda")
+    say(f"QLA_F3_r_veq_norm2_pV n={n}")
+    qla_amd.call("QLA_F3_r_veq_norm2_pV", r, pv, n)
+    qla_amd.check()
+    terms = v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]           # float32, as the kernel forms them
+    ref = np.longdouble(n) * terms.astype(np.longdouble).sum()
+    got = np.longdouble(float(r.cpu()[0]))
+    assert abs(got - ref) <= ref * 2.0 ** -23, (got, ref)
+    del pv
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("fmt", ["f16"])
+def test_compressed_spproj_at_int_max(fmt):
+    """Hh = spproj(M12 * pDh) (QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh) on
+    INT_MAX sites: 51.5 GB of contiguous links, one Dh record through the
+    pointer array, 60 GB of Hh out, each record equal to the format model's."""
+    import torch
+    n = N_INT_MAX
+    need(n * (24 + 8 + 28) + (1 << 28) * 28)
+    u = lattice.random_su3(1, 24).astype(np.float32)
+    d = lattice.random_field("D", 1, 25).astype(np.float32)
+    link = cm.pack_link(u, fmt)
+    drec = cm.to_records(*cm.pack_fermion(d))
+    mu, sign = 3, -1
+    exp = cm.spproj(link, fmt, drec, np.zeros(1, dtype=np.int64), mu, sign)
+    links = torch.from_numpy(np.ascontiguousarray(link).view(np.uint8)).cuda().reshape(1, -1).expand(n, -1).contiguous()
+    dd = torch.from_numpy(np.ascontiguousarray(drec).view(np.uint8)).cuda()
+    pd = torch.full((n,), dd.data_ptr(), dtype=torch.int64, device="cuda")
+    r = torch.full((n, 28), 0xFF, dtype=torch.uint8, device="cuda")
+    say(f"QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh {fmt} n={n}")
+    qla_amd.call("QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh", r, links, {"bf16": 0, "f16": 1}[fmt], pd, mu, sign, n)
+    qla_amd.check()
+    assert not _differs(r, torch.from_numpy(np.ascontiguousarray(exp).view(np.uint8)).cuda())
+    del links, pd, r
     torch.cuda.empty_cache()

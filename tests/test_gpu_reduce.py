@@ -157,8 +157,11 @@ def test_result_memory_kinds(orc):
         qla_amd.call("QLA_DF3_d_veq_sum_D", dsum, da, n)
         qla_amd.check()
         assert host(dsum).tolist() == list(out)
-        exp = a.astype(np.float64).reshape(n, 24).sum(axis=0)
-        assert np.allclose(list(out), exp, rtol=0, atol=1e-12 * np.abs(a).sum())
+        # each component's sum against a long-double sum, within 1e-14 of
+        # that component's sum |terms|
+        terms = a.astype(np.longdouble).reshape(n, 24)
+        err = np.abs(np.array(list(out), dtype=np.longdouble) - terms.sum(axis=0))
+        assert np.all(err <= 1e-14 * np.abs(terms).sum(axis=0)), err
         da.mul_(0.5)
         a = a * np.float32(0.5)
 
