@@ -91,7 +91,7 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 /* Reference: build_qla.pl:1002-1066 and 1663-1690 around the algorithms of */
 /* the lib/perl/code-templates sources.  QLA_FN_* take the colour count    */
 /* first (prototype.pl:594-600); this library supports nc = 3..8 (other nc: */
-/* no-op and an error is recorded; eigenvalsH: nc = 3 only).                */
+/* no-op and an error is recorded).                                         */
 /* ------------------------------------------------------------------------ */
 #include "qla_mi355x_matfuncs.h"
 

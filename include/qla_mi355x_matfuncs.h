@@ -472,8 +472,8 @@ void QLA_F3_xM_eq_log_xM(QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT r,
                          int *indexa,
                          int n);
 
-/* F3: eigenvalues of a Hermitian matrix, nc = 3 only (template order, imaginary parts 0).
- * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161. */
+/* F3: eigenvalues of a Hermitian matrix (nc = 3: closed form, template order, imaginary parts 0; QLA_FN nc = 4..8: the template's QR iteration).
+ * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161 (nc = 3), :180-484 (QR). */
 void QLA_F3_V_veq_eigenvalsH_M(QLA_F3_ColorVector *QLA_MI355X_RESTRICT r,
                                QLA_F3_ColorMatrix *QLA_MI355X_RESTRICT a,
                                int n);
@@ -960,8 +960,8 @@ void QLA_D3_xM_eq_log_xM(QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT r,
                          int *indexa,
                          int n);
 
-/* D3: eigenvalues of a Hermitian matrix, nc = 3 only (template order, imaginary parts 0).
- * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161. */
+/* D3: eigenvalues of a Hermitian matrix (nc = 3: closed form, template order, imaginary parts 0; QLA_FN nc = 4..8: the template's QR iteration).
+ * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161 (nc = 3), :180-484 (QR). */
 void QLA_D3_V_veq_eigenvalsH_M(QLA_D3_ColorVector *QLA_MI355X_RESTRICT r,
                                QLA_D3_ColorMatrix *QLA_MI355X_RESTRICT a,
                                int n);
@@ -1548,8 +1548,8 @@ void QLA_FN_xM_eq_log_xM(int nc,
                          int *indexa,
                          int n);
 
-/* FN: eigenvalues of a Hermitian matrix, nc = 3 only (template order, imaginary parts 0).
- * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161. */
+/* FN: eigenvalues of a Hermitian matrix (nc = 3: closed form, template order, imaginary parts 0; QLA_FN nc = 4..8: the template's QR iteration).
+ * Reference: code-templates/V_eq_eigenvalsH_M.c:103-161 (nc = 3), :180-484 (QR). */
 void QLA_FN_V_veq_eigenvalsH_M(int nc,
                                void *r,
                                void *a,

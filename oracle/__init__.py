@@ -373,8 +373,8 @@ def matfunc(fn: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
     if fn in ("inverse_V", "inverse_M"):
         if b is None or b.dtype != a.dtype or b.shape[0] != n:
             raise ValueError(f"{fn} needs a second operand of {n} records")
-    if fn == "eigenvalsH" and nc != 3:
-        raise ValueError("eigenvalsH: nc = 3 only")
+    if nc > 16:
+        raise ValueError("nc <= 16")
     r = np.empty(shape, dtype=a.dtype)
     getattr(lib, f"orc_{p}_matfunc")(MATFUNC[fn], nc, _ptr(r), _ptr(_need(a)),
                                      _ptr(_need(b)) if b is not None else None, n)

@@ -1651,8 +1651,9 @@ _MATFUNC_DOC = {
     "invsqrtPH": ("inverse square root of a positive Hermitian matrix",
                   "code-templates/M_eq_invsqrtPH_M.c (closed form for nc = 3, the invsqrt branch otherwise)"),
     "log": ("matrix logarithm", "code-templates/M_eq_log_M.c (4 square roots, Pade, rescale)"),
-    "eigenvalsH": ("eigenvalues of a Hermitian matrix, nc = 3 only (template order, imaginary parts 0)",
-                   "code-templates/V_eq_eigenvalsH_M.c:103-161"),
+    "eigenvalsH": ("eigenvalues of a Hermitian matrix (nc = 3: closed form, template order, imaginary parts 0; "
+                   "QLA_FN nc = 4..8: the template's QR iteration)",
+                   "code-templates/V_eq_eigenvalsH_M.c:103-161 (nc = 3), :180-484 (QR)"),
 }
 
 

@@ -277,7 +277,7 @@ void matfunc_entry(int prec, int fn, int nc, int pa, int pb, void *r, const void
                    const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
 {
     if (n <= 0) return;
-    if (nc < 3 || nc > 8 || (prec == 1 && nc != 3) || (fn == 12 && nc != 3)) {
+    if (nc < 3 || nc > 8 || (prec == 1 && nc != 3)) {
         record(hipErrorInvalidValue, name);
         return;
     }

@@ -8,8 +8,8 @@
 // (QLA_F3 and QLA_FN at nc = 3), 1 double nc = 3 (QLA_D3), 2..6 float
 // nc = 4..8 (QLA_FN).  For nc != 3 the templates' expA / expTA take the exp
 // branch and sqrtPH / invsqrtPH the sqrt / invsqrt branch, so those codes run
-// the general kernels; the Hermitian eigenvalues exist for nc = 3 only (the
-// template's QR iteration for larger nc is not provided).
+// the general kernels; the Hermitian eigenvalues take the closed form at
+// nc = 3 and the template's QR iteration at nc = 4..8.
 //
 // One thread owns one matrix (for exp, sqrt, invsqrt and log at the larger
 // nc, whose per-thread temporaries would not fit in registers, a group of nc
@@ -54,7 +54,8 @@ __device__ __forceinline__ void evaluate(cpx<T> *out, const mat<T, NC> &a, const
     } else if constexpr (FN == MF_INV_V) {
         f_inverse_v(out, a, b);
     } else if constexpr (FN == MF_EIGH) {
-        f_eigenvalsH3(out, a);
+        if constexpr (NC == 3) f_eigenvalsH3(out, a);
+        else f_eigenvalsH_qr(out, a);
     } else {
         mat<T, NC> o;
         if constexpr (FN == MF_EXP) f_exp(o, a);
@@ -142,7 +143,7 @@ template <class T, int NC, int FN> struct stage_of {
     static constexpr int RB = S::B * (int)sizeof(cpx<T>);
     static constexpr int RO = S::OUT * (int)sizeof(cpx<T>);
     static constexpr bool UNIFORM = FN == MF_DET || FN == MF_INVERSE || FN == MF_EXPA || FN == MF_EXPTA ||
-                                    FN == MF_SQRTPH || FN == MF_INVSQRTPH || FN == MF_EIGH || FN == MF_EXPA_CH ||
+                                    FN == MF_SQRTPH || FN == MF_INVSQRTPH || (FN == MF_EIGH && NC == 3) || FN == MF_EXPA_CH ||
                                     FN == MF_SQRTPH_CH || FN == MF_INVSQRTPH_CH;
     static constexpr bool OK = RA <= MF_STAGE_MAX && UNIFORM;
 };
@@ -319,6 +320,7 @@ static hipError_t launch_nc(int fn, const qla_matfunc_args &p, hipStream_t st)
         case MF_EXPTA: return launch_t<T, NC, MF_EXP>(p, st);
         case MF_SQRTPH: return launch_t<T, NC, MF_SQRT>(p, st);
         case MF_INVSQRTPH: return launch_t<T, NC, MF_INVSQRT>(p, st);
+        case MF_EIGH: return launch_t<T, NC, MF_EIGH>(p, st);        // the template's QR iteration
         default: return hipErrorInvalidValue;
         }
     }

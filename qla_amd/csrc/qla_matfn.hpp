@@ -657,6 +657,194 @@ template <class T> __device__ void f_eigenvalsH3(cpx<T> *v, const mat<T, 3> &m)
     }
 }
 
+// ---- V_eq_eigenvalsH_M, nc >= 4 (code-templates/V_eq_eigenvalsH_M.c:80-101,
+// 180-484) ----
+// Hessenberg form by Gaussian elimination with row-and-column pivoting, then
+// the template's complex implicit double-shift QR on rows l..nn: a small
+// subdiagonal element deflates one root, a 2x2 block two (solved from its
+// real trace and the real part of its determinant), exceptional shifts at
+// iterations 10 and 20.  Where the template, after 100 iterations on one
+// root, copies the remaining diagonal plus the shift into the result and
+// searches again without bound, the search is repeated at most EIGH_RESTARTS
+// times and then stops with that copy; the template's store of a zero one row
+// past the matrix (element (k+2, k-1) for k = nn-1 = nc-2) is dropped.
+// The row and column indices of the QR steps are data-dependent, so the
+// working matrix lives in the thread's private memory.
+constexpr int EIGH_RESTARTS = 8;
+
+template <class T> __device__ __forceinline__ cpx<T> cdiv_q(cpx<T> a, cpx<T> b)
+{
+    const T t = (T)(1.0 / (b.re * b.re + b.im * b.im));           // QLA_c_eq_c_div_c
+    return { (a.re * b.re + a.im * b.im) * t, (a.im * b.re - a.re * b.im) * t };
+}
+template <class T> __device__ __forceinline__ T cabs_q(cpx<T> z) { return m_sqrt(z.re * z.re + z.im * z.im); }
+
+template <class T>
+__device__ __forceinline__ void eig2x2(cpx<T> &v0, cpx<T> &v1, cpx<T> m00, cpx<T> m01, cpx<T> m10, cpx<T> m11)
+{
+    T det = m00.re * m11.re - m00.im * m11.im;
+    det = det - (m01.re * m10.re - m01.im * m10.im);
+    const T tr = m00.re + m11.re;
+    if (det == 0) {
+        v0 = { T(0), T(0) };
+        v1 = { tr, T(0) };
+    } else {
+        const T s = (T)(0.5 * tr);
+        const T d = s * s - det;
+        const T sd = m_sqrt(m_fabs(d));
+        const T l1 = s + copysign(sd, s);
+        v0 = { det / l1, T(0) };
+        v1 = { l1, T(0) };
+    }
+}
+
+template <class T, int NC> __device__ void f_eigenvalsH_qr(cpx<T> *v, const mat<T, NC> &a)
+{
+    cpx<T> h[NC * NC];
+#define H_(i, j) h[(i) * NC + (j)]
+    LOOP(k, NC * NC) h[k] = a.e[k];
+    for (int k = 0; k < NC - 2; k++) {
+        T rmax = norm2c(H_(k + 1, k));
+        int imax = k + 1;
+        for (int i = k + 2; i < NC; i++) {
+            const T n2 = norm2c(H_(i, k));
+            if (n2 > rmax) { rmax = n2; imax = i; }
+        }
+        if (rmax == 0) continue;
+        for (int j = 0; j < NC; j++) { const cpx<T> t = H_(k + 1, j); H_(k + 1, j) = H_(imax, j); H_(imax, j) = t; }
+        for (int i = 0; i < NC; i++) { const cpx<T> t = H_(i, k + 1); H_(i, k + 1) = H_(i, imax); H_(i, imax) = t; }
+        rmax = T(1) / rmax;
+        const cpx<T> s = { rmax * H_(k + 1, k).re, -(rmax * H_(k + 1, k).im) };   // QLA_c_eq_r_times_ca
+        for (int i = k + 2; i < NC; i++) {
+            const cpx<T> t = cmul(s, H_(i, k));
+            H_(i, k) = { T(0), T(0) };
+            for (int j = k + 1; j < NC; j++) acc_mul<true, false>(H_(i, j), t, H_(k + 1, j));
+            for (int j = 0; j < NC; j++) acc_mul<false, false>(H_(j, k + 1), t, H_(j, i));
+        }
+    }
+    T anorm = 0;
+    for (int i = 0; i < NC; i++)
+        for (int j = (i == 0 ? 0 : i - 1); j < NC; j++) anorm = anorm + cabs_q(H_(i, j));
+    int nn = NC - 1, restarts = 0;
+    cpx<T> t = { T(0), T(0) };
+    while (nn >= 0) {
+        int l, its = 0;
+        do {
+            for (l = nn; l > 0; l--) {
+                T s = cabs_q(H_(l - 1, l - 1)) + cabs_q(H_(l, l));
+                if (s == 0) s = anorm;
+                const T o = cabs_q(H_(l, l - 1));
+                if (s + o == s) { H_(l, l - 1) = { T(0), T(0) }; break; }
+            }
+            cpx<T> x = H_(nn, nn);
+            if (l == nn) {
+                v[nn] = { x.re + t.re, x.im + t.im };
+                nn--;
+            } else if (l == nn - 1) {
+                eig2x2(v[nn - 1], v[nn], H_(nn - 1, nn - 1), H_(nn - 1, nn), H_(nn, nn - 1), x);
+                v[nn - 1].re = v[nn - 1].re + t.re; v[nn - 1].im = v[nn - 1].im + t.im;
+                v[nn].re = v[nn].re + t.re; v[nn].im = v[nn].im + t.im;
+                nn -= 2;
+            } else {
+                if (its >= 100) {
+                    for (int i = 0; i <= nn; i++) v[i] = { H_(i, i).re + t.re, H_(i, i).im + t.im };
+                    if (++restarts > EIGH_RESTARTS) nn = -1;
+                    break;
+                }
+                cpx<T> y = H_(nn - 1, nn - 1), w = cmul(H_(nn, nn - 1), H_(nn - 1, nn));
+                if (its == 10 || its == 20) {           // exceptional shift
+                    t.re = t.re + x.re; t.im = t.im + x.im;
+                    for (int i = 0; i < nn; i++) { H_(i, i).re = H_(i, i).re - x.re; H_(i, i).im = H_(i, i).im - x.im; }
+                    H_(nn, nn) = { T(0), T(0) };
+                    const T s = cabs_q(H_(nn, nn - 1)) + cabs_q(H_(nn - 1, nn - 2));
+                    x = { (T)(0.75 * s), T(0) };
+                    y = x;
+                    w = { (T)(-0.4375 * s * s), T(0) };
+                }
+                its++;
+                const int m = l;
+                cpx<T> p, q, r, rr, ss, t1;
+                const cpx<T> z = H_(m, m);
+                rr = { x.re - z.re, x.im - z.im };
+                ss = { y.re - z.re, y.im - z.im };
+                p = cmul(rr, ss);
+                p.re = p.re - w.re; p.im = p.im - w.im;
+                acc_mul<false, false>(p, H_(m + 1, m), H_(m, m + 1));
+                t1 = { H_(m + 1, m + 1).re - z.re, H_(m + 1, m + 1).im - z.im };
+                t1.re = t1.re - (rr.re + ss.re); t1.im = t1.im - (rr.im + ss.im);
+                q = cmul(H_(m + 1, m), t1);
+                r = cmul(H_(m + 1, m), H_(m + 2, m + 1));
+                for (int k = m; k < nn; k++) {
+                    T xx = 0;
+                    if (k != m) {
+                        p = H_(k, k - 1);
+                        q = H_(k + 1, k - 1);
+                        r = { T(0), T(0) };
+                        if (k + 1 != nn) r = H_(k + 2, k - 1);
+                        xx = cabs_q(p) + cabs_q(q) + cabs_q(r);
+                        if (xx != 0) {
+                            const T xxi = T(1) / xx;
+                            p = { xxi * p.re, xxi * p.im };
+                            q = { xxi * q.re, xxi * q.im };
+                            r = { xxi * r.re, xxi * r.im };
+                        }
+                    }
+                    const T p2 = norm2c(p);
+                    const T s2 = p2 + norm2c(q) + norm2c(r);
+                    if (s2 == 0) continue;
+                    cpx<T> s;
+                    if (p2 > 0) {
+                        const double f = sqrt((double)(s2 / p2));      // C's double sqrt of a T
+                        s = { (T)(f * p.re), (T)(f * p.im) };
+                    } else {
+                        s = { (T)sqrt((double)s2), T(0) };
+                    }
+                    if (k != m) {
+                        H_(k, k - 1) = { -xx * s.re, -xx * s.im };
+                        H_(k + 1, k - 1) = { T(0), T(0) };
+                        if (k + 2 < NC) H_(k + 2, k - 1) = { T(0), T(0) };
+                    }
+                    p.re = p.re + s.re; p.im = p.im + s.im;
+                    x = cdiv_q(p, s);
+                    y = cdiv_q(q, s);
+                    w = cdiv_q(r, s);
+                    q = cdiv_q(q, p);
+                    r = cdiv_q(r, p);
+                    for (int j = k; j <= nn; j++) {             // row modification
+                        cpx<T> u = H_(k, j);
+                        const cpx<T> b1 = H_(k + 1, j);
+                        u.re = u.re + (q.re * b1.re + q.im * b1.im);        // QLA_c_peq_ca_times_c
+                        u.im = u.im + (q.re * b1.im - q.im * b1.re);
+                        if (k + 1 != nn) {
+                            const cpx<T> b2 = H_(k + 2, j);
+                            u.re = u.re + (r.re * b2.re + r.im * b2.im);
+                            u.im = u.im + (r.re * b2.im - r.im * b2.re);
+                            acc_mul<true, false>(H_(k + 2, j), u, w);
+                        }
+                        acc_mul<true, false>(H_(k + 1, j), u, y);
+                        acc_mul<true, false>(H_(k, j), u, x);
+                    }
+                    const int ie = nn < k + 3 ? nn : k + 3;
+                    for (int i = l; i <= ie; i++) {             // column modification
+                        cpx<T> u = cmul(x, H_(i, k));
+                        acc_mul<false, false>(u, y, H_(i, k + 1));
+                        if (k + 1 != nn) {
+                            acc_mul<false, false>(u, w, H_(i, k + 2));
+                            H_(i, k + 2).re = H_(i, k + 2).re - (u.re * r.re + u.im * r.im);   // QLA_c_meq_c_times_ca
+                            H_(i, k + 2).im = H_(i, k + 2).im - (u.im * r.re - u.re * r.im);
+                        }
+                        H_(i, k + 1).re = H_(i, k + 1).re - (u.re * q.re + u.im * q.im);
+                        H_(i, k + 1).im = H_(i, k + 1).im - (u.im * q.re - u.re * q.im);
+                        H_(i, k).re = H_(i, k).re - u.re;
+                        H_(i, k).im = H_(i, k).im - u.im;
+                    }
+                }
+            }
+        } while (l + 1 < nn);
+    }
+#undef H_
+}
+
 // ---- M_eq_log_M (code-templates/M_eq_log_M.c:49-70, 158-200), nc >= 3 ----
 // Scale by the Frobenius norm, take NROOTS = 4 square roots (the M_eq_sqrt_M
 // iteration), then log(1 + X) ~ X - X^2/2 + X^3 P(X)/Q(X) with the template's

@@ -180,8 +180,9 @@ def case(orc, lib, fn, nc):
 def fn_nc(v):
     # nc = 7 runs exp, sqrt, invsqrt, log (and expA, expTA, sqrtPH,
     # invsqrtPH, which take their branches) on the row kernel, one matrix per
-    # lane group, and det and the inverses on one thread per matrix
-    return 3 if v.lib != "FN" or v.fn == "eigenvalsH" else 7
+    # lane group, and det, the inverses and the eigenvalues' QR iteration on
+    # one thread per matrix
+    return 3 if v.lib != "FN" else 7
 
 
 @pytest.mark.parametrize("v", matfuncs(), ids=lambda v: v.name)
@@ -196,7 +197,7 @@ def test_matfunc_symbol(orc, v):
 
 
 @pytest.mark.parametrize("nc", [3, 4, 5, 6, 7, 8])
-@pytest.mark.parametrize("fn", [f for f in MATFUNCS if f != "eigenvalsH"])
+@pytest.mark.parametrize("fn", list(MATFUNCS))
 def test_fn_colours(orc, fn, nc):
     """Every QLA_FN v-form at each supported colour count."""
     v = [x for x in matfuncs() if x.lib == "FN" and x.fn == fn and not (x.xf or x.pa or x.pb or x.own_index)][0]
@@ -263,10 +264,10 @@ def test_errors_and_empty():
     qla_amd.call("QLA_FN_M_veq_expA_M", 2, r, dev(a), 4)
     assert lib.QLA_MI355X_last_error() != 0
     np.testing.assert_array_equal(host(r), a)
-    a4 = np.ones((4, 4, 4, 2), dtype=np.float32)
-    v = dev(np.zeros((4, 4, 2), dtype=np.float32))
-    say("QLA_FN_V_veq_eigenvalsH_M nc=4 (unsupported)")
-    qla_amd.call("QLA_FN_V_veq_eigenvalsH_M", 4, v, dev(a4), 4)
+    a9 = np.ones((4, 9, 9, 2), dtype=np.float32)
+    v = dev(np.zeros((4, 9, 2), dtype=np.float32))
+    say("QLA_FN_V_veq_eigenvalsH_M nc=9 (unsupported)")
+    qla_amd.call("QLA_FN_V_veq_eigenvalsH_M", 9, v, dev(a9), 4)
     assert lib.QLA_MI355X_last_error() != 0
     assert not host(v).any()
     a3 = np.ones((4, 3, 3, 2), dtype=np.float64)
@@ -275,3 +276,35 @@ def test_errors_and_empty():
     qla_amd.call("QLA_D3_M_veq_log_M", r3, dev(np.zeros_like(a3)), 0)
     qla_amd.check()
     np.testing.assert_array_equal(host(r3), a3)
+
+
+@pytest.mark.parametrize("nc", [4, 5, 6, 7, 8])
+def test_eigenvalsH_qr_many(orc, nc):
+    """QLA_FN_V_veq_eigenvalsH_M at nc = 4..8, the template's QR iteration,
+    on 20,000 Hermitian matrices (the batch includes matrices that take the
+    exceptional shifts), bit-exact against the oracle; then inputs with known
+    roots: diagonal and zero matrices (exact), a NaN matrix between finite
+    ones (the iteration ends; its neighbours are unaffected)."""
+    rng = np.random.default_rng(500 + nc)
+    g = rng.standard_normal((20000, nc, nc)) + 1j * rng.standard_normal((20000, nc, nc))
+    a = pack(g + np.conj(np.swapaxes(g, 1, 2))).astype(np.float32)
+    a = np.ascontiguousarray(a)
+    r = dev(np.zeros((len(a), nc, 2), dtype=np.float32))
+    say(f"QLA_FN_V_veq_eigenvalsH_M nc={nc} n={len(a)}")
+    qla_amd.call("QLA_FN_V_veq_eigenvalsH_M", nc, r, dev(a), len(a))
+    qla_amd.check()
+    assert_bitwise(host(r), orc.matfunc("eigenvalsH", a), f"eigenvalsH nc={nc}")
+    d = np.arange(1, nc + 1, dtype=np.float64)[::-1] - 2.5
+    sp = np.stack([np.diag(d), np.zeros((nc, nc)), np.diag(d), np.diag(d)]) + 0j
+    sp = np.ascontiguousarray(pack(sp).astype(np.float32))
+    sp[2, 2, 1, 0] = np.nan         # below the diagonal, where the iteration reads
+    r = dev(np.zeros((4, nc, 2), dtype=np.float32))
+    say(f"QLA_FN_V_veq_eigenvalsH_M nc={nc} special inputs")
+    qla_amd.call("QLA_FN_V_veq_eigenvalsH_M", nc, r, dev(sp), 4)
+    qla_amd.check()
+    got = host(r)
+    for i in (0, 3):
+        assert np.array_equal(got[i, :, 0], d.astype(np.float32)) and not got[i, :, 1].any()
+    assert not got[1].any()
+    assert np.isnan(got[2]).any()
+    np.testing.assert_array_equal(got, orc.matfunc("eigenvalsH", sp))   # NaN where the oracle has NaN

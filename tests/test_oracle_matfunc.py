@@ -13,7 +13,10 @@ M_eq_log_M, V_eq_eigenvalsH_M}.c) against mathematics:
 * identities: exp(A) unitary with det exp(tr A), sqrtPH(M)^2 = M,
   invsqrtPH(M) M invsqrtPH(M) = 1, exp(log(A)) = A;
 * the template's branch structure for nc != 3 (expA, expTA take the exp
-  branch, sqrtPH / invsqrtPH the sqrt / invsqrt branch).
+  branch, sqrtPH / invsqrtPH the sqrt / invsqrt branch);
+* the eigenvalues' QR iteration for nc != 3 against eigvalsh, with the
+  rare matrices where an exceptional shift makes the accumulated shift
+  complex identified by their signature rather than held to the bar.
 """
 import numpy as np
 import pytest
@@ -254,6 +257,95 @@ def test_log_zero(orc, prec):
     assert (np.diag(r).real == -np.inf).all() and not r[~np.eye(3, dtype=bool)].any()
 
 
-def test_eigenvalsH_rejects_other_nc(orc):
-    with pytest.raises(ValueError):
-        orc.matfunc("eigenvalsH", np.zeros((1, 4, 4, 2)))
+def hermitian(n, nc, seed):
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    return np.ascontiguousarray(g + np.conj(np.swapaxes(g, 1, 2)))
+
+
+@pytest.mark.parametrize("nc", [4, 5, 6, 7, 8])
+def test_eigenvalsH_qr(orc, nc):
+    """The template's QR iteration for nc != 3 (V_eq_eigenvalsH_M.c:180-484)
+    in double against numpy's eigvalsh.  Its 2x2 deflation solves from the
+    block's real trace and real determinant, exact while the accumulated
+    shift t is real; an exceptional shift (iterations 10 and 20) can make t
+    complex, and the roots the 2x2 formula then returns carry exactly Im t and
+    an error of order (Im t)^2.  So: every matrix whose roots have imaginary
+    parts at rounding level agrees with eigvalsh to 1e-13 of the spectral
+    radius, with its trace to the same; the others are rare and show that
+    signature (two or more roots with one shared imaginary part); and the
+    batch contains such a matrix, so the exceptional-shift branch ran."""
+    n = 20000
+    h = hermitian(n, nc, 200 + nc)
+    v = c(orc.matfunc("eigenvalsH", pack(h)))
+    ref = np.linalg.eigvalsh(h)
+    scale = np.abs(ref).max(axis=1)
+    err = np.abs(np.sort(v.real, axis=1) - ref).max(axis=1) / scale
+    imag = np.abs(v.imag).max(axis=1) / scale
+    clean = imag <= 1e-12
+    assert clean.mean() >= 0.995
+    assert err[clean].max() <= 1e-13
+    trace = np.abs(v.real.sum(axis=1) - np.trace(h, axis1=1, axis2=2).real) / scale
+    assert trace[clean].max() <= 1e-13
+    shifted = np.nonzero(~clean)[0]
+    assert shifted.size >= 1
+    for i in shifted:
+        im = v[i].imag[np.abs(v[i].imag) > 1e-12 * scale[i]]
+        assert im.size >= 2 and np.unique(im).size < im.size, (i, v[i])
+
+
+@pytest.mark.parametrize("nc", [4, 8])
+def test_eigenvalsH_qr_float(orc, nc):
+    """The float restatement (the QLA_FN arithmetic) tracks eigvalsh to float
+    accuracy on all but the rare complex-shift matrices."""
+    h = hermitian(2000, nc, 400 + nc)
+    a = pack(h).astype(np.float32)
+    v = c(orc.matfunc("eigenvalsH", a).astype(np.float64))
+    ref = np.linalg.eigvalsh(c(a.astype(np.float64)))
+    err = np.abs(np.sort(v.real, axis=1) - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    assert (err <= 1e-4).mean() >= 0.998
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_eigenvalsH_qr_exact_cases(orc, prec):
+    """Inputs the iteration solves without rounding: diagonal matrices (no
+    pivot, every subdiagonal already zero: the roots in diagonal order), a
+    multiple of 1, the zero matrix, nc = 1; a block-diagonal matrix of 2x2
+    Hermitian blocks deflates each block through the 2x2 formula; a NaN
+    input ends (the iteration is bounded) with NaN roots."""
+    d = np.array([[3.0, -1.0, 0.5, 2.0, -4.0, 0.25, 1.0, -2.5]])
+    for nc in (4, 6, 8):
+        m = as_dt(pack(np.diag(d[0, :nc])[None] + 0j), prec)
+        v = orc.matfunc("eigenvalsH", m)
+        assert np.array_equal(v[0, :, 0], d[0, :nc].astype(m.dtype)) and not v[..., 1].any()
+        for z in (np.zeros((1, nc, nc)), 2.5 * np.eye(nc)[None]):
+            v = orc.matfunc("eigenvalsH", as_dt(pack(z + 0j), prec))
+            assert np.array_equal(v[0, :, 0], np.diag(z[0]).astype(v.dtype)) and not v[..., 1].any()
+    v = orc.matfunc("eigenvalsH", as_dt(pack(np.array([[[2.0 - 1.0j]]])), prec))
+    assert v[0, 0, 0] == 2.0 and v[0, 0, 1] == -1.0
+    blocks = [np.array([[1.0, 2.0 - 1.0j], [2.0 + 1.0j, -3.0]]), np.array([[4.0, 0.5j], [-0.5j, 4.5]])]
+    b = np.zeros((4, 4), dtype=complex)
+    b[:2, :2], b[2:, 2:] = blocks
+    v = c(orc.matfunc("eigenvalsH", as_dt(pack(b[None]), prec)).astype(np.float64))[0]
+    tol = 1e-14 if prec == "D" else 1e-5
+    np.testing.assert_allclose(np.sort(v.real), np.linalg.eigvalsh(b), atol=tol * 6)
+    assert not v.imag.any()
+    h = hermitian(1, 6, 5)
+    h[0, 2, 3] = np.nan
+    v = orc.matfunc("eigenvalsH", as_dt(pack(h), prec))
+    assert np.isnan(v).any()
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_eigenvalsH_2x2(orc, prec):
+    """nc = 2 (V_eq_eigenvalsH_M.c:80-101): from the real trace and the real
+    determinant, det = 0 gives (0, tr); the larger-magnitude root first
+    formed, the other as det / l1."""
+    h = hermitian(500, 2, 9)
+    v = c(orc.matfunc("eigenvalsH", as_dt(pack(h), prec)).astype(np.float64))
+    ref = np.linalg.eigvalsh(h)
+    tol = (1e-14 if prec == "D" else 1e-6) * np.abs(ref).max(axis=1)
+    assert (np.abs(np.sort(v.real, axis=1) - ref).max(axis=1) <= tol).all()
+    m = as_dt(pack(np.array([[[2.0, 1.0], [1.0, 0.5]]]) + 0j), prec)     # det = 0
+    v = orc.matfunc("eigenvalsH", m)
+    assert v[0, 0, 0] == 0 and v[0, 1, 0] == 2.5
