@@ -144,6 +144,10 @@ scratch_buf scratch_for(int kind, const char *name)
     scratch_buf b;
     const size_t bytes = sizeof(double) * scratch_doubles(kind);
     e = hipMalloc((void **)&b.dev, bytes);
+    // the reduction's ticket (in its result slot) starts at 0; every call
+    // that draws it leaves it at 0
+    if (e == hipSuccess && kind == SCRATCH_REDUCE)
+        e = hipMemset(b.dev + qla_reduce_scratch_doubles(), 0, 4 * sizeof(double));
     if (e == hipSuccess) e = hipHostMalloc((void **)&b.host, sizeof(double) * HOST_SLOT_DOUBLES, hipHostMallocCoherent);
     if (e != hipSuccess) {
         if (b.dev) (void)hipFree(b.dev);
@@ -214,9 +218,10 @@ void reduce_to(L launch, void *r, size_t bytes, const char *name)
 // result slot, laid out as the allocated reduction scratch, in one static
 // device array (32 KiB; every GPU the library runs on holds its own copy
 // until the process ends).  Stage 1 writes every partial slot stage 2
-// reads, so nothing in it needs clearing between calls.  Host code never names the array in a
-// launch or copy: it takes each device's address once with
-// hipGetSymbolAddress and passes that.
+// reads, and the single-launch form's ticket starts at 0 (static storage)
+// and is left at 0 by every call, so nothing in it needs clearing.  Host
+// code never names the array in a launch or copy: it takes each device's
+// address once with hipGetSymbolAddress and passes that.
 constexpr int NOALLOC_DOUBLES = QLA_REDUCE_SCRATCH_DOUBLES + 4;
 constexpr int NOALLOC_MAX_DEVICES = 64;
 static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];

@@ -46,8 +46,10 @@ hipError_t qla_launch_multi(int prec, int eqop, int adj, int xf, const qla_multi
 // Global reductions over m complex elements.  kind: 0 = norm2 (b unused),
 // 1 = dot conj(a).b.  The result is written to the DEVICE address `out` as
 // out_kind 0 double, 1 float, 2 double complex, 3 float complex.  `scratch`
-// must hold qla_reduce_scratch_doubles() = QLA_REDUCE_SCRATCH_DOUBLES doubles
-// (the per-block partials).
+// must hold qla_reduce_scratch_doubles() + 4 = QLA_REDUCE_SCRATCH_DOUBLES + 4
+// doubles: the per-block partials, a result slot of two doubles, and in the
+// fourth double of the slot the single-launch form's ticket, which must be 0
+// before the first call (each call leaves it at 0).
 #define QLA_REDUCE_SCRATCH_DOUBLES (2 * 2048)
 int qla_reduce_scratch_doubles(void);
 // partial slots stage 1 of qla_launch_reduce2 writes for these operands

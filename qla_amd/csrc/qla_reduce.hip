@@ -14,11 +14,10 @@
 // per-block partials in a fixed tree and writes the result.  Contiguous
 // chunks keep each block's reads within few DRAM pages (6.7 TB/s measured
 // for a 510 MB norm2, against 5.4 TB/s with a grid-stride walk).
-// A single-launch form (the last block to finish, found with an atomic
-// ticket, adds the partials) was measured slower than the two launches at
-// every size: the device-scope fence each block needs before its ticket
-// writes back its XCD's L2 (3.9 us for two launches against 4.3-6.3 us at
-// 1K-65K sites, as graph replays).
+// Arrays of at most 65,536 complexes take a single launch instead (the last
+// block to finish, found with an atomic ticket, adds the partials): below
+// that size it beats the two launches, above it the device-scope fence each
+// block needs before its ticket costs more than the second launch.
 #include "qla_dev.hpp"
 #include "qla_launch.hpp"
 
@@ -79,14 +78,12 @@ __device__ __forceinline__ void acc_dot(double &sr, double &si, T ar, T ai, T br
     si += xr * yi - xi * yr;
 }
 
-// stage 1, one complex per load (any record alignment)
+// stage 1, one complex per load (any record alignment): this thread's sums
+// over its block's chunk
 template <class T, bool DOT>
-__global__ void __launch_bounds__(RBLOCK)
-k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long m,
-          double *__restrict__ part)
+__device__ __forceinline__ void stage1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long m,
+                                       double &sr, double &si)
 {
-    __shared__ double sh[RBLOCK / WAVE];
-    double sr = 0, si = 0;
     long long i, end;
     block_chunk(m, i, end);
     for (; i + 3 * RBLOCK < end; i += 4 * RBLOCK) {
@@ -107,6 +104,16 @@ k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long 
         if (DOT) { cpx<T> y = b[i]; acc_dot(sr, si, x.re, x.im, y.re, y.im); }
         else acc_norm2(sr, x.re, x.im);
     }
+}
+
+template <class T, bool DOT>
+__global__ void __launch_bounds__(RBLOCK)
+k_reduce1(const cpx<T> *__restrict__ a, const cpx<T> *__restrict__ b, long long m,
+          double *__restrict__ part)
+{
+    __shared__ double sh[RBLOCK / WAVE];
+    double sr = 0, si = 0;
+    stage1<T, DOT>(a, b, m, sr, si);
     double tr = block_sum(sr, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = tr;
     if (DOT) {
@@ -126,12 +133,9 @@ __device__ __forceinline__ float4 ld_stream(const float4 *p)
 
 // stage 1, two float complexes per 16-byte load (a, b 16-byte aligned, m even)
 template <bool DOT>
-__global__ void __launch_bounds__(RBLOCK)
-k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long long m2,
-             double *__restrict__ part)
+__device__ __forceinline__ void stage1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b,
+                                          long long m2, double &sr, double &si)
 {
-    __shared__ double sh[RBLOCK / WAVE];
-    double sr = 0, si = 0;
     long long i, end;
     block_chunk(m2, i, end);
     for (; i + 3 * RBLOCK < end; i += 4 * RBLOCK) {
@@ -163,6 +167,16 @@ k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long lo
             acc_norm2(sr, x.z, x.w);
         }
     }
+}
+
+template <bool DOT>
+__global__ void __launch_bounds__(RBLOCK)
+k_reduce1_f4(const float4 *__restrict__ a, const float4 *__restrict__ b, long long m2,
+             double *__restrict__ part)
+{
+    __shared__ double sh[RBLOCK / WAVE];
+    double sr = 0, si = 0;
+    stage1_f4<DOT>(a, b, m2, sr, si);
     double tr = block_sum(sr, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = tr;
     if (DOT) {
@@ -262,6 +276,44 @@ k_reduce2(const double *__restrict__ part, int g, int dot, void *out, int out_ki
     combine_partials(part, g, dot != 0, out, out_kind, sh);
 }
 
+// Arrays of at most RSMALL_MAX complexes in one launch of at most RSMALL_G
+// blocks: each block runs stage 1; its thread 0 stores the block's partials,
+// fences them device-wide and draws a ticket; the block with the last ticket
+// fences, adds the partials exactly as k_reduce2 does and puts the ticket
+// back to 0 for the next call on the same scratch.  The result depends only
+// on n (the grid is fixed by it).  Measured in async calls of F3 norm2_V
+// against the two launches: 4,096 sites 4.45 against 5.76 us, 16,384 sites
+// 4.87 against 5.75 us; at 65,536 sites (196,608 complexes) the one launch
+// lost at every grid of 8 to 64 blocks (5.72-6.70 against 5.56 us), the
+// fences of its blocks costing more than the second launch.
+constexpr int RSMALL_G = 64;
+constexpr long long RSMALL_MAX = 1 << 16;
+
+template <class T, bool DOT, bool F4>
+__global__ void __launch_bounds__(RBLOCK)
+k_reduce_small(const void *__restrict__ a, const void *__restrict__ b, long long m, double *__restrict__ part,
+               unsigned int *__restrict__ ticket, void *out, int out_kind)
+{
+    __shared__ double sh[RBLOCK / WAVE];
+    __shared__ int last;
+    double sr = 0, si = 0;
+    if constexpr (F4) stage1_f4<DOT>((const float4 *)a, (const float4 *)b, m / 2, sr, si);
+    else stage1<T, DOT>((const cpx<T> *)a, (const cpx<T> *)b, m, sr, si);
+    const double tr = block_sum(sr, sh);
+    const double ti = DOT ? block_sum(si, sh) : 0.0;
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = tr;
+        if (DOT) part[gridDim.x + blockIdx.x] = ti;
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    combine_partials(part, gridDim.x, DOT, out, out_kind, sh);
+    if (threadIdx.x == 0) atomicExch(ticket, 0u);
+}
+
 } // namespace qla
 
 using namespace qla;
@@ -303,6 +355,18 @@ hipError_t qla_launch_reduce2(int prec, int kind, const void *a, const void *b, 
     }
     const bool f4 = use_f4(prec, dot, a, b, m);
     const int g = grid_for(f4 ? m / 2 : m);
+    if (m <= RSMALL_MAX) {
+        const int gs = g < RSMALL_G ? g : RSMALL_G;
+        unsigned int *ticket = (unsigned int *)(scratch + QLA_REDUCE_SCRATCH_DOUBLES + 3);
+#define QLA_SMALL(T, D, F)                                                                                  \
+        hipLaunchKernelGGL((k_reduce_small<T, D, F>), dim3(gs), dim3(RBLOCK), 0, st, a, b, m, scratch, ticket, \
+                           out, out_kind)
+        if (f4) { if (dot) QLA_SMALL(float, true, true); else QLA_SMALL(float, false, true); }
+        else if (prec == 0) { if (dot) QLA_SMALL(float, true, false); else QLA_SMALL(float, false, false); }
+        else { if (dot) QLA_SMALL(double, true, false); else QLA_SMALL(double, false, false); }
+#undef QLA_SMALL
+        return hipGetLastError();
+    }
     if (f4) {
         if (dot)
             hipLaunchKernelGGL(k_reduce1_f4<true>, dim3(g), dim3(RBLOCK), 0, st, (const float4 *)a,
