@@ -817,6 +817,19 @@ def matfn_config4(lib, stream, args):
     for nc in (4, 8):
         for fn in ("exp", "sqrt", "invsqrt"):
             out[f"FN{nc}_{fn}"] = measure(fn, nc, 262144, 6)
+    # the Hermitian eigenvalues at nc = 4, 8: the template's QR iteration
+    for nc in (4, 8):
+        nn = 262144
+        rng = np.random.default_rng(48 + nc)
+        g = rng.standard_normal((nn, nc, nc)) + 1j * rng.standard_normal((nn, nc, nc))
+        h = g + np.conj(np.swapaxes(g, 1, 2))
+        a = torch.from_numpy(np.ascontiguousarray(np.stack([h.real, h.imag], -1).astype(np.float32))).cuda()
+        r = torch.empty((nn, nc, 2), dtype=torch.float32, device="cuda")
+        f = lib.QLA_FN_V_veq_eigenvalsH_M
+        ms = _timed(stream, lambda it: f(nc, r.data_ptr(), a.data_ptr(), nn), 4, 1) / 4      # noqa: B023
+        out[f"FN{nc}_eigenvalsH"] = {"n": nn, "ms": round(ms, 3), "matrices_per_s": round(nn / (ms * 1e-3), 1),
+                                     "kernel": "one thread per matrix, QR iteration in private memory"}
+        del a, r
     if args.group == "matfn":
         torch.cuda.synchronize()
         return out
