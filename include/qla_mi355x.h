@@ -90,7 +90,7 @@ typedef QLA_D_Complex QLA_D3_DiracPropagator[3][4][3][4];
 /* Hermitian eigenvalues, QLA_F3 / QLA_D3 / QLA_FN, every indexing form.    */
 /* Reference: build_qla.pl:1002-1066 and 1663-1690 around the algorithms of */
 /* the lib/perl/code-templates sources.  QLA_FN_* take the colour count    */
-/* first (prototype.pl:594-600); this library supports nc = 3..8 (other nc: */
+/* first (prototype.pl:594-600); this library supports nc = 1..8 (other nc: */
 /* no-op and an error is recorded).                                         */
 /* ------------------------------------------------------------------------ */
 #include "qla_mi355x_matfuncs.h"

@@ -46,7 +46,7 @@ CXXFLAGS = [
 MULTI_BUILD = {
     "qla_site.hip": [{"QLA_SITE_PREC": p, "QLA_SITE_KIND": k} for p in (0, 1) for k in (0, 1, 2, 3, 5, 6, 7, 8, 9, 10, 11)],
     "qla_multi.hip": [{"QLA_SITE_PREC": p} for p in (0, 1)],
-    "qla_matfn.hip": [{"QLA_MATFN_UNIT": u} for u in range(7)],
+    "qla_matfn.hip": [{"QLA_MATFN_UNIT": u} for u in range(9)],
     "qla_vecops.hip": [{"QLA_VEC_PREC": p} for p in (0, 1)],
 }
 

@@ -275,14 +275,14 @@ void convert_entry(int to_float, int ncomp, int pa, void *r, const void *a, cons
 }
 
 // One colour-matrix function (qla_matfuncs_abi.inc): fn codes of
-// qla_launch_matfunc; prec 0 takes nc = 3..8 (QLA_F3 passes 3), prec 1 nc = 3.
+// qla_launch_matfunc; prec 0 takes nc = 1..8 (QLA_F3 passes 3), prec 1 nc = 3.
 // An unsupported nc or function records hipErrorInvalidValue and writes
 // nothing.
 void matfunc_entry(int prec, int fn, int nc, int pa, int pb, void *r, const void *a, const void *b,
                    const int *index, const int *ir, const int *ia, const int *ib, int n, const char *name)
 {
     if (n <= 0) return;
-    if (nc < 3 || nc > 8 || (prec == 1 && nc != 3)) {
+    if (nc < 1 || nc > 8 || (prec == 1 && nc != 3)) {
         record(hipErrorInvalidValue, name);
         return;
     }
@@ -876,6 +876,8 @@ hipError_t qla_launch_matfunc(int prec, int fn, int nc, const qla_matfunc_args &
     case 6: return qla_matfunc_launch_u4(fn, p, st);
     case 7: return qla_matfunc_launch_u5(fn, p, st);
     case 8: return qla_matfunc_launch_u6(fn, p, st);
+    case 1: return qla_matfunc_launch_u7(fn, p, st);
+    case 2: return qla_matfunc_launch_u8(fn, p, st);
     default: return hipErrorInvalidValue;
     }
 }

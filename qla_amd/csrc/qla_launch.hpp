@@ -91,6 +91,8 @@ hipError_t qla_matfunc_launch_u3(int fn, const qla_matfunc_args &p, hipStream_t 
 hipError_t qla_matfunc_launch_u4(int fn, const qla_matfunc_args &p, hipStream_t st);
 hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, hipStream_t st);
 hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u7(int fn, const qla_matfunc_args &p, hipStream_t st);
+hipError_t qla_matfunc_launch_u8(int fn, const qla_matfunc_args &p, hipStream_t st);
 
 // Field linear algebra (qla_vecops.hip): n steps of r eqop op(x, y) over
 // records of ncomp complexes (3, 6, 9, 12 or 144).  op: 0 copy, 1 times i,

@@ -6,7 +6,8 @@
 //
 // Compiled once per (precision, nc) unit: QLA_MATFN_UNIT 0 is float nc = 3
 // (QLA_F3 and QLA_FN at nc = 3), 1 double nc = 3 (QLA_D3), 2..6 float
-// nc = 4..8 (QLA_FN).  For nc != 3 the templates' expA / expTA take the exp
+// nc = 4..8 and 7, 8 float nc = 1, 2 (QLA_FN).  At nc = 1, 2 every function
+// takes the template's own closed form (qla_matfn_small.hpp).  For nc != 3 the templates' expA / expTA take the exp
 // branch and sqrtPH / invsqrtPH the sqrt / invsqrt branch, so those codes run
 // the general kernels; the Hermitian eigenvalues take the closed form at
 // nc = 3 and the template's QR iteration at nc = 4..8.
@@ -23,6 +24,7 @@
 #include "qla_matfn.hpp"
 #include "qla_matfn_ch.hpp"
 #include "qla_matfn_rows.hpp"
+#include "qla_matfn_small.hpp"
 #include "qla_launch.hpp"
 
 #ifndef QLA_MATFN_UNIT
@@ -49,7 +51,9 @@ template <int FN, int NC> struct fn_shape {
 template <class T, int NC, int FN>
 __device__ __forceinline__ void evaluate(cpx<T> *out, const mat<T, NC> &a, const cpx<T> *b)
 {
-    if constexpr (FN == MF_DET) {
+    if constexpr (NC <= 2) {
+        f_small<T, NC, FN>(out, a, b);
+    } else if constexpr (FN == MF_DET) {
         f_det(out[0], a);
     } else if constexpr (FN == MF_INV_V) {
         f_inverse_v(out, a, b);
@@ -312,6 +316,16 @@ static hipError_t launch_nc(int fn, const qla_matfunc_args &p, hipStream_t st)
             }
         }
         return hipErrorInvalidValue;
+    } else if constexpr (NC <= 2) {
+        // every function has its own closed form at nc = 1, 2 (qla_matfn_small.hpp)
+        switch (fn) {
+        case MF_EXPA: return launch_t<T, NC, MF_EXPA>(p, st);
+        case MF_EXPTA: return launch_t<T, NC, MF_EXPTA>(p, st);
+        case MF_SQRTPH: return launch_t<T, NC, MF_SQRTPH>(p, st);
+        case MF_INVSQRTPH: return launch_t<T, NC, MF_INVSQRTPH>(p, st);
+        case MF_EIGH: return launch_t<T, NC, MF_EIGH>(p, st);
+        default: return hipErrorInvalidValue;
+        }
     } else {
         // the templates' general branches (M_eq_expA_M.c:357-420 and
         // M_eq_sqrtPH_M.c:270-310 are M_eq_exp_M / M_eq_sqrt_M's)
@@ -351,4 +365,10 @@ hipError_t qla_matfunc_launch_u5(int fn, const qla_matfunc_args &p, hipStream_t 
 #elif QLA_MATFN_UNIT == 6
 hipError_t qla_matfunc_launch_u6(int fn, const qla_matfunc_args &p, hipStream_t st)
 { return launch_nc<float, 8>(fn, p, st); }
+#elif QLA_MATFN_UNIT == 7
+hipError_t qla_matfunc_launch_u7(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 1>(fn, p, st); }
+#elif QLA_MATFN_UNIT == 8
+hipError_t qla_matfunc_launch_u8(int fn, const qla_matfunc_args &p, hipStream_t st)
+{ return launch_nc<float, 2>(fn, p, st); }
 #endif

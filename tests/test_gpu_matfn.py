@@ -41,14 +41,24 @@ def test_matfn_nc3(orc, fn, prec):
 
 
 @pytest.mark.parametrize("fn", ["exp", "sqrt", "invsqrt"])
-@pytest.mark.parametrize("nc", [3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("nc", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_matfn_fn(orc, fn, nc):
+    """Bit-exact at every nc; at nc = 1, 2 exp takes the template's closed
+    form, whose expf and sinhf the device and the host library may round
+    apart, so there it agrees to 8 ulp of the larger of 1 and the matrix's
+    largest component."""
     n = 777
     gen = lattice.random_antihermitian if fn == "exp" else lattice.random_hpd
     a = gen(n, nc, 400 + nc).astype(np.float32)
     name = f"QLA_FN_M_veq_{fn}_M"
     say(f"{name} nc={nc} n={n}")
-    assert_bitwise(run(name, nc, a, True), orc.matfn(fn, a), f"{name} nc={nc}")
+    got, exp = run(name, nc, a, True), orc.matfn(fn, a)
+    if fn == "exp" and nc <= 2:
+        scale = np.maximum(np.abs(exp).reshape(n, -1).max(axis=1), 1.0)
+        err = np.abs(got - exp).reshape(n, -1).max(axis=1)
+        assert (err <= 8 * np.finfo(np.float32).eps * scale).all(), f"{name} nc={nc}"
+    else:
+        assert_bitwise(got, exp, f"{name} nc={nc}")
 
 
 @pytest.mark.parametrize("prec", ["F", "D"])
@@ -66,12 +76,13 @@ def test_matfn_special_cases(orc, prec):
 
 def test_matfn_bad_nc_is_error():
     """nc outside the supported range leaves r untouched and records an error."""
-    a = np.ones((4, 2, 2, 2), dtype=np.float32)
-    r = dev(a)
-    say("QLA_FN_M_veq_exp_M nc=2 (unsupported)")
-    qla_amd.call("QLA_FN_M_veq_exp_M", 2, r, dev(a), 4)
-    assert qla_amd.load().QLA_MI355X_last_error() != 0
-    np.testing.assert_array_equal(host(r), a)
+    for nc in (0, 9):
+        a = np.ones((4, max(nc, 1), max(nc, 1), 2), dtype=np.float32)
+        r = dev(a)
+        say(f"QLA_FN_M_veq_exp_M nc={nc} (unsupported)")
+        qla_amd.call("QLA_FN_M_veq_exp_M", nc, r, dev(a), 4)
+        assert qla_amd.load().QLA_MI355X_last_error() != 0
+        np.testing.assert_array_equal(host(r), a)
 
 
 def general(n, nc, seed, fn, dt):

@@ -349,3 +349,100 @@ def test_eigenvalsH_2x2(orc, prec):
     m = as_dt(pack(np.array([[[2.0, 1.0], [1.0, 0.5]]]) + 0j), prec)     # det = 0
     v = orc.matfunc("eigenvalsH", m)
     assert v[0, 0, 0] == 0 and v[0, 1, 0] == 2.5
+
+
+# ------------------------------------------------------------ nc = 1 and 2
+def small_inputs(fn, n, nc, seed):
+    """(a, b) complex inputs for fn at nc = 1, 2 and the function of scipy /
+    numpy they must equal."""
+    rng = np.random.default_rng(seed)
+    g = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+    hpd = np.einsum("nki,nkj->nij", np.conj(g), g) + 0.1 * np.eye(nc)
+    ah = (g - np.conj(np.swapaxes(g, 1, 2))) / 2
+    b = None
+    if fn == "det":
+        return g, None, np.linalg.det
+    if fn == "inverse":
+        return g, None, np.linalg.inv
+    if fn == "inverse_V":
+        b = rng.standard_normal((n, nc)) + 1j * rng.standard_normal((n, nc))
+        return g, b, None
+    if fn == "inverse_M":
+        b = rng.standard_normal((n, nc, nc)) + 1j * rng.standard_normal((n, nc, nc))
+        return g, b, None
+    if fn == "exp":
+        return g, None, sl.expm
+    if fn in ("sqrt", "sqrtPH"):
+        return hpd, None, sl.sqrtm
+    if fn in ("invsqrt", "invsqrtPH"):
+        return hpd, None, lambda m: np.linalg.inv(sl.sqrtm(m))
+    if fn == "log":
+        return hpd, None, sl.logm
+    if fn == "expA":
+        return ah, None, sl.expm
+    if fn == "expTA":
+        return ah - np.trace(ah, axis1=1, axis2=2)[:, None, None] / nc * np.eye(nc), None, sl.expm
+    raise ValueError(fn)
+
+
+SMALL_FNS = ["det", "inverse", "inverse_V", "inverse_M", "exp", "sqrt", "invsqrt", "log", "expA", "expTA",
+             "sqrtPH", "invsqrtPH"]
+
+
+@pytest.mark.parametrize("fn", SMALL_FNS)
+@pytest.mark.parametrize("nc", [1, 2])
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_small_nc_against_scipy(orc, fn, nc, prec):
+    """The templates' own nc = 1, 2 branches (closed forms through lib/cmath)
+    against scipy / numpy on the same matrices: within 1e-12 (double) or
+    2e-5 (float) of the result's largest component (of 1 for log, whose
+    absolute error is the input's relative one), times the condition number
+    for the inverses and solves."""
+    n = 400
+    z, b, ref = small_inputs(fn, n, nc, 11 * nc + len(fn))
+    a = as_dt(pack(z), prec)
+    zz = c(a.astype(np.float64))
+    if b is not None:
+        bb = as_dt(pack(b), prec)
+        got = c(orc.matfunc(fn, a, bb).astype(np.float64))
+        exp = np.linalg.solve(zz, c(bb.astype(np.float64))[..., None])[..., 0] if fn == "inverse_V" \
+            else np.linalg.solve(zz, c(bb.astype(np.float64)))
+    else:
+        got = c(orc.matfunc(fn, a).astype(np.float64))
+        exp = np.stack([ref(m) for m in zz])
+    got, exp = got.reshape(n, -1), exp.reshape(n, -1)
+    scale = np.abs(exp).max(axis=1)
+    if fn == "log":
+        scale = np.maximum(scale, 1.0)       # log's absolute error is the input's relative one
+    tol = (1e-12 if prec == "D" else 2e-5) * scale
+    if fn in ("inverse", "inverse_V", "inverse_M", "invsqrt", "invsqrtPH"):
+        tol = tol * np.linalg.cond(zz)
+    err = np.abs(got - exp).max(axis=1)
+    assert (err <= tol).all(), (fn, nc, (err / scale).max())
+
+
+@pytest.mark.parametrize("prec", ["F", "D"])
+def test_small_nc_branch_cases(orc, prec):
+    """Inputs that take the nc = 2 branches' special cases, each against its
+    exact value: exp of a Jordan block (t = 0: st = ch = 1), sqrt / invsqrt
+    of a singular and of a scalar matrix (det = 0: l0 = 0, l1 = tr; equal
+    roots), log of s*1 (d = 0: c1 = log(s)/s), sqrtPH of a rank-one and of the
+    zero matrix, expTA of a matrix with det = 0 (t = 0); nc = 1 functions of a
+    positive number."""
+    tol = 1e-14 if prec == "D" else 2e-6
+    f = lambda fn, m: c(orc.matfunc(fn, as_dt(pack(np.asarray(m, dtype=complex)[None]), prec)))[0]   # noqa: E731
+    s = 0.7
+    np.testing.assert_allclose(f("exp", [[s, 1.0], [0.0, s]]), np.exp(s) * np.array([[1, 1], [0, 1]]), atol=tol * 4)
+    np.testing.assert_allclose(f("sqrt", [[4.0, 0.0], [0.0, 0.0]]), [[2, 0], [0, 0]], atol=tol)
+    np.testing.assert_allclose(f("sqrt", [[2.25, 0.0], [0.0, 2.25]]), 1.5 * np.eye(2), atol=tol)
+    np.testing.assert_allclose(f("invsqrt", [[2.25, 0.0], [0.0, 2.25]]), np.eye(2) / 1.5, atol=tol)
+    np.testing.assert_allclose(f("log", [[2.0, 0.0], [0.0, 2.0]]), np.log(2.0) * np.eye(2), atol=tol)
+    np.testing.assert_allclose(f("sqrtPH", [[1.0, 2.0], [2.0, 4.0]]), np.array([[1, 2], [2, 4]]) / np.sqrt(5), atol=tol)
+    assert not f("sqrtPH", np.zeros((2, 2))).any()
+    np.testing.assert_allclose(f("expTA", [[0.0, 1.0], [0.0, 0.0]]), [[1, 1], [0, 1]], atol=tol)
+    for fn, v in (("exp", np.exp(2.0)), ("sqrt", np.sqrt(2.0)), ("invsqrt", 1 / np.sqrt(2.0)), ("log", np.log(2.0)),
+                  ("sqrtPH", np.sqrt(2.0)), ("invsqrtPH", 1 / np.sqrt(2.0)), ("inverse", 0.5), ("det", 2.0)):
+        got = c(orc.matfunc(fn, as_dt(pack(np.full((1, 1, 1), 2.0 + 0j)), prec)).astype(np.float64)).ravel()[0]
+        assert abs(got - v) <= tol * 4, (fn, got, v)
+    got = c(orc.matfunc("expA", as_dt(pack(np.full((1, 1, 1), 0.3 + 0.5j)), prec)).astype(np.float64)).ravel()[0]
+    assert abs(got - np.exp(0.5j)) <= tol * 4
