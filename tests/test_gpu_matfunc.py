@@ -208,6 +208,19 @@ def test_matfunc_symbol(orc, v):
     check(v.fn, nc, got[rs], exp[rs], ga, v.name)
 
 
+@pytest.mark.parametrize("v", [x for x in matfuncs() if x.lib == "FN"], ids=lambda v: v.name)
+def test_matfunc_symbol_nc2(orc, v):
+    """Every QLA_FN symbol again at nc = 2, where each function takes the
+    template's own closed form, in every indexing form."""
+    nc = 2
+    cs = case(orc, v.lib, v.fn, nc)
+    got, exp, rs, ga = cs.run(v)
+    untouched = np.ones(cs.n, dtype=bool)
+    untouched[rs] = False
+    assert_bitwise(got[untouched], exp[untouched], f"{v.name} untouched records")
+    check(v.fn, nc, got[rs], exp[rs], ga, v.name)
+
+
 @pytest.mark.parametrize("nc", [1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("fn", list(MATFUNCS))
 def test_fn_colours(orc, fn, nc):
