@@ -40,7 +40,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-extra", action="store_true",
                    help="headline only (no companion measurements), e.g. for profiler runs")
-    p.add_argument("--group", choices=("vecops", "solver", "matfn", "xform"), default=None,
+    p.add_argument("--group", choices=("vecops", "solver", "matfn", "xform", "sizes"), default=None,
                    help="run one companion group alone and print its figures (profiler runs of its kernels)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also run 5 device-to-device copies of this many MiB (HBM counter calibration)")
@@ -107,6 +107,8 @@ def main():
             res, wl = vecops_layer(lib, stream, n, sets, args), "32^4 periodic"
         elif args.group == "solver":
             res, wl = wilson_cg(lib, stream, n, args), "32^4 periodic"
+        elif args.group == "sizes":
+            res, wl = size_sweep(lib, stream), "L^4 periodic, L = 8..48"
         else:
             res, wl = matfn_config4(lib, stream, args), "2,097,152 Nc=3 / 262,144 Nc=4,8 matrices"
         print(json.dumps({"group": args.group, "workload": wl, "result": res}))
@@ -861,6 +863,60 @@ def matfn_config4(lib, stream, args):
                            "GBs": round(nbytes * n3 / (ms * 1e-3) / 1e9, 1), "cache": "one set (working set <= 256 MB)"}
         del a, r
     torch.cuda.synchronize()
+    return out
+
+
+def size_sweep(lib, stream):
+    """QLA_F3_H_veq_spproj_M_times_pD and its compressed form with binary16
+    links on L^4 periodic lattices, L = 8 .. 48: per size the microseconds
+    per call over the 8 (mu, sign) combinations, algorithmic GB/s (224 and
+    112 B/site) and the compressed form's sites/s over the plain form's.
+    Input sets rotate between calls; below about 256 MB per set the working
+    set stays in the Infinity Cache, so the small sizes are cache-served and
+    launch-bound.  Each figure is the median of 5 timed blocks of 10 steps."""
+    import torch
+    import qla_amd
+    from qla_amd import lattice
+    out = {}
+    combos = [(mu, s) for mu in range(4) for s in (1, -1)]
+    for L in (8, 12, 16, 20, 24, 32, 40, 48):
+        dims = (L, L, L, L)
+        n = lattice.volume(dims)
+        fw, bw = lattice.neighbours(dims)
+        nbr = {c: torch.from_numpy((fw if c[1] == 1 else bw)[c[0]]).cuda() for c in combos}
+        per_set = n * (72 + 96 + 48 + 8 * len(combos))
+        K = max(2, min(3, -(-(600 << 20) // per_set)))
+        g = torch.Generator(device="cuda").manual_seed(L)
+        sets = []
+        for _ in range(K):
+            u = torch.randn((n, 3, 3, 2), device="cuda", generator=g)
+            d = torch.randn((n, 3, 4, 2), device="cuda", generator=g)
+            h = torch.empty((n, 3, 2, 2), device="cuda")
+            dp = {c: qla_amd.pointer_array(d, nbr[c]) for c in combos}
+            link = torch.empty((n, 12), dtype=torch.int16, device="cuda")
+            lib.QLA_MI355X_F3_M_pack12(link.data_ptr(), 1, u.data_ptr(), n)
+            drec = torch.empty((n, 52), dtype=torch.uint8, device="cuda")
+            lib.QLA_MI355X_F3_D_packh(drec.data_ptr(), d.data_ptr(), n)
+            hrec = torch.empty((n, 28), dtype=torch.uint8, device="cuda")
+            cp = {c: qla_amd.pointer_array(drec, nbr[c]) for c in combos}
+            sets.append((u, d, h, dp, link, drec, hrec, cp))
+        f = lib.QLA_F3_H_veq_spproj_M_times_pD
+        fc = lib.QLA_MI355X_F3_Hh_veq_spproj_M12_times_pDh
+
+        def block(call):
+            def step(it):
+                for j, c in enumerate(combos):
+                    call(sets[(it * len(combos) + j) % K], c)
+            return float(np.median([_timed(stream, step, 10, 2 if r == 0 else 0) for r in range(5)])) / (10 * len(combos))
+
+        ms = block(lambda st, c: f(st[2].data_ptr(), st[0].data_ptr(), st[3][c].data_ptr(), c[0], c[1], n))
+        msc = block(lambda st, c: fc(st[6].data_ptr(), st[4].data_ptr(), 1, st[7][c].data_ptr(), c[0], c[1], n))
+        out[f"{L}^4"] = {"sites": n, "us_per_call": round(ms * 1e3, 2), "GBs": round(224 * n / (ms * 1e-3) / 1e9, 1),
+                         "compressed_f16_us_per_call": round(msc * 1e3, 2),
+                         "compressed_f16_GBs": round(112 * n / (msc * 1e-3) / 1e9, 1),
+                         "compressed_speedup_sites": round(ms / msc, 3)}
+        del sets, nbr
+        torch.cuda.empty_cache()
     return out
 
 
