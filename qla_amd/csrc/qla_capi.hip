@@ -394,6 +394,7 @@ void local_entry(int prec, int kind, int eqop, int ncomp, int pa, int pb, void *
     p.n = n;
     p.px = pa;
     p.py = pb;
+    p.al = ((uintptr_t)r % 16 == 0 && (uintptr_t)a % 16 == 0 && (uintptr_t)b % 16 == 0) ? 1 : 0;
     finish(qla_launch_local(prec, kind, eqop, ncomp, p, ts.stream), name);
 }
 

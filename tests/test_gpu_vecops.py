@@ -356,6 +356,48 @@ def test_local_symbol(orc, v):
     assert_bitwise(host(r), exp, v.name)
 
 
+
+STAGE_N = 805
+
+
+def local_blocks_cases():
+    """(symbol, byte offset of the fields): the contiguous forms at offset 0
+    and, in single precision, 8 bytes off 16-byte alignment."""
+    flat = [v for v in locals_() if not (v.ir or v.pa or v.pb or v.ia or v.ib or v.xf)]
+    return [(v, 0) for v in flat] + [(v, 8) for v in flat if v.prec != "D3"]
+
+
+@pytest.mark.parametrize("v,offset", local_blocks_cases(), ids=lambda c: getattr(c, "name", str(c)))
+def test_local_symbol_blocks(orc, v, offset):
+    """The contiguous site-local forms over several block steps with a ragged
+    last one (805 sites), on 16-byte aligned fields and, in single
+    precision, on fields 8 bytes off that alignment (the 8-byte staging
+    loads)."""
+    import torch
+    code, dest = LOCALS[v.kind][0], LOCALS[v.kind][1]
+    n = STAGE_N
+    dt = np.float64 if v.prec == "D3" else np.float32
+    x = lattice.random_field(v.t, n, 41, dtype=dt)
+    y = lattice.random_field(v.t, n, 42, dtype=dt) if v.binary else None
+    r0 = np.random.default_rng(zlib.crc32(v.name.encode())).standard_normal((n,) + LOCAL_DEST[dest]).astype(dt)
+    keep = []
+
+    def place(a):
+        flat = torch.cat([torch.zeros(offset // a.itemsize, dtype=torch.float64 if dt == np.float64 else torch.float32),
+                          torch.from_numpy(np.ascontiguousarray(a)).reshape(-1)]).to("cuda")
+        keep.append(flat)
+        return flat.data_ptr() + offset
+
+    args = [place(r0), place(x)] + ([place(y)] if v.binary else [])
+    say(f"{v.name} n={n} offset={offset}")
+    qla_amd.call(v.name, *args, n)
+    qla_amd.check()
+    exp = np.ascontiguousarray(r0.copy())
+    orc.local(code, v.eqop, exp, np.ascontiguousarray(x), None if y is None else np.ascontiguousarray(y))
+    got = host(keep[0])[offset // r0.itemsize:].reshape(exp.shape)
+    assert_bitwise(got, exp, f"{v.name} offset {offset}")
+
+
 SUM_N = 1000
 
 
