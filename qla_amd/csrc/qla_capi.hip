@@ -373,6 +373,7 @@ void gamma_entry(int prec, int right, int ncomp, int pa, void *r, const void *a,
     p.ix = ia;
     p.n = n;
     p.px = pa;
+    p.al = ((uintptr_t)r % 16 == 0 && (uintptr_t)a % 16 == 0) ? 1 : 0;
     finish(qla_launch_gamma(prec, right, mu, ncomp, p, ts.stream), name);
 }
 

@@ -1418,27 +1418,68 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gamma(qla_vec_args p, gamma_tab 
     }
 }
 
-// v-form over fields: the run of m complexes, the multiplied spin index at
-// element stride SSTRIDE (1: D and a Gamma from the right; 12: P from the
-// left), so no record arithmetic is needed
-template <class T, int SSTRIDE>
-__global__ void __launch_bounds__(VBLOCK) k_vec_gamma_flat(cpx<T> *r, const cpx<T> *x, long long m, gamma_tab tab)
+// v-form over fields: a work item takes the four spin rows one destination
+// group needs, in loads of LW complexes (16 bytes: LW = 2 for 16-byte aligned
+// single-precision fields, else 1), and permutes them in registers.
+// SSTRIDE 1 (D, or P times Gamma from the right): the group is the 4 adjacent
+// spin components; SSTRIDE 12 (Gamma times P): LW adjacent complexes in each
+// of the 4 spin rows, 12 complexes apart in a 48-complex (ic, is) block.
+template <class T, int SSTRIDE, int LW>
+__global__ void __launch_bounds__(VBLOCK) k_vec_gamma_flat(cpx<T> *r, const cpx<T> *x, long long items, gamma_tab tab)
 {
-    typedef typename vtype<T, 1>::v V1;
+    typedef typename vtype<T, LW>::v VL;
+    constexpr int WW = SSTRIDE == 1 ? 1 : LW;       // complexes of one spin row in an item
+    constexpr int NCH = 4 * WW / LW;                // accesses of LW complexes per item
+    constexpr int IPB = 12 / LW;                    // items per 48-complex block (SSTRIDE 12)
     const long long stride = (long long)gridDim.x * VBLOCK;
-    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < m; g += stride) {
-        const int spin = (int)((g / SSTRIDE) & 3);
-        const int src = spin == 0 ? tab.src[0] : spin == 1 ? tab.src[1] : spin == 2 ? tab.src[2] : tab.src[3];
-        const int ph = spin == 0 ? tab.ph[0] : spin == 1 ? tab.ph[1] : spin == 2 ? tab.ph[2] : tab.ph[3];
-        const V1 v = __builtin_nontemporal_load((const V1 *)(x + g + (long long)(src - spin) * SSTRIDE));
-        V1 o;
-        switch (ph) {
-        case 0: o[0] = v[0]; o[1] = v[1]; break;
-        case 1: o[0] = -v[1]; o[1] = v[0]; break;
-        case 2: o[0] = -v[0]; o[1] = -v[1]; break;
-        default: o[0] = v[1]; o[1] = -v[0]; break;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < items; g += stride) {
+        const long long e = SSTRIDE == 1 ? g * 4 : (g / IPB) * 48 + (g % IPB) * LW;
+        T re[4 * WW], im[4 * WW];
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+            const VL v = __builtin_nontemporal_load((const VL *)(x + e + (SSTRIDE == 1 ? j * LW : j * 12)));
+#pragma unroll
+            for (int w = 0; w < LW; w++) {
+                re[j * LW + w] = v[2 * w];
+                im[j * LW + w] = v[2 * w + 1];
+            }
         }
-        __builtin_nontemporal_store(o, (V1 *)(r + g));
+        T ore[4 * WW], oim[4 * WW];
+#pragma unroll
+        for (int sp = 0; sp < 4; sp++) {
+            const int src = tab.src[sp], ph = tab.ph[sp];
+#pragma unroll
+            for (int w = 0; w < WW; w++) {
+                const T a = src == 0 ? re[w] : src == 1 ? re[WW + w] : src == 2 ? re[2 * WW + w] : re[3 * WW + w];
+                const T b = src == 0 ? im[w] : src == 1 ? im[WW + w] : src == 2 ? im[2 * WW + w] : im[3 * WW + w];
+                T orr, oi;
+                if (ph == 0) {          // QLA_c_eq_c
+                    orr = a;
+                    oi = b;
+                } else if (ph == 1) {   // QLA_c_eq_ic
+                    orr = -b;
+                    oi = a;
+                } else if (ph == 2) {   // QLA_c_eqm_c
+                    orr = -a;
+                    oi = -b;
+                } else {                // QLA_c_eqm_ic
+                    orr = b;
+                    oi = -a;
+                }
+                ore[sp * WW + w] = orr;
+                oim[sp * WW + w] = oi;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NCH; j++) {
+            VL v;
+#pragma unroll
+            for (int w = 0; w < LW; w++) {
+                v[2 * w] = ore[j * LW + w];
+                v[2 * w + 1] = oim[j * LW + w];
+            }
+            __builtin_nontemporal_store(v, (VL *)(r + e + (SSTRIDE == 1 ? j * LW : j * 12)));
+        }
     }
 }
 
@@ -1448,15 +1489,27 @@ hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hi
     if (p.n <= 0) return hipSuccess;
     const gamma_tab tab = gamma_table(g, right);
     if (!p.index && !p.ir && !p.ix && !p.px) {
+        if (ncomp != 12 && ncomp != 144) return hipErrorInvalidValue;
+        // m is a multiple of 12 (D) or 144 (P)
         const long long m = (long long)p.n * ncomp;
-        const dim3 grid(blocks_for(m));
-        if (!right && ncomp == 144)
-            hipLaunchKernelGGL((k_vec_gamma_flat<T, 12>), grid, dim3(VBLOCK), 0, st, (cpx<T> *)p.r,
-                               (const cpx<T> *)p.x, m, tab);
-        else if (ncomp == 12 || ncomp == 144)
-            hipLaunchKernelGGL((k_vec_gamma_flat<T, 1>), grid, dim3(VBLOCK), 0, st, (cpx<T> *)p.r,
-                               (const cpx<T> *)p.x, m, tab);
-        else return hipErrorInvalidValue;
+        cpx<T> *r = (cpx<T> *)p.r;
+        const cpx<T> *x = (const cpx<T> *)p.x;
+        const bool w2 = sizeof(T) == 4 && p.al;
+        if (!right && ncomp == 144) {
+            if (w2)
+                hipLaunchKernelGGL((k_vec_gamma_flat<T, 12, sizeof(T) == 4 ? 2 : 1>), dim3(blocks_for(m / 8)),
+                                   dim3(VBLOCK), 0, st, r, x, m / 8, tab);
+            else
+                hipLaunchKernelGGL((k_vec_gamma_flat<T, 12, 1>), dim3(blocks_for(m / 4)), dim3(VBLOCK), 0, st, r, x,
+                                   m / 4, tab);
+        } else {
+            if (w2)
+                hipLaunchKernelGGL((k_vec_gamma_flat<T, 1, sizeof(T) == 4 ? 2 : 1>), dim3(blocks_for(m / 4)),
+                                   dim3(VBLOCK), 0, st, r, x, m / 4, tab);
+            else
+                hipLaunchKernelGGL((k_vec_gamma_flat<T, 1, 1>), dim3(blocks_for(m / 4)), dim3(VBLOCK), 0, st, r, x,
+                                   m / 4, tab);
+        }
         return hipGetLastError();
     }
     qla_vec_args q = p;

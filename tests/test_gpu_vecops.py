@@ -306,6 +306,37 @@ def test_gamma_symbol(orc, v, mu):
     assert_bitwise(host(r), exp, f"{v.name} mu={mu}")
 
 
+
+GAMMA_OFFSET_CASES = [(v, mu, off) for v in gammas() if not (v.xf or v.pa or v.ir or v.ia) and v.prec == "F3"
+                      for mu in (3, 10, 15) for off in (0, 8)]
+
+
+@pytest.mark.parametrize("v,mu,offset", GAMMA_OFFSET_CASES, ids=lambda x: x if isinstance(x, int) else x.name)
+def test_gamma_flat_blocks(orc, v, mu, offset):
+    """The contiguous single-precision gamma forms over 1,001 sites on
+    16-byte aligned fields (two complexes an access) and on fields 8 bytes
+    off that alignment (one complex an access)."""
+    import torch
+    n = 1001
+    x = lattice.random_field(v.t, n, 51, dtype=np.float32)
+    r0 = lattice.random_field(v.t, n, 52, dtype=np.float32)
+    keep = []
+
+    def place(a):
+        flat = torch.cat([torch.zeros(offset // 4), torch.from_numpy(np.ascontiguousarray(a)).reshape(-1)]).to("cuda")
+        keep.append(flat)
+        return flat.data_ptr() + offset
+
+    rp, xp = place(r0), place(x)
+    say(f"{v.name} mu={mu} n={n} offset={offset}")
+    qla_amd.call(v.name, rp, xp, mu, n)
+    qla_amd.check()
+    exp = np.zeros_like(r0)
+    orc.gamma(v.right, mu, exp, x)
+    got = host(keep[0])[offset // 4:].reshape(exp.shape)
+    assert_bitwise(got, exp, f"{v.name} mu={mu} offset={offset}")
+
+
 def test_gamma_out_of_range_is_a_noop():
     cs = case("F3", "D")
     r = dev(cs.r0)
