@@ -449,7 +449,8 @@ hipError_t qla_launch_reduce_gather(int prec, int kind, int ncomp, const void *a
 // rpi = RBLOCK / ncr records are read per iteration, thread t taking column
 // t % ncr of record t / ncr (coalesced), and the rpi partial sums of a column
 // are added in order through LDS; records wider than the block (P: 288
-// reals) give each thread the columns t and t + RBLOCK.  Stage 2: one block
+// reals) give each thread the columns t and t + RBLOCK (contiguous fields
+// of even width read 16 bytes a lane instead: k_sum1v).  Stage 2: one block
 // per column adds that column's per-block partials in a fixed tree.
 // ---------------------------------------------------------------------------
 namespace qla {
@@ -515,6 +516,59 @@ k_sum1(const void *a, const int *index, int pa, long long n, int ncr, double *pa
     }
 }
 
+// Stage 1 over a contiguous field whose records are a whole number of
+// W-element vectors (16 bytes, or 8 for float pairs): the block's chunk
+// starts on a record, and thread t of the first TA = (RBLOCK / (ncr / W)) *
+// (ncr / W) threads reads vectors t, t + TA, ... of it, so it always holds the
+// same W columns (vector t % (ncr / W) of a record); the TA / (ncr / W)
+// partials of a column are added in thread order through LDS.
+template <class T, int W>
+__global__ void __launch_bounds__(RBLOCK)
+k_sum1v(const T *a, long long n, int ncr, double *partials)
+{
+    typedef T VT __attribute__((ext_vector_type(W)));
+    __shared__ double sh[RBLOCK * W];
+    const int t = threadIdx.x;
+    const int vpr = ncr / W;                       // vectors per record
+    const int ta = (RBLOCK / vpr) * vpr;
+    const long long per = (n + gridDim.x - 1) / gridDim.x;
+    const long long r0 = (long long)blockIdx.x * per;
+    const long long r1 = r0 + per < n ? r0 + per : n;
+    const VT *v = (const VT *)a;
+    double acc[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) acc[w] = 0.0;
+    if (t < ta && r0 < r1) {
+        const long long k1 = r1 * vpr;
+        long long k = r0 * vpr + t;
+        for (; k + 3 * ta < k1; k += 4 * ta) {
+            const VT x0 = __builtin_nontemporal_load(v + k), x1 = __builtin_nontemporal_load(v + k + ta);
+            const VT x2 = __builtin_nontemporal_load(v + k + 2 * ta), x3 = __builtin_nontemporal_load(v + k + 3 * ta);
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                acc[w] += (double)x0[w];
+                acc[w] += (double)x1[w];
+                acc[w] += (double)x2[w];
+                acc[w] += (double)x3[w];
+            }
+        }
+        for (; k < k1; k += ta) {
+            const VT x0 = __builtin_nontemporal_load(v + k);
+#pragma unroll
+            for (int w = 0; w < W; w++) acc[w] += (double)x0[w];
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++) sh[t * W + w] = acc[w];
+    __syncthreads();
+    // thread g * vpr + j holds columns j * W .. j * W + W - 1 at sh[g * ncr + j * W + w]
+    for (int c = t; c < ncr; c += RBLOCK) {
+        double s = 0.0;
+        for (int g = 0; g < ta / vpr; g++) s += sh[g * ncr + c];
+        partials[(size_t)blockIdx.x * ncr + c] = s;
+    }
+}
+
 // stage 2: block c adds column c's per-block partials in a fixed tree
 template <class TO>
 __global__ void __launch_bounds__(RBLOCK) k_sum2(const double *partials, int nb, int ncr, TO *out)
@@ -540,7 +594,16 @@ hipError_t qla_launch_sum(int prec_in, int out_double, int ncr, const void *a, c
     const int rpi = ncr <= RBLOCK ? RBLOCK / ncr : 1;
     long long want = (nn + (long long)rpi * 16 - 1) / ((long long)rpi * 16);
     const int nb = (int)(want < 1 ? 1 : (want > SUM_BLOCKS ? SUM_BLOCKS : want));
-    if (prec_in == 0) hipLaunchKernelGGL(k_sum1<float>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
+    const uintptr_t al = (uintptr_t)a;
+    const bool flat = !index && !pa;
+    if (flat && prec_in == 0 && ncr % 4 == 0 && al % 16 == 0)
+        hipLaunchKernelGGL((k_sum1v<float, 4>), dim3(nb), dim3(RBLOCK), 0, st, (const float *)a, nn, ncr, scratch);
+    else if (flat && prec_in == 0 && ncr % 2 == 0 && al % 8 == 0)
+        hipLaunchKernelGGL((k_sum1v<float, 2>), dim3(nb), dim3(RBLOCK), 0, st, (const float *)a, nn, ncr, scratch);
+    else if (flat && prec_in != 0 && ncr % 2 == 0 && al % 16 == 0)
+        hipLaunchKernelGGL((k_sum1v<double, 2>), dim3(nb), dim3(RBLOCK), 0, st, (const double *)a, nn, ncr, scratch);
+    else if (prec_in == 0)
+        hipLaunchKernelGGL(k_sum1<float>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
     else hipLaunchKernelGGL(k_sum1<double>, dim3(nb), dim3(RBLOCK), 0, st, a, index, pa, nn, ncr, scratch);
     if (out_double) hipLaunchKernelGGL(k_sum2<double>, dim3(ncr), dim3(RBLOCK), 0, st, scratch, nb, ncr, (double *)out);
     else hipLaunchKernelGGL(k_sum2<float>, dim3(ncr), dim3(RBLOCK), 0, st, scratch, nb, ncr, (float *)out);

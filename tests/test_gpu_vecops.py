@@ -467,6 +467,34 @@ def test_sum_symbol(orc, v):
     assert np.all(np.abs(out.astype(np.float64) - want) <= bound + 1e-300), v.name
 
 
+
+@pytest.mark.parametrize("name,t,n,offset", [
+    ("QLA_F3_d_veq_sum_D", "D", 100003, 0), ("QLA_F3_d_veq_sum_D", "D", 100003, 8),
+    ("QLA_DF3_d_veq_sum_D", "D", 100003, 0), ("QLA_D3_d_veq_sum_D", "D", 100003, 0),
+    ("QLA_F3_m_veq_sum_M", "M", 50001, 0), ("QLA_F3_m_veq_sum_M", "M", 50001, 4),
+    ("QLA_F3_p_veq_sum_P", "P", 3001, 0), ("QLA_D3_p_veq_sum_P", "P", 3001, 0)])
+def test_sum_contiguous_blocks(orc, name, t, n, offset):
+    """Contiguous sums over many blocks through the vector reads (16 bytes a
+    lane; 8 for float pairs on fields 8 bytes off 16-byte alignment; 4-byte
+    offset: the per-column reads), within the double-rounding bound."""
+    import torch
+    in_d = name.startswith("QLA_D3")
+    dt = np.float64 if in_d else np.float32
+    a = lattice.random_field(t, n, 93, dtype=np.float64).astype(dt)
+    flat = torch.cat([torch.zeros(offset // a.itemsize, dtype=torch.float64 if in_d else torch.float32),
+                      torch.from_numpy(np.ascontiguousarray(a)).reshape(-1)]).to("cuda")
+    out_d = not name.startswith("QLA_F3")
+    out = np.zeros(a.shape[1:], dtype=np.float64 if out_d else np.float32)
+    say(f"{name} n={n} offset={offset}")
+    qla_amd.call(name, out.ctypes.data, flat.data_ptr() + offset, n)
+    qla_amd.check()
+    want = orc.field_sum(np.ascontiguousarray(a))
+    bound = n * np.finfo(np.float64).eps * np.abs(a.astype(np.float64)).sum(axis=0)
+    if not out_d:
+        bound = bound + np.finfo(np.float32).eps * np.abs(want)
+    assert np.all(np.abs(out.astype(np.float64) - want) <= bound + 1e-300), name
+
+
 def test_sum_device_result_and_empty(orc):
     import torch
     a = lattice.random_field("D", SUM_N, 91, dtype=np.float32)
