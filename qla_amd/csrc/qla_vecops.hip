@@ -313,32 +313,62 @@ template <class T, int NC> struct tpose_shape {
     static constexpr int RB = NC == 9 ? QLA_VEC_TPOSE_M_RB : 8;
 };
 
-template <class T, int NC, int OP, int EQOP>
+// W complexes per global access: 2 (16 bytes) for 16-byte aligned
+// single-precision fields, else 1; RB is even, so a block's first complex
+// e0 is a multiple of W, and an odd complex count leaves one complex to
+// thread 0
+template <class T, int NC, int OP, int EQOP, int W>
 __global__ void __launch_bounds__(VBLOCK) k_vec_tposed(qla_vec_args p)
 {
     typedef tpose_shape<T, NC> S;
     typedef typename vtype<T, 1>::v V1;
-    __shared__ V1 tile[S::RB * NC];
+    typedef typename vtype<T, W>::v VW;
+    __shared__ VW tile[S::RB * NC / W];
+    const V1 *t1 = (const V1 *)tile;
     T sr, si;
     load_scalar<T, OP>(p, sr, si);
-    const V1 *X = (const V1 *)p.x;
-    V1 *R = (V1 *)p.r;
+    const VW *X = (const VW *)p.x;
+    VW *R = (VW *)p.r;
+    // destination complex kk of the block: element (c / D, c % D) of its
+    // record is source element (c % D, c / D)
+    auto one = [&](int kk, T &rr, T &ri) {
+        const int rec = kk / NC, c = kk - rec * NC;
+        const V1 x = t1[rec * NC + (c % S::D) * S::D + c / S::D];
+        vop<OP, EQOP, T>(rr, ri, x[0], p.conj ? -x[1] : x[1], T(0), T(0), sr, si);
+    };
     for (long long base = (long long)blockIdx.x * S::RB; base < p.n; base += (long long)gridDim.x * S::RB) {
         const int nrec = (int)(p.n - base < S::RB ? p.n - base : S::RB);
         const int tot = nrec * NC;
         const long long e0 = base * NC;
-        for (int k = threadIdx.x; k < tot; k += VBLOCK) tile[k] = __builtin_nontemporal_load(X + e0 + k);
+        for (int k = threadIdx.x; k < tot / W; k += VBLOCK) tile[k] = __builtin_nontemporal_load(X + e0 / W + k);
+        if constexpr (W == 2) {
+            if ((tot & 1) && threadIdx.x == 0)
+                ((V1 *)tile)[tot - 1] = __builtin_nontemporal_load((const V1 *)p.x + e0 + tot - 1);
+        }
         __syncthreads();
-        for (int k = threadIdx.x; k < tot; k += VBLOCK) {
-            const int rec = k / NC, c = k - rec * NC;
-            const V1 x = tile[rec * NC + (c % S::D) * S::D + c / S::D];
-            V1 r = {};
-            if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load(R + e0 + k);
-            T rr = r[0], ri = r[1];
-            vop<OP, EQOP, T>(rr, ri, x[0], p.conj ? -x[1] : x[1], T(0), T(0), sr, si);
-            r[0] = rr;
-            r[1] = ri;
-            __builtin_nontemporal_store(r, R + e0 + k);
+        for (int k = threadIdx.x; k < tot / W; k += VBLOCK) {
+            VW r = {};
+            if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load(R + e0 / W + k);
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                T rr = r[2 * w], ri = r[2 * w + 1];
+                one(k * W + w, rr, ri);
+                r[2 * w] = rr;
+                r[2 * w + 1] = ri;
+            }
+            __builtin_nontemporal_store(r, R + e0 / W + k);
+        }
+        if constexpr (W == 2) {
+            if ((tot & 1) && threadIdx.x == 0) {
+                V1 *r1 = (V1 *)p.r + e0 + tot - 1;
+                V1 r = {};
+                if constexpr (reads_r<EQOP>()) r = __builtin_nontemporal_load(r1);
+                T rr = r[0], ri = r[1];
+                one(tot - 1, rr, ri);
+                r[0] = rr;
+                r[1] = ri;
+                __builtin_nontemporal_store(r, r1);
+            }
         }
         __syncthreads();
     }
@@ -419,8 +449,14 @@ hipError_t launch_op(int ncomp, const qla_vec_args &p, hipStream_t st)
             const long long rb = ncomp == 9 ? tpose_shape<T, 9>::RB : tpose_shape<T, 144>::RB;
             const long long want = (p.n + rb - 1) / rb;
             const dim3 grid((unsigned)(want > VMAXBLOCKS ? VMAXBLOCKS : want));
-            if (ncomp == 9) hipLaunchKernelGGL((k_vec_tposed<T, 9, OP, EQOP>), grid, dim3(VBLOCK), 0, st, q);
-            else hipLaunchKernelGGL((k_vec_tposed<T, 144, OP, EQOP>), grid, dim3(VBLOCK), 0, st, q);
+            constexpr int W = sizeof(T) == 4 ? 2 : 1;
+            if (sizeof(T) == 4 && p.al) {
+                if (ncomp == 9) hipLaunchKernelGGL((k_vec_tposed<T, 9, OP, EQOP, W>), grid, dim3(VBLOCK), 0, st, q);
+                else hipLaunchKernelGGL((k_vec_tposed<T, 144, OP, EQOP, W>), grid, dim3(VBLOCK), 0, st, q);
+            } else {
+                if (ncomp == 9) hipLaunchKernelGGL((k_vec_tposed<T, 9, OP, EQOP, 1>), grid, dim3(VBLOCK), 0, st, q);
+                else hipLaunchKernelGGL((k_vec_tposed<T, 144, OP, EQOP, 1>), grid, dim3(VBLOCK), 0, st, q);
+            }
             return hipGetLastError();
         }
     }

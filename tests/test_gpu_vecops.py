@@ -273,6 +273,44 @@ def test_fill_symbol(orc, v, where):
     assert_bitwise(host(r), exp, f"{v.name} {where}")
 
 
+
+def tposed_cases():
+    """(symbol, byte offset): the contiguous single-precision transposed and
+    adjoint forms of one source, on 16-byte aligned fields (two complexes an
+    access) and 8 bytes off (one complex an access)."""
+    vs = [v for v in vecops() if v.prec == "F3" and v.xform & 2 and v.nsrc == 1
+          and not (v.ir or v.ix or v.iy or v.px or v.py or v.xf)]
+    return [(v, o) for v in vs for o in (0, 8)]
+
+
+@pytest.mark.parametrize("v,offset", tposed_cases(), ids=lambda x: x if isinstance(x, int) else x.name)
+def test_tposed_blocks(orc, v, offset):
+    """Transposed forms over 1,001 records (several LDS block steps and an
+    odd complex count for colour matrices) against the oracle, bit for bit."""
+    import torch
+    cs = Case(v.prec, v.t, n=1001)
+    keep = []
+
+    def place(a):
+        flat = torch.cat([torch.zeros(offset // 4), torch.from_numpy(np.ascontiguousarray(a)).reshape(-1)]).to("cuda")
+        keep.append(flat)
+        return flat.data_ptr() + offset
+
+    args = [place(cs.r0)]
+    sp, sval = cs.scalar(v)
+    if v.scalar:
+        args.append(sp)
+    args.append(place(cs.x))
+    say(f"{v.name} n={cs.n} offset={offset}")
+    qla_amd.call(v.name, *args, cs.n)
+    qla_amd.check()
+    exp = np.ascontiguousarray(cs.r0.copy())
+    orc.vecop(VECFAMS[v.fam][0], v.eqop, exp, np.ascontiguousarray(cs.x), None, s=sval, conj=bool(v.xform & 1),
+              dim=TDIM[v.t])
+    got = host(keep[0])[offset // 4:].reshape(exp.shape)
+    assert_bitwise(got, exp, f"{v.name} offset={offset}")
+
+
 GAMMA_CASES = [(v, mu) for v in gammas() for mu in (range(16) if not (v.xf or v.pa or v.ir or v.ia) else (5, 15))]
 
 
