@@ -1454,9 +1454,11 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gamma(qla_vec_args p, gamma_tab 
     }
 }
 
-// v-form over fields: a work item takes the four spin rows one destination
-// group needs, in loads of LW complexes (16 bytes: LW = 2 for 16-byte aligned
-// single-precision fields, else 1), and permutes them in registers.
+// v-form over 16-byte aligned single-precision fields: a work item takes the
+// four spin rows one destination group needs, in 16-byte loads of LW = 2
+// complexes, and permutes them in registers (F3 D: 5.3 TB/s at 32^4, against
+// 4.6-4.9 for one complex per work item; two groups per work item measured
+// 0.96x, four 0.92x).
 // SSTRIDE 1 (D, or P times Gamma from the right): the group is the 4 adjacent
 // spin components; SSTRIDE 12 (Gamma times P): LW adjacent complexes in each
 // of the 4 spin rows, 12 complexes apart in a 48-complex (ic, is) block.
@@ -1519,6 +1521,31 @@ __global__ void __launch_bounds__(VBLOCK) k_vec_gamma_flat(cpx<T> *r, const cpx<
     }
 }
 
+// One complex per work item, each reading its source complex: every wave
+// access is contiguous.  Double fields (whose 16-byte complexes would put a
+// group's four accesses 64 bytes apart across lanes: 2.8 TB/s against 6.3 for
+// this form at 32^4) and single-precision fields off 16-byte alignment.
+template <class T, int SSTRIDE>
+__global__ void __launch_bounds__(VBLOCK) k_vec_gamma_flat1(cpx<T> *r, const cpx<T> *x, long long m, gamma_tab tab)
+{
+    typedef typename vtype<T, 1>::v V1;
+    const long long stride = (long long)gridDim.x * VBLOCK;
+    for (long long g = (long long)blockIdx.x * VBLOCK + threadIdx.x; g < m; g += stride) {
+        const int spin = (int)((g / SSTRIDE) & 3);
+        const int src = spin == 0 ? tab.src[0] : spin == 1 ? tab.src[1] : spin == 2 ? tab.src[2] : tab.src[3];
+        const int ph = spin == 0 ? tab.ph[0] : spin == 1 ? tab.ph[1] : spin == 2 ? tab.ph[2] : tab.ph[3];
+        const V1 v = __builtin_nontemporal_load((const V1 *)(x + g + (long long)(src - spin) * SSTRIDE));
+        V1 o;
+        switch (ph) {
+        case 0: o[0] = v[0]; o[1] = v[1]; break;        // QLA_c_eq_c
+        case 1: o[0] = -v[1]; o[1] = v[0]; break;       // QLA_c_eq_ic
+        case 2: o[0] = -v[0]; o[1] = -v[1]; break;      // QLA_c_eqm_c
+        default: o[0] = v[1]; o[1] = -v[0]; break;      // QLA_c_eqm_ic
+        }
+        __builtin_nontemporal_store(o, (V1 *)(r + g));
+    }
+}
+
 template <class T>
 hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hipStream_t st)
 {
@@ -1536,15 +1563,13 @@ hipError_t launch_gamma_t(int right, int g, int ncomp, const qla_vec_args &p, hi
                 hipLaunchKernelGGL((k_vec_gamma_flat<T, 12, sizeof(T) == 4 ? 2 : 1>), dim3(blocks_for(m / 8)),
                                    dim3(VBLOCK), 0, st, r, x, m / 8, tab);
             else
-                hipLaunchKernelGGL((k_vec_gamma_flat<T, 12, 1>), dim3(blocks_for(m / 4)), dim3(VBLOCK), 0, st, r, x,
-                                   m / 4, tab);
+                hipLaunchKernelGGL((k_vec_gamma_flat1<T, 12>), dim3(blocks_for(m)), dim3(VBLOCK), 0, st, r, x, m, tab);
         } else {
             if (w2)
                 hipLaunchKernelGGL((k_vec_gamma_flat<T, 1, sizeof(T) == 4 ? 2 : 1>), dim3(blocks_for(m / 4)),
                                    dim3(VBLOCK), 0, st, r, x, m / 4, tab);
             else
-                hipLaunchKernelGGL((k_vec_gamma_flat<T, 1, 1>), dim3(blocks_for(m / 4)), dim3(VBLOCK), 0, st, r, x,
-                                   m / 4, tab);
+                hipLaunchKernelGGL((k_vec_gamma_flat1<T, 1>), dim3(blocks_for(m)), dim3(VBLOCK), 0, st, r, x, m, tab);
         }
         return hipGetLastError();
     }
